@@ -1,0 +1,783 @@
+// C ABI of libctsim_hip.so (declared in include/ctsim_amd.h).
+//
+// The MEX-signature gateways reproduce the dispatch and validation of
+// geom/fbct_os/fbct_geom_mex.c:75-326 and geom/cbct_os/cbct_geom_mex.c:83-600 on top
+// of the device plan API; the plan API keeps geometry and the per-view table resident
+// on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/ctsim_amd.h"
+#include "internal.h"
+
+using namespace ctsim;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? CTSIM_ENOMEM : CTSIM_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                           \
+    do {                                              \
+        hipError_t e_ = (expr);                       \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+void copy_err(char* errbuf, size_t errlen) {
+    if (errbuf && errlen) {
+        std::strncpy(errbuf, g_err.c_str(), errlen - 1);
+        errbuf[errlen - 1] = '\0';
+    }
+}
+
+bool valid_dtype(int dt) { return dt == CTSIM_F32 || dt == CTSIM_F16 || dt == CTSIM_BF16 || dt == CTSIM_F64; }
+
+// Scalar geometry in compute type T, rounded the way the reference rounds it:
+// w* are evaluated in double and cast (geom/cbct_os/cbct_proj.c:454-458), sdd_du and
+// (dz/dv)*sdd in T.
+template <class T>
+Geom<T> make_geom(const ctsim_geom& g) {
+    Geom<T> k;
+    k.nx = g.nx; k.ny = g.ny; k.nz = g.nz; k.nu = g.nu; k.nv = g.nv; k.dettype = g.detector_type;
+    k.dx = (T)g.dx; k.dy = (T)g.dy; k.dz = (T)g.dz; k.du = (T)g.du; k.dv = (T)g.dv;
+    k.sad = (T)g.sad; k.sdd = (T)g.sdd;
+    k.sdd_du = (T)g.sdd / (T)g.du;
+    k.zk = (T)g.dz / (T)g.dv * (T)g.sdd;
+    k.wx = (T)(-(g.nx - 1.0) / 2.0 + (double)g.ox);
+    k.wy = (T)(-(g.ny - 1.0) / 2.0 + (double)g.oy);
+    k.wu = (T)(-(g.nu - 1.0) / 2.0 + (double)g.ou);
+    k.wv = (T)(-(g.nv - 1.0) / 2.0 + (double)g.ov);
+    k.ou = (T)g.ou;
+    return k;
+}
+
+template <class T>
+std::vector<View<T>> make_views(const ctsim_geom& g, int nviews, const double* betas, const float* couchz) {
+    std::vector<View<T>> v(nviews);
+    const T dx = (T)g.dx, dy = (T)g.dy;
+    for (int b = 0; b < nviews; ++b) {
+        View<T>& e = v[b];
+        e.s = (T)std::sin(betas[b]);
+        e.c = (T)std::cos(betas[b]);
+        e.dxs2 = dx * e.s / T(2);
+        e.dxc2 = dx * e.c / T(2);
+        e.dys2 = dy * e.s / T(2);
+        e.ndyc2 = -dy * e.c / T(2);
+        const T cz = couchz ? (T)couchz[b] : T(0);
+        e.wz = (T)(-(g.nz - 1.0) / 2.0 + (double)(T)g.oz + (double)(cz / (T)g.dz));
+        e.pad = T(0);
+    }
+    return v;
+}
+
+int check_geom(int ndim, const ctsim_geom& g) {
+    if (ndim != 2 && ndim != 3) return fail(CTSIM_EINVAL, "Error: geometry must be 2D or 3D.");
+    if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0 || g.nu <= 0 || g.nv <= 0)
+        return fail(CTSIM_EINVAL, "Error: image and detector sizes must be positive.");
+    if (ndim == 2 && (g.nz != 1 || g.nv != 1)) return fail(CTSIM_EINVAL, "Error: 2D geometry needs nz = nv = 1.");
+    if (!(g.du != 0.f) || !(g.dx != 0.f) || !(g.dy != 0.f) || (ndim == 3 && (!(g.dz != 0.f) || !(g.dv != 0.f))))
+        return fail(CTSIM_EINVAL, "Error: pixel and detector spacings must be nonzero.");
+    if ((double)g.nx * g.ny * g.nz > 4.0e9 || (double)g.nu * g.nv > 2.0e9)
+        return fail(CTSIM_EINVAL, "Error: problem size exceeds 32-bit indexing of this build.");
+    return CTSIM_OK;
+}
+
+enum OpKind { OP_PROJ_DD, OP_PROJ_TF, OP_BACK_DD, OP_BACK_TF, OP_BACK_PD, OP_BACK_HPD, OP_BAD };
+
+OpKind parse_op(const char* op) {
+    if (!op) return OP_BAD;
+    if (!std::strcmp(op, "proj,dd")) return OP_PROJ_DD;
+    if (!std::strcmp(op, "proj,tf")) return OP_PROJ_TF;
+    if (!std::strcmp(op, "back,dd")) return OP_BACK_DD;
+    if (!std::strcmp(op, "back,tf")) return OP_BACK_TF;
+    if (!std::strcmp(op, "back,pd")) return OP_BACK_PD;
+    if (!std::strcmp(op, "back,hpd")) return OP_BACK_HPD;
+    return OP_BAD;
+}
+
+}  // namespace
+
+struct ctsim_plan {
+    int ndim = 0;
+    int compute = CTSIM_F32;
+    int nviews = 0;
+    int device = 0;
+    ctsim_geom g{};
+    Geom<float> gf{};
+    Geom<double> gd{};
+    View<float>* d_vf = nullptr;
+    View<double>* d_vd = nullptr;
+    uint8_t* d_map = nullptr;
+};
+
+extern "C" {
+
+const char* ctsim_last_error(void) { return g_err.c_str(); }
+int ctsim_version(void) { return 100; }
+
+// ---------------------------------------------------------------------------------
+// plans
+// ---------------------------------------------------------------------------------
+int ctsim_plan_create(ctsim_plan** out, int32_t ndim, const ctsim_geom* g, int32_t nviews, const double* betas,
+                      const float* couchz, const uint8_t* map, int32_t compute) {
+    if (!out || !g) return fail(CTSIM_EINVAL, "Error: null plan or geometry.");
+    *out = nullptr;
+    if (int rc = check_geom(ndim, *g)) return rc;
+    if (nviews <= 0 || !betas) return fail(CTSIM_EINVAL, "Error: mx_betas must have noviews X 1 double array.");
+    if (compute != CTSIM_F32 && compute != CTSIM_F64)
+        return fail(CTSIM_EINVAL, "Error: compute precision must be CTSIM_F32 or CTSIM_F64.");
+    ctsim_plan* p = new (std::nothrow) ctsim_plan;
+    if (!p) return fail(CTSIM_ENOMEM, "Error: host allocation failed.");
+    p->ndim = ndim;
+    p->compute = compute;
+    p->nviews = nviews;
+    p->g = *g;
+    HIP_TRY(hipGetDevice(&p->device), "hipGetDevice");
+    hipError_t e = reduction_scratch_init();
+    if (e != hipSuccess) {
+        delete p;
+        return hip_fail(e, "reduction scratch");
+    }
+    if (compute == CTSIM_F32) {
+        p->gf = make_geom<float>(*g);
+        auto v = make_views<float>(*g, nviews, betas, couchz);
+        e = hipMalloc((void**)&p->d_vf, v.size() * sizeof(View<float>));
+        if (e == hipSuccess) e = hipMemcpy(p->d_vf, v.data(), v.size() * sizeof(View<float>), hipMemcpyHostToDevice);
+    } else {
+        p->gd = make_geom<double>(*g);
+        auto v = make_views<double>(*g, nviews, betas, couchz);
+        e = hipMalloc((void**)&p->d_vd, v.size() * sizeof(View<double>));
+        if (e == hipSuccess) e = hipMemcpy(p->d_vd, v.data(), v.size() * sizeof(View<double>), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && map && ndim == 3) {
+        const size_t nm = (size_t)g->nx * g->ny;
+        e = hipMalloc((void**)&p->d_map, nm);
+        if (e == hipSuccess) e = hipMemcpy(p->d_map, map, nm, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        ctsim_plan_destroy(p);
+        return hip_fail(e, "plan upload");
+    }
+    *out = p;
+    return CTSIM_OK;
+}
+
+int ctsim_plan_destroy(ctsim_plan* p) {
+    if (!p) return CTSIM_OK;
+    if (p->d_vf) (void)hipFree(p->d_vf);
+    if (p->d_vd) (void)hipFree(p->d_vd);
+    if (p->d_map) (void)hipFree(p->d_map);
+    delete p;
+    return CTSIM_OK;
+}
+
+static int check_views(const ctsim_plan* p, int view0, int vstep, int nsub) {
+    if (nsub <= 0 || vstep <= 0 || view0 < 0 || (long long)view0 + (long long)(nsub - 1) * vstep >= p->nviews)
+        return fail(CTSIM_EINVAL, "Error: view selection (%d, %d, %d) outside the plan's %d views.", view0, vstep,
+                    nsub, p->nviews);
+    return CTSIM_OK;
+}
+
+static int plan_proj_impl(const ctsim_plan* p, const char* op, int view0, int vstep, int nsub, const void* d_img,
+                          int img_dt, void* d_sino, int sino_dt, void* stream, int exhaustive) {
+    if (!p) return fail(CTSIM_EINVAL, "Error: null plan.");
+    const OpKind k = parse_op(op);
+    if (k != OP_PROJ_DD && k != OP_PROJ_TF) return fail(CTSIM_EINVAL, "Error: unknown geom type.");
+    if (int rc = check_views(p, view0, vstep, nsub)) return rc;
+    if (!d_img || !d_sino) return fail(CTSIM_EINVAL, "Error: null device buffer.");
+    if (!valid_dtype(img_dt) || !valid_dtype(sino_dt)) return fail(CTSIM_EINVAL, "Error: bad dtype.");
+    hipStream_t st = (hipStream_t)stream;
+    const int foot = k == OP_PROJ_DD ? 0 : 1;
+    hipError_t e;
+    if (p->compute == CTSIM_F32) {
+        if (p->ndim == 3)
+            e = cbct_proj_launch<float>(p->gf, p->d_vf + view0, vstep, nsub, d_img, img_dt, p->d_map, d_sino, sino_dt,
+                                        foot, exhaustive, st);
+        else
+            e = fbct_proj_launch<float>(p->gf, p->d_vf + view0, vstep, nsub, d_img, img_dt, d_sino, sino_dt, foot,
+                                        exhaustive, st);
+    } else {
+        if (p->ndim == 3)
+            e = cbct_proj_launch<double>(p->gd, p->d_vd + view0, vstep, nsub, d_img, img_dt, p->d_map, d_sino, sino_dt,
+                                         foot, exhaustive, st);
+        else
+            e = fbct_proj_launch<double>(p->gd, p->d_vd + view0, vstep, nsub, d_img, img_dt, d_sino, sino_dt, foot,
+                                         exhaustive, st);
+    }
+    if (e == hipErrorInvalidValue)
+        return fail(CTSIM_EUNSUP, "Error: dtype combination (img %d, sino %d) not supported for compute %d.", img_dt,
+                    sino_dt, p->compute);
+    HIP_TRY(e, "forward projection launch");
+    return CTSIM_OK;
+}
+
+int ctsim_plan_proj(const ctsim_plan* p, const char* op, int32_t view0, int32_t vstep, int32_t nsub, const void* d_img,
+                    int32_t img_dt, void* d_sino, int32_t sino_dt, void* stream) {
+    return plan_proj_impl(p, op, view0, vstep, nsub, d_img, img_dt, d_sino, sino_dt, stream, 0);
+}
+
+int ctsim_plan_proj_exhaustive(const ctsim_plan* p, const char* op, int32_t view0, int32_t vstep, int32_t nsub,
+                               const void* d_img, int32_t img_dt, void* d_sino, int32_t sino_dt, void* stream) {
+    return plan_proj_impl(p, op, view0, vstep, nsub, d_img, img_dt, d_sino, sino_dt, stream, 1);
+}
+
+int ctsim_plan_back(const ctsim_plan* p, const char* op, int32_t view0, int32_t vstep, int32_t nsub,
+                    const void* d_sino, int32_t sino_dt, void* d_img, int32_t img_dt, double scale, int32_t accumulate,
+                    void* stream) {
+    if (!p) return fail(CTSIM_EINVAL, "Error: null plan.");
+    const OpKind k = parse_op(op);
+    if (k != OP_BACK_DD && k != OP_BACK_TF && k != OP_BACK_PD && k != OP_BACK_HPD)
+        return fail(CTSIM_EINVAL, "Error: unknown geom type.");
+    if (p->ndim == 2 && k == OP_BACK_HPD) return fail(CTSIM_EINVAL, "Error: unknown geom type.");
+    if (int rc = check_views(p, view0, vstep, nsub)) return rc;
+    if (!d_img || !d_sino) return fail(CTSIM_EINVAL, "Error: null device buffer.");
+    if (!valid_dtype(img_dt) || !valid_dtype(sino_dt)) return fail(CTSIM_EINVAL, "Error: bad dtype.");
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    const int kind = k == OP_BACK_DD ? 0 : k == OP_BACK_TF ? 1 : 2;
+    const int off_boundary = k == OP_BACK_PD ? 1 : 0;
+    if (p->compute == CTSIM_F32) {
+        if (p->ndim == 3) {
+            if (kind == 2)
+                e = cbct_back_pd_launch<float>(p->gf, p->d_vf + view0, vstep, nsub, d_sino, sino_dt, p->d_map, d_img,
+                                               img_dt, off_boundary, (float)scale, accumulate, st);
+            else
+                e = cbct_back_launch<float>(p->gf, p->d_vf + view0, vstep, nsub, d_sino, sino_dt, p->d_map, d_img,
+                                            img_dt, kind, (float)scale, accumulate, st);
+        } else {
+            if (accumulate) return fail(CTSIM_EUNSUP, "Error: 2D backprojection does not accumulate.");
+            e = fbct_back_launch<float>(p->gf, p->d_vf + view0, vstep, nsub, d_sino, sino_dt, d_img, img_dt, kind,
+                                        (float)scale, st);
+        }
+    } else {
+        if (p->ndim == 3) {
+            if (kind == 2)
+                e = cbct_back_pd_launch<double>(p->gd, p->d_vd + view0, vstep, nsub, d_sino, sino_dt, p->d_map, d_img,
+                                                img_dt, off_boundary, scale, accumulate, st);
+            else
+                e = cbct_back_launch<double>(p->gd, p->d_vd + view0, vstep, nsub, d_sino, sino_dt, p->d_map, d_img,
+                                             img_dt, kind, scale, accumulate, st);
+        } else {
+            if (accumulate) return fail(CTSIM_EUNSUP, "Error: 2D backprojection does not accumulate.");
+            e = fbct_back_launch<double>(p->gd, p->d_vd + view0, vstep, nsub, d_sino, sino_dt, d_img, img_dt, kind,
+                                         scale, st);
+        }
+    }
+    if (e == hipErrorInvalidValue)
+        return fail(CTSIM_EUNSUP, "Error: dtype combination (sino %d, img %d) not supported for compute %d.", sino_dt,
+                    img_dt, p->compute);
+    HIP_TRY(e, "backprojection launch");
+    return CTSIM_OK;
+}
+
+int ctsim_xyz_to_zxy(const void* src, int32_t sdt, void* dst, int32_t ddt, int32_t nx, int32_t ny, int32_t nz,
+                     void* stream) {
+    if (!src || !dst || nx <= 0 || ny <= 0 || nz <= 0) return fail(CTSIM_EINVAL, "Error: bad layout arguments.");
+    hipError_t e = xyz_zxy_launch(src, sdt, dst, ddt, nx * ny, nz, true, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype pair not supported.");
+    HIP_TRY(e, "layout conversion");
+    return CTSIM_OK;
+}
+
+int ctsim_zxy_to_xyz(const void* src, int32_t sdt, void* dst, int32_t ddt, int32_t nx, int32_t ny, int32_t nz,
+                     void* stream) {
+    if (!src || !dst || nx <= 0 || ny <= 0 || nz <= 0) return fail(CTSIM_EINVAL, "Error: bad layout arguments.");
+    hipError_t e = xyz_zxy_launch(src, sdt, dst, ddt, nx * ny, nz, false, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype pair not supported.");
+    HIP_TRY(e, "layout conversion");
+    return CTSIM_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// MEX-signature gateways
+// ---------------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+// device buffers reused across gateway calls (grown on demand, freed at exit)
+struct Pool {
+    void* buf[3] = {nullptr, nullptr, nullptr};
+    size_t cap[3] = {0, 0, 0};
+    int dev[3] = {-1, -1, -1};
+    ~Pool() {
+        for (int i = 0; i < 3; ++i)
+            if (buf[i]) (void)hipFree(buf[i]);
+    }
+    hipError_t get(int i, size_t bytes, void** out) {
+        int d = 0;
+        hipError_t e = hipGetDevice(&d);
+        if (e != hipSuccess) return e;
+        if (cap[i] < bytes || dev[i] != d) {
+            if (buf[i]) (void)hipFree(buf[i]);
+            buf[i] = nullptr;
+            cap[i] = 0;
+            e = hipMalloc(&buf[i], bytes);
+            if (e != hipSuccess) return e;
+            cap[i] = bytes;
+            dev[i] = d;
+        }
+        *out = buf[i];
+        return hipSuccess;
+    }
+};
+Pool g_pool;
+std::mutex g_gateway_mutex;  // the MEX gateway is not re-entrant; neither is this one
+
+struct PlanGuard {
+    ctsim_plan* p = nullptr;
+    ~PlanGuard() { ctsim_plan_destroy(p); }
+};
+
+int gateway(int ndim, const char* op, const ctsim_geom& g, int32_t noviews, const double* betas, const float* couchz,
+            const uint8_t* map, const float* in, float* out) {
+    const OpKind k = parse_op(op);
+    if (k == OP_BAD || (ndim == 2 && k == OP_BACK_HPD)) return fail(CTSIM_EINVAL, "Error: unknown geom type.");
+    if (!in || !out) return fail(CTSIM_EINVAL, "Error: null data pointer.");
+    if (noviews <= 0 || !betas) return fail(CTSIM_EINVAL, "Error: mx_betas must have noviews X 1 double array.");
+    std::lock_guard<std::mutex> lock(g_gateway_mutex);
+    PlanGuard pg;
+    if (int rc = ctsim_plan_create(&pg.p, ndim, &g, noviews, betas, couchz, map, CTSIM_F32)) return rc;
+    const bool proj = k == OP_PROJ_DD || k == OP_PROJ_TF;
+    const size_t nimg = (size_t)g.nx * g.ny * g.nz;
+    const size_t nsino = (size_t)g.nu * g.nv * noviews;
+    void *d_in = nullptr, *d_out = nullptr, *d_tmp = nullptr;
+    HIP_TRY(g_pool.get(0, (proj ? nimg : nsino) * 4, &d_in), "device buffer");
+    HIP_TRY(g_pool.get(1, (proj ? nsino : nimg) * 4, &d_out), "device buffer");
+    if (ndim == 3) HIP_TRY(g_pool.get(2, nimg * 4, &d_tmp), "device buffer");
+    hipStream_t st = nullptr;
+    HIP_TRY(hipMemcpy(d_in, in, (proj ? nimg : nsino) * 4, hipMemcpyHostToDevice), "upload");
+    int rc;
+    if (proj) {
+        const void* vol = d_in;
+        if (ndim == 3) {
+            HIP_TRY(xyz_zxy_launch(d_in, DT_F32, d_tmp, DT_F32, g.nx * g.ny, g.nz, true, st), "layout");
+            vol = d_tmp;
+        }
+        rc = ctsim_plan_proj(pg.p, op, 0, 1, noviews, vol, CTSIM_F32, d_out, CTSIM_F32, st);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(out, d_out, nsino * 4, hipMemcpyDeviceToHost), "download");
+    } else {
+        void* vol = ndim == 3 ? d_tmp : d_out;
+        rc = ctsim_plan_back(pg.p, op, 0, 1, noviews, d_in, CTSIM_F32, vol, CTSIM_F32, 1.0, 0, st);
+        if (rc) return rc;
+        if (ndim == 3) HIP_TRY(xyz_zxy_launch(d_tmp, DT_F32, d_out, DT_F32, g.nx * g.ny, g.nz, false, st), "layout");
+        HIP_TRY(hipMemcpy(out, d_out, nimg * 4, hipMemcpyDeviceToHost), "download");
+    }
+    return CTSIM_OK;
+}
+
+ctsim_geom geom3_from_mex(const int32_t* nxyz, const float* dxyz, const float* cxyz, const int32_t* nuv,
+                          const float* duv, const float* cuv, float sad, float add, float sdd, int32_t dettype) {
+    ctsim_geom g;
+    g.nx = nxyz[0]; g.ny = nxyz[1]; g.nz = nxyz[2];
+    g.dx = dxyz[0]; g.dy = dxyz[1]; g.dz = dxyz[2];
+    g.ox = cxyz[0]; g.oy = cxyz[1]; g.oz = cxyz[2];
+    g.nu = nuv[0]; g.nv = nuv[1];
+    g.du = duv[0]; g.dv = duv[1];
+    g.ou = cuv[0]; g.ov = cuv[1];
+    g.sad = sad; g.add = add; g.sdd = sdd;
+    g.detector_type = dettype;
+    return g;
+}
+
+int cbct_gateway_restricted(const char* allowed0, const char* allowed1, const char* op, const int32_t* nxyz,
+                            const float* dxyz, const float* cxyz, const int32_t* nuv, const float* duv,
+                            const float* cuv, float sad, float add, float sdd, int32_t noviews, const double* betas,
+                            const float* couchz, int32_t dettype, const float* in, const uint8_t* map, float* out,
+                            char* errbuf, size_t errlen) {
+    int rc;
+    if (!op || (std::strcmp(op, allowed0) && std::strcmp(op, allowed1)))
+        rc = fail(CTSIM_EINVAL, "Error: unknown geom type.");
+    else if (!nxyz || !dxyz || !cxyz || !nuv || !duv || !cuv || !couchz || !map)
+        rc = fail(CTSIM_EINVAL, "Error: incorrect number of arguments.");
+    else
+        rc = gateway(3, op, geom3_from_mex(nxyz, dxyz, cxyz, nuv, duv, cuv, sad, add, sdd, dettype), noviews, betas,
+                     couchz, map, in, out);
+    if (rc) copy_err(errbuf, errlen);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ctsim_fbct_geom_mex(const char* op, const int32_t nxy[2], const float dxy[2], const float oxy[2], int32_t nu,
+                        float du, float ou, float sad, float add, float sdd, int32_t noviews, const double* betas,
+                        int32_t dettype, const float* in, float* out, char* errbuf, size_t errlen) {
+    int rc;
+    if (!nxy || !dxy || !oxy) {
+        rc = fail(CTSIM_EINVAL, "Error: incorrect number of arguments.");
+    } else {
+        ctsim_geom g;
+        g.nx = nxy[0]; g.ny = nxy[1]; g.nz = 1;
+        g.dx = dxy[0]; g.dy = dxy[1]; g.dz = 1.f;
+        g.ox = oxy[0]; g.oy = oxy[1]; g.oz = 0.f;
+        g.nu = nu; g.nv = 1; g.du = du; g.dv = 1.f; g.ou = ou; g.ov = 0.f;
+        g.sad = sad; g.add = add; g.sdd = sdd; g.detector_type = dettype;
+        rc = gateway(2, op, g, noviews, betas, nullptr, nullptr, in, out);
+    }
+    if (rc) copy_err(errbuf, errlen);
+    return rc;
+}
+
+int ctsim_cbct_geom_mex(const char* op, const int32_t nxyz[3], const float dxyz[3], const float cxyz[3],
+                        const int32_t nuv[2], const float duv[2], const float cuv[2], float sad, float add, float sdd,
+                        int32_t noviews, const double* betas, const float* couchz, int32_t dettype, const float* in,
+                        const uint8_t* map, float* out, char* errbuf, size_t errlen) {
+    int rc;
+    if (!nxyz || !dxyz || !cxyz || !nuv || !duv || !cuv || !couchz)
+        rc = fail(CTSIM_EINVAL, "Error: incorrect number of arguments.");
+    else
+        rc = gateway(3, op, geom3_from_mex(nxyz, dxyz, cxyz, nuv, duv, cuv, sad, add, sdd, dettype), noviews, betas,
+                     couchz, map, in, out);
+    if (rc) copy_err(errbuf, errlen);
+    return rc;
+}
+
+int ctsim_cbct_dd_cuda(const char* op, const int32_t nxyz[3], const float dxyz[3], const float cxyz[3],
+                       const int32_t nuv[2], const float duv[2], const float cuv[2], float sad, float add, float sdd,
+                       int32_t noviews, const double* betas, const float* couchz, int32_t dettype, const float* in,
+                       const uint8_t* map, float* out, char* errbuf, size_t errlen) {
+    return cbct_gateway_restricted("proj,dd", "back,dd", op, nxyz, dxyz, cxyz, nuv, duv, cuv, sad, add, sdd, noviews,
+                                   betas, couchz, dettype, in, map, out, errbuf, errlen);
+}
+
+int ctsim_cbct_sf_cuda(const char* op, const int32_t nxyz[3], const float dxyz[3], const float cxyz[3],
+                       const int32_t nuv[2], const float duv[2], const float cuv[2], float sad, float add, float sdd,
+                       int32_t noviews, const double* betas, const float* couchz, int32_t dettype, const float* in,
+                       const uint8_t* map, float* out, char* errbuf, size_t errlen) {
+    return cbct_gateway_restricted("proj,tf", "back,tf", op, nxyz, dxyz, cxyz, nuv, duv, cuv, sad, add, sdd, noviews,
+                                   betas, couchz, dettype, in, map, out, errbuf, errlen);
+}
+
+int ctsim_cbct_back_pd_cuda(const char* op, const int32_t nxyz[3], const float dxyz[3], const float cxyz[3],
+                            const int32_t nuv[2], const float duv[2], const float cuv[2], float sad, float add,
+                            float sdd, int32_t noviews, const double* betas, const float* couchz, int32_t dettype,
+                            const float* in, const uint8_t* map, float* out, char* errbuf, size_t errlen) {
+    return cbct_gateway_restricted("back,pd", "back,hpd", op, nxyz, dxyz, cxyz, nuv, duv, cuv, sad, add, sdd, noviews,
+                                   betas, couchz, dettype, in, map, out, errbuf, errlen);
+}
+
+// ---------------------------------------------------------------------------------
+// batched uneven 2D slices
+// ---------------------------------------------------------------------------------
+}  // extern "C"
+
+struct ctsim_batch2 {
+    int nslices = 0;
+    int compute = CTSIM_F32;
+    int nitems_proj = 0, nitems_back = 0;
+    long long img_elems = 0, sino_elems = 0;
+    void* d_slices_proj = nullptr;  // Slice2 table, item0 counted for the forward work list
+    void* d_slices_back = nullptr;  // same slices, item0 counted for the back work list
+    std::vector<void*> d_views;
+};
+
+namespace {
+template <class T>
+int batch2_build(ctsim_batch2* b, int n, const ctsim_geom* geoms, const int32_t* nviews, const double* const* betas) {
+    std::vector<Slice2<T>> sp(n), sb(n);
+    long long img = 0, sino = 0;
+    int ip = 0, ib = 0;
+    for (int i = 0; i < n; ++i) {
+        const ctsim_geom& g = geoms[i];
+        if (int rc = check_geom(2, g)) return rc;
+        if (nviews[i] <= 0 || !betas[i]) return fail(CTSIM_EINVAL, "Error: slice %d has no views.", i);
+        auto v = make_views<T>(g, nviews[i], betas[i], nullptr);
+        void* dv = nullptr;
+        HIP_TRY(hipMalloc(&dv, v.size() * sizeof(View<T>)), "batch views");
+        b->d_views.push_back(dv);
+        HIP_TRY(hipMemcpy(dv, v.data(), v.size() * sizeof(View<T>), hipMemcpyHostToDevice), "batch views");
+        Slice2<T> s{};
+        s.g = make_geom<T>(g);
+        s.views = (const View<T>*)dv;
+        s.nviews = nviews[i];
+        s.img_off = img;
+        s.sino_off = sino;
+        sp[i] = s;
+        sb[i] = s;
+        sp[i].item0 = ip;
+        sb[i].item0 = ib;
+        ip += nviews[i] * ((g.nu + 63) / 64);
+        ib += g.ny * ((g.nx + 63) / 64);
+        img += (long long)g.nx * g.ny;
+        sino += (long long)g.nu * nviews[i];
+    }
+    b->nitems_proj = ip;
+    b->nitems_back = ib;
+    b->img_elems = img;
+    b->sino_elems = sino;
+    HIP_TRY(hipMalloc(&b->d_slices_proj, n * sizeof(Slice2<T>)), "batch table");
+    HIP_TRY(hipMalloc(&b->d_slices_back, n * sizeof(Slice2<T>)), "batch table");
+    HIP_TRY(hipMemcpy(b->d_slices_proj, sp.data(), n * sizeof(Slice2<T>), hipMemcpyHostToDevice), "batch table");
+    HIP_TRY(hipMemcpy(b->d_slices_back, sb.data(), n * sizeof(Slice2<T>), hipMemcpyHostToDevice), "batch table");
+    return CTSIM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ctsim_batch2_create(ctsim_batch2** out, int32_t nslices, const ctsim_geom* geoms, const int32_t* nviews,
+                        const double* const* betas, int32_t compute) {
+    if (!out || nslices <= 0 || !geoms || !nviews || !betas) return fail(CTSIM_EINVAL, "Error: bad batch arguments.");
+    *out = nullptr;
+    if (compute != CTSIM_F32 && compute != CTSIM_F64) return fail(CTSIM_EINVAL, "Error: bad compute precision.");
+    ctsim_batch2* b = new (std::nothrow) ctsim_batch2;
+    if (!b) return fail(CTSIM_ENOMEM, "Error: host allocation failed.");
+    b->nslices = nslices;
+    b->compute = compute;
+    const int rc = compute == CTSIM_F32 ? batch2_build<float>(b, nslices, geoms, nviews, betas)
+                                        : batch2_build<double>(b, nslices, geoms, nviews, betas);
+    if (rc) {
+        ctsim_batch2_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return CTSIM_OK;
+}
+
+int ctsim_batch2_destroy(ctsim_batch2* b) {
+    if (!b) return CTSIM_OK;
+    for (void* p : b->d_views) (void)hipFree(p);
+    if (b->d_slices_proj) (void)hipFree(b->d_slices_proj);
+    if (b->d_slices_back) (void)hipFree(b->d_slices_back);
+    delete b;
+    return CTSIM_OK;
+}
+
+int ctsim_batch2_sizes(const ctsim_batch2* b, int64_t* img_elems, int64_t* sino_elems) {
+    if (!b) return fail(CTSIM_EINVAL, "Error: null batch.");
+    if (img_elems) *img_elems = b->img_elems;
+    if (sino_elems) *sino_elems = b->sino_elems;
+    return CTSIM_OK;
+}
+
+int ctsim_batch2_proj(const ctsim_batch2* b, const char* op, const void* d_img, int32_t img_dt, void* d_sino,
+                      int32_t sino_dt, void* stream) {
+    if (!b || !d_img || !d_sino) return fail(CTSIM_EINVAL, "Error: null argument.");
+    const OpKind k = parse_op(op);
+    if (k != OP_PROJ_DD && k != OP_PROJ_TF) return fail(CTSIM_EINVAL, "Error: unknown geom type.");
+    const int kind = k == OP_PROJ_DD ? 0 : 1;
+    hipError_t e = b->compute == CTSIM_F32
+                       ? fbct_proj_batch_launch<float>(b->d_slices_proj, b->nslices, b->nitems_proj, d_img, img_dt,
+                                                       d_sino, sino_dt, kind, (hipStream_t)stream)
+                       : fbct_proj_batch_launch<double>(b->d_slices_proj, b->nslices, b->nitems_proj, d_img, img_dt,
+                                                        d_sino, sino_dt, kind, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype combination not supported.");
+    HIP_TRY(e, "batched forward projection");
+    return CTSIM_OK;
+}
+
+int ctsim_batch2_back(const ctsim_batch2* b, const char* op, const void* d_sino, int32_t sino_dt, void* d_img,
+                      int32_t img_dt, void* stream) {
+    if (!b || !d_img || !d_sino) return fail(CTSIM_EINVAL, "Error: null argument.");
+    const OpKind k = parse_op(op);
+    if (k != OP_BACK_DD && k != OP_BACK_TF && k != OP_BACK_PD) return fail(CTSIM_EINVAL, "Error: unknown geom type.");
+    const int kind = k == OP_BACK_DD ? 0 : k == OP_BACK_TF ? 1 : 2;
+    hipError_t e = b->compute == CTSIM_F32
+                       ? fbct_back_batch_launch<float>(b->d_slices_back, b->nslices, b->nitems_back, d_sino, sino_dt,
+                                                       d_img, img_dt, kind, (hipStream_t)stream)
+                       : fbct_back_batch_launch<double>(b->d_slices_back, b->nslices, b->nitems_back, d_sino, sino_dt,
+                                                        d_img, img_dt, kind, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype combination not supported.");
+    HIP_TRY(e, "batched backprojection");
+    return CTSIM_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// memory, streams, events, graphs
+// ---------------------------------------------------------------------------------
+int ctsim_malloc(void** p, size_t bytes) {
+    if (!p) return fail(CTSIM_EINVAL, "Error: null pointer.");
+    *p = nullptr;
+    if (bytes == 0) return CTSIM_OK;
+    HIP_TRY(hipMalloc(p, bytes), "hipMalloc");
+    return CTSIM_OK;
+}
+int ctsim_free(void* p) {
+    if (p) HIP_TRY(hipFree(p), "hipFree");
+    return CTSIM_OK;
+}
+int ctsim_memcpy_h2d(void* d, const void* h, size_t bytes, void* stream) {
+    if (!bytes) return CTSIM_OK;
+    HIP_TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, (hipStream_t)stream), "h2d copy");
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream), "h2d copy");
+    return CTSIM_OK;
+}
+int ctsim_memcpy_d2h(void* h, const void* d, size_t bytes, void* stream) {
+    if (!bytes) return CTSIM_OK;
+    HIP_TRY(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream), "d2h copy");
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream), "d2h copy");
+    return CTSIM_OK;
+}
+int ctsim_memcpy_d2d(void* d, const void* s, size_t bytes, void* stream) {
+    if (!bytes) return CTSIM_OK;
+    HIP_TRY(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream), "d2d copy");
+    return CTSIM_OK;
+}
+int ctsim_fill(void* p, int32_t dt, size_t n, double value, void* stream) {
+    if (!p && n) return fail(CTSIM_EINVAL, "Error: null pointer.");
+    hipError_t e = fill_launch(p, dt, n, value, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EINVAL, "Error: bad dtype.");
+    HIP_TRY(e, "fill");
+    return CTSIM_OK;
+}
+int ctsim_convert(const void* s, int32_t sdt, void* d, int32_t ddt, size_t n, void* stream) {
+    hipError_t e = convert_launch(s, sdt, d, ddt, n, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype pair not supported.");
+    HIP_TRY(e, "convert");
+    return CTSIM_OK;
+}
+int ctsim_stream_create(void** s) {
+    if (!s) return fail(CTSIM_EINVAL, "Error: null pointer.");
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream create");
+    *s = st;
+    return CTSIM_OK;
+}
+int ctsim_stream_destroy(void* s) {
+    if (s) HIP_TRY(hipStreamDestroy((hipStream_t)s), "stream destroy");
+    return CTSIM_OK;
+}
+int ctsim_stream_synchronize(void* s) {
+    HIP_TRY(hipStreamSynchronize((hipStream_t)s), "stream synchronize");
+    return CTSIM_OK;
+}
+int ctsim_device_synchronize(void) {
+    HIP_TRY(hipDeviceSynchronize(), "device synchronize");
+    return CTSIM_OK;
+}
+int ctsim_event_create(void** ev) {
+    if (!ev) return fail(CTSIM_EINVAL, "Error: null pointer.");
+    hipEvent_t e = nullptr;
+    HIP_TRY(hipEventCreate(&e), "event create");
+    *ev = e;
+    return CTSIM_OK;
+}
+int ctsim_event_destroy(void* ev) {
+    if (ev) HIP_TRY(hipEventDestroy((hipEvent_t)ev), "event destroy");
+    return CTSIM_OK;
+}
+int ctsim_event_record(void* ev, void* s) {
+    HIP_TRY(hipEventRecord((hipEvent_t)ev, (hipStream_t)s), "event record");
+    return CTSIM_OK;
+}
+int ctsim_event_elapsed_ms(void* a, void* b, float* ms) {
+    if (!ms) return fail(CTSIM_EINVAL, "Error: null pointer.");
+    HIP_TRY(hipEventSynchronize((hipEvent_t)b), "event synchronize");
+    HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b), "event elapsed");
+    return CTSIM_OK;
+}
+int ctsim_graph_begin(void* s) {
+    if (!s) return fail(CTSIM_EINVAL, "Error: graph capture needs a non-default stream.");
+    HIP_TRY(hipStreamBeginCapture((hipStream_t)s, hipStreamCaptureModeThreadLocal), "begin capture");
+    return CTSIM_OK;
+}
+int ctsim_graph_end(void* s, void** exec) {
+    if (!exec) return fail(CTSIM_EINVAL, "Error: null pointer.");
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamEndCapture((hipStream_t)s, &graph), "end capture");
+    hipGraphExec_t ge = nullptr;
+    hipError_t e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_TRY(e, "graph instantiate");
+    *exec = ge;
+    return CTSIM_OK;
+}
+int ctsim_graph_launch(void* exec, void* s) {
+    HIP_TRY(hipGraphLaunch((hipGraphExec_t)exec, (hipStream_t)s), "graph launch");
+    return CTSIM_OK;
+}
+int ctsim_graph_destroy(void* exec) {
+    if (exec) HIP_TRY(hipGraphExecDestroy((hipGraphExec_t)exec), "graph destroy");
+    return CTSIM_OK;
+}
+int ctsim_device_count(int32_t* n) {
+    if (!n) return fail(CTSIM_EINVAL, "Error: null pointer.");
+    int c = 0;
+    HIP_TRY(hipGetDeviceCount(&c), "device count");
+    *n = c;
+    return CTSIM_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// PWLS building blocks
+// ---------------------------------------------------------------------------------
+int ctsim_pwls_residual(const void* p, int32_t pdt, const void* y, int32_t ydt, const void* w, int32_t wdt, void* r,
+                        int32_t rdt, size_t n, double* phi, void* stream) {
+    if (!p || !y || !w || !r || !phi) return fail(CTSIM_EINVAL, "Error: null argument.");
+    hipError_t e = residual_launch(p, pdt, y, ydt, w, wdt, r, rdt, n, phi, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype combination not supported.");
+    HIP_TRY(e, "pwls residual");
+    return CTSIM_OK;
+}
+
+int ctsim_huber(const float* x, int32_t nx, int32_t ny, int32_t nz, double delta, int32_t mode, float* out,
+                double* cost, void* stream) {
+    if (!x || nx <= 0 || ny <= 0 || nz <= 0) return fail(CTSIM_EINVAL, "Error: bad volume.");
+    if ((mode == 1 || mode == 2) && !out) return fail(CTSIM_EINVAL, "Error: null output.");
+    if (mode == 0 && !cost) return fail(CTSIM_EINVAL, "Error: null cost output.");
+    hipError_t e = huber_launch(x, nx, ny, nz, delta, mode, out, cost, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EINVAL, "Error: wrong penalty mode.");
+    HIP_TRY(e, "huber");
+    return CTSIM_OK;
+}
+
+int ctsim_pwls_nesterov_update(const float* x, const float* z, float* s, const float* g, const float* precom,
+                               float* x_out, float* z_out, void* zcopy, int32_t zcopy_dtype, int32_t nx, int32_t ny,
+                               int32_t nz, double beta, double delta, double t, double t_next, void* stream) {
+    if (!x || !z || !s || !g || !precom || !x_out || !z_out) return fail(CTSIM_EINVAL, "Error: null argument.");
+    if (x_out == x || z_out == z || x_out == z || z_out == x)
+        return fail(CTSIM_EINVAL, "Error: outputs must not alias the stencil inputs.");
+    hipError_t e = nesterov_launch(x, z, s, g, precom, x_out, z_out, zcopy, zcopy_dtype, nx, ny, nz, beta, delta, t,
+                                   t_next, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: z copy dtype not supported.");
+    HIP_TRY(e, "nesterov update");
+    return CTSIM_OK;
+}
+
+int ctsim_pwls_sqs_update(const float* x, float* x_out, const float* g, const float* precom, int32_t nx, int32_t ny,
+                          int32_t nz, double beta, double delta, void* stream) {
+    if (!x || !x_out || !g || !precom) return fail(CTSIM_EINVAL, "Error: null argument.");
+    if (x == x_out) return fail(CTSIM_EINVAL, "Error: output must not alias the input.");
+    HIP_TRY(sqs_launch(x, x_out, g, precom, nx, ny, nz, beta, delta, (hipStream_t)stream), "sqs update");
+    return CTSIM_OK;
+}
+
+int ctsim_sqdiff_sum(const float* a, const float* b, size_t n, double* out, void* stream) {
+    if (!a || !b || !out) return fail(CTSIM_EINVAL, "Error: null argument.");
+    HIP_TRY(sqdiff_launch(a, b, n, out, (hipStream_t)stream), "sqdiff");
+    return CTSIM_OK;
+}
+
+int ctsim_mul(const void* a, int32_t adt, const void* b, int32_t bdt, void* o, int32_t odt, size_t n, void* stream) {
+    if (!a || !b || !o) return fail(CTSIM_EINVAL, "Error: null argument.");
+    hipError_t e = mul_launch(a, adt, b, bdt, o, odt, n, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype combination not supported.");
+    HIP_TRY(e, "mul");
+    return CTSIM_OK;
+}
+
+}  // extern "C"

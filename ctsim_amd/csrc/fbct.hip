@@ -1,0 +1,420 @@
+// 2D fan-beam projector family for gfx950: distance-driven and separable-footprint
+// forward projection (geom/fbct_os/fbct_proj.c fbct_proj_dd :155-291, fbct_proj_tf
+// :294-466), their matched backprojections (geom/fbct_os/fbct_back.c fbct_back_dd
+// :82-225, fbct_back_tf :227-399) and the pixel-driven FBP backprojection
+// (fbct_back_pd :3-80).  2D footprints are not capped (the 3D code caps at 16 cells).
+//
+// Layouts as in MATLAB: image [nx ny] (x fastest), sinogram [nu nviews].
+// Batched forms take a table of independent slices, each with its own geometry,
+// and walk a flat work list of (slice, view, tile) or (slice, pixel block) items,
+// so slices of uneven size share one launch.
+#include "internal.h"
+
+namespace ctsim {
+
+constexpr int kW2 = 64;
+
+enum Foot2 { F2_DD = 0, F2_TF = 1 };
+
+// ---------------------------------------------------------------------------------
+// forward: one wavefront per (view, 64 detector cells); lanes own cells.  Candidate
+// pixels of the strip are evaluated 64 at a time (one per lane); each surviving pixel
+// is broadcast and every lane adds its cell's share.
+// ---------------------------------------------------------------------------------
+template <class Tin, class Tout, class T, int FOOT>
+__device__ __forceinline__ void fbct_proj_wave(const Geom<T>& g, const View<T>& v, int ut, const Tin* __restrict__ img,
+                                               Tout* __restrict__ out_row, int exhaustive) {
+    const int lane = lane_id();
+    const int iu_a = ut * kW2;
+    const int iu_b = min(iu_a + kW2, g.nu) - 1;
+    const int iu = iu_a + lane;
+    const bool row_major = fabs_t(v.c) >= fabs_t(v.s);
+    const int nmajor = row_major ? g.ny : g.nx;
+    const T kL = u_to_slope(g, T(iu_a) - T(0.5));
+    const T kR = u_to_slope(g, T(iu_b) + T(0.5));
+    T acc = T(0);
+    for (int m0 = 0; m0 < nmajor; m0 += kW2) {
+        const int m = m0 + lane;
+        int lo = 0, hi = -1;
+        if (m < nmajor) {
+            if (!strip_range(g, v, row_major, m, kL, kR, exhaustive != 0, lo, hi)) hi = lo - 1;
+        }
+        const int cnt = hi - lo + 1 > 0 ? hi - lo + 1 : 0;
+        const int incl = wave_incl_scan(cnt);
+        const int excl = incl - cnt;
+        const int total = rdl_i(incl, kW2 - 1);
+        for (int k0 = 0; k0 < total; k0 += kW2) {
+            const int k = k0 + lane;
+            int j = 0;
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1) {
+                const int e = __shfl(excl, j + step, 64);
+                if (e <= k) j += step;
+            }
+            const int lo_j = __shfl(lo, j, 64);
+            const int ex_j = __shfl(excl, j, 64);
+            const int major = m0 + j, minor = lo_j + (k - ex_j);
+            const int ix = row_major ? minor : major;
+            const int iy = row_major ? major : minor;
+            bool ok = k < total;
+            int ilo = 0, ihi = -1;
+            T p0 = T(0), p1 = T(0), p2 = T(0), p3 = T(0), val = T(0);
+            if (ok) {
+                const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
+                const Column<T> cg = column_geom(xc, yc, v, g.sad, g.dx);
+                T clo, chi;
+                if (FOOT == F2_DD) {
+                    T umin, umax;
+                    dd_bounds(cg, v, g, umin, umax);
+                    ok = clamp_cells(umin, umax, g.nu, 0, clo, chi, ilo, ihi);
+                    p0 = clo;
+                    p1 = chi;
+                } else {
+                    T tau[4];
+                    sf_taus(cg, v, g, tau);
+                    ok = clamp_cells(tau[0], tau[3], g.nu, 0, clo, chi, ilo, ihi);
+                    p0 = tau[0]; p1 = tau[1]; p2 = tau[2]; p3 = tau[3];
+                }
+                ok = ok && ihi >= iu_a && ilo <= iu_b;
+                if (ok) val = cg.amp * ld<T>(img + (size_t)iy * g.nx + ix);
+            }
+            unsigned long long mask = __ballot(ok);
+            while (mask) {
+                const int src = __ffsll((long long)mask) - 1;
+                mask &= mask - 1;
+                const int cilo = rdl_i(ilo, src), cihi = rdl_i(ihi, src);
+                const T cval = rdl(val, src);
+                const T cp0 = rdl(p0, src), cp1 = rdl(p1, src);
+                T cp2 = T(0), cp3 = T(0);
+                if (FOOT == F2_TF) {
+                    cp2 = rdl(p2, src);
+                    cp3 = rdl(p3, src);
+                }
+                if (iu >= cilo && iu <= cihi) {
+                    const T w = FOOT == F2_DD ? dd_weight(iu, cp0, cp1, cilo, cihi) : sf_weight(iu, cp0, cp1, cp2, cp3);
+                    acc += w * cval;
+                }
+            }
+        }
+    }
+    if (iu < g.nu) out_row[iu] = cvt_store<Tout>(acc);
+}
+
+template <class Tin, class Tout, class T, int FOOT>
+__global__ __launch_bounds__(256) void fbct_proj_kernel(Geom<T> g, const View<T>* __restrict__ views, int vstride,
+                                                        int nviews, const Tin* __restrict__ img,
+                                                        Tout* __restrict__ proj, int exhaustive) {
+    const int nut = (g.nu + kW2 - 1) / kW2;
+    const int wid = blockIdx.x * 4 + (int)(threadIdx.x / kW2);
+    if (wid >= nviews * nut) return;
+    const int ut = wid % nut, b = wid / nut;
+    fbct_proj_wave<Tin, Tout, T, FOOT>(g, views[(size_t)b * vstride], ut, img, proj + (size_t)b * g.nu, exhaustive);
+}
+
+// ---------------------------------------------------------------------------------
+// matched backprojection (DD / SF) and pixel-driven backprojection: lanes own 64
+// consecutive pixels of a row; the 4 waves of a block split the views into four
+// contiguous ranges and their partial sums are added in a fixed order.
+// ---------------------------------------------------------------------------------
+template <class Tin, class T, int KIND>  // KIND 0 dd, 1 tf, 2 pd
+__device__ __forceinline__ T fbct_back_pixel(const Geom<T>& g, const View<T>* __restrict__ views, int vstride, int b0,
+                                             int b1, int ix, int iy, const Tin* __restrict__ proj) {
+    T img = T(0);
+    if (ix >= g.nx) return img;
+    const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
+    for (int b = b0; b < b1; ++b) {
+        const View<T> v = views[(size_t)b * vstride];
+        const Tin* row = proj + (size_t)b * g.nu;
+        if (KIND == 2) {
+            const T xr = xc * v.c - yc * v.s;
+            const T yr = xc * v.s + yc * v.c;
+            T u, w2s;
+            if (g.dettype == 1) {
+                const T w2 = g.sdd / (g.sad + yr);
+                w2s = w2 * w2;
+                u = xr * w2 / g.du - g.wu;
+            } else {
+                const T roff = g.ou * g.sad / g.sdd;
+                u = atan((xr - roff) / (g.sad + yr)) * g.sdd / g.du - g.wu;
+                w2s = g.sdd * g.sdd / ((g.sad + yr) * (g.sad + yr) + (xr - roff) * (xr - roff));
+            }
+            if (!((double)u >= 1e-6 && (double)u <= (double)(g.nu - 1) - 1e-6)) continue;  // also skips NaN
+            const int iu = (int)floor(u);
+            const T wl = u - T(iu);
+            img += w2s * ((T(1) - wl) * ld<T>(row + iu) + wl * ld<T>(row + iu + 1));
+            continue;
+        }
+        const Column<T> cg = column_geom(xc, yc, v, g.sad, g.dx);
+        T clo, chi, p0, p1, p2 = T(0), p3 = T(0);
+        int ilo, ihi;
+        bool ok;
+        if (KIND == 0) {
+            T umin, umax;
+            dd_bounds(cg, v, g, umin, umax);
+            ok = clamp_cells(umin, umax, g.nu, 0, clo, chi, ilo, ihi);
+            p0 = clo;
+            p1 = chi;
+        } else {
+            T tau[4];
+            sf_taus(cg, v, g, tau);
+            ok = clamp_cells(tau[0], tau[3], g.nu, 0, clo, chi, ilo, ihi);
+            p0 = tau[0]; p1 = tau[1]; p2 = tau[2]; p3 = tau[3];
+        }
+        if (!ok) continue;
+        T s = T(0);
+        for (int iu = ilo; iu <= ihi; ++iu) {
+            const T w = KIND == 0 ? dd_weight(iu, p0, p1, ilo, ihi) : sf_weight(iu, p0, p1, p2, p3);
+            s += w * ld<T>(row + iu);
+        }
+        img += cg.amp * s;
+    }
+    return img;
+}
+
+template <class Tin, class Tout, class T, int KIND>
+__global__ __launch_bounds__(256) void fbct_back_kernel(Geom<T> g, const View<T>* __restrict__ views, int vstride,
+                                                        int nviews, const Tin* __restrict__ proj,
+                                                        Tout* __restrict__ img, T scale) {
+    __shared__ T part[3][kW2];
+    const int nxb = (g.nx + kW2 - 1) / kW2;
+    const int iy = blockIdx.x / nxb;
+    const int ix = (blockIdx.x % nxb) * kW2 + lane_id();
+    const int w = threadIdx.x / kW2;
+    const int per = (nviews + 3) / 4;
+    const int b0 = min(w * per, nviews), b1 = min(b0 + per, nviews);
+    const T s = fbct_back_pixel<Tin, T, KIND>(g, views, vstride, b0, b1, ix, iy, proj);
+    if (w > 0) part[w - 1][lane_id()] = s;
+    __syncthreads();
+    if (w == 0 && ix < g.nx) {
+        const T tot = (s + part[0][lane_id()]) + (part[1][lane_id()] + part[2][lane_id()]);
+        img[(size_t)iy * g.nx + ix] = cvt_store<Tout>(scale * tot);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Batched uneven slices in one launch.  items are (slice, view, u-tile) for the
+// forward and (slice, row, 64-pixel block) for the back pass; a wave finds its slice
+// by binary search over the slices' first item.  One launch covers the whole batch.
+// ---------------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ int find_slice(const Slice2<T>* __restrict__ sl, int nslices, int item) {
+    int lo = 0, hi = nslices - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sl[mid].item0 <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class Tin, class Tout, class T, int FOOT>
+__global__ __launch_bounds__(256) void fbct_proj_batch_kernel(const Slice2<T>* __restrict__ sl, int nslices,
+                                                              int nitems, const Tin* __restrict__ img,
+                                                              Tout* __restrict__ proj) {
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int item = bid * 4 + (int)(threadIdx.x / kW2);
+    if (item >= nitems) return;
+    const int si = find_slice(sl, nslices, item);
+    const Slice2<T>& s = sl[si];
+    const int local = item - s.item0;
+    const int nut = (s.g.nu + kW2 - 1) / kW2;
+    const int ut = local % nut, b = local / nut;
+    fbct_proj_wave<Tin, Tout, T, FOOT>(s.g, s.views[b], ut, img + s.img_off, proj + s.sino_off + (size_t)b * s.g.nu,
+                                       0);
+}
+
+template <class Tin, class Tout, class T, int KIND>
+__global__ __launch_bounds__(256) void fbct_back_batch_kernel(const Slice2<T>* __restrict__ sl, int nslices,
+                                                              int nitems, const Tin* __restrict__ proj,
+                                                              Tout* __restrict__ img) {
+    __shared__ T part[3][kW2];
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    if (bid >= nitems) return;  // uniform per block
+    const int si = find_slice(sl, nslices, bid);
+    const Slice2<T>& s = sl[si];
+    const int local = bid - s.item0;
+    const int nxb = (s.g.nx + kW2 - 1) / kW2;
+    const int iy = local / nxb;
+    const int ix = (local % nxb) * kW2 + lane_id();
+    const int w = threadIdx.x / kW2;
+    const int per = (s.nviews + 3) / 4;
+    const int b0 = min(w * per, s.nviews), b1 = min(b0 + per, s.nviews);
+    const T v = fbct_back_pixel<Tin, T, KIND>(s.g, s.views, 1, b0, b1, ix, iy, proj + s.sino_off);
+    if (w > 0) part[w - 1][lane_id()] = v;
+    __syncthreads();
+    if (w == 0 && ix < s.g.nx) {
+        const T tot = (v + part[0][lane_id()]) + (part[1][lane_id()] + part[2][lane_id()]);
+        img[s.img_off + (size_t)iy * s.g.nx + ix] = cvt_store<Tout>(tot);
+    }
+}
+
+}  // namespace ctsim
+
+// ===================================================================================
+// host launchers
+// ===================================================================================
+namespace ctsim {
+
+template <class Tin, class Tout, class T>
+static hipError_t proj2(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* img, void* proj,
+                        int kind, int exhaustive, hipStream_t st) {
+    const long long nut = (g.nu + kW2 - 1) / kW2;
+    const unsigned blocks = (unsigned)(((long long)nviews * nut + 3) / 4);
+    if (blocks == 0) return hipSuccess;
+    if (kind == F2_DD)
+        hipLaunchKernelGGL((fbct_proj_kernel<Tin, Tout, T, F2_DD>), dim3(blocks), dim3(256), 0, st, g, views, vstride,
+                           nviews, (const Tin*)img, (Tout*)proj, exhaustive);
+    else
+        hipLaunchKernelGGL((fbct_proj_kernel<Tin, Tout, T, F2_TF>), dim3(blocks), dim3(256), 0, st, g, views, vstride,
+                           nviews, (const Tin*)img, (Tout*)proj, exhaustive);
+    return hipGetLastError();
+}
+
+template <class Tin, class Tout, class T>
+static hipError_t back2(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* proj, void* img,
+                        int kind, T scale, hipStream_t st) {
+    const unsigned blocks = (unsigned)(((g.nx + kW2 - 1) / kW2) * g.ny);
+    if (blocks == 0) return hipSuccess;
+    if (kind == 0)
+        hipLaunchKernelGGL((fbct_back_kernel<Tin, Tout, T, 0>), dim3(blocks), dim3(256), 0, st, g, views, vstride,
+                           nviews, (const Tin*)proj, (Tout*)img, scale);
+    else if (kind == 1)
+        hipLaunchKernelGGL((fbct_back_kernel<Tin, Tout, T, 1>), dim3(blocks), dim3(256), 0, st, g, views, vstride,
+                           nviews, (const Tin*)proj, (Tout*)img, scale);
+    else
+        hipLaunchKernelGGL((fbct_back_kernel<Tin, Tout, T, 2>), dim3(blocks), dim3(256), 0, st, g, views, vstride,
+                           nviews, (const Tin*)proj, (Tout*)img, scale);
+    return hipGetLastError();
+}
+
+#define CT_DISPATCH_IN(dt, MACRO)                   \
+    switch (dt) {                                   \
+        case DT_F32: MACRO(float);                  \
+        case DT_F16: MACRO(__half);                 \
+        case DT_BF16: MACRO(__hip_bfloat16);        \
+        default: return hipErrorInvalidValue;       \
+    }
+
+template <>
+hipError_t fbct_proj_launch<float>(const Geom<float>& g, const View<float>* views, int vstride, int nviews,
+                                   const void* img, int img_dt, void* proj, int proj_dt, int kind, int exhaustive,
+                                   hipStream_t st) {
+#define OUT_F32(TI) return proj2<TI, float, float>(g, views, vstride, nviews, img, proj, kind, exhaustive, st)
+#define OUT_F16(TI) return proj2<TI, __half, float>(g, views, vstride, nviews, img, proj, kind, exhaustive, st)
+#define OUT_BF16(TI) return proj2<TI, __hip_bfloat16, float>(g, views, vstride, nviews, img, proj, kind, exhaustive, st)
+    switch (proj_dt) {
+        case DT_F32: CT_DISPATCH_IN(img_dt, OUT_F32);
+        case DT_F16: CT_DISPATCH_IN(img_dt, OUT_F16);
+        case DT_BF16: CT_DISPATCH_IN(img_dt, OUT_BF16);
+        default: return hipErrorInvalidValue;
+    }
+#undef OUT_F32
+#undef OUT_F16
+#undef OUT_BF16
+}
+
+template <>
+hipError_t fbct_proj_launch<double>(const Geom<double>& g, const View<double>* views, int vstride, int nviews,
+                                    const void* img, int img_dt, void* proj, int proj_dt, int kind, int exhaustive,
+                                    hipStream_t st) {
+    if (img_dt != DT_F64 || proj_dt != DT_F64) return hipErrorInvalidValue;
+    return proj2<double, double, double>(g, views, vstride, nviews, img, proj, kind, exhaustive, st);
+}
+
+template <>
+hipError_t fbct_back_launch<float>(const Geom<float>& g, const View<float>* views, int vstride, int nviews,
+                                   const void* proj, int proj_dt, void* img, int img_dt, int kind, float scale,
+                                   hipStream_t st) {
+    if (img_dt != DT_F32) return hipErrorInvalidValue;
+#define B2(TI) return back2<TI, float, float>(g, views, vstride, nviews, proj, img, kind, scale, st)
+    CT_DISPATCH_IN(proj_dt, B2);
+#undef B2
+}
+
+template <>
+hipError_t fbct_back_launch<double>(const Geom<double>& g, const View<double>* views, int vstride, int nviews,
+                                    const void* proj, int proj_dt, void* img, int img_dt, int kind, double scale,
+                                    hipStream_t st) {
+    if (img_dt != DT_F64 || proj_dt != DT_F64) return hipErrorInvalidValue;
+    return back2<double, double, double>(g, views, vstride, nviews, proj, img, kind, scale, st);
+}
+
+size_t slice2_size(bool dbl) { return dbl ? sizeof(Slice2<double>) : sizeof(Slice2<float>); }
+
+template <class Tin, class Tout, class T>
+static hipError_t proj2b(const void* sl, int nslices, int nitems, const void* img, void* proj, int kind,
+                         hipStream_t st) {
+    const unsigned blocks = (unsigned)((nitems + 3) / 4);
+    if (blocks == 0) return hipSuccess;
+    if (kind == F2_DD)
+        hipLaunchKernelGGL((fbct_proj_batch_kernel<Tin, Tout, T, F2_DD>), dim3(blocks), dim3(256), 0, st,
+                           (const Slice2<T>*)sl, nslices, nitems, (const Tin*)img, (Tout*)proj);
+    else
+        hipLaunchKernelGGL((fbct_proj_batch_kernel<Tin, Tout, T, F2_TF>), dim3(blocks), dim3(256), 0, st,
+                           (const Slice2<T>*)sl, nslices, nitems, (const Tin*)img, (Tout*)proj);
+    return hipGetLastError();
+}
+
+template <class Tin, class Tout, class T>
+static hipError_t back2b(const void* sl, int nslices, int nitems, const void* proj, void* img, int kind,
+                         hipStream_t st) {
+    const unsigned blocks = (unsigned)nitems;
+    if (blocks == 0) return hipSuccess;
+    if (kind == 0)
+        hipLaunchKernelGGL((fbct_back_batch_kernel<Tin, Tout, T, 0>), dim3(blocks), dim3(256), 0, st,
+                           (const Slice2<T>*)sl, nslices, nitems, (const Tin*)proj, (Tout*)img);
+    else if (kind == 1)
+        hipLaunchKernelGGL((fbct_back_batch_kernel<Tin, Tout, T, 1>), dim3(blocks), dim3(256), 0, st,
+                           (const Slice2<T>*)sl, nslices, nitems, (const Tin*)proj, (Tout*)img);
+    else
+        hipLaunchKernelGGL((fbct_back_batch_kernel<Tin, Tout, T, 2>), dim3(blocks), dim3(256), 0, st,
+                           (const Slice2<T>*)sl, nslices, nitems, (const Tin*)proj, (Tout*)img);
+    return hipGetLastError();
+}
+
+template <>
+hipError_t fbct_proj_batch_launch<float>(const void* sl, int nslices, int nitems, const void* img, int img_dt,
+                                         void* proj, int proj_dt, int kind, hipStream_t st) {
+#define OUT_F32(TI) return proj2b<TI, float, float>(sl, nslices, nitems, img, proj, kind, st)
+#define OUT_F16(TI) return proj2b<TI, __half, float>(sl, nslices, nitems, img, proj, kind, st)
+#define OUT_BF16(TI) return proj2b<TI, __hip_bfloat16, float>(sl, nslices, nitems, img, proj, kind, st)
+    switch (proj_dt) {
+        case DT_F32: CT_DISPATCH_IN(img_dt, OUT_F32);
+        case DT_F16: CT_DISPATCH_IN(img_dt, OUT_F16);
+        case DT_BF16: CT_DISPATCH_IN(img_dt, OUT_BF16);
+        default: return hipErrorInvalidValue;
+    }
+#undef OUT_F32
+#undef OUT_F16
+#undef OUT_BF16
+}
+
+template <>
+hipError_t fbct_proj_batch_launch<double>(const void* sl, int nslices, int nitems, const void* img, int img_dt,
+                                          void* proj, int proj_dt, int kind, hipStream_t st) {
+    if (img_dt != DT_F64 || proj_dt != DT_F64) return hipErrorInvalidValue;
+    return proj2b<double, double, double>(sl, nslices, nitems, img, proj, kind, st);
+}
+
+template <>
+hipError_t fbct_back_batch_launch<float>(const void* sl, int nslices, int nitems, const void* proj, int proj_dt,
+                                         void* img, int img_dt, int kind, hipStream_t st) {
+#define OUT_F32(TI) return back2b<TI, float, float>(sl, nslices, nitems, proj, img, kind, st)
+#define OUT_F16(TI) return back2b<TI, __half, float>(sl, nslices, nitems, proj, img, kind, st)
+    switch (img_dt) {
+        case DT_F32: CT_DISPATCH_IN(proj_dt, OUT_F32);
+        case DT_F16: CT_DISPATCH_IN(proj_dt, OUT_F16);
+        default: return hipErrorInvalidValue;
+    }
+#undef OUT_F32
+#undef OUT_F16
+}
+
+template <>
+hipError_t fbct_back_batch_launch<double>(const void* sl, int nslices, int nitems, const void* proj, int proj_dt,
+                                          void* img, int img_dt, int kind, hipStream_t st) {
+    if (img_dt != DT_F64 || proj_dt != DT_F64) return hipErrorInvalidValue;
+    return back2b<double, double, double>(sl, nslices, nitems, proj, img, kind, st);
+}
+
+}  // namespace ctsim

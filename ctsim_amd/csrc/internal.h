@@ -1,0 +1,80 @@
+// Host-side launchers shared between the kernel translation units and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "geom.h"
+
+namespace ctsim {
+
+enum DType { DT_F32 = 0, DT_F16 = 1, DT_BF16 = 2, DT_F64 = 3 };
+
+inline size_t dtype_size(int dt) { return dt == DT_F64 ? 8 : dt == DT_F32 ? 4 : 2; }
+
+// 3D cone beam.  foot: 0 = DD, 1 = SF.  Volume in ZXY order.
+template <class T>
+hipError_t cbct_proj_launch(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* vol,
+                            int vol_dt, const uint8_t* map, void* proj, int proj_dt, int foot, int exhaustive,
+                            hipStream_t st);
+template <class T>
+hipError_t cbct_back_launch(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* proj,
+                            int proj_dt, const uint8_t* map, void* vol, int vol_dt, int foot, T scale, int accumulate,
+                            hipStream_t st);
+template <class T>
+hipError_t cbct_back_pd_launch(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* proj,
+                               int proj_dt, const uint8_t* map, void* vol, int vol_dt, int off_boundary, T scale,
+                               int accumulate, hipStream_t st);
+hipError_t xyz_zxy_launch(const void* src, int src_dt, void* dst, int dst_dt, int nxy, int nz, bool to_zxy,
+                          hipStream_t st);
+
+// 2D fan beam.  kind: 0 = DD, 1 = SF, 2 = pd (back only).
+template <class T>
+hipError_t fbct_proj_launch(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* img,
+                            int img_dt, void* proj, int proj_dt, int kind, int exhaustive, hipStream_t st);
+template <class T>
+hipError_t fbct_back_launch(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* proj,
+                            int proj_dt, void* img, int img_dt, int kind, T scale, hipStream_t st);
+
+// batched 2D: slice table on the device
+// One 2D slice of a batch: geometry, view table and data offsets.
+template <class T>
+struct Slice2 {
+    Geom<T> g;
+    const View<T>* views;
+    int nviews;
+    long long img_off;   // element offset of the image
+    long long sino_off;  // element offset of the sinogram
+    int item0;           // first work item of this slice in the flat work list
+    int pad;
+};
+
+template <class T>
+hipError_t fbct_proj_batch_launch(const void* slices, int nslices, int nitems, const void* img, int img_dt,
+                                  void* proj, int proj_dt, int kind, hipStream_t st);
+template <class T>
+hipError_t fbct_back_batch_launch(const void* slices, int nslices, int nitems, const void* proj, int proj_dt,
+                                  void* img, int img_dt, int kind, hipStream_t st);
+size_t slice2_size(bool dbl);
+
+// utility kernels
+hipError_t fill_launch(void* p, int dt, size_t n, double value, hipStream_t st);
+hipError_t convert_launch(const void* src, int sdt, void* dst, int ddt, size_t n, hipStream_t st);
+
+}  // namespace ctsim
+
+namespace ctsim {
+// PWLS / reduction launchers (pwls.hip)
+hipError_t reduction_scratch_init();
+hipError_t residual_launch(const void* p, int pdt, const void* y, int ydt, const void* w, int wdt, void* r, int rdt,
+                           size_t n, double* phi, hipStream_t st);
+hipError_t huber_launch(const float* x, int nx, int ny, int nz, double delta, int mode, float* out, double* cost,
+                        hipStream_t st);
+hipError_t nesterov_launch(const float* x, const float* z, float* s, const float* g, const float* precom, float* x_out,
+                           float* z_out, void* zcopy, int zcdt, int nx, int ny, int nz, double beta, double delta,
+                           double t, double t_next, hipStream_t st);
+hipError_t sqs_launch(const float* x, float* x_out, const float* g, const float* precom, int nx, int ny, int nz,
+                      double beta, double delta, hipStream_t st);
+hipError_t sqdiff_launch(const float* a, const float* b, size_t n, double* out, hipStream_t st);
+hipError_t mul_launch(const void* a, int adt, const void* b, int bdt, void* o, int odt, size_t n, hipStream_t st);
+}  // namespace ctsim

@@ -1,0 +1,463 @@
+// Elementwise, stencil and reduction kernels of the PWLS ordered-subset loop
+// (statsrecon/pwls/reconPwlsSeNesterovSqs.m:59-137, statsrecon/pwls/reconPwlsSeSqs.m:51-110)
+// and the Huber penalty (penalty/huberPenalty.m:101-161, helpers :222-238).
+//
+// Volumes are float32 in device ZXY order (z fastest).  Sums that the loop reports
+// (data-fit cost, penalty cost, RMSD) accumulate in FP64 with a fixed-order two-level
+// reduction: every block writes a partial, the last block to finish adds the partials
+// in index order.  Results are therefore bit-reproducible run to run.
+#include "internal.h"
+
+namespace ctsim {
+
+constexpr int kRB = 256;        // reduction block size
+constexpr int kRedBlocks = 1024;  // fixed grid of reduction kernels
+
+// ---------------------------------------------------------------------------------
+// fill / convert
+// ---------------------------------------------------------------------------------
+template <class D>
+__global__ void fill_kernel(D* __restrict__ p, size_t n, double value) {
+    const D v = cvt_store<D>((float)value);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void fill_f64_kernel(double* __restrict__ p, size_t n, double value) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = value;
+}
+
+template <class S, class D>
+__global__ void convert_kernel(const S* __restrict__ s, D* __restrict__ d, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = cvt_store<D>(ld_f(s + i));
+}
+__global__ void f64_to_f32_kernel(const double* __restrict__ s, float* __restrict__ d, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = (float)s[i];
+}
+__global__ void f32_to_f64_kernel(const float* __restrict__ s, double* __restrict__ d, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = (double)s[i];
+}
+
+static unsigned grid_for(size_t n) {
+    size_t b = (n + 255) / 256;
+    return (unsigned)(b < 8192 ? (b == 0 ? 1 : b) : 8192);
+}
+
+hipError_t fill_launch(void* p, int dt, size_t n, double value, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    switch (dt) {
+        case DT_F32: hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n)), dim3(256), 0, st, (float*)p, n, value); break;
+        case DT_F16: hipLaunchKernelGGL(fill_kernel<__half>, dim3(grid_for(n)), dim3(256), 0, st, (__half*)p, n, value); break;
+        case DT_BF16: hipLaunchKernelGGL(fill_kernel<__hip_bfloat16>, dim3(grid_for(n)), dim3(256), 0, st, (__hip_bfloat16*)p, n, value); break;
+        case DT_F64: hipLaunchKernelGGL(fill_f64_kernel, dim3(grid_for(n)), dim3(256), 0, st, (double*)p, n, value); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class S>
+static hipError_t convert_from(const void* s, void* d, int ddt, size_t n, hipStream_t st) {
+    switch (ddt) {
+        case DT_F32: hipLaunchKernelGGL((convert_kernel<S, float>), dim3(grid_for(n)), dim3(256), 0, st, (const S*)s, (float*)d, n); break;
+        case DT_F16: hipLaunchKernelGGL((convert_kernel<S, __half>), dim3(grid_for(n)), dim3(256), 0, st, (const S*)s, (__half*)d, n); break;
+        case DT_BF16: hipLaunchKernelGGL((convert_kernel<S, __hip_bfloat16>), dim3(grid_for(n)), dim3(256), 0, st, (const S*)s, (__hip_bfloat16*)d, n); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t convert_launch(const void* s, int sdt, void* d, int ddt, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    if (sdt == DT_F64 && ddt == DT_F32) {
+        hipLaunchKernelGGL(f64_to_f32_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const double*)s, (float*)d, n);
+        return hipGetLastError();
+    }
+    if (sdt == DT_F32 && ddt == DT_F64) {
+        hipLaunchKernelGGL(f32_to_f64_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const float*)s, (double*)d, n);
+        return hipGetLastError();
+    }
+    if (sdt == DT_F64 && ddt == DT_F64) return hipMemcpyAsync(d, s, n * 8, hipMemcpyDeviceToDevice, st);
+    switch (sdt) {
+        case DT_F32: return convert_from<float>(s, d, ddt, n, st);
+        case DT_F16: return convert_from<__half>(s, d, ddt, n, st);
+        case DT_BF16: return convert_from<__hip_bfloat16>(s, d, ddt, n, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// deterministic FP64 reduction tail: block partial -> last block sums in order
+// ---------------------------------------------------------------------------------
+struct RedScratch {
+    double* partial;       // kRedBlocks doubles
+    unsigned int* count;   // one counter, returns to 0 after every reduction
+};
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double sh[kRB / 64];
+    // wave reduction
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_down(v, off, 64);
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kRB / 64; ++i) t += sh[i];
+    return t;  // valid in thread 0
+}
+
+// called by every block; the last one adds `sum(partials)` to *out
+__device__ __forceinline__ void finish_reduction(double block_value, RedScratch rs, double* out, int accumulate) {
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        rs.partial[blockIdx.x] = block_value;
+        __threadfence();
+        const unsigned prev = atomicAdd(rs.count, 1u);
+        last = prev == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double v = 0.0;
+    const volatile double* part = rs.partial;  // bypass this CU's L1: written by other blocks
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kRB) v += part[i];
+    // fixed-order tree over the strided partial sums
+    const double tot = block_sum(v);
+    if (threadIdx.x == 0) {
+        *out = accumulate ? *out + tot : tot;
+        *rs.count = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// residual r = w .* (p - y) and phi += sum(w .* (p - y).^2) / 2
+// ---------------------------------------------------------------------------------
+template <class P, class Y, class W, class R>
+__global__ __launch_bounds__(kRB) void residual_kernel(const P* __restrict__ p, const Y* __restrict__ y,
+                                                       const W* __restrict__ w, R* __restrict__ r, size_t n,
+                                                       RedScratch rs, double* __restrict__ phi) {
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (size_t)gridDim.x * kRB) {
+        const float d = ld_f(p + i) - ld_f(y + i);
+        const float wi = ld_f(w + i);
+        r[i] = cvt_store<R>(wi * d);
+        acc += (double)wi * (double)d * (double)d;
+    }
+    const double bs = block_sum(acc);
+    finish_reduction(0.5 * bs, rs, phi, 1);
+}
+
+// ---------------------------------------------------------------------------------
+// Huber penalty on a ZXY volume.  For voxel i with neighbours k (weights 1 along x
+// and y, 1/sqrt(2) on the two xy diagonals, 1/2 along z):
+//   S(x)_i = sum_k w_k clip(x_i - x_k, +-delta)      (mode 1, dHuber)
+//   T(x)_i = sum_k w_k [|x_i - x_k| <= delta]        (mode 2, cHuber)
+//   R(x)   = sum over pairs w_k H(x_i - x_k)          (mode 0, sumHuber)
+// which is what huberPenalty.m builds by adding the pair terms into both voxels.
+// ---------------------------------------------------------------------------------
+struct Nbr {
+    int dx, dy, dz;
+    float w;
+};
+
+__device__ __forceinline__ float dhuber(float d, float delta) { return d > delta ? delta : (d < -delta ? -delta : d); }
+__device__ __forceinline__ float chuber(float d, float delta) { return fabsf(d) <= delta ? 1.f : 0.f; }
+__device__ __forceinline__ double hcost(double d, double delta) {
+    d = fabs(d);
+    return d > delta ? delta * d - delta * delta / 2 : d * d / 2;
+}
+
+// S and T of voxel (ix, iy, iz) of volume v (ZXY), neighbours in the reference's order
+template <int MODE>
+__device__ __forceinline__ float huber_point(const float* __restrict__ v, int nx, int ny, int nz, int ix, int iy, int iz,
+                                             float delta) {
+    const float axy = 0.70710678118654752f;  // 1/sqrt(2)
+    const size_t sx = nz, sy = (size_t)nx * nz;
+    const size_t i = (size_t)iy * sy + (size_t)ix * sx + iz;
+    const float xi = v[i];
+    float acc = 0.f;
+    auto term = [&](bool inside, size_t j, float w) {
+        if (!inside) return;
+        const float d = xi - v[j];
+        acc += MODE == 1 ? w * dhuber(d, delta) : w * chuber(d, delta);
+    };
+    // x neighbours
+    term(ix + 1 < nx, i + sx, 1.f);
+    term(ix > 0, i - sx, 1.f);
+    // y neighbours
+    term(iy + 1 < ny, i + sy, 1.f);
+    term(iy > 0, i - sy, 1.f);
+    // diagonal (+1,+1) / (-1,-1)
+    term(ix + 1 < nx && iy + 1 < ny, i + sx + sy, axy);
+    term(ix > 0 && iy > 0, i - sx - sy, axy);
+    // anti-diagonal (-1,+1) / (+1,-1)
+    term(ix > 0 && iy + 1 < ny, i - sx + sy, axy);
+    term(ix + 1 < nx && iy > 0, i + sx - sy, axy);
+    // z neighbours
+    term(iz + 1 < nz, i + 1, 0.5f);
+    term(iz > 0, i - 1, 0.5f);
+    return acc;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void huber_map_kernel(const float* __restrict__ x, int nx, int ny, int nz, float delta,
+                                                        float* __restrict__ out) {
+    const size_t n = (size_t)nx * ny * nz;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(i % nz);
+        const size_t c = i / nz;
+        const int ix = (int)(c % nx), iy = (int)(c / nx);
+        out[i] = huber_point<MODE>(x, nx, ny, nz, ix, iy, iz, delta);
+    }
+}
+
+// penalty cost: each pair counted once (forward neighbours only)
+__global__ __launch_bounds__(kRB) void huber_cost_kernel(const float* __restrict__ x, int nx, int ny, int nz,
+                                                         double delta, RedScratch rs, double* __restrict__ out) {
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t sx = nz, sy = (size_t)nx * nz;
+    const double axy = 0.70710678118654752440;
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (size_t)gridDim.x * kRB) {
+        const int iz = (int)(i % nz);
+        const size_t c = i / nz;
+        const int ix = (int)(c % nx), iy = (int)(c / nx);
+        const double xi = x[i];
+        if (ix + 1 < nx) acc += hcost(xi - x[i + sx], delta);
+        if (iy + 1 < ny) acc += hcost(xi - x[i + sy], delta);
+        if (ix + 1 < nx && iy + 1 < ny) acc += axy * hcost(xi - x[i + sx + sy], delta);
+        // anti-diagonal pair {(ix, iy), (ix-1, iy+1)}, counted at its (ix, iy) member
+        if (ix > 0 && iy + 1 < ny) acc += axy * hcost(xi - x[i - sx + sy], delta);
+        if (iz + 1 < nz) acc += 0.5 * hcost(xi - x[i + 1], delta);
+    }
+    finish_reduction(block_sum(acc), rs, out, 0);
+}
+
+__device__ __forceinline__ float scrub(float v) { return (v < 0.f || !isfinite(v)) ? 0.f : v; }
+
+// fused Nesterov-SQS subset step; reads x, z (stencils) and writes x_out, z_out
+template <class ZC>
+__global__ __launch_bounds__(256) void nesterov_kernel(const float* __restrict__ x, const float* __restrict__ z,
+                                                       float* __restrict__ s, const float* __restrict__ g,
+                                                       const float* __restrict__ precom, float* __restrict__ x_out,
+                                                       float* __restrict__ z_out, ZC* __restrict__ zcopy, int nx, int ny,
+                                                       int nz, double beta, float delta, double t, double t_next) {
+    const size_t n = (size_t)nx * ny * nz;
+    const double a = 1.0 - 1.0 / t_next, b = 1.0 / t_next;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(i % nz);
+        const size_t c = i / nz;
+        const int ix = (int)(c % nx), iy = (int)(c / nx);
+        const float sz = huber_point<1>(z, nx, ny, nz, ix, iy, iz, delta);
+        const float tx = huber_point<2>(x, nx, ny, nz, ix, iy, iz, delta);
+        const float grad = g[i] + (float)(beta * (double)sz);
+        const double den = (double)precom[i] + beta * (double)tx;
+        const float du = (float)((double)grad / den);
+        const float xn = z[i] - du;
+        const float sn = s[i] - (float)(t * (double)du);
+        const float zn = (float)(a * (double)xn + b * (double)sn);
+        x_out[i] = scrub(xn);
+        s[i] = scrub(sn);
+        const float zs = scrub(zn);
+        z_out[i] = zs;
+        if (zcopy) zcopy[i] = cvt_store<ZC>(zs);
+    }
+}
+
+__global__ __launch_bounds__(256) void sqs_kernel(const float* __restrict__ x, float* __restrict__ x_out,
+                                                  const float* __restrict__ g, const float* __restrict__ precom, int nx,
+                                                  int ny, int nz, double beta, float delta) {
+    const size_t n = (size_t)nx * ny * nz;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(i % nz);
+        const size_t c = i / nz;
+        const int ix = (int)(c % nx), iy = (int)(c / nx);
+        const float sx = huber_point<1>(x, nx, ny, nz, ix, iy, iz, delta);
+        const float tx = huber_point<2>(x, nx, ny, nz, ix, iy, iz, delta);
+        const float num = g[i] + (float)(beta * (double)sx);
+        const double den = (double)precom[i] + beta * (double)tx;
+        x_out[i] = scrub((float)((double)x[i] - (double)num / den));
+    }
+}
+
+__global__ __launch_bounds__(kRB) void sqdiff_kernel(const float* __restrict__ a, const float* __restrict__ b, size_t n,
+                                                     RedScratch rs, double* __restrict__ out) {
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (size_t)gridDim.x * kRB) {
+        const double d = (double)a[i] - (double)b[i];
+        acc += d * d;
+    }
+    finish_reduction(block_sum(acc), rs, out, 0);
+}
+
+template <class A, class B, class O>
+__global__ void mul_kernel(const A* __restrict__ a, const B* __restrict__ b, O* __restrict__ o, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        o[i] = cvt_store<O>(ld_f(a + i) * ld_f(b + i));
+}
+
+// ---------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------
+static RedScratch g_scratch[16];
+
+hipError_t reduction_scratch_init();
+
+static hipError_t scratch(RedScratch& rs) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 16) return hipErrorInvalidDevice;
+    if (!g_scratch[dev].partial) {
+        // allocated once per device and kept until the process ends; the counter
+        // starts at zero and every reduction returns it to zero
+        void* p = nullptr;
+        e = hipMalloc(&p, kRedBlocks * sizeof(double) + 256);
+        if (e != hipSuccess) return e;
+        e = hipMemset(p, 0, kRedBlocks * sizeof(double) + 256);
+        if (e != hipSuccess) return e;
+        g_scratch[dev].partial = (double*)p;
+        g_scratch[dev].count = (unsigned int*)((char*)p + kRedBlocks * sizeof(double));
+    }
+    rs = g_scratch[dev];
+    return hipSuccess;
+}
+
+hipError_t reduction_scratch_init() {
+    RedScratch rs;
+    return scratch(rs);
+}
+
+static unsigned red_grid(size_t n) {
+    size_t b = (n + kRB - 1) / kRB;
+    return (unsigned)(b < (size_t)kRedBlocks ? (b == 0 ? 1 : b) : kRedBlocks);
+}
+
+template <class P, class Y, class W>
+static hipError_t residual_r(const void* p, const void* y, const void* w, void* r, int rdt, size_t n, RedScratch rs,
+                             double* phi, hipStream_t st) {
+    const unsigned gb = red_grid(n);
+    switch (rdt) {
+        case DT_F32: hipLaunchKernelGGL((residual_kernel<P, Y, W, float>), dim3(gb), dim3(kRB), 0, st, (const P*)p, (const Y*)y, (const W*)w, (float*)r, n, rs, phi); break;
+        case DT_F16: hipLaunchKernelGGL((residual_kernel<P, Y, W, __half>), dim3(gb), dim3(kRB), 0, st, (const P*)p, (const Y*)y, (const W*)w, (__half*)r, n, rs, phi); break;
+        case DT_BF16: hipLaunchKernelGGL((residual_kernel<P, Y, W, __hip_bfloat16>), dim3(gb), dim3(kRB), 0, st, (const P*)p, (const Y*)y, (const W*)w, (__hip_bfloat16*)r, n, rs, phi); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class P, class Y>
+static hipError_t residual_w(const void* p, const void* y, const void* w, int wdt, void* r, int rdt, size_t n,
+                             RedScratch rs, double* phi, hipStream_t st) {
+    switch (wdt) {
+        case DT_F32: return residual_r<P, Y, float>(p, y, w, r, rdt, n, rs, phi, st);
+        case DT_F16: return residual_r<P, Y, __half>(p, y, w, r, rdt, n, rs, phi, st);
+        case DT_BF16: return residual_r<P, Y, __hip_bfloat16>(p, y, w, r, rdt, n, rs, phi, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <class P>
+static hipError_t residual_y(const void* p, const void* y, int ydt, const void* w, int wdt, void* r, int rdt, size_t n,
+                             RedScratch rs, double* phi, hipStream_t st) {
+    switch (ydt) {
+        case DT_F32: return residual_w<P, float>(p, y, w, wdt, r, rdt, n, rs, phi, st);
+        case DT_F16: return residual_w<P, __half>(p, y, w, wdt, r, rdt, n, rs, phi, st);
+        case DT_BF16: return residual_w<P, __hip_bfloat16>(p, y, w, wdt, r, rdt, n, rs, phi, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t residual_launch(const void* p, int pdt, const void* y, int ydt, const void* w, int wdt, void* r, int rdt,
+                           size_t n, double* phi, hipStream_t st) {
+    RedScratch rs;
+    hipError_t e = scratch(rs);
+    if (e != hipSuccess) return e;
+    switch (pdt) {
+        case DT_F32: return residual_y<float>(p, y, ydt, w, wdt, r, rdt, n, rs, phi, st);
+        case DT_F16: return residual_y<__half>(p, y, ydt, w, wdt, r, rdt, n, rs, phi, st);
+        case DT_BF16: return residual_y<__hip_bfloat16>(p, y, ydt, w, wdt, r, rdt, n, rs, phi, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t huber_launch(const float* x, int nx, int ny, int nz, double delta, int mode, float* out, double* cost,
+                        hipStream_t st) {
+    const size_t n = (size_t)nx * ny * nz;
+    if (mode == 1) {
+        hipLaunchKernelGGL(huber_map_kernel<1>, dim3(grid_for(n)), dim3(256), 0, st, x, nx, ny, nz, (float)delta, out);
+    } else if (mode == 2) {
+        hipLaunchKernelGGL(huber_map_kernel<2>, dim3(grid_for(n)), dim3(256), 0, st, x, nx, ny, nz, (float)delta, out);
+    } else if (mode == 0) {
+        RedScratch rs;
+        hipError_t e = scratch(rs);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(huber_cost_kernel, dim3(red_grid(n)), dim3(kRB), 0, st, x, nx, ny, nz, delta, rs, cost);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t nesterov_launch(const float* x, const float* z, float* s, const float* g, const float* precom, float* x_out,
+                           float* z_out, void* zcopy, int zcdt, int nx, int ny, int nz, double beta, double delta,
+                           double t, double t_next, hipStream_t st) {
+    const size_t n = (size_t)nx * ny * nz;
+    const dim3 gr(grid_for(n)), bl(256);
+    switch (zcopy ? zcdt : DT_F32) {
+        case DT_F32: hipLaunchKernelGGL(nesterov_kernel<float>, gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (float*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next); break;
+        case DT_F16: hipLaunchKernelGGL(nesterov_kernel<__half>, gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__half*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next); break;
+        case DT_BF16: hipLaunchKernelGGL(nesterov_kernel<__hip_bfloat16>, gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__hip_bfloat16*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t sqs_launch(const float* x, float* x_out, const float* g, const float* precom, int nx, int ny, int nz,
+                      double beta, double delta, hipStream_t st) {
+    const size_t n = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL(sqs_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, x_out, g, precom, nx, ny, nz, beta, (float)delta);
+    return hipGetLastError();
+}
+
+hipError_t sqdiff_launch(const float* a, const float* b, size_t n, double* out, hipStream_t st) {
+    RedScratch rs;
+    hipError_t e = scratch(rs);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sqdiff_kernel, dim3(red_grid(n)), dim3(kRB), 0, st, a, b, n, rs, out);
+    return hipGetLastError();
+}
+
+template <class A, class B>
+static hipError_t mul_o(const void* a, const void* b, void* o, int odt, size_t n, hipStream_t st) {
+    switch (odt) {
+        case DT_F32: hipLaunchKernelGGL((mul_kernel<A, B, float>), dim3(grid_for(n)), dim3(256), 0, st, (const A*)a, (const B*)b, (float*)o, n); break;
+        case DT_F16: hipLaunchKernelGGL((mul_kernel<A, B, __half>), dim3(grid_for(n)), dim3(256), 0, st, (const A*)a, (const B*)b, (__half*)o, n); break;
+        case DT_BF16: hipLaunchKernelGGL((mul_kernel<A, B, __hip_bfloat16>), dim3(grid_for(n)), dim3(256), 0, st, (const A*)a, (const B*)b, (__hip_bfloat16*)o, n); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class A>
+static hipError_t mul_b(const void* a, const void* b, int bdt, void* o, int odt, size_t n, hipStream_t st) {
+    switch (bdt) {
+        case DT_F32: return mul_o<A, float>(a, b, o, odt, n, st);
+        case DT_F16: return mul_o<A, __half>(a, b, o, odt, n, st);
+        case DT_BF16: return mul_o<A, __hip_bfloat16>(a, b, o, odt, n, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t mul_launch(const void* a, int adt, const void* b, int bdt, void* o, int odt, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    switch (adt) {
+        case DT_F32: return mul_b<float>(a, b, bdt, o, odt, n, st);
+        case DT_F16: return mul_b<__half>(a, b, bdt, o, odt, n, st);
+        case DT_BF16: return mul_b<__hip_bfloat16>(a, b, bdt, o, odt, n, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace ctsim

@@ -1,0 +1,248 @@
+/*
+ * ctsim_amd.h - C ABI of the MI355X (gfx950) projector / backprojector library
+ * libctsim_hip.so.  Plain C: pointers, sizes and status codes only.
+ *
+ * Two layers:
+ *
+ *  1. MEX-signature gateways.  Host pointers, MATLAB column-major layout, the exact
+ *     argument lists of CTSim's MEX functions.  A mexFunction shim (see
+ *     ctsim_amd/csrc/mex/) marshals mxArrays into these calls, so geom/shell's
+ *     forwardProjectMex / backProjectMex, and through them reconFDK, reconFBP and the
+ *     statsrecon/pwls ordered-subset loops, run unchanged.
+ *
+ *  2. A device-resident plan API for loops that keep volumes and sinograms in HBM
+ *     (ordered-subset PWLS, FDK, benchmarks): geometry and the per-view table are
+ *     uploaded once; operators take device pointers, storage dtypes and a stream.
+ *
+ * Every function returns CTSIM_OK (0) or a nonzero status; ctsim_last_error()
+ * returns the message of the calling thread's last failure.  Gateways additionally
+ * copy the message into (errbuf, errlen) when errbuf is not NULL.
+ *
+ * Units and conventions follow the reference: lengths in mm, offsets in cell units,
+ * projections scaled by dx/10 (attenuation in 1/cm), detector_type 1 = flat panel,
+ * anything else = arc.
+ */
+#ifndef CTSIM_AMD_H
+#define CTSIM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define CTSIM_OK 0
+#define CTSIM_EINVAL 1   /* bad argument: op string, size, class, count */
+#define CTSIM_EDEVICE 2  /* HIP runtime or launch failure */
+#define CTSIM_ENOMEM 3   /* device allocation failed */
+#define CTSIM_EUNSUP 4   /* valid request this build does not implement */
+
+/* storage dtypes for the plan API */
+#define CTSIM_F32 0
+#define CTSIM_F16 1
+#define CTSIM_BF16 2
+#define CTSIM_F64 3
+
+const char* ctsim_last_error(void);
+int ctsim_version(void); /* 100 * major + minor */
+
+/* ------------------------------------------------------------------------------
+ * 1. MEX-signature gateways (host pointers, MATLAB layout, caller-allocated output
+ *    that the callee overwrites completely, inputs never modified).
+ * ---------------------------------------------------------------------------- */
+
+/* Replaces fbct_geom_mex (geom/fbct_os/fbct_geom_mex.c:250-326), the 14-argument 2D
+ * fan-beam gateway called by geom/shell/forwardProjectMex.m:52-55 and
+ * geom/shell/backProjectMex.m:145-148.
+ * op: "proj,dd" "proj,tf" "back,dd" "back,tf" "back,pd".
+ * in:  image [nx ny] for proj, sinogram [nu noviews] for back (float32).
+ * out: sinogram [nu noviews] for proj, image [nx ny] for back (float32). */
+int ctsim_fbct_geom_mex(const char* op, const int32_t nxy[2], const float dxy[2], const float oxy[2], int32_t nu,
+                        float du, float ou, float sad, float add, float sdd, int32_t noviews, const double* betas,
+                        int32_t detector_type, const float* in, float* out, char* errbuf, size_t errlen);
+
+/* Replaces cbct_geom_mex (geom/cbct_os/cbct_geom_mex.c:516-600), the 3D cone-beam
+ * gateway called by geom/shell/forwardProjectMex.m:82-85 and
+ * geom/shell/backProjectMex.m:181-184.  map == NULL is the 15-argument (no map)
+ * family, otherwise the 16-argument family with logical map [nx ny] (one byte per
+ * column).  op: "proj,dd" "proj,tf" "back,dd" "back,tf" "back,pd" "back,hpd".
+ * in:  image [nx ny nz] for proj, sinogram [nv nu noviews] for back (float32).
+ * out: sinogram [nv nu noviews] for proj, image [nx ny nz] for back (float32). */
+int ctsim_cbct_geom_mex(const char* op, const int32_t nxyz[3], const float dxyz[3], const float cxyz[3],
+                        const int32_t nuv[2], const float duv[2], const float cuv[2], float sad, float add,
+                        float sdd, int32_t noviews, const double* betas, const float* couchz,
+                        int32_t detector_type, const float* in, const uint8_t* map, float* out, char* errbuf,
+                        size_t errlen);
+
+/* Same arguments as ctsim_cbct_geom_mex, restricted to the ops of the reference's
+ * CUDA gateways, which take the 16-argument list only:
+ *   cbct_dd_cuda      (geom/dd_cuda/cbct_dd_cuda.cu:21-299)        "proj,dd" "back,dd"
+ *   cbct_sf_cuda      (geom/sf_cuda/cbct_sf_cuda.cu)               "proj,tf" "back,tf"
+ *   cbct_back_pd_cuda (geom/fdk_cuda/cbct_back_pd_cuda.cu:21-193)  "back,pd" "back,hpd" */
+int ctsim_cbct_dd_cuda(const char* op, const int32_t nxyz[3], const float dxyz[3], const float cxyz[3],
+                       const int32_t nuv[2], const float duv[2], const float cuv[2], float sad, float add, float sdd,
+                       int32_t noviews, const double* betas, const float* couchz, int32_t detector_type,
+                       const float* in, const uint8_t* map, float* out, char* errbuf, size_t errlen);
+int ctsim_cbct_sf_cuda(const char* op, const int32_t nxyz[3], const float dxyz[3], const float cxyz[3],
+                       const int32_t nuv[2], const float duv[2], const float cuv[2], float sad, float add, float sdd,
+                       int32_t noviews, const double* betas, const float* couchz, int32_t detector_type,
+                       const float* in, const uint8_t* map, float* out, char* errbuf, size_t errlen);
+int ctsim_cbct_back_pd_cuda(const char* op, const int32_t nxyz[3], const float dxyz[3], const float cxyz[3],
+                            const int32_t nuv[2], const float duv[2], const float cuv[2], float sad, float add,
+                            float sdd, int32_t noviews, const double* betas, const float* couchz,
+                            int32_t detector_type, const float* in, const uint8_t* map, float* out, char* errbuf,
+                            size_t errlen);
+
+/* ------------------------------------------------------------------------------
+ * 2. Device-resident plan API.
+ * ---------------------------------------------------------------------------- */
+
+/* Geometry of one operator; mirrors the fields of CTSim's geom struct that the
+ * gateways receive (reconSize, reconSpacing, reconOffset, detSize, detSpacing,
+ * detOffset, SAD, ADD, SDD, flatPanel; parameters/loadProjectionGeometryCT.m).
+ * 2D plans use nz = nv = 1. */
+typedef struct ctsim_geom {
+    int32_t nx, ny, nz;
+    float dx, dy, dz;
+    float ox, oy, oz;
+    int32_t nu, nv;
+    float du, dv;
+    float ou, ov;
+    float sad, add, sdd;
+    int32_t detector_type;
+} ctsim_geom;
+
+typedef struct ctsim_plan ctsim_plan;
+
+/* ndim 2 or 3.  betas[nviews] (radians, double), couchz[nviews] (mm, may be NULL =
+ * zeros), map [nx ny] host bytes (may be NULL = all columns active).  compute is
+ * CTSIM_F32 (FP32 footprints and accumulation) or CTSIM_F64 (everything in double). */
+int ctsim_plan_create(ctsim_plan** plan, int32_t ndim, const ctsim_geom* g, int32_t nviews, const double* betas,
+                      const float* couchz, const uint8_t* map, int32_t compute);
+int ctsim_plan_destroy(ctsim_plan* plan);
+
+/* Forward projection of the views view0, view0 + vstep, ... (nsub of them), the
+ * ordered-subset selection of geom/shell/forwardProjectMex.m:35-39.
+ * op "proj,dd" or "proj,tf".  d_img: 3D volume in device ZXY order (z fastest,
+ * vol[(iy*nx+ix)*nz+iz]) or 2D image [nx ny]; d_sino: [nsub][nu][nv] (v fastest),
+ * every element written. */
+int ctsim_plan_proj(const ctsim_plan* plan, const char* op, int32_t view0, int32_t vstep, int32_t nsub,
+                    const void* d_img, int32_t img_dtype, void* d_sino, int32_t sino_dtype, void* stream);
+
+/* Backprojection of the same view selection.  op "back,dd" "back,tf" "back,pd"
+ * "back,hpd".  out = scale * A^T sino (+ out when accumulate != 0); scale carries the
+ * ordered-subset factor M of geom/shell/backProjectMex.m:191-193. */
+int ctsim_plan_back(const ctsim_plan* plan, const char* op, int32_t view0, int32_t vstep, int32_t nsub,
+                    const void* d_sino, int32_t sino_dtype, void* d_img, int32_t img_dtype, double scale,
+                    int32_t accumulate, void* stream);
+
+/* Exhaustive-candidate variant of ctsim_plan_proj used by tests: every image column
+ * is considered for every detector tile (no strip pruning); results must be
+ * identical to ctsim_plan_proj. */
+int ctsim_plan_proj_exhaustive(const ctsim_plan* plan, const char* op, int32_t view0, int32_t vstep, int32_t nsub,
+                               const void* d_img, int32_t img_dtype, void* d_sino, int32_t sino_dtype,
+                               void* stream);
+
+/* Volume layout changes between MATLAB order [nx ny nz] and device ZXY order, with
+ * dtype conversion (src and dst dtypes may differ; F64 only to F64). */
+int ctsim_xyz_to_zxy(const void* d_src, int32_t src_dtype, void* d_dst, int32_t dst_dtype, int32_t nx, int32_t ny,
+                     int32_t nz, void* stream);
+int ctsim_zxy_to_xyz(const void* d_src, int32_t src_dtype, void* d_dst, int32_t dst_dtype, int32_t nx, int32_t ny,
+                     int32_t nz, void* stream);
+
+/* ------------------------------------------------------------------------------
+ * Batched uneven 2D slices: nslices independent fan-beam slices, each with its own
+ * geometry and views, projected (or backprojected) in one launch.  Images and
+ * sinograms of all slices are packed back to back in d_img / d_sino in slice order
+ * ([nx ny] and [nu nviews] each).
+ * ---------------------------------------------------------------------------- */
+typedef struct ctsim_batch2 ctsim_batch2;
+int ctsim_batch2_create(ctsim_batch2** batch, int32_t nslices, const ctsim_geom* geoms, const int32_t* nviews,
+                        const double* const* betas, int32_t compute);
+int ctsim_batch2_destroy(ctsim_batch2* batch);
+/* element counts of the packed image / sinogram buffers */
+int ctsim_batch2_sizes(const ctsim_batch2* batch, int64_t* img_elems, int64_t* sino_elems);
+int ctsim_batch2_proj(const ctsim_batch2* batch, const char* op, const void* d_img, int32_t img_dtype, void* d_sino,
+                      int32_t sino_dtype, void* stream);
+int ctsim_batch2_back(const ctsim_batch2* batch, const char* op, const void* d_sino, int32_t sino_dtype,
+                      void* d_img, int32_t img_dtype, void* stream);
+
+/* ------------------------------------------------------------------------------
+ * Device memory, streams, events and graphs (thin HIP wrappers so that host code in
+ * any language can drive the library through this ABI alone).
+ * ---------------------------------------------------------------------------- */
+int ctsim_malloc(void** d_ptr, size_t bytes);
+int ctsim_free(void* d_ptr);
+int ctsim_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes, void* stream);
+int ctsim_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes, void* stream);
+int ctsim_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes, void* stream);
+/* fills with a kernel (never hipMemsetAsync), so it is safe inside graph capture */
+int ctsim_fill(void* d_ptr, int32_t dtype, size_t count, double value, void* stream);
+int ctsim_convert(const void* d_src, int32_t src_dtype, void* d_dst, int32_t dst_dtype, size_t count, void* stream);
+int ctsim_stream_create(void** stream);
+int ctsim_stream_destroy(void* stream);
+int ctsim_stream_synchronize(void* stream);
+int ctsim_device_synchronize(void);
+int ctsim_event_create(void** event);
+int ctsim_event_destroy(void* event);
+int ctsim_event_record(void* event, void* stream);
+int ctsim_event_elapsed_ms(void* start, void* stop, float* ms);
+int ctsim_graph_begin(void* stream);
+int ctsim_graph_end(void* stream, void** graph_exec);
+int ctsim_graph_launch(void* graph_exec, void* stream);
+int ctsim_graph_destroy(void* graph_exec);
+int ctsim_device_count(int32_t* count);
+
+/* ------------------------------------------------------------------------------
+ * PWLS ordered-subset building blocks (statsrecon/pwls/reconPwlsSeNesterovSqs.m:59-137,
+ * statsrecon/pwls/reconPwlsSeSqs.m:51-110, penalty/huberPenalty.m:101-161).  Volumes
+ * are device ZXY float32 (nx, ny, nz) unless stated; sinogram-side arrays carry a
+ * dtype.  Reductions accumulate in FP64 and write one double to d_out.
+ * ---------------------------------------------------------------------------- */
+
+/* r = w .* (p - y) written as dtype r_dtype, and phi += sum(w .* (p - y).^2) / 2 in
+ * FP64 at d_phi (statsrecon/pwls/reconPwlsSeNesterovSqs.m:65,95-96). */
+int ctsim_pwls_residual(const void* d_p, int32_t p_dtype, const void* d_y, int32_t y_dtype, const void* d_w,
+                        int32_t w_dtype, void* d_r, int32_t r_dtype, size_t n, double* d_phi, void* stream);
+
+/* Huber penalty of a ZXY volume: mode 1 gradient S(x), mode 2 curvature T(x), written
+ * to d_out (float32); mode 0 cost R(x) accumulated in FP64 into d_cost
+ * (penalty/huberPenalty.m:101-161: neighbours +-x, +-y weight 1, xy diagonals 1/sqrt2,
+ * +-z weight 1/2). */
+int ctsim_huber(const float* d_x, int32_t nx, int32_t ny, int32_t nz, double delta, int32_t mode, float* d_out,
+                double* d_cost, void* stream);
+
+/* One Nesterov-SQS subset update (reconPwlsSeNesterovSqs.m:67-87), fused:
+ *   grad = g + beta*S(z);  du = grad ./ (precom + beta*T(x));  x' = z - du;
+ *   s = s - t*du;  z' = (1 - 1/t_next)*x' + (1/t_next)*s;  clamp x', s, z' to >= 0
+ *   and NaN/Inf -> 0.  S(z) and T(x) are stencils evaluated in-kernel, so x' and z'
+ *   go to separate buffers (d_x_out, d_z_out must not alias d_x or d_z); s is
+ *   updated in place.  t and t_next are the momentum scalars of the current and next
+ *   step.  The denominator and the step are formed in FP64.  d_zcopy (may be NULL)
+ *   receives z' again in zcopy_dtype, e.g. an FP16 copy for the next forward
+ *   projection. */
+int ctsim_pwls_nesterov_update(const float* d_x, const float* d_z, float* d_s, const float* d_g,
+                               const float* d_precom, float* d_x_out, float* d_z_out, void* d_zcopy,
+                               int32_t zcopy_dtype, int32_t nx, int32_t ny, int32_t nz, double beta, double delta,
+                               double t, double t_next, void* stream);
+
+/* One plain SQS subset update (reconPwlsSeSqs.m:57-66): x = max(x - (g + beta S(x)) ./
+ * (precom + beta T(x)), 0) with NaN/Inf -> 0; the denominator in FP64.  Writes
+ * d_xnew (must not alias d_x). */
+int ctsim_pwls_sqs_update(const float* d_x, float* d_xnew, const float* d_g, const float* d_precom, int32_t nx,
+                          int32_t ny, int32_t nz, double beta, double delta, void* stream);
+
+/* sum((a - b).^2) in FP64 into d_out (RMSD numerator, reconPwlsSeNesterovSqs.m:103). */
+int ctsim_sqdiff_sum(const float* d_a, const float* d_b, size_t n, double* d_out, void* stream);
+
+/* out = a .* b elementwise with dtypes (precom = At(w .* A(ones)), reconPwlsSeNesterovSqs.m:42-43) */
+int ctsim_mul(const void* d_a, int32_t a_dtype, const void* d_b, int32_t b_dtype, void* d_out, int32_t out_dtype,
+              size_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CTSIM_AMD_H */

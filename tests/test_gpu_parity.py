@@ -1,0 +1,332 @@
+"""GPU parity of the HIP projector family against the oracle and the reference's golden
+vectors, through the C ABI.
+
+Tolerances (stated by the target for FP32 footprints): relative L2 <= 1e-5 against the
+reference CPU path, and matched-pair adjoint mismatch |<Ax,y> - <x,A^T y>| / (|Ax| |y|)
+< 1e-6.  The FP64 kernels are held to 1e-11 against the FP64 oracle.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden_files, load_golden, rel_l2
+from helpers import ellipse_phantom2, g2_from, g3_from, geometry_from_g2, geometry_from_g3
+
+pytestmark = pytest.mark.gpu
+
+OPS2 = ("proj,dd", "proj,tf", "back,dd", "back,tf", "back,pd")
+OPS3 = ("proj,dd", "proj,tf", "back,dd", "back,tf", "back,pd", "back,hpd")
+TOL = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu_present():
+    from ctsim_amd import device
+    if device.device_count() < 1:
+        pytest.fail("no GPU visible to the HIP runtime")
+
+
+# ---------------------------------------------------------------------------------
+# golden vectors of the reference's compiled kernels, through the MEX gateways
+# ---------------------------------------------------------------------------------
+@pytest.mark.parametrize("path", golden_files("fan_"), ids=os.path.basename)
+def test_fbct_gateway_vs_reference_golden(path):
+    from ctsim_amd import mex
+    rec = load_golden(path)
+    say("golden 2D", os.path.basename(path))
+    for op in OPS2:
+        x = rec["img"] if op.startswith("proj") else rec["sino"]
+        got = mex.fbct_geom_mex(op, rec["nxy"], rec["dxy"], rec["oxy"], rec["nu"], rec["du"], rec["ou"], rec["sad"],
+                                rec["add"], rec["sdd"], len(rec["betas"]), rec["betas"], rec["dettype"],
+                                np.asfortranarray(x))
+        want = rec[op.replace(",", "_")]
+        assert got.shape == want.shape
+        assert rel_l2(got, want) <= TOL, (op, rel_l2(got, want))
+
+
+@pytest.mark.parametrize("path", golden_files("cone_"), ids=os.path.basename)
+def test_cbct_gateway_vs_reference_golden(path):
+    from ctsim_amd import mex
+    rec = load_golden(path)
+    say("golden 3D", os.path.basename(path))
+    mp = rec["map"] if bool(rec["has_map"]) else None
+    for op in OPS3:
+        x = rec["img"] if op.startswith("proj") else rec["sino"]
+        got = mex.cbct_geom_mex(op, rec["n"], rec["d"], rec["o"], rec["nuv"], rec["duv"], rec["ouv"], rec["sad"],
+                                rec["add"], rec["sdd"], len(rec["betas"]), rec["betas"], rec["couchz"],
+                                rec["dettype"], np.asfortranarray(x), mp)
+        want = rec[op.replace(",", "_")]
+        assert got.shape == want.shape
+        assert rel_l2(got, want) <= TOL, (op, rel_l2(got, want))
+
+
+def test_cuda_gateway_names():
+    from ctsim_amd import mex
+    rec = load_golden(golden_files("cone_flat_map")[0])
+    args = (rec["n"], rec["d"], rec["o"], rec["nuv"], rec["duv"], rec["ouv"], rec["sad"], rec["add"], rec["sdd"],
+            len(rec["betas"]), rec["betas"], rec["couchz"], rec["dettype"])
+    mp = rec["map"]
+    assert rel_l2(mex.cbct_dd_cuda("proj,dd", *args, rec["img"], mp), rec["proj_dd"]) <= TOL
+    assert rel_l2(mex.cbct_sf_cuda("back,tf", *args, rec["sino"], mp), rec["back_tf"]) <= TOL
+    assert rel_l2(mex.cbct_back_pd_cuda("back,hpd", *args, rec["sino"], mp), rec["back_hpd"]) <= TOL
+
+
+# ---------------------------------------------------------------------------------
+# random seeded cases against the oracle, larger than the fixtures
+# ---------------------------------------------------------------------------------
+def _cone_case(seed, n=(48, 44, 24), nuv=(96, 40), nviews=20, dettype=1, offsets=False, couch=False, map_=True):
+    from oracle import oracle as O
+    rng = np.random.default_rng(seed)
+    betas = np.sort(rng.uniform(0, 2 * np.pi, nviews))
+    o = (0.7, -1.2, 0.5) if offsets else (0, 0, 0)
+    ouv = (1.5, -2.5) if offsets else (0, 0)
+    g = O.G3(n, (0.5, 0.5, 0.6), o, nuv, (0.4 if dettype == 1 else 0.45, 0.4), ouv, 1000.0, 0.0, 1500.0, betas,
+             couchz=rng.uniform(-3, 3, nviews) if couch else None, dettype=dettype)
+    vol = rng.uniform(0, 1, n).astype(np.float32)
+    vol[rng.uniform(size=n) < 0.3] = 0
+    sino = rng.uniform(0, 1, (nuv[1], nuv[0], nviews)).astype(np.float32)
+    mp = (rng.uniform(size=n[:2]) < 0.9) if map_ else None
+    return g, vol, sino, mp
+
+
+CONE_CASES = [
+    dict(seed=1),
+    dict(seed=2, offsets=True, map_=False),
+    dict(seed=3, dettype=0, couch=True),
+    dict(seed=4, n=(37, 53, 70), nuv=(70, 130), nviews=9),  # nv > 64: several row chunks, odd sizes
+    dict(seed=5, n=(64, 64, 8), nuv=(200, 12), nviews=7),  # thin detector, wide fan
+]
+
+
+@pytest.mark.parametrize("case", CONE_CASES, ids=[str(c["seed"]) for c in CONE_CASES])
+def test_cbct_plan_vs_oracle(case):
+    from oracle import oracle as O
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    g, vol, sino, mp = _cone_case(**case)
+    say("cone case", case)
+    P = Projector(geometry_from_g3(g, mp))
+    d_vol = P.upload_volume(vol)
+    d_sino = P.upload_sino(sino)
+    out_s = DeviceArray(P.n_sino(), "f32")
+    out_v = DeviceArray(P.n_img, "f32")
+    for kind in ("dd", "tf"):
+        P.proj(d_vol, out_s, f"proj,{kind}")
+        got = P.download_sino(out_s)
+        want = O.cbct(f"proj,{kind}", g, vol, mp)
+        assert rel_l2(got, want) <= TOL, (kind, rel_l2(got, want))
+        # strip pruning must not drop any column: exhaustive candidate lists give the same bits
+        ex = DeviceArray(P.n_sino(), "f32")
+        P.proj(d_vol, ex, f"proj,{kind}", exhaustive=True)
+        assert np.array_equal(P.download_sino(ex), got), kind
+    for op in ("back,dd", "back,tf", "back,pd", "back,hpd"):
+        P.back(d_sino, out_v, op)
+        got = P.download_volume(out_v)
+        want = O.cbct(op, g, sino, mp)
+        assert rel_l2(got, want) <= TOL, (op, rel_l2(got, want))
+
+
+FAN_CASES = [
+    dict(seed=11, n=(64, 64), nu=128, nviews=36, dettype=1),
+    dict(seed=12, n=(45, 77), nu=100, nviews=25, dettype=1, offsets=True),
+    dict(seed=13, n=(50, 50), nu=90, nviews=30, dettype=0),
+]
+
+
+def _fan_case(seed, n, nu, nviews, dettype, offsets=False):
+    from oracle import oracle as O
+    rng = np.random.default_rng(seed)
+    betas = np.sort(rng.uniform(0, 2 * np.pi, nviews))
+    g = O.G2(n[0], n[1], 0.8, 0.7 if offsets else 0.8, 1.2 if offsets else 0.0, -0.7 if offsets else 0.0, nu, 1.0,
+             2.0 if offsets else 0.0, 1000.0, 0.0, 1500.0, betas, dettype)
+    img = rng.uniform(0, 1, n).astype(np.float32)
+    sino = rng.uniform(0, 1, (nu, nviews)).astype(np.float32)
+    return g, img, sino
+
+
+@pytest.mark.parametrize("case", FAN_CASES, ids=[str(c["seed"]) for c in FAN_CASES])
+def test_fbct_plan_vs_oracle(case):
+    from oracle import oracle as O
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    g, img, sino = _fan_case(**case)
+    say("fan case", case)
+    P = Projector(geometry_from_g2(g))
+    d_img = P.upload_volume(img)
+    d_sino = P.upload_sino(sino)
+    out_s = DeviceArray(P.n_sino(), "f32")
+    out_i = DeviceArray(P.n_img, "f32")
+    for kind in ("dd", "tf"):
+        P.proj(d_img, out_s, f"proj,{kind}")
+        got = P.download_sino(out_s)
+        assert rel_l2(got, O.fbct(f"proj,{kind}", g, img)) <= TOL, kind
+        ex = DeviceArray(P.n_sino(), "f32")
+        P.proj(d_img, ex, f"proj,{kind}", exhaustive=True)
+        assert np.array_equal(P.download_sino(ex), got), kind
+    for op in ("back,dd", "back,tf", "back,pd"):
+        P.back(d_sino, out_i, op)
+        got = P.download_volume(out_i)
+        assert rel_l2(got, O.fbct(op, g, sino)) <= TOL, op
+
+
+# ---------------------------------------------------------------------------------
+# adjointness of the GPU pairs, ordered subsets, fp64 kernels, storage dtypes
+# ---------------------------------------------------------------------------------
+def _adjoint_mismatch(ax, y, x, aty):
+    ax, y, x, aty = (np.asarray(a, np.float64).ravel() for a in (ax, y, x, aty))
+    return abs(ax.dot(y) - x.dot(aty)) / (np.linalg.norm(ax) * np.linalg.norm(y))
+
+
+@pytest.mark.parametrize("kind", ["dd", "tf"])
+def test_gpu_pairs_are_adjoint(kind):
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    g, vol, sino, mp = _cone_case(21, n=(40, 40, 32), nuv=(90, 48), nviews=24)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(g.img_shape()).astype(np.float32)
+    y = rng.standard_normal(g.sino_shape()).astype(np.float32)
+    P = Projector(geometry_from_g3(g, mp))
+    ax = DeviceArray(P.n_sino(), "f32")
+    aty = DeviceArray(P.n_img, "f32")
+    P.proj(P.upload_volume(x), ax, f"proj,{kind}")
+    P.back(P.upload_sino(y), aty, f"back,{kind}")
+    m = _adjoint_mismatch(P.download_sino(ax), y, x, P.download_volume(aty))
+    say("adjoint 3D", kind, m)
+    assert m < 1e-6
+    g2, img, s2 = _fan_case(31, (64, 64), 128, 40, 1)
+    x2 = rng.standard_normal(g2.img_shape()).astype(np.float32)
+    y2 = rng.standard_normal(g2.sino_shape()).astype(np.float32)
+    P2 = Projector(geometry_from_g2(g2))
+    a2 = DeviceArray(P2.n_sino(), "f32")
+    t2 = DeviceArray(P2.n_img, "f32")
+    P2.proj(P2.upload_volume(x2), a2, f"proj,{kind}")
+    P2.back(P2.upload_sino(y2), t2, f"back,{kind}")
+    m2 = _adjoint_mismatch(P2.download_sino(a2), y2, x2, P2.download_volume(t2))
+    say("adjoint 2D", kind, m2)
+    assert m2 < 1e-6
+
+
+def test_ordered_subsets_and_scale():
+    from oracle import oracle as O
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    from ctsim_amd.shell import subset_views
+    g, vol, sino, mp = _cone_case(41, nviews=24)
+    P = Projector(geometry_from_g3(g, mp))
+    d_vol = P.upload_volume(vol)
+    M = 5
+    for isub in (1, 3, 5):
+        v0, vs, ns = Projector.subset(g.nviews, M, isub)
+        idx = subset_views(g.nviews, M, isub)
+        gs = O.G3(g.n, g.d, g.o, g.nuv, g.duv, g.ouv, g.sad, g.add, g.sdd, g.betas[idx], couchz=g.couchz[idx],
+                  dettype=g.dettype)
+        out = DeviceArray(P.n_sino(ns), "f32")
+        P.proj(d_vol, out, "proj,tf", v0, vs, ns)
+        assert rel_l2(P.download_sino(out, ns), O.cbct("proj,tf", gs, vol, mp)) <= TOL
+        sub = np.asfortranarray(sino[:, :, idx])
+        img = DeviceArray(P.n_img, "f32")
+        P.back(P.upload_sino(sub), img, "back,tf", v0, vs, ns, scale=M)
+        want = np.float32(M) * O.cbct("back,tf", gs, sub, mp)
+        assert rel_l2(P.download_volume(img), want) <= TOL
+
+
+def test_fp64_kernels_vs_fp64_oracle():
+    from oracle import oracle as O
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    g, vol, sino, mp = _cone_case(51, n=(32, 30, 20), nuv=(70, 36), nviews=12)
+    P = Projector(geometry_from_g3(g, mp), compute="f64")
+    d_vol = P.upload_volume(vol.astype(np.float64), dtype="f64")
+    d_sino = P.upload_sino(sino.astype(np.float64), dtype="f64")
+    for kind in ("dd", "tf"):
+        out = DeviceArray(P.n_sino(), "f64")
+        P.proj(d_vol, out, f"proj,{kind}")
+        assert rel_l2(P.download_sino(out), O.cbct(f"proj,{kind}", g, vol, mp, prec=64)) < 1e-11
+    for op in ("back,dd", "back,tf", "back,hpd"):
+        out = DeviceArray(P.n_img, "f64")
+        P.back(d_sino, out, op)
+        assert rel_l2(P.download_volume(out), O.cbct(op, g, sino, mp, prec=64)) < 1e-11, op
+    g2, img, s2 = _fan_case(52, (40, 40), 80, 16, 1)
+    P2 = Projector(geometry_from_g2(g2), compute="f64")
+    out = DeviceArray(P2.n_sino(), "f64")
+    P2.proj(P2.upload_volume(img.astype(np.float64), "f64"), out, "proj,tf")
+    assert rel_l2(P2.download_sino(out), O.fbct("proj,tf", g2, img, prec=64)) < 1e-11
+
+
+@pytest.mark.parametrize("dt,tol", [("f16", 2e-3), ("bf16", 1.2e-2)])
+def test_half_storage_close_to_fp32(dt, tol):
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    g, vol, sino, mp = _cone_case(61)
+    P = Projector(geometry_from_g3(g, mp))
+    ref = DeviceArray(P.n_sino(), "f32")
+    P.proj(P.upload_volume(vol), ref, "proj,tf")
+    half = DeviceArray(P.n_sino(), dt)
+    P.proj(P.upload_volume(vol, dtype=dt), half, "proj,tf")
+    assert rel_l2(half.to_numpy(), ref.to_numpy()) < tol
+    v32 = DeviceArray(P.n_img, "f32")
+    P.back(P.upload_sino(sino), v32, "back,tf")
+    v16 = DeviceArray(P.n_img, "f32")
+    P.back(P.upload_sino(sino, dtype=dt), v16, "back,tf")
+    assert rel_l2(v16.to_numpy(), v32.to_numpy()) < tol
+
+
+def test_batched_uneven_2d_matches_single_slices():
+    import ctypes as C
+    from ctsim_amd import _lib
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    rng = np.random.default_rng(71)
+    geoms, imgs, betas_list = [], [], []
+    for k in range(6):
+        n = int(rng.choice([32, 48, 64]))
+        nv = int(rng.choice([10, 20, 30]))
+        nu = int(rng.choice([64, 96]))
+        b = np.sort(rng.uniform(0, 2 * np.pi, nv))
+        from oracle import oracle as O
+        g = O.G2(n, n, 0.8, 0.8, 0, 0, nu, 1.0, 0, 1000.0, 0, 1500.0, b, 1)
+        geoms.append(g)
+        imgs.append(ellipse_phantom2(n, n, 0.8, 4, rng))
+        betas_list.append(b)
+    cg = (_lib.CtsimGeom * len(geoms))(*[geometry_from_g2(g).to_c() for g in geoms])
+    nvs = np.array([g.nviews for g in geoms], np.int32)
+    bptrs = (C.POINTER(C.c_double) * len(geoms))(*[b.ctypes.data_as(C.POINTER(C.c_double)) for b in betas_list])
+    h = C.c_void_p()
+    _lib.call("ctsim_batch2_create", C.byref(h), len(geoms), cg, nvs.ctypes.data_as(C.POINTER(C.c_int32)), bptrs, 0)
+    ni, ns = C.c_int64(), C.c_int64()
+    _lib.call("ctsim_batch2_sizes", h, C.byref(ni), C.byref(ns))
+    packed = np.concatenate([im.ravel(order="F") for im in imgs])
+    assert packed.size == ni.value
+    d_img = DeviceArray.from_numpy(packed, "f32")
+    d_sino = DeviceArray(ns.value, "f32")
+    for kind in ("dd", "tf"):
+        _lib.call("ctsim_batch2_proj", h, f"proj,{kind}".encode(), d_img.vp, 0, d_sino.vp, 0, None)
+        got = d_sino.to_numpy()
+        off = 0
+        for g, im in zip(geoms, imgs):
+            P = Projector(geometry_from_g2(g))
+            one = DeviceArray(P.n_sino(), "f32")
+            P.proj(P.upload_volume(im), one, f"proj,{kind}")
+            np.testing.assert_array_equal(got[off:off + one.count], one.to_numpy())
+            off += one.count
+        d_back = DeviceArray(ni.value, "f32")
+        _lib.call("ctsim_batch2_back", h, f"back,{kind}".encode(), d_sino.vp, 0, d_back.vp, 0, None)
+        gb = d_back.to_numpy()
+        off_i, off_s = 0, 0
+        for g in geoms:
+            P = Projector(geometry_from_g2(g))
+            n_s = P.n_sino()
+            s = DeviceArray.from_numpy(got[off_s:off_s + n_s], "f32")
+            one = DeviceArray(P.n_img, "f32")
+            P.back(s, one, f"back,{kind}")
+            np.testing.assert_array_equal(gb[off_i:off_i + P.n_img], one.to_numpy())
+            off_i += P.n_img
+            off_s += n_s
+    _lib.call("ctsim_batch2_destroy", h)
