@@ -1,0 +1,281 @@
+"""Benchmark: PWLS-OS iterations per second at 512x512x128 (984 views, 1024x64 detector,
+24 ordered subsets, Huber beta 1e4 delta 1e-3), mixed-precision graph form vs plain FP32,
+plus DD / SF forward+back projection GUPS at 512x512x256 (720 views, 1024x384).
+
+A step is one PWLS-OS iteration: 24 x (forward projection of the subset, residual,
+backprojection x 24, fused Nesterov-SQS update) plus the per-iteration penalty cost and
+RMSD, all resident in HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--no-pair] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_counts():
+    with open(os.path.join(ROOT, "tests", "golden", "work_counts.json")) as f:
+        return json.load(f)
+
+
+def flops(c):
+    """algorithmic FLOPs of one 3D DD/SF operator: 17 U + 3 C + 60 K"""
+    return 17.0 * c["U"] + 3.0 * c["C"] + 60.0 * c["K"]
+
+
+def simulate_pwls_data(geom, seed=4):
+    """phantom -> SF projection on the GPU -> Poisson counts at I0 = 2e4 -> y, w = counts"""
+    from ctsim_amd import workloads as W
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    vol = W.ellipsoid_phantom(geom.reconSize, geom.reconSpacing, 20, seed)
+    P = Projector(geom)
+    d = DeviceArray(P.n_sino(), "f32")
+    P.proj(P.upload_volume(vol), d, "proj,tf")
+    p = P.download_sino(d).astype(np.float64)
+    rng = np.random.default_rng(seed + 1)
+    I0 = 2.0e4
+    counts = rng.poisson(I0 * np.exp(-p)).astype(np.float64)
+    y = -np.log(np.maximum(counts, 10.0) / I0)  # dummyLowPhotonCount = 10
+    w = counts  # computeWeightsPwls with r = 0, v = 0
+    return vol, y.astype(np.float32), w.astype(np.float32)
+
+
+class Timer:
+    def __init__(self, stream):
+        from ctsim_amd.device import Event
+        self.stream = stream
+        self.a, self.b = Event(), Event()
+
+    def __enter__(self):
+        self.a.record(self.stream)
+        return self
+
+    def __exit__(self, *exc):
+        self.b.record(self.stream)
+        self.ms = self.a.elapsed_ms(self.b)
+        return False
+
+
+def kernel_times(solver, reps=1):
+    """average launch duration (ms) of each PWLS kernel over the 24 subsets, by HIP events
+    on the solver's stream"""
+    import ctypes as C
+    from ctsim_amd._lib import call
+    st = solver.stream
+    nx, ny, nz = solver.nxyz
+    t = {"proj": 0.0, "back": 0.0, "residual": 0.0, "update": 0.0}
+    src = solver.zc if solver.mixed else solver.z[solver.cur]
+    for _ in range(reps):
+        for k, (v0, vs, ns, off) in enumerate(solver.subsets):
+            pw = solver._window(solver.p, 0, ns)
+            with Timer(st) as tm:
+                solver.P.proj(src, pw, f"proj,{solver.kind}", v0, vs, ns, stream=st)
+            t["proj"] += tm.ms
+            with Timer(st) as tm:
+                call("ctsim_pwls_residual", pw.vp, solver.p.dtype, solver._sub_ptr(solver.y, off), solver.y.dtype,
+                     solver._sub_ptr(solver.w, off), solver.w.dtype, solver.r.vp, solver.r.dtype, ns * solver.cell,
+                     C.c_void_p(solver.scal.ptr), solver._sv())
+            t["residual"] += tm.ms
+            with Timer(st) as tm:
+                solver.P.back(solver._window(solver.r, 0, ns), solver.g, f"back,{solver.kind}", v0, vs, ns,
+                              scale=float(solver.M), stream=st)
+            t["back"] += tm.ms
+            a, b = solver.cur, 1 - solver.cur
+            with Timer(st) as tm:
+                call("ctsim_pwls_nesterov_update", solver.x[a].vp, solver.z[a].vp, solver.s.vp, solver.g.vp,
+                     solver.precom.vp, solver.x[b].vp, solver.z[b].vp,
+                     solver.zc.vp if solver.zc is not None else None, solver.zc.dtype if solver.zc is not None else 0,
+                     nx, ny, nz, solver.beta, solver.delta, 1.0, 1.0, solver._sv())
+            t["update"] += tm.ms
+    n = reps * len(solver.subsets)
+    return {k: v / n for k, v in t.items()}
+
+
+def run_pwls(form, geom, y, w, x0, steps, warmup, numos=24, beta=1e4, delta=1e-3):
+    from ctsim_amd.device import synchronize
+    from ctsim_amd.pwls import PwlsOS
+    mixed = form == "mixed"
+    solver = PwlsOS(geom, y, w, numos, beta, delta, mixed=mixed, sino_dtype="f16", zcopy_dtype="f16")
+    solver.setup(x0)
+    solver.prepare(warmup + steps)
+    for _ in range(warmup):
+        solver.iterate()
+    synchronize()
+    t0 = time.perf_counter()
+    outs = [solver.iterate() for _ in range(steps)]
+    synchronize()
+    dt = time.perf_counter() - t0
+    return solver, dt, outs
+
+
+def pair_gups(kinds=("dd", "tf"), reps=2):
+    """forward + back projection of the 512x512x256 / 720-view / 1024x384 pair"""
+    from ctsim_amd import workloads as W
+    from ctsim_amd.device import DeviceArray, Stream
+    from ctsim_amd.projector import Projector
+    geom = W.sf_pair_512()
+    P = Projector(geom)
+    st = Stream()
+    rng = np.random.default_rng(1)
+    vol = DeviceArray.from_numpy(rng.uniform(0, 1, P.n_img).astype(np.float32), "f32", stream=st)
+    sino = DeviceArray(P.n_sino(), "f32")
+    img = DeviceArray(P.n_img, "f32")
+    counts = load_counts()
+    out = {}
+    for kind in kinds:
+        P.proj(vol, sino, f"proj,{kind}", stream=st)
+        P.back(sino, img, f"back,{kind}", stream=st)
+        st.synchronize()
+        tp = tb = 0.0
+        for _ in range(reps):
+            with Timer(st) as a:
+                P.proj(vol, sino, f"proj,{kind}", stream=st)
+            with Timer(st) as b:
+                P.back(sino, img, f"back,{kind}", stream=st)
+            tp += a.ms
+            tb += b.ms
+        tp /= reps
+        tb /= reps
+        c = counts[f"sf_pair_512/{kind}"]
+        out[kind] = dict(proj_ms=round(tp, 3), back_ms=round(tb, 3),
+                         gups_pair=round(2 * c["U"] / ((tp + tb) * 1e-3) / 1e9, 2),
+                         proj_tflops=round(flops(c) / (tp * 1e-3) / 1e12, 2),
+                         back_tflops=round(flops(c) / (tb * 1e-3) / 1e12, 2))
+        log(f"pair {kind}: proj {tp:.1f} ms  back {tb:.1f} ms  {out[kind]['gups_pair']} GUPS")
+    return out
+
+
+def cpu_baseline(geom, nsample=8):
+    """reference CPU projector (oracle/_ref, else the oracle port), one core, on nsample
+    evenly spaced views of the PWLS geometry; extrapolated to one iteration (every view
+    forward and back once)."""
+    from oracle import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    idx = np.linspace(0, geom.noViews - 1, nsample).astype(int)
+    g = O.G3(geom.reconSize, geom.reconSpacing, geom.reconOffset, geom.detSize, geom.detSpacing, geom.detOffset,
+             geom.SAD, geom.ADD, geom.SDD, geom.betas[idx], couchz=geom.couchZ[idx], dettype=geom.flatPanel)
+    rng = np.random.default_rng(2)
+    vol = rng.uniform(0, 1, geom.reconSize).astype(np.float32)
+    sino = rng.uniform(0, 1, g.sino_shape()).astype(np.float32)
+    mp = np.asarray(geom.map, bool)
+    run = (lambda op, x: O.ref_cbct(op, g, x, mp)) if kind == "reference" else (
+        lambda op, x: O.cbct(op, g, x, mp, nthreads=1))
+    t0 = time.perf_counter()
+    run("proj,tf", vol)
+    t1 = time.perf_counter()
+    run("back,tf", sino)
+    t2 = time.perf_counter()
+    per_iter = (t2 - t0) * geom.noViews / nsample
+    return dict(value=round(1.0 / per_iter, 6), unit="iter/s", cores=1, kind=kind,
+                sample=f"proj,tf + back,tf of {nsample} of 984 views at 512x512x128 with the FOV map, "
+                       f"{t1 - t0:.1f} s + {t2 - t1:.1f} s, scaled x{geom.noViews / nsample:.0f} to one iteration "
+                       f"(elementwise PWLS work not included); host {os.uname().machine}, 1 thread")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-pair", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from ctsim_amd import workloads as W
+    from ctsim_amd.device import device_count
+    if device_count() < 1:
+        raise SystemExit("no GPU visible")
+    geom = W.pwls_512x128()
+    log("simulating PWLS data ...")
+    vol, y, w = simulate_pwls_data(geom)
+    # FDK initial image (reconPwlsSeNesterovSqs initialises from FBP)
+    from ctsim_amd.fdk import recon_fdk
+    x0 = recon_fdk(y, geom, "hamming")
+    x0 = np.maximum(x0, 0).astype(np.float32)
+    log("FDK init done")
+
+    counts = load_counts()
+    c_it = counts["pwls_512x128/tf"]
+    f_iter = 2 * flops(c_it)  # forward + back over all views per iteration
+
+    solver, dt, outs = run_pwls("mixed", geom, y, w, x0, args.steps, args.warmup)
+    ms_mixed = dt / args.steps * 1e3
+    log(f"mixed: {ms_mixed:.1f} ms/iter, phi {outs[-1]['phi']:.6e}, rmsd {outs[-1]['rmsd']:.3e}")
+    kt = kernel_times(solver)
+    log("kernel ms (avg per subset):", {k: round(v, 3) for k, v in kt.items()})
+    del solver
+
+    plain = None
+    if not args.no_plain:
+        sp, dtp, outs_p = run_pwls("plain", geom, y, w, x0, args.steps, args.warmup)
+        plain = dtp / args.steps * 1e3
+        log(f"plain: {plain:.1f} ms/iter, phi {outs_p[-1]['phi']:.6e}")
+        kt_plain = kernel_times(sp)
+        del sp
+    M = 24
+    # dominant kernel: the forward projector (per-subset launch)
+    f_launch = flops(c_it) / M
+    achieved = f_launch / (kt["proj"] * 1e-3) / 1e12
+    roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=None,
+                kernel="cbct_proj_kernel<__half,__half,float,SF,16> (one 41-view subset)",
+                note="FP32 vector ALU bound (no MFMA); achieved = (17U+3C+60K)/24 per launch / event time")
+    res = {
+        "metric": "PWLS-OS iter/s at 512²×128 and DD proj+backproj GUPS, mixed vs plain FP32",
+        "value": round(1e3 / ms_mixed, 4),
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_mixed, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (20 random ellipsoids, Poisson I0=2e4, FDK init)",
+        "config": {"workload": "pwls_os_sf_512x512x128_984views_1024x64det_24subsets", "volume": [512, 512, 128],
+                   "views": 984, "detector": [1024, 64], "subsets": 24, "beta": 1e4, "delta": 1e-3,
+                   "storage": "fp16 sinogram/weights/residual/z-copy, fp32 x/z/s, fp64 reductions",
+                   "graph": True},
+        "plain_fp32_iter_per_s": round(1e3 / plain, 4) if plain else None,
+        "mixed_over_plain": round(plain / ms_mixed, 3) if plain else None,
+        "gups_iteration": round(2 * c_it["U"] / (ms_mixed * 1e-3) / 1e9, 2),
+        "iteration_tflops": round(f_iter / (ms_mixed * 1e-3) / 1e12, 3),
+        "kernel_ms_per_subset": {k: round(v, 4) for k, v in kt.items()},
+        "roofline": roof,
+    }
+    if plain:
+        res["kernel_ms_per_subset_plain"] = {k: round(v, 4) for k, v in kt_plain.items()}
+    if not args.no_pair:
+        res["pair_512x512x256"] = pair_gups()
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        res["cpu_baseline"] = cpu_baseline(geom)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

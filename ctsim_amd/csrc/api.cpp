@@ -752,9 +752,30 @@ int ctsim_pwls_nesterov_update(const float* x, const float* z, float* s, const f
     if (x_out == x || z_out == z || x_out == z || z_out == x)
         return fail(CTSIM_EINVAL, "Error: outputs must not alias the stencil inputs.");
     hipError_t e = nesterov_launch(x, z, s, g, precom, x_out, z_out, zcopy, zcopy_dtype, nx, ny, nz, beta, delta, t,
-                                   t_next, (hipStream_t)stream);
+                                   t_next, nullptr, nullptr, 0, (hipStream_t)stream);
     if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: z copy dtype not supported.");
     HIP_TRY(e, "nesterov update");
+    return CTSIM_OK;
+}
+
+int ctsim_pwls_nesterov_update_seq(const float* x, const float* z, float* s, const float* g, const float* precom,
+                                   float* x_out, float* z_out, void* zcopy, int32_t zcopy_dtype, int32_t nx,
+                                   int32_t ny, int32_t nz, double beta, double delta, const double* d_tseq,
+                                   const int32_t* d_base, int32_t k, void* stream) {
+    if (!x || !z || !s || !g || !precom || !x_out || !z_out || !d_tseq || !d_base)
+        return fail(CTSIM_EINVAL, "Error: null argument.");
+    if (x_out == x || z_out == z || x_out == z || z_out == x)
+        return fail(CTSIM_EINVAL, "Error: outputs must not alias the stencil inputs.");
+    hipError_t e = nesterov_launch(x, z, s, g, precom, x_out, z_out, zcopy, zcopy_dtype, nx, ny, nz, beta, delta, 1.0,
+                                   1.0, d_tseq, (const int*)d_base, k, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: z copy dtype not supported.");
+    HIP_TRY(e, "nesterov update");
+    return CTSIM_OK;
+}
+
+int ctsim_add_i32(int32_t* d_counter, int32_t increment, void* stream) {
+    if (!d_counter) return fail(CTSIM_EINVAL, "Error: null argument.");
+    HIP_TRY(add_i32_launch((int*)d_counter, increment, (hipStream_t)stream), "counter add");
     return CTSIM_OK;
 }
 

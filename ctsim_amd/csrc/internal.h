@@ -72,7 +72,8 @@ hipError_t huber_launch(const float* x, int nx, int ny, int nz, double delta, in
                         hipStream_t st);
 hipError_t nesterov_launch(const float* x, const float* z, float* s, const float* g, const float* precom, float* x_out,
                            float* z_out, void* zcopy, int zcdt, int nx, int ny, int nz, double beta, double delta,
-                           double t, double t_next, hipStream_t st);
+                           double t, double t_next, const double* tseq, const int* base, int k, hipStream_t st);
+hipError_t add_i32_launch(int* p, int inc, hipStream_t st);
 hipError_t sqs_launch(const float* x, float* x_out, const float* g, const float* precom, int nx, int ny, int nz,
                       double beta, double delta, hipStream_t st);
 hipError_t sqdiff_launch(const float* a, const float* b, size_t n, double* out, hipStream_t st);

@@ -237,13 +237,23 @@ __global__ __launch_bounds__(kRB) void huber_cost_kernel(const float* __restrict
 __device__ __forceinline__ float scrub(float v) { return (v < 0.f || !isfinite(v)) ? 0.f : v; }
 
 // fused Nesterov-SQS subset step; reads x, z (stencils) and writes x_out, z_out
+// t and t_next come either from the arguments or, when tseq is not null, from
+// tseq[*base + k] and tseq[*base + k + 1] (device memory, so a captured graph can be
+// replayed across iterations while the momentum sequence advances)
 template <class ZC>
 __global__ __launch_bounds__(256) void nesterov_kernel(const float* __restrict__ x, const float* __restrict__ z,
                                                        float* __restrict__ s, const float* __restrict__ g,
                                                        const float* __restrict__ precom, float* __restrict__ x_out,
                                                        float* __restrict__ z_out, ZC* __restrict__ zcopy, int nx, int ny,
-                                                       int nz, double beta, float delta, double t, double t_next) {
+                                                       int nz, double beta, float delta, double t, double t_next,
+                                                       const double* __restrict__ tseq, const int* __restrict__ base,
+                                                       int k) {
     const size_t n = (size_t)nx * ny * nz;
+    if (tseq) {
+        const int i0 = *base + k;
+        t = tseq[i0];
+        t_next = tseq[i0 + 1];
+    }
     const double a = 1.0 - 1.0 / t_next, b = 1.0 / t_next;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int iz = (int)(i % nz);
@@ -402,15 +412,24 @@ hipError_t huber_launch(const float* x, int nx, int ny, int nz, double delta, in
 
 hipError_t nesterov_launch(const float* x, const float* z, float* s, const float* g, const float* precom, float* x_out,
                            float* z_out, void* zcopy, int zcdt, int nx, int ny, int nz, double beta, double delta,
-                           double t, double t_next, hipStream_t st) {
+                           double t, double t_next, const double* tseq, const int* base, int k, hipStream_t st) {
     const size_t n = (size_t)nx * ny * nz;
     const dim3 gr(grid_for(n)), bl(256);
     switch (zcopy ? zcdt : DT_F32) {
-        case DT_F32: hipLaunchKernelGGL(nesterov_kernel<float>, gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (float*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next); break;
-        case DT_F16: hipLaunchKernelGGL(nesterov_kernel<__half>, gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__half*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next); break;
-        case DT_BF16: hipLaunchKernelGGL(nesterov_kernel<__hip_bfloat16>, gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__hip_bfloat16*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next); break;
+        case DT_F32: hipLaunchKernelGGL(nesterov_kernel<float>, gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (float*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
+        case DT_F16: hipLaunchKernelGGL(nesterov_kernel<__half>, gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__half*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
+        case DT_BF16: hipLaunchKernelGGL(nesterov_kernel<__hip_bfloat16>, gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__hip_bfloat16*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+__global__ void add_i32_kernel(int* p, int inc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p += inc;
+}
+
+hipError_t add_i32_launch(int* p, int inc, hipStream_t st) {
+    hipLaunchKernelGGL(add_i32_kernel, dim3(1), dim3(64), 0, st, p, inc);
     return hipGetLastError();
 }
 

@@ -1,0 +1,235 @@
+"""Device-resident PWLS ordered-subset reconstruction.
+
+NesterovSqs restates statsrecon/pwls/reconPwlsSeNesterovSqs.m:37-137 (and Sqs the plain
+SQS loop of statsrecon/pwls/reconPwlsSeSqs.m:36-110) with the separable-footprint
+operators of geom/shell/loadPojectors.m and the Huber penalty of penalty/huberPenalty.m:
+
+    precom = At(w .* A(ones))
+    per subset:  d = Aos(z) - Os(y);  g = Atos(Os(w) .* d) + beta S(z);
+                 du = g ./ (precom + beta T(x));  x = z - du;  s -= t du;
+                 t' = (1 + sqrt(1 + 4 t^2)) / 2;  z = (1 - 1/t') x + s / t';  clamp
+    per iteration: phi = sum_sub sum(w d^2)/2 + beta R(x);  rmsd = rms(x - xold)
+
+Everything stays in HBM.  Sinogram-side data (y, w, the residual) is stored in
+subset-major order, so Os(., isub) is a contiguous slice.
+
+Two forms:
+  * plain:  FP32 storage everywhere, one kernel launch per operator call;
+  * mixed:  y, w and the residual in FP16 or BF16, the forward projector reads an
+            FP16 copy of z written by the update kernel, FP32 footprint sums, FP64
+            step denominators and reductions, and each iteration replayed as one
+            captured hipGraph.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from ._lib import F32, call
+from .device import DeviceArray, Graph, Stream, dtype_code
+from .geometry import Geometry
+from .projector import Projector
+from .shell import subset_views
+
+
+def nesterov_t_sequence(n_steps: int, restart_every: int | None = None, steps_per_iter: int = 1) -> np.ndarray:
+    """t_0 = 1, t_{k+1} = (1 + sqrt(1 + 4 t_k^2)) / 2 (reconPwlsSeNesterovSqs.m:55,74); restarts to 1."""
+    t = np.empty(n_steps + 1, np.float64)
+    t[0] = 1.0
+    for k in range(n_steps):
+        t[k + 1] = (1.0 + math.sqrt(1.0 + 4.0 * t[k] * t[k])) / 2.0
+    return t
+
+
+class PwlsOS:
+    def __init__(self, geom: Geometry, y: np.ndarray, w: np.ndarray, numos: int, beta: float, delta: float,
+                 mixed: bool = True, sino_dtype: str = "f16", zcopy_dtype: str = "f16", graph: bool | None = None,
+                 op: str = "tf", stream: Stream | None = None):
+        if geom.ndim != 3:
+            raise ValueError("PwlsOS drives the 3D cone-beam operators")
+        self.geom = geom
+        self.P = Projector(geom)
+        self.M = int(numos)
+        self.beta = float(beta)
+        self.delta = float(delta)
+        self.mixed = bool(mixed)
+        self.graph_mode = self.mixed if graph is None else bool(graph)
+        self.kind = op
+        self.stream = stream or Stream()
+        nx, ny, nz = geom.reconSize
+        self.nxyz = (nx, ny, nz)
+        nv, nu, nviews = geom.sino_shape()
+        self.cell = nu * nv
+        self.sdt = dtype_code(sino_dtype) if self.mixed else F32
+        self.zdt = dtype_code(zcopy_dtype) if self.mixed else None
+        # subset-major permutation of the views
+        self.subsets = []
+        order = []
+        off = 0
+        for isub in range(1, self.M + 1):
+            v0, vs, ns = Projector.subset(nviews, self.M, isub)
+            self.subsets.append((v0, vs, ns, off))
+            order.append(subset_views(nviews, self.M, isub))
+            off += ns
+        self.order = np.concatenate(order)
+        y = np.asarray(y, np.float32).reshape((nv, nu, nviews), order="F")[:, :, self.order]
+        w = np.asarray(w, np.float32).reshape((nv, nu, nviews), order="F")[:, :, self.order]
+        st = self.stream
+        self.y = DeviceArray.from_numpy(y.ravel(order="F"), self.sdt, stream=st)
+        self.w = DeviceArray.from_numpy(w.ravel(order="F"), self.sdt, stream=st)
+        nmax = max(s[2] for s in self.subsets)
+        nvox = nx * ny * nz
+        self.p = DeviceArray(nmax * self.cell, self.sdt if self.mixed else F32)  # Aos(z)
+        self.r = DeviceArray(nmax * self.cell, self.sdt)  # Os(w) .* d
+        self.x = [DeviceArray(nvox, F32), DeviceArray(nvox, F32)]
+        self.z = [DeviceArray(nvox, F32), DeviceArray(nvox, F32)]
+        self.s = DeviceArray(nvox, F32)
+        self.g = DeviceArray(nvox, F32)
+        self.precom = DeviceArray(nvox, F32)
+        self.xold = DeviceArray(nvox, F32)
+        self.zc = DeviceArray(nvox, self.zdt) if self.mixed else None
+        self.scal = DeviceArray(4, "f64")  # phi, R, rmsd numerator, spare
+        self.cur = 0  # which of the x / z double buffers is current
+        self.t = 1.0
+        self.itn = 0
+        self._graph = None
+        self._tseq = None
+        self._base = None
+
+    # ------------------------------------------------------------------------------
+    def _sv(self):
+        return C.c_void_p(self.stream.handle)
+
+    def _sub_ptr(self, arr: DeviceArray, off_views: int) -> C.c_void_p:
+        esz = 2 if arr.dtype in (1, 2) else (8 if arr.dtype == 3 else 4)
+        return C.c_void_p(arr.ptr + off_views * self.cell * esz)
+
+    class _View:
+        """a DeviceArray-like window for the plan API (pointer, count, dtype)"""
+
+        def __init__(self, ptr, count, dtype):
+            self.ptr, self.count, self.dtype = ptr, count, dtype
+
+        @property
+        def vp(self):
+            return C.c_void_p(self.ptr)
+
+    def _window(self, arr: DeviceArray, off_views: int, nviews: int):
+        return PwlsOS._View(self._sub_ptr(arr, off_views).value, nviews * self.cell, arr.dtype)
+
+    def setup(self, x0: np.ndarray) -> None:
+        """x = z = s = x0 (MATLAB order), precom = At(w .* A(ones))."""
+        st = self.stream
+        x0d = self.P.upload_volume(np.asarray(x0, np.float32), "f32", stream=st)
+        for a in (self.x[0], self.z[0], self.s):
+            a.copy_from(x0d, stream=st)
+        if self.zc is not None:
+            self.zc.copy_from(self.z[0], stream=st)
+        ones = DeviceArray(self.P.n_img, F32).fill(1.0, stream=st)
+        a = DeviceArray(max(s[2] for s in self.subsets) * self.cell, F32)
+        wa = DeviceArray(a.count, F32)
+        # At(w .* A(ones)) summed subset by subset: every view is used once, scale 1
+        for k, (v0, vs, ns, off) in enumerate(self.subsets):
+            self.P.proj(ones, a, f"proj,{self.kind}", v0, vs, ns, stream=st)
+            call("ctsim_mul", a.vp, F32, self._sub_ptr(self.w, off), self.w.dtype, wa.vp, F32, ns * self.cell,
+                 self._sv())
+            self.P.back(self._View(wa.ptr, ns * self.cell, F32), self.precom, f"back,{self.kind}", v0, vs, ns,
+                        scale=1.0, accumulate=k > 0, stream=st)
+        st.synchronize()
+        self.cur = 0
+        self.t = 1.0
+        self.itn = 0
+
+    # ------------------------------------------------------------------------------
+    def _subset_step(self, k: int, t: float | None, t_next: float | None) -> None:
+        """one ordered subset (index k into self.subsets) on the stream"""
+        nx, ny, nz = self.nxyz
+        v0, vs, ns, off = self.subsets[k]
+        src = self.zc if self.mixed else self.z[self.cur]
+        pw = self._window(self.p, 0, ns)
+        self.P.proj(src, pw, f"proj,{self.kind}", v0, vs, ns, stream=self.stream)
+        call("ctsim_pwls_residual", pw.vp, self.p.dtype, self._sub_ptr(self.y, off), self.y.dtype,
+             self._sub_ptr(self.w, off), self.w.dtype, self.r.vp, self.r.dtype, ns * self.cell,
+             C.c_void_p(self.scal.ptr), self._sv())
+        self.P.back(self._window(self.r, 0, ns), self.g, f"back,{self.kind}", v0, vs, ns, scale=float(self.M),
+                    stream=self.stream)
+        a, b = self.cur, 1 - self.cur
+        zc = self.zc.vp if self.zc is not None else None
+        zcdt = self.zc.dtype if self.zc is not None else 0
+        if t is None:
+            call("ctsim_pwls_nesterov_update_seq", self.x[a].vp, self.z[a].vp, self.s.vp, self.g.vp, self.precom.vp,
+                 self.x[b].vp, self.z[b].vp, zc, zcdt, nx, ny, nz, self.beta, self.delta, self._tseq.vp,
+                 self._base.vp, k, self._sv())
+        else:
+            call("ctsim_pwls_nesterov_update", self.x[a].vp, self.z[a].vp, self.s.vp, self.g.vp, self.precom.vp,
+                 self.x[b].vp, self.z[b].vp, zc, zcdt, nx, ny, nz, self.beta, self.delta, t, t_next, self._sv())
+        self.cur = b
+
+    def _iteration_body(self, device_t: bool) -> None:
+        st = self.stream
+        nx, ny, nz = self.nxyz
+        self.xold.copy_from(self.x[self.cur], stream=st)
+        call("ctsim_fill", C.c_void_p(self.scal.ptr), 3, 1, 0.0, self._sv())  # phi = 0 at isub == 1
+        for k in range(self.M):
+            if device_t:
+                self._subset_step(k, None, None)
+            else:
+                t_next = (1.0 + math.sqrt(1.0 + 4.0 * self.t * self.t)) / 2.0
+                self._subset_step(k, self.t, t_next)
+                self.t = t_next
+        xc = self.x[self.cur]
+        call("ctsim_huber", xc.vp, nx, ny, nz, self.delta, 0, None, C.c_void_p(self.scal.ptr + 8), self._sv())
+        call("ctsim_sqdiff_sum", xc.vp, self.xold.vp, xc.count, C.c_void_p(self.scal.ptr + 16), self._sv())
+        if device_t:
+            call("ctsim_add_i32", self._base.vp, self.M, self._sv())
+
+    def prepare(self, itnlim: int) -> None:
+        """for the graph form: momentum sequence on the device, one iteration captured"""
+        if not self.graph_mode:
+            return
+        seq = nesterov_t_sequence(itnlim * self.M + 1)
+        self._tseq = DeviceArray.from_numpy(seq, "f64", stream=self.stream)
+        self._base = DeviceArray(1, "f32")  # one int32 counter
+        call("ctsim_fill", self._base.vp, F32, 1, 0.0, self._sv())  # 0.0f has the bits of int 0
+        if self.M % 2:
+            raise ValueError("graph replay needs an even number of subsets (x/z double buffers)")
+        cur0 = self.cur
+        with Graph(self.stream) as gr:
+            self._iteration_body(device_t=True)
+        self._graph = gr
+        self.cur = cur0  # capture did not execute anything
+
+    def iterate(self) -> dict:
+        """one PWLS-OS iteration; returns phi, beta R(x), rmsd (read back after the iteration)"""
+        if self.graph_mode:
+            if self._graph is None:
+                raise RuntimeError("call prepare(itnlim) before iterating the graph form")
+            self._graph.launch()
+        else:
+            self._iteration_body(device_t=False)
+        self.itn += 1
+        self.stream.synchronize()
+        sc = self.scal.to_numpy()
+        nvox = self.P.n_img
+        return dict(phi=float(sc[0] + self.beta * sc[1]), penalty=float(self.beta * sc[1]),
+                    rmsd=float(math.sqrt(sc[2] / nvox)))
+
+    def image(self) -> np.ndarray:
+        """current x in MATLAB order"""
+        self.stream.synchronize()
+        return self.P.download_volume(self.x[self.cur], stream=self.stream)
+
+    def run(self, x0: np.ndarray, itnlim: int, stop_crit: float = 1e-10):
+        """reconPwlsSeNesterovSqs main loop (restart every 2*itnlim never triggers within itnlim)"""
+        self.setup(x0)
+        self.prepare(itnlim)
+        phis, rmsds = [], []
+        for itn in range(1, itnlim + 1):
+            out = self.iterate()
+            phis.append(out["phi"])
+            rmsds.append(out["rmsd"])
+            if itn > 5 and out["rmsd"] < stop_crit:
+                break
+        return self.image(), np.array(phis), np.array(rmsds)

@@ -228,6 +228,18 @@ int ctsim_pwls_nesterov_update(const float* d_x, const float* d_z, float* d_s, c
                                int32_t zcopy_dtype, int32_t nx, int32_t ny, int32_t nz, double beta, double delta,
                                double t, double t_next, void* stream);
 
+/* Same update with the momentum scalars read from device memory:
+ * t = d_tseq[*d_base + k], t_next = d_tseq[*d_base + k + 1].  Lets one captured graph
+ * per iteration be replayed while the sequence advances (ctsim_add_i32 moves the
+ * base at the end of the graph). */
+int ctsim_pwls_nesterov_update_seq(const float* d_x, const float* d_z, float* d_s, const float* d_g,
+                                   const float* d_precom, float* d_x_out, float* d_z_out, void* d_zcopy,
+                                   int32_t zcopy_dtype, int32_t nx, int32_t ny, int32_t nz, double beta, double delta,
+                                   const double* d_tseq, const int32_t* d_base, int32_t k, void* stream);
+
+/* *d_counter += increment, as a kernel (graph-capturable). */
+int ctsim_add_i32(int32_t* d_counter, int32_t increment, void* stream);
+
 /* One plain SQS subset update (reconPwlsSeSqs.m:57-66): x = max(x - (g + beta S(x)) ./
  * (precom + beta T(x)), 0) with NaN/Inf -> 0; the denominator in FP64.  Writes
  * d_xnew (must not alias d_x). */

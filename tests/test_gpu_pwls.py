@@ -1,0 +1,100 @@
+"""GPU tests of the PWLS-OS building blocks and loop against the FP64 oracle
+(oracle/pwls_ref.py)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(seed=3, n=(24, 20, 12), nuv=(48, 16), nviews=24):
+    from oracle import oracle as O
+    rng = np.random.default_rng(seed)
+    betas = np.linspace(0, 2 * np.pi, nviews, endpoint=False) + 0.1
+    g = O.G3(n, (1.0, 1.0, 1.0), (0, 0, 0), nuv, (0.9, 1.1), (0, 0), 500.0, 0, 850.0, betas)
+    xt = np.zeros(n)
+    xt[5:18, 4:15, 2:10] = 0.2
+    xt[8:12, 7:10, 4:7] = 0.35
+    p = O.cbct("proj,tf", g, xt, prec=64)
+    counts = rng.poisson(2e4 * np.exp(-p)).astype(np.float64)
+    y = -np.log(np.maximum(counts, 10) / 2e4)
+    w = counts
+    x0 = np.clip(xt + rng.normal(0, 0.02, n), 0, None)
+    return g, y, w, x0
+
+
+def _geometry(g):
+    from helpers import geometry_from_g3
+    return geometry_from_g3(g)
+
+
+def test_huber_kernels_match_oracle():
+    from oracle import pwls_ref as R
+    from ctsim_amd._lib import call
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    g, y, w, x0 = _problem()
+    P = Projector(_geometry(g))
+    rng = np.random.default_rng(9)
+    x = (rng.uniform(0, 0.01, g.img_shape())).astype(np.float32)
+    dx = P.upload_volume(x)
+    nx, ny, nz = g.img_shape()
+    delta = 0.003
+    for mode in (1, 2):
+        out = DeviceArray(P.n_img, "f32")
+        call("ctsim_huber", dx.vp, nx, ny, nz, delta, mode, out.vp, None, None)
+        want = R.huber(x.astype(np.float64), mode, delta)
+        assert rel_l2(P.download_volume(out), want) < 1e-6, mode
+    cost = DeviceArray(1, "f64")
+    call("ctsim_huber", dx.vp, nx, ny, nz, delta, 0, None, cost.vp, None)
+    assert cost.to_numpy()[0] == pytest.approx(R.huber(x.astype(np.float64), 0, delta), rel=1e-6)
+
+
+def test_residual_kernel_and_fp64_cost():
+    from ctsim_amd._lib import call
+    from ctsim_amd.device import DeviceArray
+    rng = np.random.default_rng(4)
+    n = 300001
+    p, yy, ww = (rng.uniform(0, 2, n).astype(np.float32) for _ in range(3))
+    dp, dy, dw = (DeviceArray.from_numpy(a, "f32") for a in (p, yy, ww))
+    r = DeviceArray(n, "f32")
+    phi = DeviceArray(1, "f64").fill(0.0)
+    call("ctsim_pwls_residual", dp.vp, 0, dy.vp, 0, dw.vp, 0, r.vp, 0, n, phi.vp, None)
+    call("ctsim_pwls_residual", dp.vp, 0, dy.vp, 0, dw.vp, 0, r.vp, 0, n, phi.vp, None)  # accumulates
+    d = p.astype(np.float64) - yy
+    np.testing.assert_allclose(r.to_numpy(), (ww * (p - yy)).astype(np.float32), rtol=1e-6)
+    assert phi.to_numpy()[0] == pytest.approx(2 * np.sum(ww * d * d) / 2, rel=1e-9)
+
+
+def test_pwls_plain_matches_fp64_oracle():
+    from oracle import pwls_ref as R
+    from ctsim_amd.pwls import PwlsOS
+    g, y, w, x0 = _problem()
+    beta, delta, M, its = 50.0, 1e-3, 4, 3
+    want, phis, rmsds = R.nesterov_sqs(y, w, g, beta, delta, its, M, x0)
+    sol = PwlsOS(_geometry(g), y, w, M, beta, delta, mixed=False)
+    x, ph, rm = sol.run(x0.astype(np.float32), its)
+    print("plain vs fp64 oracle:", rel_l2(x, want), ph, phis)
+    assert rel_l2(x, want) < 1e-4
+    np.testing.assert_allclose(ph, phis, rtol=1e-4)
+    np.testing.assert_allclose(rm, rmsds, rtol=1e-3)
+
+
+def test_pwls_mixed_graph_equals_stream_and_is_close_to_plain():
+    from ctsim_amd.pwls import PwlsOS
+    g, y, w, x0 = _problem(seed=5)
+    beta, delta, M, its = 50.0, 1e-3, 4, 3
+    a = PwlsOS(_geometry(g), y, w, M, beta, delta, mixed=True, graph=True)
+    xa, pa, _ = a.run(x0.astype(np.float32), its)
+    b = PwlsOS(_geometry(g), y, w, M, beta, delta, mixed=True, graph=False)
+    xb, pb, _ = b.run(x0.astype(np.float32), its)
+    # the graph replays the same kernels with the same momentum scalars
+    assert np.array_equal(xa, xb)
+    np.testing.assert_array_equal(pa, pb)
+    c = PwlsOS(_geometry(g), y, w, M, beta, delta, mixed=False)
+    xc, _, _ = c.run(x0.astype(np.float32), its)
+    print("mixed vs plain:", rel_l2(xa, xc))
+    assert rel_l2(xa, xc) < 2e-3
