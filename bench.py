@@ -92,11 +92,11 @@ def kernel_times(solver, reps=1):
             with Timer(st) as tm:
                 call("ctsim_pwls_residual", pw.vp, solver.p.dtype, solver._sub_ptr(solver.y, off), solver.y.dtype,
                      solver._sub_ptr(solver.w, off), solver.w.dtype, solver.r.vp, solver.r.dtype, ns * solver.cell,
-                     C.c_void_p(solver.scal.ptr), solver._sv())
+                     solver.wscale, C.c_void_p(solver.scal.ptr), solver._sv())
             t["residual"] += tm.ms
             with Timer(st) as tm:
                 solver.P.back(solver._window(solver.r, 0, ns), solver.g, f"back,{solver.kind}", v0, vs, ns,
-                              scale=float(solver.M), stream=st)
+                              scale=float(solver.M) * solver.wscale, stream=st)
             t["back"] += tm.ms
             a, b = solver.cur, 1 - solver.cur
             with Timer(st) as tm:

@@ -287,6 +287,8 @@ int ctsim_plan_back(const ctsim_plan* p, const char* op, int32_t view0, int32_t 
     if (e == hipErrorInvalidValue)
         return fail(CTSIM_EUNSUP, "Error: dtype combination (sino %d, img %d) not supported for compute %d.", sino_dt,
                     img_dt, p->compute);
+    if (e == hipErrorNotSupported)
+        return fail(CTSIM_EUNSUP, "Error: 3D backprojection supports up to 512 slices (nz = %d).", p->g.nz);
     HIP_TRY(e, "backprojection launch");
     return CTSIM_OK;
 }
@@ -726,9 +728,9 @@ int ctsim_device_count(int32_t* n) {
 // PWLS building blocks
 // ---------------------------------------------------------------------------------
 int ctsim_pwls_residual(const void* p, int32_t pdt, const void* y, int32_t ydt, const void* w, int32_t wdt, void* r,
-                        int32_t rdt, size_t n, double* phi, void* stream) {
+                        int32_t rdt, size_t n, double wscale, double* phi, void* stream) {
     if (!p || !y || !w || !r || !phi) return fail(CTSIM_EINVAL, "Error: null argument.");
-    hipError_t e = residual_launch(p, pdt, y, ydt, w, wdt, r, rdt, n, phi, (hipStream_t)stream);
+    hipError_t e = residual_launch(p, pdt, y, ydt, w, wdt, r, rdt, n, wscale, phi, (hipStream_t)stream);
     if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype combination not supported.");
     HIP_TRY(e, "pwls residual");
     return CTSIM_OK;

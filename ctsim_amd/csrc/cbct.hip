@@ -21,47 +21,78 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 enum Foot { FOOT_DD = 0, FOOT_TF = 1 };
 
 // ---------------------------------------------------------------------------------
-// Forward projection: one wavefront per (view, tile of TU detector columns, 64 detector
-// rows).  Lanes own detector rows iv and TU accumulators.  The wave enumerates the
-// image columns whose footprint can reach its tile (a strip, found by inverse-mapping
-// the tile edges line by line), evaluates 64 candidate footprints at a time (one per
-// lane), and then walks the surviving columns one by one with their parameters
-// broadcast from the owning lane.  For each column every lane gathers the voxels whose
-// v footprint covers its row, with the exact clamps of the reference.
+// Forward projection.  Work item = (view, tile of TU detector columns, RPL*64 detector
+// rows); SPLIT waves share an item, each walking a contiguous share of the image lines,
+// and combine their partial sums through LDS in a fixed order (deterministic).  Lanes
+// own detector rows (RPL of them) and TU accumulators each.
+//
+// The wave enumerates the image columns whose footprint can reach its tile (the strip,
+// found by inverse-mapping the tile edges line by line), evaluates 64 candidate
+// footprints at a time (one per lane, including the tile's u weights), and walks the
+// surviving columns with their parameters broadcast from the owning lane.  For each
+// column the lanes first stage the needed voxels (one coalesced load each, with the
+// per-voxel cone factor ampv) in LDS; then every row gathers the voxels whose v
+// footprint covers it, with the exact clamps of the reference.
 // ---------------------------------------------------------------------------------
+constexpr int kZWin = 512;  // voxels staged per LDS window and wave
 
-template <class Tin, class Tout, class T, int FOOT, int TU>
+template <class T>
+struct ColParams {
+    int col, iz0, iz1, ilo, ihi;
+    T amp, r, zinc, vstart, w0, w1, w2, w3, p0, p1, p2, p3;
+};
+
+// weight of tile cell iu for a column whose u footprint is (ilo, ihi, p*): DD uses
+// (clo, chi) in p0, p1; SF the sorted corners p0..p3
+template <class T, int FOOT>
+__device__ __forceinline__ T u_weight(int iu, int ilo, int ihi, T p0, T p1, T p2, T p3) {
+    if (iu < ilo || iu > ihi) return T(0);
+    return FOOT == FOOT_DD ? dd_weight(iu, p0, p1, ilo, ihi) : sf_weight(iu, p0, p1, p2, p3);
+}
+
+template <class Tin, class Tout, class T, int FOOT, int TU, int RPL, int SPLIT>
 __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View<T>* __restrict__ views,
                                                            int vstride, int nviews, const Tin* __restrict__ vol,
                                                            const uint8_t* __restrict__ map,
                                                            Tout* __restrict__ proj, int exhaustive) {
+    static_assert(SPLIT == 1 || SPLIT == kWavesPerBlock, "SPLIT is 1 or the block's wave count");
+    __shared__ T zbuf[kWavesPerBlock][kZWin];
+    __shared__ T part[SPLIT == 1 ? 1 : SPLIT - 1][TU * RPL][kWave];
+    const int rows = RPL * kWave;
     const int nut = (g.nu + TU - 1) / TU;
-    const int nvc = (g.nv + kWave - 1) / kWave;
+    const int nvc = (g.nv + rows - 1) / rows;
+    const int items = nviews * nut * nvc;
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int wid = bid * kWavesPerBlock + (int)(threadIdx.x / kWave);
-    if (wid >= nviews * nut * nvc) return;  // whole wave exits together
-    const int vc = wid % nvc;
-    const int ut = (wid / nvc) % nut;
-    const int b = wid / (nvc * nut);
+    const int wv = (int)(threadIdx.x / kWave);
+    const int item = SPLIT == 1 ? bid * kWavesPerBlock + wv : bid;
+    const int share = SPLIT == 1 ? 0 : wv;
+    if (item >= items) return;  // SPLIT > 1: the whole block leaves together
+    const int vc = item % nvc;
+    const int ut = (item / nvc) % nut;
+    const int b = item / (nvc * nut);
     const int lane = lane_id();
-    const int iv = vc * kWave + lane;
+    const int iv_a = vc * rows;
     const int iu_a = ut * TU;
-    const int iu_b = min(iu_a + TU, g.nu) - 1;  // last cell of the tile
+    const int iu_b = min(iu_a + TU, g.nu) - 1;
     const View<T> v = views[(size_t)b * vstride];
     const bool row_major = fabs_t(v.c) >= fabs_t(v.s);
     const int nmajor = row_major ? g.ny : g.nx;
+    const int m_begin = (nmajor * share) / SPLIT, m_end = (nmajor * (share + 1)) / SPLIT;
     const T kL = u_to_slope(g, T(iu_a) - T(0.5));
     const T kR = u_to_slope(g, T(iu_b) + T(0.5));
-    const T v_lo_chunk = T(vc * kWave), v_hi_chunk = T(vc * kWave + kWave);
+    const int iv_last = min(iv_a + rows, g.nv) - 1;
+    T* zw = zbuf[wv];
 
-    T acc[TU];
+    T acc[RPL][TU];
 #pragma unroll
-    for (int t = 0; t < TU; ++t) acc[t] = T(0);
+    for (int q = 0; q < RPL; ++q)
+#pragma unroll
+        for (int t = 0; t < TU; ++t) acc[q][t] = T(0);
 
-    for (int m0 = 0; m0 < nmajor; m0 += kWave) {
+    for (int m0 = m_begin; m0 < m_end; m0 += kWave) {
         const int m = m0 + lane;
         int lo = 0, hi = -1;
-        if (m < nmajor) {
+        if (m < m_end) {
             if (!strip_range(g, v, row_major, m, kL, kR, exhaustive != 0, lo, hi)) hi = lo - 1;
         }
         const int cnt = hi - lo + 1 > 0 ? hi - lo + 1 : 0;
@@ -70,7 +101,6 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
         const int total = rdl_i(incl, kWave - 1);
         for (int k0 = 0; k0 < total; k0 += kWave) {
             const int k = k0 + lane;
-            // owning line: the largest j with excl_j <= k
             int j = 0;
 #pragma unroll
             for (int step = 32; step >= 1; step >>= 1) {
@@ -84,10 +114,11 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
             const int ix = row_major ? minor : major;
             const int iy = row_major ? major : minor;
 
-            // exact footprint of the candidate column
+            // exact footprint of the candidate column and its weights on this tile
             bool ok = k < total;
-            int ilo = 0, ihi = -1, iz0 = 0, iz1 = 0;
-            T p0 = T(0), p1 = T(0), p2 = T(0), p3 = T(0), amp = T(0), r = T(1), zinc = T(1), vstart = T(0);
+            ColParams<T> cp{};
+            cp.r = T(1);
+            cp.zinc = T(1);
             if (ok && map != nullptr) ok = map[iy * g.nx + ix] != 0;
             if (ok) {
                 const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
@@ -96,118 +127,175 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                 if (FOOT == FOOT_DD) {
                     T umin, umax;
                     dd_bounds(cg, v, g, umin, umax);
-                    ok = clamp_cells(umin, umax, g.nu, 16, clo, chi, ilo, ihi);
-                    p0 = clo;
-                    p1 = chi;
+                    ok = clamp_cells(umin, umax, g.nu, 16, clo, chi, cp.ilo, cp.ihi);
+                    cp.p0 = clo;
+                    cp.p1 = chi;
                 } else {
                     T tau[4];
                     sf_taus(cg, v, g, tau);
-                    ok = clamp_cells(tau[0], tau[3], g.nu, 16, clo, chi, ilo, ihi);
-                    p0 = tau[0]; p1 = tau[1]; p2 = tau[2]; p3 = tau[3];
+                    ok = clamp_cells(tau[0], tau[3], g.nu, 16, clo, chi, cp.ilo, cp.ihi);
+                    cp.p0 = tau[0]; cp.p1 = tau[1]; cp.p2 = tau[2]; cp.p3 = tau[3];
                 }
-                ok = ok && ihi >= iu_a && ilo <= iu_b;
+                ok = ok && cp.ihi >= iu_a && cp.ilo <= iu_b;
                 if (ok) {
-                    ok = v_window(g, cg.d, v.wz, zinc, vstart, iz0, iz1) && iz0 < iz1;
-                    amp = cg.amp;
-                    r = cg.r;
+                    ok = v_window(g, cg.d, v.wz, cp.zinc, cp.vstart, cp.iz0, cp.iz1) && cp.iz0 < cp.iz1 &&
+                         cp.zinc > T(0);
+                    cp.amp = cg.amp;
+                    cp.r = cg.r;
                 }
                 if (ok) {
-                    // does the column's v extent reach this wave's rows at all?
-                    const T e0 = vstart + T(iz0) * zinc + T(0.5);
-                    const T e1 = vstart + T(iz1) * zinc + T(0.5);
-                    ok = e1 > v_lo_chunk - T(1) && e0 < v_hi_chunk + T(1);
+                    // rows covered by the column's voxels vs this wave's rows
+                    const T e0 = cp.vstart + T(cp.iz0) * cp.zinc + T(0.5);
+                    const T e1 = cp.vstart + T(cp.iz1) * cp.zinc + T(0.5);
+                    ok = e1 > T(iv_a) - T(1) && e0 < T(iv_last + 2);
+                }
+                if (ok) {
+                    // tile weights of the first four footprint cells inside the tile
+                    const int c0 = max(cp.ilo, iu_a);
+                    cp.w0 = u_weight<T, FOOT>(c0 + 0, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
+                    cp.w1 = u_weight<T, FOOT>(c0 + 1, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
+                    cp.w2 = u_weight<T, FOOT>(c0 + 2, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
+                    cp.w3 = u_weight<T, FOOT>(c0 + 3, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
+                    cp.col = iy * g.nx + ix;
                 }
             }
             unsigned long long mask = __ballot(ok);
             while (mask) {
                 const int src = __ffsll((long long)mask) - 1;
                 mask &= mask - 1;
-                const int cix = rdl_i(ix, src), ciy = rdl_i(iy, src);
-                const int cilo = rdl_i(ilo, src), cihi = rdl_i(ihi, src);
-                const int ciz0 = rdl_i(iz0, src), ciz1 = rdl_i(iz1, src);
-                const T cp0 = rdl(p0, src), cp1 = rdl(p1, src);
-                T cp2 = T(0), cp3 = T(0);
-                if (FOOT == FOOT_TF) {
-                    cp2 = rdl(p2, src);
-                    cp3 = rdl(p3, src);
+                const int col = rdl_i(cp.col, src);
+                const int iz0 = rdl_i(cp.iz0, src), iz1 = rdl_i(cp.iz1, src);
+                const int ilo = rdl_i(cp.ilo, src), ihi = rdl_i(cp.ihi, src);
+                const T amp = rdl(cp.amp, src), r = rdl(cp.r, src);
+                const T zinc = rdl(cp.zinc, src), vstart = rdl(cp.vstart, src);
+                const T w0 = rdl(cp.w0, src), w1 = rdl(cp.w1, src), w2 = rdl(cp.w2, src), w3 = rdl(cp.w3, src);
+                const int c0 = max(ilo, iu_a) - iu_a;  // first tile cell of the footprint
+                const int c1 = min(ihi, iu_b) - iu_a;  // last
+                T p0 = T(0), p1 = T(0), p2 = T(0), p3 = T(0);
+                if (c1 - c0 >= 4) {  // footprint wider than four tile cells: weights computed below
+                    p0 = rdl(cp.p0, src); p1 = rdl(cp.p1, src);
+                    if (FOOT == FOOT_TF) { p2 = rdl(cp.p2, src); p3 = rdl(cp.p3, src); }
                 }
-                const T camp = rdl(amp, src), cr = rdl(r, src);
-                const T czinc = rdl(zinc, src), cvstart = rdl(vstart, src);
-                const Tin* col = vol + (size_t)(ciy * g.nx + cix) * g.nz;
-
-                // v gather for this lane's detector row
-                T q = T(0);
-                if (iv < g.nv) {
-                    const T inv = T(1) / czinc;
-                    const T f = (T(iv) - cvstart) * inv;
-                    int za = (int)floor(f - T(0.5) * inv) - 1;
-                    int zb = (int)floor(f + T(0.5) * inv) + 1;
-                    za = za > ciz0 ? za : ciz0;
-                    zb = zb < ciz1 - 1 ? zb : ciz1 - 1;
-                    const T inv_r = T(1) / cr;
-                    for (int iz = za; iz <= zb; ++iz) {
-                        const T vmin = cvstart + T(iz) * czinc;
-                        T clo, chi;
-                        int vlo, vhi;
-                        if (!clamp_cells(vmin, vmin + czinc, g.nv, 16, clo, chi, vlo, vhi)) continue;
-                        const T wv = dd_weight(iv, clo, chi, vlo, vhi);
-                        if (wv == T(0)) continue;
-                        const T zc = (T(iz) + v.wz) * g.dz;
-                        const T ampv = sqrt(cr * cr + zc * zc) * inv_r;
-                        q += wv * ampv * ld<T>(col + iz);
-                    }
-                }
-                const T aq = camp * q;
+                const Tin* colp = vol + (size_t)col * g.nz;
+                const T inv_r = T(1) / r;
+                const T rz = T(1) / zinc;
+                // voxel range needed by this wave's rows (exact ranges; a voxel left out
+                // at the rounding edge has a weight of the order of the rounding error)
+                int za = (int)ceil((T(iv_a) - T(0.5) - vstart) * rz - T(1));
+                int zb = (int)ceil((T(iv_last) + T(0.5) - vstart) * rz) - 1;
+                za = max(za, iz0);
+                zb = min(zb, iz1 - 1);
+                T q[RPL];
 #pragma unroll
-                for (int t = 0; t < TU; ++t) {
-                    const int iu = iu_a + t;
-                    if (iu >= cilo && iu <= cihi) {
-                        T wu;
-                        if (FOOT == FOOT_DD)
-                            wu = dd_weight(iu, cp0, cp1, cilo, cihi);
-                        else
-                            wu = sf_weight(iu, cp0, cp1, cp2, cp3);
-                        acc[t] += wu * aq;
+                for (int s = 0; s < RPL; ++s) q[s] = T(0);
+                for (int zw0 = za; zw0 <= zb; zw0 += kZWin) {
+                    const int zw1 = min(zb, zw0 + kZWin - 1);
+                    // stage voxel values ampv * img of [zw0, zw1] in LDS
+                    for (int zz = zw0 + lane; zz <= zw1; zz += kWave) {
+                        const T zc = (T(zz) + v.wz) * g.dz;
+                        zw[zz - zw0] = sqrt(r * r + zc * zc) * inv_r * ld<T>(colp + zz);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                    for (int s = 0; s < RPL; ++s) {
+                        const int iv = iv_a + s * kWave + lane;
+                        if (iv > iv_last) continue;
+                        const T fiv = T(iv);
+                        int ja = (int)ceil((fiv - T(0.5) - vstart) * rz - T(1));
+                        int jb = (int)ceil((fiv + T(0.5) - vstart) * rz) - 1;
+                        ja = max(ja, zw0);
+                        jb = min(jb, zw1);
+                        for (int jj = ja; jj <= jb; ++jj) {
+                            const T vmin = vstart + T(jj) * zinc;
+                            T clo, chi;
+                            int vlo, vhi;
+                            if (!clamp_cells(vmin, vmin + zinc, g.nv, 16, clo, chi, vlo, vhi)) continue;
+                            q[s] += dd_weight(iv, clo, chi, vlo, vhi) * zw[jj - zw0];
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) {
+                    const T aq = amp * q[s];
+#pragma unroll
+                    for (int t = 0; t < TU; ++t) {
+                        T wt;
+                        if (c1 - c0 < 4) {
+                            const int d = t - c0;
+                            wt = d == 0 ? w0 : d == 1 ? w1 : d == 2 ? w2 : d == 3 ? w3 : T(0);
+                            wt = (t >= c0 && t <= c1) ? wt : T(0);
+                        } else {
+                            wt = (t >= c0 && t <= c1) ? u_weight<T, FOOT>(iu_a + t, ilo, ihi, p0, p1, p2, p3) : T(0);
+                        }
+                        acc[s][t] += wt * aq;
                     }
                 }
             }
         }
     }
-    if (iv < g.nv) {
-        Tout* out = proj + (size_t)b * g.nu * g.nv;
+    if (SPLIT > 1) {
+        // fixed-order combine of the shares: ((s0 + s1) + s2) + s3
+        if (wv > 0) {
+#pragma unroll
+            for (int s = 0; s < RPL; ++s)
+#pragma unroll
+                for (int t = 0; t < TU; ++t) part[wv - 1][s * TU + t][lane] = acc[s][t];
+        }
+        __syncthreads();
+        if (wv > 0) return;
+#pragma unroll
+        for (int s = 0; s < RPL; ++s)
+#pragma unroll
+            for (int t = 0; t < TU; ++t)
+#pragma unroll
+                for (int w = 0; w < SPLIT - 1; ++w) acc[s][t] += part[w][s * TU + t][lane];
+    }
+    Tout* out = proj + (size_t)b * g.nu * g.nv;
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+        const int iv = iv_a + s * kWave + lane;
+        if (iv > iv_last) continue;
 #pragma unroll
         for (int t = 0; t < TU; ++t) {
             const int iu = iu_a + t;
-            if (iu < g.nu) out[(size_t)iu * g.nv + iv] = cvt_store<Tout>(acc[t]);
+            if (iu < g.nu) out[(size_t)iu * g.nv + iv] = cvt_store<Tout>(acc[s][t]);
         }
     }
 }
 
 // ---------------------------------------------------------------------------------
-// Matched backprojection: one wavefront per (image column, 64 consecutive z).  The
-// column's u footprint is wave-uniform; each lane gathers its voxel's v footprint
-// from the sinogram rows of the footprint cells (v fastest, so loads coalesce) and
-// accumulates all views in registers; one store per voxel at the end.
+// Matched backprojection: one wavefront per image column, NZL voxels per lane (z
+// fastest).  Per view the column's u footprint is wave-uniform; the wave first combines
+// the footprint's detector columns row by row, s(iv) = sum_u wu * proj(u, iv), into
+// LDS for the rows the column covers, then every voxel gathers its v footprint from
+// LDS.  All views accumulate in registers; one store per voxel.
 // ---------------------------------------------------------------------------------
-template <class Tin, class Tout, class T, int FOOT>
-__global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View<T>* __restrict__ views, int vstride, int nviews,
-                                                           const Tin* __restrict__ proj,
+constexpr int kRowWin = 1024;  // detector rows staged per wave (nv larger: windowed)
+
+template <class Tin, class Tout, class T, int FOOT, int NZL>
+__global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View<T>* __restrict__ views, int vstride,
+                                                           int nviews, const Tin* __restrict__ proj,
                                                            const uint8_t* __restrict__ map,
                                                            Tout* __restrict__ vol, T scale, int accumulate) {
-    const int nzc = (g.nz + kWave - 1) / kWave;
+    __shared__ T srow[kWavesPerBlock][kRowWin];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int wid = bid * kWavesPerBlock + (int)(threadIdx.x / kWave);
-    if (wid >= g.nx * g.ny * nzc) return;
-    const int zc = wid % nzc;
-    const int colid = wid / nzc;
+    const int wv = (int)(threadIdx.x / kWave);
+    const int colid = bid * kWavesPerBlock + wv;
+    if (colid >= g.nx * g.ny) return;
     const int ix = colid % g.nx, iy = colid / g.nx;
-    const int iz = zc * kWave + lane_id();
+    const int lane = lane_id();
+    Tout* ocol = vol + (size_t)colid * g.nz;
     if (map != nullptr && map[iy * g.nx + ix] == 0) {
-        if (iz < g.nz && !accumulate) vol[(size_t)colid * g.nz + iz] = cvt_store<Tout>(T(0));
+        if (!accumulate)
+            for (int iz = lane; iz < g.nz; iz += kWave) ocol[iz] = cvt_store<Tout>(T(0));
         return;
     }
+    T* sw = srow[wv];
     const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
-    T img = T(0);
+    T img[NZL];
+#pragma unroll
+    for (int c = 0; c < NZL; ++c) img[c] = T(0);
     for (int b = 0; b < nviews; ++b) {
         const View<T> v = views[(size_t)b * vstride];
         const Column<T> cg = column_geom(xc, yc, v, g.sad, g.dx);
@@ -229,55 +317,90 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
         if (!ok) continue;
         T zinc, vstart;
         int iz0, iz1;
-        if (!v_window(g, cg.d, v.wz, zinc, vstart, iz0, iz1)) continue;
-        if (iz < iz0 || iz >= iz1) continue;
-        const T vmin = vstart + T(iz) * zinc;
-        int vlo, vhi;
-        if (!clamp_cells(vmin, vmin + zinc, g.nv, 16, clo, chi, vlo, vhi)) continue;
-        const T zcm = (T(iz) + v.wz) * g.dz;
-        const T ampv = sqrt(cg.r * cg.r + zcm * zcm) / cg.r;
+        if (!v_window(g, cg.d, v.wz, zinc, vstart, iz0, iz1) || iz0 >= iz1 || !(zinc > T(0))) continue;
+        // rows touched by voxels iz0 .. iz1-1
+        T a0 = vstart + T(iz0) * zinc + T(0.5), a1 = vstart + T(iz1 - 1) * zinc + zinc + T(0.5);
+        a0 = a0 > T(0) ? a0 : T(0);
+        a1 = a1 < T(g.nv) ? a1 : T(g.nv);
+        if (a0 > a1) continue;
+        const int r0 = min((int)floor(a0), g.nv - 1);
+        const int r1 = min((int)floor(a1), g.nv - 1);
+        const T wu0 = u_weight<T, FOOT>(ilo + 0, ilo, ihi, p0, p1, p2, p3);
+        const T wu1 = u_weight<T, FOOT>(ilo + 1, ilo, ihi, p0, p1, p2, p3);
+        const T wu2 = u_weight<T, FOOT>(ilo + 2, ilo, ihi, p0, p1, p2, p3);
+        const T wu3 = u_weight<T, FOOT>(ilo + 3, ilo, ihi, p0, p1, p2, p3);
+        const int nw = ihi - ilo + 1;
         const Tin* pv = proj + (size_t)b * g.nu * g.nv;
-        T s = T(0);
-        for (int iu = ilo; iu <= ihi; ++iu) {
-            const T wu = FOOT == FOOT_DD ? dd_weight(iu, p0, p1, ilo, ihi) : sf_weight(iu, p0, p1, p2, p3);
-            const Tin* row = pv + (size_t)iu * g.nv;
-            for (int ivv = vlo; ivv <= vhi; ++ivv) s += ld<T>(row + ivv) * wu * dd_weight(ivv, clo, chi, vlo, vhi);
+        const T inv_r = T(1) / cg.r;
+        for (int rw = r0; rw <= r1; rw += kRowWin) {
+            const int rwe = min(r1, rw + kRowWin - 1);
+            // u combination of the footprint's detector columns, row by row
+            for (int row = rw + lane; row <= rwe; row += kWave) {
+                const Tin* pr = pv + (size_t)ilo * g.nv + row;
+                T s = wu0 * ld<T>(pr);
+                if (nw > 1) s += wu1 * ld<T>(pr + g.nv);
+                if (nw > 2) s += wu2 * ld<T>(pr + 2 * g.nv);
+                if (nw > 3) s += wu3 * ld<T>(pr + 3 * g.nv);
+                for (int k = 4; k < nw; ++k)
+                    s += u_weight<T, FOOT>(ilo + k, ilo, ihi, p0, p1, p2, p3) * ld<T>(pr + (size_t)k * g.nv);
+                sw[row - rw] = s;
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int c = 0; c < NZL; ++c) {
+                const int iz = c * kWave + lane;
+                if (iz < iz0 || iz >= iz1) continue;
+                const T vmin = vstart + T(iz) * zinc;
+                int vlo, vhi;
+                if (!clamp_cells(vmin, vmin + zinc, g.nv, 16, clo, chi, vlo, vhi)) continue;
+                const int ra = max(vlo, rw), rb = min(vhi, rwe);
+                if (ra > rb) continue;
+                T s = T(0);
+                for (int iv = ra; iv <= rb; ++iv) s += dd_weight(iv, clo, chi, vlo, vhi) * sw[iv - rw];
+                const T zcm = (T(iz) + v.wz) * g.dz;
+                img[c] += cg.amp * (sqrt(cg.r * cg.r + zcm * zcm) * inv_r) * s;
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        img += cg.amp * ampv * s;
     }
-    if (iz < g.nz) {
-        Tout* o = vol + (size_t)colid * g.nz + iz;
-        const T val = scale * img;
-        *o = cvt_store<Tout>(accumulate ? ld<T>(o) + val : val);
+#pragma unroll
+    for (int c = 0; c < NZL; ++c) {
+        const int iz = c * kWave + lane;
+        if (iz < g.nz) {
+            const T val = scale * img[c];
+            ocol[iz] = cvt_store<Tout>(accumulate ? ld<T>(ocol + iz) + val : val);
+        }
     }
 }
 
 // ---------------------------------------------------------------------------------
 // FDK voxel-driven bilinear backprojection (cbct_back1_pd).  off_boundary = 1 is
 // 'back,pd' (skip below the detector, stop above it), 0 is 'back,hpd' (clamp v).
+// One wavefront per image column, NZL voxels per lane.
 // ---------------------------------------------------------------------------------
-template <class Tin, class Tout, class T>
+template <class Tin, class Tout, class T, int NZL>
 __global__ __launch_bounds__(kBlock) void cbct_back_pd_kernel(Geom<T> g, const View<T>* __restrict__ views,
                                                               int vstride, int nviews, const Tin* __restrict__ proj,
                                                               const uint8_t* __restrict__ map,
                                                               Tout* __restrict__ vol, T scale, int accumulate,
                                                               int off_boundary) {
-    const int nzc = (g.nz + kWave - 1) / kWave;
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int wid = bid * kWavesPerBlock + (int)(threadIdx.x / kWave);
-    if (wid >= g.nx * g.ny * nzc) return;
-    const int zc = wid % nzc;
-    const int colid = wid / nzc;
+    const int colid = bid * kWavesPerBlock + (int)(threadIdx.x / kWave);
+    if (colid >= g.nx * g.ny) return;
     const int ix = colid % g.nx, iy = colid / g.nx;
-    const int iz = zc * kWave + lane_id();
+    const int lane = lane_id();
+    Tout* ocol = vol + (size_t)colid * g.nz;
     if (map != nullptr && map[iy * g.nx + ix] == 0) {
-        if (iz < g.nz && !accumulate) vol[(size_t)colid * g.nz + iz] = cvt_store<Tout>(T(0));
+        if (!accumulate)
+            for (int iz = lane; iz < g.nz; iz += kWave) ocol[iz] = cvt_store<Tout>(T(0));
         return;
     }
     const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
     const double lim_u = (double)(g.nu - 1) - 1e-6;
     const double lim_v = (double)(g.nv - 1) - 1e-6;
-    T img = T(0);
+    T img[NZL];
+#pragma unroll
+    for (int c = 0; c < NZL; ++c) img[c] = T(0);
     for (int b = 0; b < nviews; ++b) {
         const View<T> v = views[(size_t)b * vstride];
         const T xr = xc * v.c - yc * v.s;
@@ -296,33 +419,43 @@ __global__ __launch_bounds__(kBlock) void cbct_back_pd_kernel(Geom<T> g, const V
         if (!((double)u >= 1e-6 && (double)u <= lim_u)) continue;  // also skips NaN
         const int iu = (int)floor(u);
         const T wul = u - T(iu);
-        if (iz >= g.nz) continue;
-        T vv = (T(iz) + v.wz) * g.dz * w2 / g.dv - g.wv;
-        if (off_boundary) {
-            // the reference walks iz upwards and stops at the first voxel above the
-            // detector; v is monotone in iz, so that is a per-voxel test when v grows,
-            // and "nothing if voxel 0 is already above" when v shrinks
-            if (!((double)vv >= 1e-6 && (double)vv <= lim_v)) continue;
-            if (!(g.dz * w2 > T(0))) {
-                const T v0 = v.wz * g.dz * w2 / g.dv - g.wv;
-                if ((double)v0 > lim_v) continue;
-            }
-        } else {
-            if ((double)vv < 1e-6) vv = T(0.5);
-            if ((double)vv > lim_v) vv = T(g.nv) - T(1.5);
-            if (vv != vv) continue;  // NaN
+        // 'back,pd' stops at the first voxel above the detector; v is monotone in iz, so
+        // that is a per-voxel test when v grows and "nothing if voxel 0 is already above"
+        // when it shrinks
+        bool none = false;
+        if (off_boundary && !(g.dz * w2 > T(0))) {
+            const T v0 = v.wz * g.dz * w2 / g.dv - g.wv;
+            none = (double)v0 > lim_v;
         }
-        int iv = (int)floor(vv);
-        iv = iv < 0 ? 0 : (iv > g.nv - 2 ? g.nv - 2 : iv);  // no-op for finite v; keeps NaN-free reads in range
-        const T wvl = vv - T(iv);
-        const Tin* p = proj + ((size_t)b * g.nu + iu) * g.nv + iv;
-        const T a0 = ld<T>(p), a1 = ld<T>(p + 1), a2 = ld<T>(p + g.nv), a3 = ld<T>(p + g.nv + 1);
-        img += w2s * ((T(1) - wul) * ((T(1) - wvl) * a0 + wvl * a1) + wul * ((T(1) - wvl) * a2 + wvl * a3));
+        if (none) continue;
+        const Tin* p = proj + ((size_t)b * g.nu + iu) * g.nv;
+#pragma unroll
+        for (int c = 0; c < NZL; ++c) {
+            const int iz = c * kWave + lane;
+            if (iz >= g.nz) continue;
+            T vv = (T(iz) + v.wz) * g.dz * w2 / g.dv - g.wv;
+            if (off_boundary) {
+                if (!((double)vv >= 1e-6 && (double)vv <= lim_v)) continue;
+            } else {
+                if ((double)vv < 1e-6) vv = T(0.5);
+                if ((double)vv > lim_v) vv = T(g.nv) - T(1.5);
+                if (vv != vv) continue;  // NaN
+            }
+            int iv = (int)floor(vv);
+            iv = iv < 0 ? 0 : (iv > g.nv - 2 ? g.nv - 2 : iv);  // no-op for finite v; keeps reads in range
+            const T wvl = vv - T(iv);
+            const T a0 = ld<T>(p + iv), a1 = ld<T>(p + iv + 1), a2 = ld<T>(p + g.nv + iv),
+                    a3 = ld<T>(p + g.nv + iv + 1);
+            img[c] += w2s * ((T(1) - wul) * ((T(1) - wvl) * a0 + wvl * a1) + wul * ((T(1) - wvl) * a2 + wvl * a3));
+        }
     }
-    if (iz < g.nz) {
-        Tout* o = vol + (size_t)colid * g.nz + iz;
-        const T val = scale * img;
-        *o = cvt_store<Tout>(accumulate ? ld<T>(o) + val : val);
+#pragma unroll
+    for (int c = 0; c < NZL; ++c) {
+        const int iz = c * kWave + lane;
+        if (iz < g.nz) {
+            const T val = scale * img[c];
+            ocol[iz] = cvt_store<Tout>(accumulate ? ld<T>(ocol + iz) + val : val);
+        }
     }
 }
 
@@ -406,40 +539,72 @@ namespace ctsim {
 
 constexpr int kTU = 16;  // detector columns per forward-projection wave
 
+template <class Tin, class Tout, class T, int FOOT, int RPL, int SPLIT>
+static void launch_proj(unsigned blocks, hipStream_t st, const Geom<T>& g, const View<T>* views, int vstride,
+                        int nviews, const void* vol, const uint8_t* map, void* proj, int exhaustive) {
+    hipLaunchKernelGGL((cbct_proj_kernel<Tin, Tout, T, FOOT, kTU, RPL, SPLIT>), dim3(blocks), dim3(kBlock), 0, st, g,
+                       views, vstride, nviews, (const Tin*)vol, map, (Tout*)proj, exhaustive);
+}
+
+template <class Tin, class Tout, class T, int FOOT>
+static void proj3_foot(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* vol,
+                       const uint8_t* map, void* proj, int exhaustive, hipStream_t st) {
+    const int rpl = g.nv > kWave ? 2 : 1;
+    const long long nut = (g.nu + kTU - 1) / kTU, nvc = (g.nv + rpl * kWave - 1) / (rpl * kWave);
+    const long long items = (long long)nviews * nut * nvc;
+    // few work items (short view subsets, thin detectors): split each item's strip over
+    // the four waves of a block so that the chip still holds several waves per SIMD
+    const bool split = items < 16384;
+    const unsigned blocks = (unsigned)(split ? items : (items + kWavesPerBlock - 1) / kWavesPerBlock);
+    if (rpl == 2) {
+        if (split) launch_proj<Tin, Tout, T, FOOT, 2, kWavesPerBlock>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
+        else launch_proj<Tin, Tout, T, FOOT, 2, 1>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
+    } else {
+        if (split) launch_proj<Tin, Tout, T, FOOT, 1, kWavesPerBlock>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
+        else launch_proj<Tin, Tout, T, FOOT, 1, 1>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
+    }
+}
+
 template <class Tin, class Tout, class T>
 static hipError_t proj3(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* vol,
                         const uint8_t* map, void* proj, int foot, int exhaustive, hipStream_t st) {
-    const long long nut = (g.nu + kTU - 1) / kTU, nvc = (g.nv + kWave - 1) / kWave;
-    const long long waves = (long long)nviews * nut * nvc;
-    const unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
-    if (blocks == 0) return hipSuccess;
+    if (nviews <= 0) return hipSuccess;
     if (foot == FOOT_DD)
-        hipLaunchKernelGGL((cbct_proj_kernel<Tin, Tout, T, FOOT_DD, kTU>), dim3(blocks), dim3(kBlock), 0, st, g, views,
-                           vstride, nviews, (const Tin*)vol, map, (Tout*)proj, exhaustive);
+        proj3_foot<Tin, Tout, T, FOOT_DD>(g, views, vstride, nviews, vol, map, proj, exhaustive, st);
     else
-        hipLaunchKernelGGL((cbct_proj_kernel<Tin, Tout, T, FOOT_TF, kTU>), dim3(blocks), dim3(kBlock), 0, st, g, views,
-                           vstride, nviews, (const Tin*)vol, map, (Tout*)proj, exhaustive);
+        proj3_foot<Tin, Tout, T, FOOT_TF>(g, views, vstride, nviews, vol, map, proj, exhaustive, st);
     return hipGetLastError();
+}
+
+template <class Tin, class Tout, class T, int NZL>
+static void launch_back(unsigned blocks, hipStream_t st, const Geom<T>& g, const View<T>* views, int vstride,
+                        int nviews, const void* proj, const uint8_t* map, void* vol, int foot, int pd_mode, T scale,
+                        int accumulate) {
+    if (foot == 2)
+        hipLaunchKernelGGL((cbct_back_pd_kernel<Tin, Tout, T, NZL>), dim3(blocks), dim3(kBlock), 0, st, g, views,
+                           vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate, pd_mode);
+    else if (foot == FOOT_DD)
+        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_DD, NZL>), dim3(blocks), dim3(kBlock), 0, st, g,
+                           views, vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate);
+    else
+        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_TF, NZL>), dim3(blocks), dim3(kBlock), 0, st, g,
+                           views, vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate);
 }
 
 template <class Tin, class Tout, class T>
 static hipError_t back3(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* proj,
                         const uint8_t* map, void* vol, int foot, int pd_mode, T scale, int accumulate,
                         hipStream_t st) {
-    const long long nzc = (g.nz + kWave - 1) / kWave;
-    const long long waves = (long long)g.nx * g.ny * nzc;
-    const unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    const long long cols = (long long)g.nx * g.ny;
+    const unsigned blocks = (unsigned)((cols + kWavesPerBlock - 1) / kWavesPerBlock);
     if (blocks == 0) return hipSuccess;
     if (foot == 2 && (g.nu < 2 || g.nv < 2)) return hipErrorInvalidValue;  // bilinear needs 2x2 cells
-    if (foot == 2)
-        hipLaunchKernelGGL((cbct_back_pd_kernel<Tin, Tout, T>), dim3(blocks), dim3(kBlock), 0, st, g, views, vstride,
-                           nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate, pd_mode);
-    else if (foot == FOOT_DD)
-        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_DD>), dim3(blocks), dim3(kBlock), 0, st, g, views,
-                           vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate);
-    else
-        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_TF>), dim3(blocks), dim3(kBlock), 0, st, g, views,
-                           vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate);
+    if (g.nz > 8 * kWave) return hipErrorNotSupported;                      // voxel columns up to 512
+    const int nzl = (g.nz + kWave - 1) / kWave;
+    if (nzl <= 1) launch_back<Tin, Tout, T, 1>(blocks, st, g, views, vstride, nviews, proj, map, vol, foot, pd_mode, scale, accumulate);
+    else if (nzl <= 2) launch_back<Tin, Tout, T, 2>(blocks, st, g, views, vstride, nviews, proj, map, vol, foot, pd_mode, scale, accumulate);
+    else if (nzl <= 4) launch_back<Tin, Tout, T, 4>(blocks, st, g, views, vstride, nviews, proj, map, vol, foot, pd_mode, scale, accumulate);
+    else launch_back<Tin, Tout, T, 8>(blocks, st, g, views, vstride, nviews, proj, map, vol, foot, pd_mode, scale, accumulate);
     return hipGetLastError();
 }
 

@@ -67,7 +67,7 @@ namespace ctsim {
 // PWLS / reduction launchers (pwls.hip)
 hipError_t reduction_scratch_init();
 hipError_t residual_launch(const void* p, int pdt, const void* y, int ydt, const void* w, int wdt, void* r, int rdt,
-                           size_t n, double* phi, hipStream_t st);
+                           size_t n, double wscale, double* phi, hipStream_t st);
 hipError_t huber_launch(const float* x, int nx, int ny, int nz, double delta, int mode, float* out, double* cost,
                         hipStream_t st);
 hipError_t nesterov_launch(const float* x, const float* z, float* s, const float* g, const float* precom, float* x_out,

@@ -136,7 +136,7 @@ __device__ __forceinline__ void finish_reduction(double block_value, RedScratch 
 template <class P, class Y, class W, class R>
 __global__ __launch_bounds__(kRB) void residual_kernel(const P* __restrict__ p, const Y* __restrict__ y,
                                                        const W* __restrict__ w, R* __restrict__ r, size_t n,
-                                                       RedScratch rs, double* __restrict__ phi) {
+                                                       double wscale, RedScratch rs, double* __restrict__ phi) {
     double acc = 0.0;
     for (size_t i = (size_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (size_t)gridDim.x * kRB) {
         const float d = ld_f(p + i) - ld_f(y + i);
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kRB) void residual_kernel(const P* __restrict__ p, 
         acc += (double)wi * (double)d * (double)d;
     }
     const double bs = block_sum(acc);
-    finish_reduction(0.5 * bs, rs, phi, 1);
+    finish_reduction(0.5 * wscale * bs, rs, phi, 1);
 }
 
 // ---------------------------------------------------------------------------------
@@ -345,13 +345,13 @@ static unsigned red_grid(size_t n) {
 }
 
 template <class P, class Y, class W>
-static hipError_t residual_r(const void* p, const void* y, const void* w, void* r, int rdt, size_t n, RedScratch rs,
-                             double* phi, hipStream_t st) {
+static hipError_t residual_r(const void* p, const void* y, const void* w, void* r, int rdt, size_t n, double ws,
+                             RedScratch rs, double* phi, hipStream_t st) {
     const unsigned gb = red_grid(n);
     switch (rdt) {
-        case DT_F32: hipLaunchKernelGGL((residual_kernel<P, Y, W, float>), dim3(gb), dim3(kRB), 0, st, (const P*)p, (const Y*)y, (const W*)w, (float*)r, n, rs, phi); break;
-        case DT_F16: hipLaunchKernelGGL((residual_kernel<P, Y, W, __half>), dim3(gb), dim3(kRB), 0, st, (const P*)p, (const Y*)y, (const W*)w, (__half*)r, n, rs, phi); break;
-        case DT_BF16: hipLaunchKernelGGL((residual_kernel<P, Y, W, __hip_bfloat16>), dim3(gb), dim3(kRB), 0, st, (const P*)p, (const Y*)y, (const W*)w, (__hip_bfloat16*)r, n, rs, phi); break;
+        case DT_F32: hipLaunchKernelGGL((residual_kernel<P, Y, W, float>), dim3(gb), dim3(kRB), 0, st, (const P*)p, (const Y*)y, (const W*)w, (float*)r, n, ws, rs, phi); break;
+        case DT_F16: hipLaunchKernelGGL((residual_kernel<P, Y, W, __half>), dim3(gb), dim3(kRB), 0, st, (const P*)p, (const Y*)y, (const W*)w, (__half*)r, n, ws, rs, phi); break;
+        case DT_BF16: hipLaunchKernelGGL((residual_kernel<P, Y, W, __hip_bfloat16>), dim3(gb), dim3(kRB), 0, st, (const P*)p, (const Y*)y, (const W*)w, (__hip_bfloat16*)r, n, ws, rs, phi); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -359,35 +359,35 @@ static hipError_t residual_r(const void* p, const void* y, const void* w, void* 
 
 template <class P, class Y>
 static hipError_t residual_w(const void* p, const void* y, const void* w, int wdt, void* r, int rdt, size_t n,
-                             RedScratch rs, double* phi, hipStream_t st) {
+                             double ws, RedScratch rs, double* phi, hipStream_t st) {
     switch (wdt) {
-        case DT_F32: return residual_r<P, Y, float>(p, y, w, r, rdt, n, rs, phi, st);
-        case DT_F16: return residual_r<P, Y, __half>(p, y, w, r, rdt, n, rs, phi, st);
-        case DT_BF16: return residual_r<P, Y, __hip_bfloat16>(p, y, w, r, rdt, n, rs, phi, st);
+        case DT_F32: return residual_r<P, Y, float>(p, y, w, r, rdt, n, ws, rs, phi, st);
+        case DT_F16: return residual_r<P, Y, __half>(p, y, w, r, rdt, n, ws, rs, phi, st);
+        case DT_BF16: return residual_r<P, Y, __hip_bfloat16>(p, y, w, r, rdt, n, ws, rs, phi, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <class P>
 static hipError_t residual_y(const void* p, const void* y, int ydt, const void* w, int wdt, void* r, int rdt, size_t n,
-                             RedScratch rs, double* phi, hipStream_t st) {
+                             double ws, RedScratch rs, double* phi, hipStream_t st) {
     switch (ydt) {
-        case DT_F32: return residual_w<P, float>(p, y, w, wdt, r, rdt, n, rs, phi, st);
-        case DT_F16: return residual_w<P, __half>(p, y, w, wdt, r, rdt, n, rs, phi, st);
-        case DT_BF16: return residual_w<P, __hip_bfloat16>(p, y, w, wdt, r, rdt, n, rs, phi, st);
+        case DT_F32: return residual_w<P, float>(p, y, w, wdt, r, rdt, n, ws, rs, phi, st);
+        case DT_F16: return residual_w<P, __half>(p, y, w, wdt, r, rdt, n, ws, rs, phi, st);
+        case DT_BF16: return residual_w<P, __hip_bfloat16>(p, y, w, wdt, r, rdt, n, ws, rs, phi, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t residual_launch(const void* p, int pdt, const void* y, int ydt, const void* w, int wdt, void* r, int rdt,
-                           size_t n, double* phi, hipStream_t st) {
+                           size_t n, double ws, double* phi, hipStream_t st) {
     RedScratch rs;
     hipError_t e = scratch(rs);
     if (e != hipSuccess) return e;
     switch (pdt) {
-        case DT_F32: return residual_y<float>(p, y, ydt, w, wdt, r, rdt, n, rs, phi, st);
-        case DT_F16: return residual_y<__half>(p, y, ydt, w, wdt, r, rdt, n, rs, phi, st);
-        case DT_BF16: return residual_y<__hip_bfloat16>(p, y, ydt, w, wdt, r, rdt, n, rs, phi, st);
+        case DT_F32: return residual_y<float>(p, y, ydt, w, wdt, r, rdt, n, ws, rs, phi, st);
+        case DT_F16: return residual_y<__half>(p, y, ydt, w, wdt, r, rdt, n, ws, rs, phi, st);
+        case DT_BF16: return residual_y<__hip_bfloat16>(p, y, ydt, w, wdt, r, rdt, n, ws, rs, phi, st);
         default: return hipErrorInvalidValue;
     }
 }

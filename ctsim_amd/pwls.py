@@ -76,12 +76,16 @@ class PwlsOS:
         self.order = np.concatenate(order)
         y = np.asarray(y, np.float32).reshape((nv, nu, nviews), order="F")[:, :, self.order]
         w = np.asarray(w, np.float32).reshape((nv, nu, nviews), order="F")[:, :, self.order]
+        # half-precision storage keeps w / max(w) (counts reach 2e4; w .* d would overflow
+        # FP16); the scale returns in the backprojection factor and in the FP64 cost
+        self.wscale = float(w.max()) if (self.mixed and w.size and w.max() > 0) else 1.0
+        w = w / np.float32(self.wscale) if self.wscale != 1.0 else w
         st = self.stream
         self.y = DeviceArray.from_numpy(y.ravel(order="F"), self.sdt, stream=st)
         self.w = DeviceArray.from_numpy(w.ravel(order="F"), self.sdt, stream=st)
         nmax = max(s[2] for s in self.subsets)
         nvox = nx * ny * nz
-        self.p = DeviceArray(nmax * self.cell, self.sdt if self.mixed else F32)  # Aos(z)
+        self.p = DeviceArray(nmax * self.cell, F32)  # Aos(z), FP32 so that d = p - y keeps its bits
         self.r = DeviceArray(nmax * self.cell, self.sdt)  # Os(w) .* d
         self.x = [DeviceArray(nvox, F32), DeviceArray(nvox, F32)]
         self.z = [DeviceArray(nvox, F32), DeviceArray(nvox, F32)]
@@ -136,7 +140,7 @@ class PwlsOS:
             call("ctsim_mul", a.vp, F32, self._sub_ptr(self.w, off), self.w.dtype, wa.vp, F32, ns * self.cell,
                  self._sv())
             self.P.back(self._View(wa.ptr, ns * self.cell, F32), self.precom, f"back,{self.kind}", v0, vs, ns,
-                        scale=1.0, accumulate=k > 0, stream=st)
+                        scale=self.wscale, accumulate=k > 0, stream=st)
         st.synchronize()
         self.cur = 0
         self.t = 1.0
@@ -151,10 +155,10 @@ class PwlsOS:
         pw = self._window(self.p, 0, ns)
         self.P.proj(src, pw, f"proj,{self.kind}", v0, vs, ns, stream=self.stream)
         call("ctsim_pwls_residual", pw.vp, self.p.dtype, self._sub_ptr(self.y, off), self.y.dtype,
-             self._sub_ptr(self.w, off), self.w.dtype, self.r.vp, self.r.dtype, ns * self.cell,
+             self._sub_ptr(self.w, off), self.w.dtype, self.r.vp, self.r.dtype, ns * self.cell, self.wscale,
              C.c_void_p(self.scal.ptr), self._sv())
-        self.P.back(self._window(self.r, 0, ns), self.g, f"back,{self.kind}", v0, vs, ns, scale=float(self.M),
-                    stream=self.stream)
+        self.P.back(self._window(self.r, 0, ns), self.g, f"back,{self.kind}", v0, vs, ns,
+                    scale=float(self.M) * self.wscale, stream=self.stream)
         a, b = self.cur, 1 - self.cur
         zc = self.zc.vp if self.zc is not None else None
         zcdt = self.zc.dtype if self.zc is not None else 0

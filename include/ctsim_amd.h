@@ -202,10 +202,12 @@ int ctsim_device_count(int32_t* count);
  * dtype.  Reductions accumulate in FP64 and write one double to d_out.
  * ---------------------------------------------------------------------------- */
 
-/* r = w .* (p - y) written as dtype r_dtype, and phi += sum(w .* (p - y).^2) / 2 in
- * FP64 at d_phi (statsrecon/pwls/reconPwlsSeNesterovSqs.m:65,95-96). */
+/* r = w .* (p - y) written as dtype r_dtype, and phi += wscale * sum(w .* (p - y).^2) / 2
+ * in FP64 at d_phi (statsrecon/pwls/reconPwlsSeNesterovSqs.m:65,95-96).  wscale lets
+ * half-precision storage keep w / max(w): the true weights are wscale * w. */
 int ctsim_pwls_residual(const void* d_p, int32_t p_dtype, const void* d_y, int32_t y_dtype, const void* d_w,
-                        int32_t w_dtype, void* d_r, int32_t r_dtype, size_t n, double* d_phi, void* stream);
+                        int32_t w_dtype, void* d_r, int32_t r_dtype, size_t n, double wscale, double* d_phi,
+                        void* stream);
 
 /* Huber penalty of a ZXY volume: mode 1 gradient S(x), mode 2 curvature T(x), written
  * to d_out (float32); mode 0 cost R(x) accumulated in FP64 into d_cost

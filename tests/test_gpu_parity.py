@@ -81,13 +81,14 @@ def test_cuda_gateway_names():
 # ---------------------------------------------------------------------------------
 # random seeded cases against the oracle, larger than the fixtures
 # ---------------------------------------------------------------------------------
-def _cone_case(seed, n=(48, 44, 24), nuv=(96, 40), nviews=20, dettype=1, offsets=False, couch=False, map_=True):
+def _cone_case(seed, n=(48, 44, 24), nuv=(96, 40), nviews=20, dettype=1, offsets=False, couch=False, map_=True,
+               dz=0.6, dv=0.4):
     from oracle import oracle as O
     rng = np.random.default_rng(seed)
     betas = np.sort(rng.uniform(0, 2 * np.pi, nviews))
     o = (0.7, -1.2, 0.5) if offsets else (0, 0, 0)
     ouv = (1.5, -2.5) if offsets else (0, 0)
-    g = O.G3(n, (0.5, 0.5, 0.6), o, nuv, (0.4 if dettype == 1 else 0.45, 0.4), ouv, 1000.0, 0.0, 1500.0, betas,
+    g = O.G3(n, (0.5, 0.5, dz), o, nuv, (0.4 if dettype == 1 else 0.45, dv), ouv, 1000.0, 0.0, 1500.0, betas,
              couchz=rng.uniform(-3, 3, nviews) if couch else None, dettype=dettype)
     vol = rng.uniform(0, 1, n).astype(np.float32)
     vol[rng.uniform(size=n) < 0.3] = 0
@@ -102,6 +103,8 @@ CONE_CASES = [
     dict(seed=3, dettype=0, couch=True),
     dict(seed=4, n=(37, 53, 70), nuv=(70, 130), nviews=9),  # nv > 64: several row chunks, odd sizes
     dict(seed=5, n=(64, 64, 8), nuv=(200, 12), nviews=7),  # thin detector, wide fan
+    dict(seed=6, n=(40, 40, 150), nuv=(90, 60), nviews=8, dz=0.3, dv=1.2),  # slices finer than rows: multi-pass
+    dict(seed=7, n=(30, 30, 6), nuv=(70, 140), nviews=6, dz=4.0, dv=0.3),  # 20 rows per slice: 16-cell cap
 ]
 
 
@@ -131,6 +134,20 @@ def test_cbct_plan_vs_oracle(case):
         got = P.download_volume(out_v)
         want = O.cbct(op, g, sino, mp)
         assert rel_l2(got, want) <= TOL, (op, rel_l2(got, want))
+
+
+def test_cbct_unsplit_launch_vs_oracle():
+    """>= 16384 work items: each wave walks its whole strip (no LDS combine)"""
+    from oracle import oracle as O
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    g, vol, sino, mp = _cone_case(8, n=(24, 24, 12), nuv=(1024, 200), nviews=64, dz=0.9, dv=0.4)
+    P = Projector(geometry_from_g3(g, mp))
+    out = DeviceArray(P.n_sino(), "f32")
+    d_vol = P.upload_volume(vol)
+    for kind in ("dd", "tf"):
+        P.proj(d_vol, out, f"proj,{kind}")
+        assert rel_l2(P.download_sino(out), O.cbct(f"proj,{kind}", g, vol, mp)) <= TOL, kind
 
 
 FAN_CASES = [

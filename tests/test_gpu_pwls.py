@@ -62,11 +62,11 @@ def test_residual_kernel_and_fp64_cost():
     dp, dy, dw = (DeviceArray.from_numpy(a, "f32") for a in (p, yy, ww))
     r = DeviceArray(n, "f32")
     phi = DeviceArray(1, "f64").fill(0.0)
-    call("ctsim_pwls_residual", dp.vp, 0, dy.vp, 0, dw.vp, 0, r.vp, 0, n, phi.vp, None)
-    call("ctsim_pwls_residual", dp.vp, 0, dy.vp, 0, dw.vp, 0, r.vp, 0, n, phi.vp, None)  # accumulates
+    call("ctsim_pwls_residual", dp.vp, 0, dy.vp, 0, dw.vp, 0, r.vp, 0, n, 1.0, phi.vp, None)
+    call("ctsim_pwls_residual", dp.vp, 0, dy.vp, 0, dw.vp, 0, r.vp, 0, n, 3.0, phi.vp, None)  # accumulates
     d = p.astype(np.float64) - yy
     np.testing.assert_allclose(r.to_numpy(), (ww * (p - yy)).astype(np.float32), rtol=1e-6)
-    assert phi.to_numpy()[0] == pytest.approx(2 * np.sum(ww * d * d) / 2, rel=1e-9)
+    assert phi.to_numpy()[0] == pytest.approx(4 * np.sum(ww * d * d) / 2, rel=1e-9)
 
 
 def test_pwls_plain_matches_fp64_oracle():
