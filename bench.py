@@ -41,11 +41,11 @@ def flops(c):
 
 
 def simulate_pwls_data(geom, seed=4):
-    """phantom -> SF projection on the GPU -> Poisson counts at I0 = 2e4 -> y, w = counts"""
+    """body phantom -> SF projection on the GPU -> Poisson counts at I0 = 2e4 -> y, w = counts"""
     from ctsim_amd import workloads as W
     from ctsim_amd.device import DeviceArray
     from ctsim_amd.projector import Projector
-    vol = W.ellipsoid_phantom(geom.reconSize, geom.reconSpacing, 20, seed)
+    vol = W.body_phantom(geom.reconSize, geom.reconSpacing, 10, seed)
     P = Projector(geom)
     d = DeviceArray(P.n_sino(), "f32")
     P.proj(P.upload_volume(vol), d, "proj,tf")
@@ -255,7 +255,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (20 random ellipsoids, Poisson I0=2e4, FDK init)",
+        "data": "synthetic (water cylinder + 10 random ellipsoid inserts, Poisson I0=2e4, FDK init)",
         "config": {"workload": "pwls_os_sf_512x512x128_984views_1024x64det_24subsets", "volume": [512, 512, 128],
                    "views": 984, "detector": [1024, 64], "subsets": 24, "beta": 1e4, "delta": 1e-3,
                    "storage": "fp16 sinogram/weights/residual/z-copy, fp32 x/z/s, fp64 reductions",

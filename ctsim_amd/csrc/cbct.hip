@@ -28,17 +28,19 @@ enum Foot { FOOT_DD = 0, FOOT_TF = 1 };
 //
 // The wave enumerates the image columns whose footprint can reach its tile (the strip,
 // found by inverse-mapping the tile edges line by line), evaluates 64 candidate
-// footprints at a time (one per lane, including the tile's u weights), and walks the
-// surviving columns with their parameters broadcast from the owning lane.  For each
-// column the lanes first stage the needed voxels (one coalesced load each, with the
-// per-voxel cone factor ampv) in LDS; then every row gathers the voxels whose v
+// footprints at a time (one per lane, including the tile's u weights) and parks the
+// survivors' parameters in LDS.  It then walks them in a software pipeline: the voxel
+// loads of the next column are issued before the current column is staged (value times
+// the cone factor ampv, in LDS) and gathered: every row sums the voxels whose v
 // footprint covers it, with the exact clamps of the reference.
 // ---------------------------------------------------------------------------------
 constexpr int kZWin = 512;  // voxels staged per LDS window and wave
+constexpr int kPre = 3;     // voxel loads per lane issued one column ahead (3 x 64 voxels)
+static_assert(kPre == 3, "the pipeline below holds three prefetch registers");
 
 template <class T>
 struct ColParams {
-    int col, iz0, iz1, ilo, ihi;
+    int col, iz0, iz1, ilo, ihi, za, zb, pad;
     T amp, r, zinc, vstart, w0, w1, w2, w3, p0, p1, p2, p3;
 };
 
@@ -56,7 +58,9 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                                                            const uint8_t* __restrict__ map,
                                                            Tout* __restrict__ proj, int exhaustive) {
     static_assert(SPLIT == 1 || SPLIT == kWavesPerBlock, "SPLIT is 1 or the block's wave count");
+    static_assert(RPL == 1, "one detector row per lane");
     __shared__ T zbuf[kWavesPerBlock][kZWin];
+    __shared__ ColParams<T> cbuf[kWavesPerBlock][kWave];
     __shared__ T part[SPLIT == 1 ? 1 : SPLIT - 1][TU * RPL][kWave];
     const int rows = RPL * kWave;
     const int nut = (g.nu + TU - 1) / TU;
@@ -82,6 +86,7 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
     const T kR = u_to_slope(g, T(iu_b) + T(0.5));
     const int iv_last = min(iv_a + rows, g.nv) - 1;
     T* zw = zbuf[wv];
+    ColParams<T>* cw = cbuf[wv];
 
     T acc[RPL][TU];
 #pragma unroll
@@ -100,137 +105,164 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
         const int excl = incl - cnt;
         const int total = rdl_i(incl, kWave - 1);
         for (int k0 = 0; k0 < total; k0 += kWave) {
-            const int k = k0 + lane;
-            int j = 0;
+            {
+                const int k = k0 + lane;
+                int j = 0;
 #pragma unroll
-            for (int step = 32; step >= 1; step >>= 1) {
-                const int e = __shfl(excl, j + step, 64);
-                if (e <= k) j += step;
-            }
-            const int lo_j = __shfl(lo, j, 64);
-            const int ex_j = __shfl(excl, j, 64);
-            const int major = m0 + j;
-            const int minor = lo_j + (k - ex_j);
-            const int ix = row_major ? minor : major;
-            const int iy = row_major ? major : minor;
+                for (int step = 32; step >= 1; step >>= 1) {
+                    const int e = __shfl(excl, j + step, 64);
+                    if (e <= k) j += step;
+                }
+                const int lo_j = __shfl(lo, j, 64);
+                const int ex_j = __shfl(excl, j, 64);
+                const int major = m0 + j;
+                const int minor = lo_j + (k - ex_j);
+                const int ix = row_major ? minor : major;
+                const int iy = row_major ? major : minor;
 
-            // exact footprint of the candidate column and its weights on this tile
-            bool ok = k < total;
-            ColParams<T> cp{};
-            cp.r = T(1);
-            cp.zinc = T(1);
-            if (ok && map != nullptr) ok = map[iy * g.nx + ix] != 0;
-            if (ok) {
-                const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
-                const Column<T> cg = column_geom(xc, yc, v, g.sad, g.dx);
-                T clo, chi;
-                if (FOOT == FOOT_DD) {
-                    T umin, umax;
-                    dd_bounds(cg, v, g, umin, umax);
-                    ok = clamp_cells(umin, umax, g.nu, 16, clo, chi, cp.ilo, cp.ihi);
-                    cp.p0 = clo;
-                    cp.p1 = chi;
-                } else {
-                    T tau[4];
-                    sf_taus(cg, v, g, tau);
-                    ok = clamp_cells(tau[0], tau[3], g.nu, 16, clo, chi, cp.ilo, cp.ihi);
-                    cp.p0 = tau[0]; cp.p1 = tau[1]; cp.p2 = tau[2]; cp.p3 = tau[3];
-                }
-                ok = ok && cp.ihi >= iu_a && cp.ilo <= iu_b;
+                // exact footprint of the candidate column and its weights on this tile
+                bool ok = k < total;
+                ColParams<T> cp{};
+                cp.r = T(1);
+                cp.zinc = T(1);
+                if (ok && map != nullptr) ok = map[iy * g.nx + ix] != 0;
                 if (ok) {
-                    ok = v_window(g, cg.d, v.wz, cp.zinc, cp.vstart, cp.iz0, cp.iz1) && cp.iz0 < cp.iz1 &&
-                         cp.zinc > T(0);
-                    cp.amp = cg.amp;
-                    cp.r = cg.r;
-                }
-                if (ok) {
-                    // rows covered by the column's voxels vs this wave's rows
-                    const T e0 = cp.vstart + T(cp.iz0) * cp.zinc + T(0.5);
-                    const T e1 = cp.vstart + T(cp.iz1) * cp.zinc + T(0.5);
-                    ok = e1 > T(iv_a) - T(1) && e0 < T(iv_last + 2);
-                }
-                if (ok) {
-                    // tile weights of the first four footprint cells inside the tile
-                    const int c0 = max(cp.ilo, iu_a);
-                    cp.w0 = u_weight<T, FOOT>(c0 + 0, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
-                    cp.w1 = u_weight<T, FOOT>(c0 + 1, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
-                    cp.w2 = u_weight<T, FOOT>(c0 + 2, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
-                    cp.w3 = u_weight<T, FOOT>(c0 + 3, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
-                    cp.col = iy * g.nx + ix;
-                }
-            }
-            unsigned long long mask = __ballot(ok);
-            while (mask) {
-                const int src = __ffsll((long long)mask) - 1;
-                mask &= mask - 1;
-                const int col = rdl_i(cp.col, src);
-                const int iz0 = rdl_i(cp.iz0, src), iz1 = rdl_i(cp.iz1, src);
-                const int ilo = rdl_i(cp.ilo, src), ihi = rdl_i(cp.ihi, src);
-                const T amp = rdl(cp.amp, src), r = rdl(cp.r, src);
-                const T zinc = rdl(cp.zinc, src), vstart = rdl(cp.vstart, src);
-                const T w0 = rdl(cp.w0, src), w1 = rdl(cp.w1, src), w2 = rdl(cp.w2, src), w3 = rdl(cp.w3, src);
-                const int c0 = max(ilo, iu_a) - iu_a;  // first tile cell of the footprint
-                const int c1 = min(ihi, iu_b) - iu_a;  // last
-                T p0 = T(0), p1 = T(0), p2 = T(0), p3 = T(0);
-                if (c1 - c0 >= 4) {  // footprint wider than four tile cells: weights computed below
-                    p0 = rdl(cp.p0, src); p1 = rdl(cp.p1, src);
-                    if (FOOT == FOOT_TF) { p2 = rdl(cp.p2, src); p3 = rdl(cp.p3, src); }
-                }
-                const Tin* colp = vol + (size_t)col * g.nz;
-                const T inv_r = T(1) / r;
-                const T rz = T(1) / zinc;
-                // voxel range needed by this wave's rows (exact ranges; a voxel left out
-                // at the rounding edge has a weight of the order of the rounding error)
-                int za = (int)ceil((T(iv_a) - T(0.5) - vstart) * rz - T(1));
-                int zb = (int)ceil((T(iv_last) + T(0.5) - vstart) * rz) - 1;
-                za = max(za, iz0);
-                zb = min(zb, iz1 - 1);
-                T q[RPL];
-#pragma unroll
-                for (int s = 0; s < RPL; ++s) q[s] = T(0);
-                for (int zw0 = za; zw0 <= zb; zw0 += kZWin) {
-                    const int zw1 = min(zb, zw0 + kZWin - 1);
-                    // stage voxel values ampv * img of [zw0, zw1] in LDS
-                    for (int zz = zw0 + lane; zz <= zw1; zz += kWave) {
-                        const T zc = (T(zz) + v.wz) * g.dz;
-                        zw[zz - zw0] = sqrt(r * r + zc * zc) * inv_r * ld<T>(colp + zz);
+                    const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
+                    const Column<T> cg = column_geom(xc, yc, v, g.sad, g.dx);
+                    T clo, chi;
+                    if (FOOT == FOOT_DD) {
+                        T umin, umax;
+                        dd_bounds(cg, v, g, umin, umax);
+                        ok = clamp_cells(umin, umax, g.nu, 16, clo, chi, cp.ilo, cp.ihi);
+                        cp.p0 = clo;
+                        cp.p1 = chi;
+                    } else {
+                        T tau[4];
+                        sf_taus(cg, v, g, tau);
+                        ok = clamp_cells(tau[0], tau[3], g.nu, 16, clo, chi, cp.ilo, cp.ihi);
+                        cp.p0 = tau[0]; cp.p1 = tau[1]; cp.p2 = tau[2]; cp.p3 = tau[3];
                     }
-                    __builtin_amdgcn_wave_barrier();
+                    ok = ok && cp.ihi >= iu_a && cp.ilo <= iu_b;
+                    if (ok) {
+                        ok = v_window(g, cg.d, v.wz, cp.zinc, cp.vstart, cp.iz0, cp.iz1) && cp.iz0 < cp.iz1 &&
+                             cp.zinc > T(0);
+                        cp.amp = cg.amp;
+                        cp.r = cg.r;
+                    }
+                    if (ok) {
+                        // voxel range needed by this wave's rows (exact ranges; a voxel left
+                        // out at the rounding edge has a weight of the order of the rounding)
+                        const T rz = frcp(cp.zinc);
+                        cp.za = max((int)ceil((T(iv_a) - T(0.5) - cp.vstart) * rz - T(1)), cp.iz0);
+                        cp.zb = min((int)ceil((T(iv_last) + T(0.5) - cp.vstart) * rz) - 1, cp.iz1 - 1);
+                        ok = cp.za <= cp.zb;
+                    }
+                    if (ok) {
+                        // tile weights of the first four footprint cells inside the tile
+                        const int c0 = max(cp.ilo, iu_a);
+                        cp.w0 = u_weight<T, FOOT>(c0 + 0, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
+                        cp.w1 = u_weight<T, FOOT>(c0 + 1, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
+                        cp.w2 = u_weight<T, FOOT>(c0 + 2, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
+                        cp.w3 = u_weight<T, FOOT>(c0 + 3, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
+                        cp.col = iy * g.nx + ix;
+                    }
+                }
+                cw[lane] = cp;
+                __builtin_amdgcn_wave_barrier();
+                unsigned long long mask = __ballot(ok);
+                // software pipeline over the surviving columns: loads of column i+1 are in
+                // flight while column i is staged and gathered
+                int cur = mask ? __ffsll((long long)mask) - 1 : -1;
+                if (cur >= 0) mask &= mask - 1;
+                Tin pre0{}, pre1{}, pre2{};
+                if (cur >= 0) {
+                    const ColParams<T>& c = cw[cur];
+                    const Tin* colp = vol + (size_t)c.col * g.nz;
+                    const int zz = c.za + lane;
+                    pre0 = colp[zz <= c.zb ? zz : c.za];
+                    pre1 = colp[zz + kWave <= c.zb ? zz + kWave : c.za];
+                    pre2 = colp[zz + 2 * kWave <= c.zb ? zz + 2 * kWave : c.za];
+                }
+                while (cur >= 0) {
+                    const ColParams<T>& c = cw[cur];
+                    const int za = c.za, zb = c.zb, ilo = c.ilo, ihi = c.ihi;
+                    const T amp = c.amp, r = c.r, zinc = c.zinc, vstart = c.vstart;
+                    const T inv_r = frcp(r);
+                    const T rz = frcp(zinc);
+                    const Tin* colp = vol + (size_t)c.col * g.nz;
+                    const int iv = iv_a + lane;
+                    T q = T(0);
+                    for (int zw0 = za; zw0 <= zb; zw0 += kZWin) {
+                        const int zw1 = min(zb, zw0 + kZWin - 1);
+                        // stage ampv * img of voxels [zw0, zw1] in LDS: the first window from
+                        // the registers loaded one column earlier, the rest directly
+                        int zfirst = zw0 + lane;
+                        if (zw0 == za) {
+                            const Tin pv[kPre] = {pre0, pre1, pre2};
 #pragma unroll
-                    for (int s = 0; s < RPL; ++s) {
-                        const int iv = iv_a + s * kWave + lane;
-                        if (iv > iv_last) continue;
-                        const T fiv = T(iv);
-                        int ja = (int)ceil((fiv - T(0.5) - vstart) * rz - T(1));
-                        int jb = (int)ceil((fiv + T(0.5) - vstart) * rz) - 1;
-                        ja = max(ja, zw0);
-                        jb = min(jb, zw1);
-                        for (int jj = ja; jj <= jb; ++jj) {
-                            const T vmin = vstart + T(jj) * zinc;
-                            T clo, chi;
-                            int vlo, vhi;
-                            if (!clamp_cells(vmin, vmin + zinc, g.nv, 16, clo, chi, vlo, vhi)) continue;
-                            q[s] += dd_weight(iv, clo, chi, vlo, vhi) * zw[jj - zw0];
+                            for (int i = 0; i < kPre; ++i) {
+                                const int zz = za + i * kWave + lane;
+                                if (zz <= zw1) {
+                                    const T zc = (T(zz) + v.wz) * g.dz;
+                                    zw[zz - zw0] = fsqrt(r * r + zc * zc) * inv_r * T(to_f(pv[i]));
+                                }
+                            }
+                            zfirst = za + kPre * kWave + lane;
                         }
+                        for (int zz = zfirst; zz <= zw1; zz += kWave) {
+                            const T zc = (T(zz) + v.wz) * g.dz;
+                            zw[zz - zw0] = fsqrt(r * r + zc * zc) * inv_r * ld<T>(colp + zz);
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        if (zw0 == za) {
+                            // the next column's voxel loads fly while this one is gathered
+                            const int nxt = mask ? __ffsll((long long)mask) - 1 : -1;
+                            if (nxt >= 0) {
+                                mask &= mask - 1;
+                                const ColParams<T>& cn = cw[nxt];
+                                const Tin* coln = vol + (size_t)cn.col * g.nz;
+                                const int zz = cn.za + lane;
+                                pre0 = coln[zz <= cn.zb ? zz : cn.za];
+                                pre1 = coln[zz + kWave <= cn.zb ? zz + kWave : cn.za];
+                                pre2 = coln[zz + 2 * kWave <= cn.zb ? zz + 2 * kWave : cn.za];
+                            }
+                            cur = nxt;
+                        }
+                        if (iv <= iv_last) {
+                            const T fiv = T(iv);
+                            int ja = (int)ceil((fiv - T(0.5) - vstart) * rz - T(1));
+                            int jb = (int)ceil((fiv + T(0.5) - vstart) * rz) - 1;
+                            ja = max(ja, zw0);
+                            jb = min(jb, zw1);
+                            for (int jj = ja; jj <= jb; ++jj) {
+                                const T vmin = vstart + T(jj) * zinc;
+                                q += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * zw[jj - zw0];
+                            }
+                        }
+                        __builtin_amdgcn_wave_barrier();
                     }
-                    __builtin_amdgcn_wave_barrier();
-                }
+                    const int c0 = max(ilo, iu_a) - iu_a;  // first tile cell of the footprint
+                    const int c1 = min(ihi, iu_b) - iu_a;  // last
+                    const T aq = amp * q;
+                    if (c1 - c0 < 4) {
+                        const T w0 = c.w0, w1 = c.w1, w2 = c.w2, w3 = c.w3;
 #pragma unroll
-                for (int s = 0; s < RPL; ++s) {
-                    const T aq = amp * q[s];
-#pragma unroll
-                    for (int t = 0; t < TU; ++t) {
-                        T wt;
-                        if (c1 - c0 < 4) {
+                        for (int t = 0; t < TU; ++t) {
                             const int d = t - c0;
-                            wt = d == 0 ? w0 : d == 1 ? w1 : d == 2 ? w2 : d == 3 ? w3 : T(0);
-                            wt = (t >= c0 && t <= c1) ? wt : T(0);
-                        } else {
-                            wt = (t >= c0 && t <= c1) ? u_weight<T, FOOT>(iu_a + t, ilo, ihi, p0, p1, p2, p3) : T(0);
+                            const T wt = d == 0 ? w0 : d == 1 ? w1 : d == 2 ? w2 : d == 3 ? w3 : T(0);
+                            acc[0][t] += wt * aq;
                         }
-                        acc[s][t] += wt * aq;
+                    } else {  // footprint wider than four tile cells
+                        const T p0 = c.p0, p1 = c.p1, p2 = c.p2, p3 = c.p3;
+                        for (int t = c0; t <= c1; ++t) {
+                            const T wt = u_weight<T, FOOT>(iu_a + t, ilo, ihi, p0, p1, p2, p3);
+#pragma unroll
+                            for (int tt = 0; tt < TU; ++tt)
+                                if (tt == t) acc[0][tt] += wt * aq;
+                        }
                     }
                 }
+                __builtin_amdgcn_wave_barrier();
             }
         }
     }
@@ -331,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
         const T wu3 = u_weight<T, FOOT>(ilo + 3, ilo, ihi, p0, p1, p2, p3);
         const int nw = ihi - ilo + 1;
         const Tin* pv = proj + (size_t)b * g.nu * g.nv;
-        const T inv_r = T(1) / cg.r;
+        const T inv_r = frcp(cg.r);
         for (int rw = r0; rw <= r1; rw += kRowWin) {
             const int rwe = min(r1, rw + kRowWin - 1);
             // u combination of the footprint's detector columns, row by row
@@ -356,9 +388,9 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 const int ra = max(vlo, rw), rb = min(vhi, rwe);
                 if (ra > rb) continue;
                 T s = T(0);
-                for (int iv = ra; iv <= rb; ++iv) s += dd_weight(iv, clo, chi, vlo, vhi) * sw[iv - rw];
+                for (int iv = ra; iv <= rb; ++iv) s += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * sw[iv - rw];
                 const T zcm = (T(iz) + v.wz) * g.dz;
-                img[c] += cg.amp * (sqrt(cg.r * cg.r + zcm * zcm) * inv_r) * s;
+                img[c] += cg.amp * (fsqrt(cg.r * cg.r + zcm * zcm) * inv_r) * s;
             }
             __builtin_amdgcn_wave_barrier();
         }
@@ -549,20 +581,14 @@ static void launch_proj(unsigned blocks, hipStream_t st, const Geom<T>& g, const
 template <class Tin, class Tout, class T, int FOOT>
 static void proj3_foot(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* vol,
                        const uint8_t* map, void* proj, int exhaustive, hipStream_t st) {
-    const int rpl = g.nv > kWave ? 2 : 1;
-    const long long nut = (g.nu + kTU - 1) / kTU, nvc = (g.nv + rpl * kWave - 1) / (rpl * kWave);
+    const long long nut = (g.nu + kTU - 1) / kTU, nvc = (g.nv + kWave - 1) / kWave;
     const long long items = (long long)nviews * nut * nvc;
     // few work items (short view subsets, thin detectors): split each item's strip over
     // the four waves of a block so that the chip still holds several waves per SIMD
     const bool split = items < 16384;
     const unsigned blocks = (unsigned)(split ? items : (items + kWavesPerBlock - 1) / kWavesPerBlock);
-    if (rpl == 2) {
-        if (split) launch_proj<Tin, Tout, T, FOOT, 2, kWavesPerBlock>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
-        else launch_proj<Tin, Tout, T, FOOT, 2, 1>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
-    } else {
-        if (split) launch_proj<Tin, Tout, T, FOOT, 1, kWavesPerBlock>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
-        else launch_proj<Tin, Tout, T, FOOT, 1, 1>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
-    }
+    if (split) launch_proj<Tin, Tout, T, FOOT, 1, kWavesPerBlock>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
+    else launch_proj<Tin, Tout, T, FOOT, 1, 1>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
 }
 
 template <class Tin, class Tout, class T>

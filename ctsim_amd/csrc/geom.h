@@ -26,6 +26,12 @@ __device__ __forceinline__ float ld_f<__hip_bfloat16>(const __hip_bfloat16* p) {
 
 template <class C, class S>
 __device__ __forceinline__ C ld(const S* p) { return (C)ld_f(p); }
+
+// value conversions (no address taken, so values stay in registers)
+__device__ __forceinline__ float to_f(float x) { return x; }
+__device__ __forceinline__ float to_f(__half x) { return __half2float(x); }
+__device__ __forceinline__ float to_f(__hip_bfloat16 x) { return __bfloat162float(x); }
+__device__ __forceinline__ double to_f(double x) { return x; }
 template <>
 __device__ __forceinline__ double ld<double, double>(const double* p) { return *p; }
 
@@ -74,24 +80,36 @@ struct Column {
 template <class T>
 __device__ __forceinline__ T fabs_t(T x) { return x < T(0) ? -x : x; }
 
+// Division and square root of the footprint arithmetic.  The FP32 path uses the
+// hardware's 1-ulp v_rcp_f32 / v_sqrt_f32: every weight is a continuous function of
+// the quantities they produce, so the result moves by O(1e-7) relative, far inside the
+// 1e-5 parity budget.  The FP64 path stays IEEE-exact.
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float frcp(float b) { return __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double frcp(double b) { return 1.0 / b; }
+__device__ __forceinline__ float fsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+__device__ __forceinline__ double fsqrt(double a) { return sqrt(a); }
+
 template <class T>
 __device__ __forceinline__ Column<T> column_geom(T xc, T yc, const View<T>& v, T sad, T dx) {
     Column<T> k;
     k.t = xc * v.c - yc * v.s;
     k.d = sad + xc * v.s + yc * v.c;
-    k.r = sqrt(k.t * k.t + k.d * k.d);
-    T cg = k.d / k.r;
-    T sg = k.t / k.r;
+    k.r = fsqrt(k.t * k.t + k.d * k.d);
+    const T ir = frcp(k.r);
+    T cg = k.d * ir;
+    T sg = k.t * ir;
     T cphi = fabs_t(v.c * cg - v.s * sg);
     T sphi = fabs_t(v.s * cg + v.c * sg);
     k.xdom = cphi > sphi;
-    k.amp = (k.xdom ? dx / cphi : dx / sphi) / T(10);
+    k.amp = fdiv(dx, k.xdom ? cphi : sphi) * T(0.1);
     return k;
 }
 
 template <class T>
 __device__ __forceinline__ T det_map(T num, T den, T sdd_du, T wu, int dettype) {
-    return dettype == 1 ? sdd_du * num / den - wu : sdd_du * atan(num / den) - wu;
+    return dettype == 1 ? sdd_du * fdiv(num, den) - wu : sdd_du * atan(fdiv(num, den)) - wu;
 }
 
 // Distance-driven boundaries (unshifted), sorted (fbct_proj.c:216-248).
@@ -141,13 +159,13 @@ __device__ __forceinline__ T sf_weight(int iu, T t0, T t1, T t2, T t3) {
     T w = T(0);
     const T bl = T(iu) - T(0.5), br = T(iu) + T(0.5);
     T xl = bl > t0 ? bl : t0, xr = br < t1 ? br : t1;
-    if (xr > xl) w += (((xr - t0) * (xr - t0) - (xl - t0) * (xl - t0)) / (t1 - t0)) / T(2);
+    if (xr > xl) w += fdiv((xr - t0) * (xr - t0) - (xl - t0) * (xl - t0), t1 - t0) * T(0.5);
     xl = bl > t1 ? bl : t1;
     xr = br < t2 ? br : t2;
     if (xr > xl) w += (xr - xl);
     xl = bl > t2 ? bl : t2;
     xr = br < t3 ? br : t3;
-    if (xr > xl) w += (((xl - t3) * (xl - t3) - (xr - t3) * (xr - t3)) / (t3 - t2)) / T(2);
+    if (xr > xl) w += fdiv((xl - t3) * (xl - t3) - (xr - t3) * (xr - t3), t3 - t2) * T(0.5);
     return w;
 }
 
@@ -178,14 +196,33 @@ __device__ __forceinline__ T dd_weight(int i, T clo, T chi, int ilo, int ihi) {
     return T(1);
 }
 
+// Branch-free DD weight of detector cell i for the unshifted footprint [lo, hi] on a
+// detector of n cells with the 16-cell cap: the reference's first / middle / last /
+// single-cell rules (cbct_proj.c:597-623) are all the overlap min(chi, i+1) - max(clo, i)
+// of the clamped interval (same float operations); beyond the cap the cells get nothing
+// and the capped last cell gets chi - i.  Returns 0 for an empty footprint.
+template <class T>
+__device__ __forceinline__ T dd_overlap16(int i, T lo, T hi, int n) {
+    const T a = lo + T(0.5), b = hi + T(0.5);
+    const T clo = a > T(0) ? a : T(0);
+    const T chi = b < T(n) ? b : T(n);
+    const T fi = T(i);
+    T w = (chi < fi + T(1) ? chi : fi + T(1)) - (clo > fi ? clo : fi);
+    w = w > T(0) ? w : T(0);
+    const T capc = floor(clo) + T(15);
+    w = fi > capc ? T(0) : (fi == capc ? chi - fi : w);
+    return clo > chi ? T(0) : w;
+}
+
 // v window of a column (cbct_proj.c:569-580): magnified slice pitch zinc,
 // first voxel edge vstart and the voxel range [iz0, iz1).
 template <class T>
 __device__ __forceinline__ bool v_window(const Geom<T>& g, T d, T wz, T& zinc, T& vstart, int& iz0, int& iz1) {
-    zinc = g.zk / d;
+    zinc = fdiv(g.zk, d);
     vstart = (T)(((double)wz - 0.5) * (double)zinc - (double)g.wv);
-    iz0 = (int)floor((T(-1) - vstart) / zinc);
-    iz1 = (int)ceil((T(g.nv + 1) - vstart) / zinc);
+    const T rz = frcp(zinc);
+    iz0 = (int)floor((T(-1) - vstart) * rz);
+    iz1 = (int)ceil((T(g.nv + 1) - vstart) * rz);
     iz0 = iz0 > 0 ? iz0 : 0;
     iz1 = iz1 < g.nz ? iz1 : g.nz;
     return iz0 <= iz1;

@@ -79,3 +79,39 @@ def ellipsoid_phantom(shape, spacing, n, seed, semi=(5.0, 120.0), mu=(0.015, 0.0
             if m.any():
                 vol[:, :, k][m] += val
     return vol.astype(dtype)
+
+
+def body_phantom(shape, spacing, n_inserts, seed, water=0.02, mu=(0.015, 0.06), dtype=np.float32):
+    """Water cylinder (radius 40% of the field, mu 0.02 /mm) with n random ellipsoidal
+    inserts (semi-axes 5-60 mm, mu 0.015-0.06 /mm) that replace the water, as
+    phantom/util/addEllipses.m does; values x10 into 1/cm.  Line integrals stay below ~8,
+    so rays keep tens of photons at I0 = 2e4.  MATLAB order [nx ny nz]."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    dx, dy, dz = spacing
+    x = (np.arange(nx) - (nx - 1) / 2) * dx
+    y = (np.arange(ny) - (ny - 1) / 2) * dy
+    z = (np.arange(nz) - (nz - 1) / 2) * dz
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    R = 0.4 * min(nx * dx, ny * dy)
+    H = 0.5 * nz * dz
+    slab = ((X / R) ** 2 + (Y / R) ** 2 <= 1.0).astype(np.float32) * np.float32(water * 10.0)
+    vol = np.repeat(slab[:, :, None], nz, axis=2)
+    for _ in range(n_inserts):
+        r = R * 0.7 * np.sqrt(rng.uniform())
+        ph = rng.uniform(0, 2 * np.pi)
+        cx, cy, cz = r * np.cos(ph), r * np.sin(ph), rng.uniform(-0.5, 0.5) * H
+        a, b = rng.uniform(5.0, 60.0, 2)
+        c = rng.uniform(5.0, 60.0)
+        th = rng.uniform(0, np.pi)
+        val = np.float32(rng.uniform(*mu) * 10.0)
+        ct, st = np.cos(th), np.sin(th)
+        xr = (X - cx) * ct + (Y - cy) * st
+        yr = -(X - cx) * st + (Y - cy) * ct
+        inplane = (xr / a) ** 2 + (yr / b) ** 2
+        for k in range(nz):
+            if abs(z[k] - cz) > c:
+                continue
+            m = inplane + ((z[k] - cz) / c) ** 2 <= 1.0
+            vol[:, :, k][m] = val
+    return vol.astype(dtype)
