@@ -43,6 +43,8 @@ class CtsimGeom(C.Structure):
 _GATEWAY3 = [cp, P_i32, P_f32, P_f32, P_i32, P_f32, P_f32, f32, f32, f32, i32, P_f64, P_f32, i32, P_f32, P_u8, P_f32,
              C.c_char_p, sz]
 
+_PER_OP = [C.POINTER(CtsimGeom), i32, P_f64, P_f32, P_u8, P_f32, P_f32, C.c_char_p, sz]
+
 SIGNATURES = {
     "ctsim_last_error": (C.c_char_p, []),
     "ctsim_version": (C.c_int, []),
@@ -52,6 +54,17 @@ SIGNATURES = {
     "ctsim_cbct_dd_cuda": (C.c_int, _GATEWAY3),
     "ctsim_cbct_sf_cuda": (C.c_int, _GATEWAY3),
     "ctsim_cbct_back_pd_cuda": (C.c_int, _GATEWAY3),
+    "ctsim_fbct_proj_dd": (C.c_int, _PER_OP),
+    "ctsim_fbct_proj_tf": (C.c_int, _PER_OP),
+    "ctsim_fbct_back_dd": (C.c_int, _PER_OP),
+    "ctsim_fbct_back_tf": (C.c_int, _PER_OP),
+    "ctsim_fbct_back_pd": (C.c_int, _PER_OP),
+    "ctsim_cbct_proj_dd": (C.c_int, _PER_OP),
+    "ctsim_cbct_proj_tf": (C.c_int, _PER_OP),
+    "ctsim_cbct_back_dd": (C.c_int, _PER_OP),
+    "ctsim_cbct_back_tf": (C.c_int, _PER_OP),
+    "ctsim_cbct_back_pd": (C.c_int, _PER_OP),
+    "ctsim_cbct_back_hpd": (C.c_int, _PER_OP),
     "ctsim_plan_create": (C.c_int, [C.POINTER(vp), i32, C.POINTER(CtsimGeom), i32, P_f64, P_f32, P_u8, i32]),
     "ctsim_plan_destroy": (C.c_int, [vp]),
     "ctsim_plan_proj": (C.c_int, [vp, cp, i32, i32, i32, vp, i32, vp, i32, vp]),

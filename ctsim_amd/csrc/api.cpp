@@ -482,6 +482,38 @@ int ctsim_cbct_back_pd_cuda(const char* op, const int32_t nxyz[3], const float d
                                    betas, couchz, dettype, in, map, out, errbuf, errlen);
 }
 
+// one entry point per (dimension, operator): the gateway with the op string fixed
+#define CTSIM_PER_OP(NAME, NDIM, OPSTR)                                                                            \
+    int NAME(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz, const uint8_t* map,   \
+             const float* in, float* out, char* errbuf, size_t errlen) {                                           \
+        int rc;                                                                                                    \
+        if (!g) {                                                                                                  \
+            rc = fail(CTSIM_EINVAL, "Error: null geometry.");                                                      \
+        } else {                                                                                                   \
+            ctsim_geom gg = *g;                                                                                    \
+            if (NDIM == 2) {                                                                                       \
+                gg.nz = 1; gg.nv = 1; gg.dz = 1.f; gg.dv = 1.f; gg.oz = 0.f; gg.ov = 0.f;                           \
+            }                                                                                                      \
+            rc = gateway(NDIM, OPSTR, gg, nviews, betas, NDIM == 3 ? couchz : nullptr, NDIM == 3 ? map : nullptr,  \
+                         in, out);                                                                                 \
+        }                                                                                                          \
+        if (rc) copy_err(errbuf, errlen);                                                                          \
+        return rc;                                                                                                 \
+    }
+
+CTSIM_PER_OP(ctsim_fbct_proj_dd, 2, "proj,dd")
+CTSIM_PER_OP(ctsim_fbct_proj_tf, 2, "proj,tf")
+CTSIM_PER_OP(ctsim_fbct_back_dd, 2, "back,dd")
+CTSIM_PER_OP(ctsim_fbct_back_tf, 2, "back,tf")
+CTSIM_PER_OP(ctsim_fbct_back_pd, 2, "back,pd")
+CTSIM_PER_OP(ctsim_cbct_proj_dd, 3, "proj,dd")
+CTSIM_PER_OP(ctsim_cbct_proj_tf, 3, "proj,tf")
+CTSIM_PER_OP(ctsim_cbct_back_dd, 3, "back,dd")
+CTSIM_PER_OP(ctsim_cbct_back_tf, 3, "back,tf")
+CTSIM_PER_OP(ctsim_cbct_back_pd, 3, "back,pd")
+CTSIM_PER_OP(ctsim_cbct_back_hpd, 3, "back,hpd")
+#undef CTSIM_PER_OP
+
 // ---------------------------------------------------------------------------------
 // batched uneven 2D slices
 // ---------------------------------------------------------------------------------

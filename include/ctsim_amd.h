@@ -95,10 +95,6 @@ int ctsim_cbct_back_pd_cuda(const char* op, const int32_t nxyz[3], const float d
                             int32_t detector_type, const float* in, const uint8_t* map, float* out, char* errbuf,
                             size_t errlen);
 
-/* ------------------------------------------------------------------------------
- * 2. Device-resident plan API.
- * ---------------------------------------------------------------------------- */
-
 /* Geometry of one operator; mirrors the fields of CTSim's geom struct that the
  * gateways receive (reconSize, reconSpacing, reconOffset, detSize, detSpacing,
  * detOffset, SAD, ADD, SDD, flatPanel; parameters/loadProjectionGeometryCT.m).
@@ -113,6 +109,44 @@ typedef struct ctsim_geom {
     float sad, add, sdd;
     int32_t detector_type;
 } ctsim_geom;
+
+/* One entry point per (dimension, operator), the gateways above with the op string
+ * fixed and the geometry in a struct: the reference's per-operator workers
+ * fbct_{proj,back}_{dd,tf} and fbct_back_pd (geom/fbct_os/fbct_proj.c:155-466,
+ * geom/fbct_os/fbct_back.c:82-399), cbct_proj1_{dd,tf} and cbct_back1_{dd,tf,pd}
+ * (geom/cbct_os/cbct_proj.c:427-642,912-1169; geom/cbct_os/cbct_back.c:135-262,
+ * 479-692,945-1197), as selected by the gateways' op-string dispatch
+ * (geom/fbct_os/fbct_geom_mex.c:250-326, geom/cbct_os/cbct_geom_mex.c:516-600).
+ * Same layouts and ownership as the gateways: host pointers in MATLAB order, out
+ * allocated by the caller and overwritten completely, in never modified.  2D entry
+ * points ignore nz, nv, dz, dv, oz, ov, couchz and map; 3D ones take couchz [nviews]
+ * (NULL = zeros) and map [nx ny] (NULL = all columns). */
+int ctsim_fbct_proj_dd(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
+                       const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
+int ctsim_fbct_proj_tf(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
+                       const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
+int ctsim_fbct_back_dd(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
+                       const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
+int ctsim_fbct_back_tf(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
+                       const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
+int ctsim_fbct_back_pd(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
+                       const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
+int ctsim_cbct_proj_dd(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
+                       const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
+int ctsim_cbct_proj_tf(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
+                       const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
+int ctsim_cbct_back_dd(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
+                       const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
+int ctsim_cbct_back_tf(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
+                       const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
+int ctsim_cbct_back_pd(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
+                       const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
+int ctsim_cbct_back_hpd(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
+                        const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
+
+/* ------------------------------------------------------------------------------
+ * 2. Device-resident plan API.
+ * ---------------------------------------------------------------------------- */
 
 typedef struct ctsim_plan ctsim_plan;
 

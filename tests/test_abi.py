@@ -63,6 +63,21 @@ def test_gateway_rejects_unknown_op_without_device():
     assert rc == 1 and "unknown geom type" in err.value.decode()
 
 
+def test_per_operator_entry_points_validate_without_device():
+    from ctsim_amd import _lib
+    L = _lib.lib()
+    err = C.create_string_buffer(256)
+    buf = np.zeros(64, np.float32)
+    fp = buf.ctypes.data_as(C.POINTER(C.c_float))
+    betas = np.zeros(2)
+    bp = betas.ctypes.data_as(C.POINTER(C.c_double))
+    assert L.ctsim_cbct_proj_tf(None, 2, bp, None, None, fp, fp, err, 256) == 1
+    assert err.value.decode() == "Error: null geometry."
+    g = _lib.CtsimGeom(4, 4, 4, 1.0, 1.0, 1.0, 0, 0, 0, 8, 8, 1.0, 1.0, 0, 0, 500.0, 0.0, 900.0, 1)
+    assert L.ctsim_fbct_back_pd(C.byref(g), 0, bp, None, None, fp, fp, err, 256) == 1
+    assert "noviews" in err.value.decode()
+
+
 def test_plan_rejects_bad_geometry_without_device():
     from ctsim_amd import _lib
     L = _lib.lib()

@@ -78,6 +78,48 @@ def test_cuda_gateway_names():
     assert rel_l2(mex.cbct_back_pd_cuda("back,hpd", *args, rec["sino"], mp), rec["back_hpd"]) <= TOL
 
 
+def _per_op(name, g, betas, couchz, mp, x, out_shape):
+    import ctypes as C
+    from ctsim_amd._lib import lib
+    x = np.asfortranarray(x, dtype=np.float32)
+    out = np.zeros(out_shape, dtype=np.float32, order="F")
+    betas = np.ascontiguousarray(betas, dtype=np.float64)
+    err = C.create_string_buffer(256)
+    f32p = C.POINTER(C.c_float)
+    cz = None if couchz is None else np.ascontiguousarray(couchz, dtype=np.float32)
+    mpp = None if mp is None else np.ascontiguousarray(np.asarray(mp, dtype=np.uint8).ravel(order="F"))
+    rc = getattr(lib(), name)(C.byref(g), len(betas), betas.ctypes.data_as(C.POINTER(C.c_double)),
+                              None if cz is None else cz.ctypes.data_as(f32p),
+                              None if mpp is None else mpp.ctypes.data_as(C.POINTER(C.c_uint8)),
+                              x.ctypes.data_as(f32p), out.ctypes.data_as(f32p), err, 256)
+    assert rc == 0, err.value
+    return out
+
+
+def test_per_operator_entry_points_match_golden():
+    from ctsim_amd._lib import CtsimGeom
+    rec = load_golden(golden_files("cone_flat_map")[0])
+    n, nuv = [int(v) for v in rec["n"]], [int(v) for v in rec["nuv"]]
+    g = CtsimGeom(*n, *map(float, rec["d"]), *map(float, rec["o"]), *nuv, *map(float, rec["duv"]),
+                  *map(float, rec["ouv"]), float(rec["sad"]), float(rec["add"]), float(rec["sdd"]), int(rec["dettype"]))
+    sino_shape = (nuv[1], nuv[0], len(rec["betas"]))
+    for op in OPS3:
+        proj = op.startswith("proj")
+        got = _per_op("ctsim_cbct_" + op.replace(",", "_"), g, rec["betas"], rec["couchz"], rec["map"],
+                      rec["img"] if proj else rec["sino"], sino_shape if proj else tuple(n))
+        assert rel_l2(got, rec[op.replace(",", "_")]) <= TOL, op
+    rec = load_golden(golden_files("fan_flat_small")[0])
+    n = [int(v) for v in rec["nxy"]]
+    g = CtsimGeom(n[0], n[1], 1, *map(float, rec["dxy"]), 1.0, *map(float, rec["oxy"]), 0.0, int(rec["nu"]), 1,
+                  float(rec["du"]), 1.0, float(rec["ou"]), 0.0, float(rec["sad"]), float(rec["add"]),
+                  float(rec["sdd"]), int(rec["dettype"]))
+    for op in OPS2:
+        proj = op.startswith("proj")
+        got = _per_op("ctsim_fbct_" + op.replace(",", "_"), g, rec["betas"], None, None,
+                      rec["img"] if proj else rec["sino"], (int(rec["nu"]), len(rec["betas"])) if proj else tuple(n))
+        assert rel_l2(got, rec[op.replace(",", "_")]) <= TOL, op
+
+
 # ---------------------------------------------------------------------------------
 # random seeded cases against the oracle, larger than the fixtures
 # ---------------------------------------------------------------------------------
