@@ -6,7 +6,13 @@ A step is one PWLS-OS iteration: 24 x (forward projection of the subset, residua
 backprojection x 24, fused Nesterov-SQS update) plus the per-iteration penalty cost and
 RMSD, all resident in HBM.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--no-pair] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--sino-dtype bf16|f16]
+                    [--accuracy-iters 20] [--no-pair] [--no-cpu] [--no-plain]
+
+Accuracy: after the timed iterations each form continues, untimed, to --accuracy-iters
+iterations, and the all-FP64 form (FP64 plan, FP64 elementwise kernels) runs the same
+iterations from the same FDK image; the JSON line reports the relative RMSE of the mixed
+and plain images against it, ||x - x64|| / ||x64|| over the volume.
 """
 from __future__ import annotations
 
@@ -38,6 +44,21 @@ def load_counts():
 def flops(c):
     """algorithmic FLOPs of one 3D DD/SF operator: 17 U + 3 C + 60 K"""
     return 17.0 * c["U"] + 3.0 * c["C"] + 60.0 * c["K"]
+
+
+def measured_traffic():
+    """HBM bytes per launch of the forward projector on one subset, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes, each scaled by what the counter reads for a
+    copy kernel of known size (profiles/cbct_proj_tf_pwls_subset_traffic.json); None when
+    no such measurement is committed"""
+    path = os.path.join(ROOT, "profiles", "cbct_proj_tf_pwls_subset_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    fetch = t["fetch_kib"] * t["copy_bytes"] / t["copy_fetch_kib"]
+    write = t["write_kib"] * t["copy_bytes"] / t["copy_write_kib"]
+    return round(fetch + write)
 
 
 def simulate_pwls_data(geom, seed=4):
@@ -109,13 +130,18 @@ def kernel_times(solver, reps=1):
     return {k: v / n for k, v in t.items()}
 
 
-def run_pwls(form, geom, y, w, x0, steps, warmup, numos=24, beta=1e4, delta=1e-3):
+def run_pwls(form, geom, y, w, x0, steps, warmup, sino_dtype="bf16", numos=24, beta=1e4, delta=1e-3,
+             total_iters=0):
+    """warmup + steps iterations (steps timed) of the form 'mixed', 'plain' or 'f64', then
+    untimed iterations up to total_iters"""
     from ctsim_amd.device import synchronize
     from ctsim_amd.pwls import PwlsOS
-    mixed = form == "mixed"
-    solver = PwlsOS(geom, y, w, numos, beta, delta, mixed=mixed, sino_dtype="f16", zcopy_dtype="f16")
+    precision = {"mixed": "mixed", "plain": "f32", "f64": "f64"}[form]
+    n_all = max(warmup + steps, total_iters)
+    solver = PwlsOS(geom, y, w, numos, beta, delta, precision=precision, sino_dtype=sino_dtype,
+                    zcopy_dtype="f16", graph=form == "mixed")
     solver.setup(x0)
-    solver.prepare(warmup + steps)
+    solver.prepare(n_all)
     for _ in range(warmup):
         solver.iterate()
     synchronize()
@@ -123,7 +149,15 @@ def run_pwls(form, geom, y, w, x0, steps, warmup, numos=24, beta=1e4, delta=1e-3
     outs = [solver.iterate() for _ in range(steps)]
     synchronize()
     dt = time.perf_counter() - t0
+    for _ in range(n_all - warmup - steps):
+        outs.append(solver.iterate())
     return solver, dt, outs
+
+
+def rel_rmse(x, ref):
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
 
 
 def pair_gups(kinds=("dd", "tf"), reps=2):
@@ -193,8 +227,10 @@ def cpu_baseline(geom, nsample=8):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--sino-dtype", default="bf16", choices=["bf16", "f16"])
+    ap.add_argument("--accuracy-iters", type=int, default=20)
     ap.add_argument("--no-pair", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -221,26 +257,41 @@ def main():
     c_it = counts["pwls_512x128/tf"]
     f_iter = 2 * flops(c_it)  # forward + back over all views per iteration
 
-    solver, dt, outs = run_pwls("mixed", geom, y, w, x0, args.steps, args.warmup)
+    n_acc = args.accuracy_iters
+    solver, dt, outs = run_pwls("mixed", geom, y, w, x0, args.steps, args.warmup, args.sino_dtype,
+                                total_iters=n_acc)
     ms_mixed = dt / args.steps * 1e3
     log(f"mixed: {ms_mixed:.1f} ms/iter, phi {outs[-1]['phi']:.6e}, rmsd {outs[-1]['rmsd']:.3e}")
+    x_mixed = solver.image() if n_acc else None
     kt = kernel_times(solver)
     log("kernel ms (avg per subset):", {k: round(v, 3) for k, v in kt.items()})
     del solver
 
     plain = None
+    x_plain = None
     if not args.no_plain:
-        sp, dtp, outs_p = run_pwls("plain", geom, y, w, x0, args.steps, args.warmup)
+        sp, dtp, outs_p = run_pwls("plain", geom, y, w, x0, args.steps, args.warmup, total_iters=n_acc)
         plain = dtp / args.steps * 1e3
         log(f"plain: {plain:.1f} ms/iter, phi {outs_p[-1]['phi']:.6e}")
+        x_plain = sp.image() if n_acc else None
         kt_plain = kernel_times(sp)
         del sp
+    accuracy = None
+    if n_acc:
+        s64, dt64, outs64 = run_pwls("f64", geom, y, w, x0, 1, 0, total_iters=n_acc)
+        x64 = s64.image()
+        del s64
+        accuracy = {"iterations": n_acc, "fp64_first_iter_ms": round(dt64 * 1e3, 1),
+                    "rel_rmse_mixed_vs_fp64": rel_rmse(x_mixed, x64),
+                    "rel_rmse_plain_vs_fp64": rel_rmse(x_plain, x64) if x_plain is not None else None,
+                    "phi_fp64": outs64[-1]["phi"], "phi_mixed": outs[-1]["phi"]}
+        log("accuracy:", accuracy)
     M = 24
     # dominant kernel: the forward projector (per-subset launch)
     f_launch = flops(c_it) / M
     achieved = f_launch / (kt["proj"] * 1e-3) / 1e12
     roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=None,
+                frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=measured_traffic(),
                 kernel="cbct_proj_kernel<__half,__half,float,SF,16> (one 41-view subset)",
                 note="FP32 vector ALU bound (no MFMA); achieved = (17U+3C+60K)/24 per launch / event time")
     res = {
@@ -258,7 +309,8 @@ def main():
         "data": "synthetic (water cylinder + 10 random ellipsoid inserts, Poisson I0=2e4, FDK init)",
         "config": {"workload": "pwls_os_sf_512x512x128_984views_1024x64det_24subsets", "volume": [512, 512, 128],
                    "views": 984, "detector": [1024, 64], "subsets": 24, "beta": 1e4, "delta": 1e-3,
-                   "storage": "fp16 sinogram/weights/residual/z-copy, fp32 x/z/s, fp64 reductions",
+                   "storage": f"{args.sino_dtype} sinogram/weights/residual, fp16 z-copy for the projector, "
+                              "fp32 x/z/s and footprint sums, fp64 denominators and reductions",
                    "graph": True},
         "plain_fp32_iter_per_s": round(1e3 / plain, 4) if plain else None,
         "mixed_over_plain": round(plain / ms_mixed, 3) if plain else None,
@@ -266,6 +318,7 @@ def main():
         "iteration_tflops": round(f_iter / (ms_mixed * 1e-3) / 1e12, 3),
         "kernel_ms_per_subset": {k: round(v, 4) for k, v in kt.items()},
         "roofline": roof,
+        "accuracy": accuracy,
     }
     if plain:
         res["kernel_ms_per_subset_plain"] = {k: round(v, 4) for k, v in kt_plain.items()}

@@ -106,6 +106,9 @@ SIGNATURES = {
     "ctsim_add_i32": (C.c_int, [vp, i32, vp]),
     "ctsim_pwls_sqs_update": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, f64, f64, vp]),
     "ctsim_sqdiff_sum": (C.c_int, [vp, vp, sz, vp, vp]),
+    "ctsim_huber_f64": (C.c_int, [vp, i32, i32, i32, f64, i32, vp, vp, vp]),
+    "ctsim_pwls_nesterov_update_f64": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, f64, f64, f64, f64, vp]),
+    "ctsim_sqdiff_sum_f64": (C.c_int, [vp, vp, sz, vp, vp]),
     "ctsim_mul": (C.c_int, [vp, i32, vp, i32, vp, i32, sz, vp]),
 }
 

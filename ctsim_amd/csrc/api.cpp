@@ -100,6 +100,8 @@ int check_geom(int ndim, const ctsim_geom& g) {
         return fail(CTSIM_EINVAL, "Error: pixel and detector spacings must be nonzero.");
     if ((double)g.nx * g.ny * g.nz > 4.0e9 || (double)g.nu * g.nv > 2.0e9)
         return fail(CTSIM_EINVAL, "Error: problem size exceeds 32-bit indexing of this build.");
+    if (ndim == 3 && g.nu >= 32768)
+        return fail(CTSIM_EUNSUP, "Error: 3D detectors up to 32767 columns are supported (nu = %d).", g.nu);
     return CTSIM_OK;
 }
 
@@ -822,6 +824,36 @@ int ctsim_pwls_sqs_update(const float* x, float* x_out, const float* g, const fl
 }
 
 int ctsim_sqdiff_sum(const float* a, const float* b, size_t n, double* out, void* stream) {
+    if (!a || !b || !out) return fail(CTSIM_EINVAL, "Error: null argument.");
+    HIP_TRY(sqdiff_launch(a, b, n, out, (hipStream_t)stream), "sqdiff");
+    return CTSIM_OK;
+}
+
+int ctsim_huber_f64(const double* x, int32_t nx, int32_t ny, int32_t nz, double delta, int32_t mode, double* out,
+                    double* cost, void* stream) {
+    if (!x || nx <= 0 || ny <= 0 || nz <= 0) return fail(CTSIM_EINVAL, "Error: bad volume.");
+    if ((mode == 1 || mode == 2) && !out) return fail(CTSIM_EINVAL, "Error: null output.");
+    if (mode == 0 && !cost) return fail(CTSIM_EINVAL, "Error: null cost output.");
+    hipError_t e = huber_launch(x, nx, ny, nz, delta, mode, out, cost, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EINVAL, "Error: wrong penalty mode.");
+    HIP_TRY(e, "huber");
+    return CTSIM_OK;
+}
+
+int ctsim_pwls_nesterov_update_f64(const double* x, const double* z, double* s, const double* g,
+                                   const double* precom, double* x_out, double* z_out, int32_t nx, int32_t ny,
+                                   int32_t nz, double beta, double delta, double t, double t_next, void* stream) {
+    if (!x || !z || !s || !g || !precom || !x_out || !z_out) return fail(CTSIM_EINVAL, "Error: null argument.");
+    if (nx <= 0 || ny <= 0 || nz <= 0) return fail(CTSIM_EINVAL, "Error: bad volume.");
+    if (x_out == x || z_out == z || x_out == z || z_out == x)
+        return fail(CTSIM_EINVAL, "Error: outputs must not alias the stencil inputs.");
+    HIP_TRY(nesterov_launch(x, z, s, g, precom, x_out, z_out, nullptr, nx, ny, nz, beta, delta, t, t_next, nullptr,
+                            nullptr, 0, (hipStream_t)stream),
+            "nesterov update");
+    return CTSIM_OK;
+}
+
+int ctsim_sqdiff_sum_f64(const double* a, const double* b, size_t n, double* out, void* stream) {
     if (!a || !b || !out) return fail(CTSIM_EINVAL, "Error: null argument.");
     HIP_TRY(sqdiff_launch(a, b, n, out, (hipStream_t)stream), "sqdiff");
     return CTSIM_OK;

@@ -34,13 +34,13 @@ enum Foot { FOOT_DD = 0, FOOT_TF = 1 };
 // the cone factor ampv, in LDS) and gathered: every row sums the voxels whose v
 // footprint covers it, with the exact clamps of the reference.
 // ---------------------------------------------------------------------------------
-constexpr int kZWin = 512;  // voxels staged per LDS window and wave
+constexpr int kZWin = 256;  // voxels staged per LDS window and wave
 constexpr int kPre = 3;     // voxel loads per lane issued one column ahead (3 x 64 voxels)
 static_assert(kPre == 3, "the pipeline below holds three prefetch registers");
 
 template <class T>
 struct ColParams {
-    int col, iz0, iz1, ilo, ihi, za, zb, pad;
+    int col, cells, za, zb;  // cells = ilo | ihi << 16 (detectors narrower than 32768 cells)
     T amp, r, zinc, vstart, w0, w1, w2, w3, p0, p1, p2, p3;
 };
 
@@ -52,6 +52,34 @@ __device__ __forceinline__ T u_weight(int iu, int ilo, int ihi, T p0, T p1, T p2
     return FOOT == FOOT_DD ? dd_weight(iu, p0, p1, ilo, ihi) : sf_weight(iu, p0, p1, p2, p3);
 }
 
+// sum over the K voxels ja .. ja+K-1 of (overlap with the row) * staged value, for
+// voxels inside the staged window [zw0, zw1]; x_j = j zinc - tt is the voxel's lower
+// edge relative to the row's (both in detector rows), so the overlap with the row
+// [0, 1] is min(x_j + zinc, 1) - max(x_j, 0), the DD rule of dd_overlap
+template <class T, int K>
+__device__ __forceinline__ T gather_row(const T* zwin, int zw0, int zw1, int ja, T tt, T zinc) {
+    T x = T(ja) * zinc - tt;
+    T s = T(0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int j = ja + k;
+        const bool in = j >= zw0 && j <= zw1;
+        const T val = zwin[min(max(j, zw0), zw1) - zw0];
+        const T hi = x + zinc < T(1) ? x + zinc : T(1);
+        const T lo = x > T(0) ? x : T(0);
+        const T w = hi - lo > T(0) ? hi - lo : T(0);
+        s = in ? s + w * val : s;
+        x += zinc;
+    }
+    return s;
+}
+
+// x added to an LDS word that only the calling lane touches (ds_add, no return value)
+template <class T>
+__device__ __forceinline__ void lds_add(T* p, T x) {
+    __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 template <class Tin, class Tout, class T, int FOOT, int TU, int RPL, int SPLIT>
 __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View<T>* __restrict__ views,
                                                            int vstride, int nviews, const Tin* __restrict__ vol,
@@ -61,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
     static_assert(RPL == 1, "one detector row per lane");
     __shared__ T zbuf[kWavesPerBlock][kZWin];
     __shared__ ColParams<T> cbuf[kWavesPerBlock][kWave];
-    __shared__ T part[SPLIT == 1 ? 1 : SPLIT - 1][TU * RPL][kWave];
+    __shared__ T accs[kWavesPerBlock][TU][kWave];  // tile accumulators, one column per lane
     const int rows = RPL * kWave;
     const int nut = (g.nu + TU - 1) / TU;
     const int nvc = (g.nv + rows - 1) / rows;
@@ -88,11 +116,9 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
     T* zw = zbuf[wv];
     ColParams<T>* cw = cbuf[wv];
 
-    T acc[RPL][TU];
+    T(*accw)[kWave] = accs[wv];
 #pragma unroll
-    for (int q = 0; q < RPL; ++q)
-#pragma unroll
-        for (int t = 0; t < TU; ++t) acc[q][t] = T(0);
+    for (int t = 0; t < TU; ++t) accw[t][lane] = T(0);
 
     for (int m0 = m_begin; m0 < m_end; m0 += kWave) {
         const int m = m0 + lane;
@@ -130,22 +156,23 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                     const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
                     const Column<T> cg = column_geom(xc, yc, v, g.sad, g.dx);
                     T clo, chi;
+                    int ilo = 0, ihi = -1, iz0 = 0, iz1 = 0;
                     if (FOOT == FOOT_DD) {
                         T umin, umax;
                         dd_bounds(cg, v, g, umin, umax);
-                        ok = clamp_cells(umin, umax, g.nu, 16, clo, chi, cp.ilo, cp.ihi);
+                        ok = clamp_cells(umin, umax, g.nu, 16, clo, chi, ilo, ihi);
                         cp.p0 = clo;
                         cp.p1 = chi;
                     } else {
                         T tau[4];
                         sf_taus(cg, v, g, tau);
-                        ok = clamp_cells(tau[0], tau[3], g.nu, 16, clo, chi, cp.ilo, cp.ihi);
+                        ok = clamp_cells(tau[0], tau[3], g.nu, 16, clo, chi, ilo, ihi);
                         cp.p0 = tau[0]; cp.p1 = tau[1]; cp.p2 = tau[2]; cp.p3 = tau[3];
                     }
-                    ok = ok && cp.ihi >= iu_a && cp.ilo <= iu_b;
+                    ok = ok && ihi >= iu_a && ilo <= iu_b;
+                    cp.cells = ilo | (ihi << 16);
                     if (ok) {
-                        ok = v_window(g, cg.d, v.wz, cp.zinc, cp.vstart, cp.iz0, cp.iz1) && cp.iz0 < cp.iz1 &&
-                             cp.zinc > T(0);
+                        ok = v_window(g, cg.d, v.wz, cp.zinc, cp.vstart, iz0, iz1) && iz0 < iz1 && cp.zinc > T(0);
                         cp.amp = cg.amp;
                         cp.r = cg.r;
                     }
@@ -153,17 +180,17 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                         // voxel range needed by this wave's rows (exact ranges; a voxel left
                         // out at the rounding edge has a weight of the order of the rounding)
                         const T rz = frcp(cp.zinc);
-                        cp.za = max((int)ceil((T(iv_a) - T(0.5) - cp.vstart) * rz - T(1)), cp.iz0);
-                        cp.zb = min((int)ceil((T(iv_last) + T(0.5) - cp.vstart) * rz) - 1, cp.iz1 - 1);
+                        cp.za = max((int)ceil((T(iv_a) - T(0.5) - cp.vstart) * rz - T(1)), iz0);
+                        cp.zb = min((int)ceil((T(iv_last) + T(0.5) - cp.vstart) * rz) - 1, iz1 - 1);
                         ok = cp.za <= cp.zb;
                     }
                     if (ok) {
                         // tile weights of the first four footprint cells inside the tile
-                        const int c0 = max(cp.ilo, iu_a);
-                        cp.w0 = u_weight<T, FOOT>(c0 + 0, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
-                        cp.w1 = u_weight<T, FOOT>(c0 + 1, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
-                        cp.w2 = u_weight<T, FOOT>(c0 + 2, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
-                        cp.w3 = u_weight<T, FOOT>(c0 + 3, cp.ilo, cp.ihi, cp.p0, cp.p1, cp.p2, cp.p3);
+                        const int c0 = max(ilo, iu_a);
+                        cp.w0 = u_weight<T, FOOT>(c0 + 0, ilo, ihi, cp.p0, cp.p1, cp.p2, cp.p3);
+                        cp.w1 = u_weight<T, FOOT>(c0 + 1, ilo, ihi, cp.p0, cp.p1, cp.p2, cp.p3);
+                        cp.w2 = u_weight<T, FOOT>(c0 + 2, ilo, ihi, cp.p0, cp.p1, cp.p2, cp.p3);
+                        cp.w3 = u_weight<T, FOOT>(c0 + 3, ilo, ihi, cp.p0, cp.p1, cp.p2, cp.p3);
                         cp.col = iy * g.nx + ix;
                     }
                 }
@@ -185,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                 }
                 while (cur >= 0) {
                     const ColParams<T>& c = cw[cur];
-                    const int za = c.za, zb = c.zb, ilo = c.ilo, ihi = c.ihi;
+                    const int za = c.za, zb = c.zb, ilo = c.cells & 0xffff, ihi = c.cells >> 16;
                     const T amp = c.amp, r = c.r, zinc = c.zinc, vstart = c.vstart;
                     const T inv_r = frcp(r);
                     const T rz = frcp(zinc);
@@ -229,37 +256,51 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                             cur = nxt;
                         }
                         if (iv <= iv_last) {
-                            const T fiv = T(iv);
-                            int ja = (int)ceil((fiv - T(0.5) - vstart) * rz - T(1));
-                            int jb = (int)ceil((fiv + T(0.5) - vstart) * rz) - 1;
-                            ja = max(ja, zw0);
-                            jb = min(jb, zw1);
-                            for (int jj = ja; jj <= jb; ++jj) {
-                                const T vmin = vstart + T(jj) * zinc;
-                                q += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * zw[jj - zw0];
+                            // row iv against the column's voxels: voxel j covers detector rows
+                            // [x_j, x_j + zinc] relative to the row's lower edge, with
+                            // x_j = j zinc - (iv - 1/2 - vstart); at most ceil(1/zinc) + 1 of
+                            // them overlap the row, so thick voxels take a fixed unrolled count
+                            const T tt = T(iv) - T(0.5) - vstart;
+                            const int ja = (int)ceil(tt * rz - T(1));
+                            const T zu = uni(zinc);
+                            if (zu > T(14)) {  // footprints that may reach the 16-cell cap
+                                const int jb = min((int)ceil((T(iv) + T(0.5) - vstart) * rz) - 1, zw1);
+                                for (int jj = max(ja, zw0); jj <= jb; ++jj) {
+                                    const T vmin = vstart + T(jj) * zinc;
+                                    q += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * zw[jj - zw0];
+                                }
+                            } else if (zu >= T(1)) {
+                                q += gather_row<T, 2>(zw, zw0, zw1, ja, tt, zinc);
+                            } else if (zu >= T(0.5)) {
+                                q += gather_row<T, 3>(zw, zw0, zw1, ja, tt, zinc);
+                            } else if (zu >= T(1) / T(3)) {
+                                q += gather_row<T, 4>(zw, zw0, zw1, ja, tt, zinc);
+                            } else {
+                                const int jb = min((int)ceil((T(iv) + T(0.5) - vstart) * rz) - 1, zw1);
+                                for (int jj = max(ja, zw0); jj <= jb; ++jj) {
+                                    const T vmin = vstart + T(jj) * zinc;
+                                    q += dd_overlap(iv, vmin, vmin + zinc) * zw[jj - zw0];
+                                }
                             }
                         }
                         __builtin_amdgcn_wave_barrier();
                     }
-                    const int c0 = max(ilo, iu_a) - iu_a;  // first tile cell of the footprint
-                    const int c1 = min(ihi, iu_b) - iu_a;  // last
+                    // first and last tile cell of the footprint; the same on every lane, so
+                    // the four weighted adds go to registers picked by a uniform branch
+                    const int c0 = uni(max(ilo, iu_a) - iu_a);
+                    const int c1 = uni(min(ihi, iu_b) - iu_a);
                     const T aq = amp * q;
                     if (c1 - c0 < 4) {
-                        const T w0 = c.w0, w1 = c.w1, w2 = c.w2, w3 = c.w3;
-#pragma unroll
-                        for (int t = 0; t < TU; ++t) {
-                            const int d = t - c0;
-                            const T wt = d == 0 ? w0 : d == 1 ? w1 : d == 2 ? w2 : d == 3 ? w3 : T(0);
-                            acc[0][t] += wt * aq;
-                        }
+                        const T w0 = uni(c.w0), w1 = uni(c.w1), w2 = uni(c.w2), w3 = uni(c.w3);
+                        T* a = &accw[c0][lane];
+                        lds_add(a, w0 * aq);
+                        if (c0 + 1 < TU) lds_add(a + kWave, w1 * aq);
+                        if (c0 + 2 < TU) lds_add(a + 2 * kWave, w2 * aq);
+                        if (c0 + 3 < TU) lds_add(a + 3 * kWave, w3 * aq);
                     } else {  // footprint wider than four tile cells
                         const T p0 = c.p0, p1 = c.p1, p2 = c.p2, p3 = c.p3;
-                        for (int t = c0; t <= c1; ++t) {
-                            const T wt = u_weight<T, FOOT>(iu_a + t, ilo, ihi, p0, p1, p2, p3);
-#pragma unroll
-                            for (int tt = 0; tt < TU; ++tt)
-                                if (tt == t) acc[0][tt] += wt * aq;
-                        }
+                        for (int t = c0; t <= c1; ++t)
+                            lds_add(&accw[t][lane], u_weight<T, FOOT>(iu_a + t, ilo, ihi, p0, p1, p2, p3) * aq);
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
@@ -268,42 +309,60 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
     }
     if (SPLIT > 1) {
         // fixed-order combine of the shares: ((s0 + s1) + s2) + s3
-        if (wv > 0) {
-#pragma unroll
-            for (int s = 0; s < RPL; ++s)
-#pragma unroll
-                for (int t = 0; t < TU; ++t) part[wv - 1][s * TU + t][lane] = acc[s][t];
-        }
         __syncthreads();
         if (wv > 0) return;
-#pragma unroll
-        for (int s = 0; s < RPL; ++s)
-#pragma unroll
-            for (int t = 0; t < TU; ++t)
-#pragma unroll
-                for (int w = 0; w < SPLIT - 1; ++w) acc[s][t] += part[w][s * TU + t][lane];
     }
     Tout* out = proj + (size_t)b * g.nu * g.nv;
-#pragma unroll
-    for (int s = 0; s < RPL; ++s) {
-        const int iv = iv_a + s * kWave + lane;
-        if (iv > iv_last) continue;
+    const int iv = iv_a + lane;
+    if (iv <= iv_last) {
 #pragma unroll
         for (int t = 0; t < TU; ++t) {
             const int iu = iu_a + t;
-            if (iu < g.nu) out[(size_t)iu * g.nv + iv] = cvt_store<Tout>(acc[s][t]);
+            // SPLIT = 1: the wave's own tile; otherwise wave 0 combines the block's shares
+            T sum = SPLIT == 1 ? accw[t][lane] : accs[0][t][lane];
+#pragma unroll
+            for (int w = 1; w < SPLIT; ++w) sum += accs[w][t][lane];
+            if (iu < g.nu) out[(size_t)iu * g.nv + iv] = cvt_store<Tout>(sum);
         }
     }
 }
 
 // ---------------------------------------------------------------------------------
 // Matched backprojection: one wavefront per image column, NZL voxels per lane (z
-// fastest).  Per view the column's u footprint is wave-uniform; the wave first combines
-// the footprint's detector columns row by row, s(iv) = sum_u wu * proj(u, iv), into
-// LDS for the rows the column covers, then every voxel gathers its v footprint from
-// LDS.  All views accumulate in registers; one store per voxel.
+// fastest).  The views' footprints of the column are evaluated 64 at a time (one view
+// per lane) into LDS.  Per view the wave then combines the footprint's detector columns
+// row by row, s(iv) = sum_u w_u proj(u, iv), into LDS for the rows the column covers
+// (the next view's rows are already loading), and every voxel gathers its v footprint
+// from LDS.  All views accumulate in registers; one store per voxel.
 // ---------------------------------------------------------------------------------
 constexpr int kRowWin = 1024;  // detector rows staged per wave (nv larger: windowed)
+
+template <class T>
+struct ViewCol {          // one view's footprint on one image column
+    int ilo, nw, r0, r1;  // first footprint cell, cell count, rows covered
+    int iz0, iz1, pad0, pad1;
+    T amp, r, zinc, vstart, wz, w0, w1, w2, w3, p0, p1, p2, p3, padt[3];
+};
+
+// sum over the rows ra .. ra+K-1 inside the staged window [rw, rwe] of the row's
+// overlap with the voxel's edge interval [a, b] times the staged row sum
+template <class T, int K>
+__device__ __forceinline__ T gather_rows(const T* sw, int rw, int rwe, T a, T b) {
+    const int ra = (int)floor(a);
+    T s = T(0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int row = ra + k;
+        const bool in = row >= rw && row <= rwe;
+        const T val = sw[min(max(row, rw), rwe) - rw];
+        const T fr = T(row);
+        const T hi = b < fr + T(1) ? b : fr + T(1);
+        const T lo = a > fr ? a : fr;
+        const T w = hi - lo > T(0) ? hi - lo : T(0);
+        s = in ? s + w * val : s;
+    }
+    return s;
+}
 
 template <class Tin, class Tout, class T, int FOOT, int NZL>
 __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View<T>* __restrict__ views, int vstride,
@@ -311,6 +370,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                                                            const uint8_t* __restrict__ map,
                                                            Tout* __restrict__ vol, T scale, int accumulate) {
     __shared__ T srow[kWavesPerBlock][kRowWin];
+    __shared__ ViewCol<T> vbuf[kWavesPerBlock][kWave];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int wv = (int)(threadIdx.x / kWave);
     const int colid = bid * kWavesPerBlock + wv;
@@ -324,75 +384,157 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
         return;
     }
     T* sw = srow[wv];
+    ViewCol<T>* vw = vbuf[wv];
     const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
     T img[NZL];
 #pragma unroll
     for (int c = 0; c < NZL; ++c) img[c] = T(0);
-    for (int b = 0; b < nviews; ++b) {
-        const View<T> v = views[(size_t)b * vstride];
-        const Column<T> cg = column_geom(xc, yc, v, g.sad, g.dx);
-        T clo, chi, p0, p1, p2 = T(0), p3 = T(0);
-        int ilo, ihi;
-        bool ok;
-        if (FOOT == FOOT_DD) {
-            T umin, umax;
-            dd_bounds(cg, v, g, umin, umax);
-            ok = clamp_cells(umin, umax, g.nu, 16, clo, chi, ilo, ihi);
-            p0 = clo;
-            p1 = chi;
-        } else {
-            T tau[4];
-            sf_taus(cg, v, g, tau);
-            ok = clamp_cells(tau[0], tau[3], g.nu, 16, clo, chi, ilo, ihi);
-            p0 = tau[0]; p1 = tau[1]; p2 = tau[2]; p3 = tau[3];
+    for (int b0 = 0; b0 < nviews; b0 += kWave) {
+        // footprint of view b0 + lane on this column
+        bool ok = b0 + lane < nviews;
+        ViewCol<T> vc{};
+        vc.r = T(1);
+        vc.zinc = T(1);
+        if (ok) {
+            const View<T> v = views[(size_t)(b0 + lane) * vstride];
+            const Column<T> cg = column_geom(xc, yc, v, g.sad, g.dx);
+            T clo, chi;
+            int ilo = 0, ihi = -1;
+            if (FOOT == FOOT_DD) {
+                T umin, umax;
+                dd_bounds(cg, v, g, umin, umax);
+                ok = clamp_cells(umin, umax, g.nu, 16, clo, chi, ilo, ihi);
+                vc.p0 = clo;
+                vc.p1 = chi;
+            } else {
+                T tau[4];
+                sf_taus(cg, v, g, tau);
+                ok = clamp_cells(tau[0], tau[3], g.nu, 16, clo, chi, ilo, ihi);
+                vc.p0 = tau[0]; vc.p1 = tau[1]; vc.p2 = tau[2]; vc.p3 = tau[3];
+            }
+            if (ok)
+                ok = v_window(g, cg.d, v.wz, vc.zinc, vc.vstart, vc.iz0, vc.iz1) && vc.iz0 < vc.iz1 &&
+                     vc.zinc > T(0);
+            if (ok) {
+                // rows touched by voxels iz0 .. iz1-1
+                T a0 = vc.vstart + T(vc.iz0) * vc.zinc + T(0.5);
+                T a1 = vc.vstart + T(vc.iz1 - 1) * vc.zinc + vc.zinc + T(0.5);
+                a0 = a0 > T(0) ? a0 : T(0);
+                a1 = a1 < T(g.nv) ? a1 : T(g.nv);
+                ok = a0 <= a1;
+                vc.r0 = min((int)floor(a0), g.nv - 1);
+                vc.r1 = min((int)floor(a1), g.nv - 1);
+            }
+            if (ok) {
+                vc.ilo = ilo;
+                vc.nw = ihi - ilo + 1;
+                vc.w0 = u_weight<T, FOOT>(ilo + 0, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3);
+                vc.w1 = u_weight<T, FOOT>(ilo + 1, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3);
+                vc.w2 = u_weight<T, FOOT>(ilo + 2, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3);
+                vc.w3 = u_weight<T, FOOT>(ilo + 3, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3);
+                vc.amp = cg.amp;
+                vc.r = cg.r;
+                vc.wz = v.wz;
+                vc.pad0 = b0 + lane;
+            }
         }
-        if (!ok) continue;
-        T zinc, vstart;
-        int iz0, iz1;
-        if (!v_window(g, cg.d, v.wz, zinc, vstart, iz0, iz1) || iz0 >= iz1 || !(zinc > T(0))) continue;
-        // rows touched by voxels iz0 .. iz1-1
-        T a0 = vstart + T(iz0) * zinc + T(0.5), a1 = vstart + T(iz1 - 1) * zinc + zinc + T(0.5);
-        a0 = a0 > T(0) ? a0 : T(0);
-        a1 = a1 < T(g.nv) ? a1 : T(g.nv);
-        if (a0 > a1) continue;
-        const int r0 = min((int)floor(a0), g.nv - 1);
-        const int r1 = min((int)floor(a1), g.nv - 1);
-        const T wu0 = u_weight<T, FOOT>(ilo + 0, ilo, ihi, p0, p1, p2, p3);
-        const T wu1 = u_weight<T, FOOT>(ilo + 1, ilo, ihi, p0, p1, p2, p3);
-        const T wu2 = u_weight<T, FOOT>(ilo + 2, ilo, ihi, p0, p1, p2, p3);
-        const T wu3 = u_weight<T, FOOT>(ilo + 3, ilo, ihi, p0, p1, p2, p3);
-        const int nw = ihi - ilo + 1;
-        const Tin* pv = proj + (size_t)b * g.nu * g.nv;
-        const T inv_r = frcp(cg.r);
-        for (int rw = r0; rw <= r1; rw += kRowWin) {
-            const int rwe = min(r1, rw + kRowWin - 1);
-            // u combination of the footprint's detector columns, row by row
-            for (int row = rw + lane; row <= rwe; row += kWave) {
-                const Tin* pr = pv + (size_t)ilo * g.nv + row;
-                T s = wu0 * ld<T>(pr);
-                if (nw > 1) s += wu1 * ld<T>(pr + g.nv);
-                if (nw > 2) s += wu2 * ld<T>(pr + 2 * g.nv);
-                if (nw > 3) s += wu3 * ld<T>(pr + 3 * g.nv);
-                for (int k = 4; k < nw; ++k)
-                    s += u_weight<T, FOOT>(ilo + k, ilo, ihi, p0, p1, p2, p3) * ld<T>(pr + (size_t)k * g.nv);
-                sw[row - rw] = s;
-            }
-            __builtin_amdgcn_wave_barrier();
+        vw[lane] = vc;
+        __builtin_amdgcn_wave_barrier();
+        unsigned long long mask = __ballot(ok);
+        // the first row chunk of the next view is loaded while the current one is used
+        int cur = mask ? __ffsll((long long)mask) - 1 : -1;
+        if (cur >= 0) mask &= mask - 1;
+        Tin pre0{}, pre1{}, pre2{}, pre3{};
+        if (cur >= 0) {
+            const ViewCol<T>& c = vw[cur];
+            const int nw = uni(c.nw), r0 = uni(c.r0), r1 = uni(c.r1);
+            const Tin* pr = proj + ((size_t)uni(c.pad0) * g.nu + uni(c.ilo)) * g.nv + min(r0 + lane, r1);
+            pre0 = pr[0];
+            if (nw > 1) pre1 = pr[g.nv];
+            if (nw > 2) pre2 = pr[2 * g.nv];
+            if (nw > 3) pre3 = pr[3 * g.nv];
+        }
+        while (cur >= 0) {
+            const ViewCol<T>& c = vw[cur];
+            const int ilo = uni(c.ilo), nw = uni(c.nw), r0 = uni(c.r0), r1 = uni(c.r1);
+            const int iz0 = uni(c.iz0), iz1 = uni(c.iz1), vb = uni(c.pad0);
+            const T w0 = uni(c.w0), w1 = uni(c.w1), w2 = uni(c.w2), w3 = uni(c.w3);
+            const T amp = c.amp, r = c.r, zinc = c.zinc, vstart = c.vstart, wz = c.wz;
+            const T inv_r = frcp(r);
+            const Tin* pv = proj + ((size_t)vb * g.nu + ilo) * g.nv;
+            for (int rw = r0; rw <= r1; rw += kRowWin) {
+                const int rwe = min(r1, rw + kRowWin - 1);
+                // u combination of the footprint's detector columns, row by row
+                int rfirst = rw + lane;
+                if (rw == r0) {
+                    if (rfirst <= rwe) {
+                        T s = w0 * T(to_f(pre0));
+                        if (nw > 1) s += w1 * T(to_f(pre1));
+                        if (nw > 2) s += w2 * T(to_f(pre2));
+                        if (nw > 3) s += w3 * T(to_f(pre3));
+                        for (int k = 4; k < nw; ++k)
+                            s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) *
+                                 ld<T>(pv + (size_t)k * g.nv + rfirst);
+                        sw[rfirst - rw] = s;
+                    }
+                    rfirst += kWave;
+                }
+                for (int row = rfirst; row <= rwe; row += kWave) {
+                    const Tin* pr = pv + row;
+                    T s = w0 * ld<T>(pr);
+                    if (nw > 1) s += w1 * ld<T>(pr + g.nv);
+                    if (nw > 2) s += w2 * ld<T>(pr + 2 * g.nv);
+                    if (nw > 3) s += w3 * ld<T>(pr + 3 * g.nv);
+                    for (int k = 4; k < nw; ++k)
+                        s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) *
+                             ld<T>(pr + (size_t)k * g.nv);
+                    sw[row - rw] = s;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (rw == r0) {
+                    const int nxt = mask ? __ffsll((long long)mask) - 1 : -1;
+                    if (nxt >= 0) {
+                        mask &= mask - 1;
+                        const ViewCol<T>& cn = vw[nxt];
+                        const int nwn = uni(cn.nw), r0n = uni(cn.r0), r1n = uni(cn.r1);
+                        const Tin* pr =
+                            proj + ((size_t)uni(cn.pad0) * g.nu + uni(cn.ilo)) * g.nv + min(r0n + lane, r1n);
+                        pre0 = pr[0];
+                        if (nwn > 1) pre1 = pr[g.nv];
+                        if (nwn > 2) pre2 = pr[2 * g.nv];
+                        if (nwn > 3) pre3 = pr[3 * g.nv];
+                    }
+                    cur = nxt;
+                }
+                const T zu = uni(zinc);
 #pragma unroll
-            for (int c = 0; c < NZL; ++c) {
-                const int iz = c * kWave + lane;
-                if (iz < iz0 || iz >= iz1) continue;
-                const T vmin = vstart + T(iz) * zinc;
-                int vlo, vhi;
-                if (!clamp_cells(vmin, vmin + zinc, g.nv, 16, clo, chi, vlo, vhi)) continue;
-                const int ra = max(vlo, rw), rb = min(vhi, rwe);
-                if (ra > rb) continue;
-                T s = T(0);
-                for (int iv = ra; iv <= rb; ++iv) s += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * sw[iv - rw];
-                const T zcm = (T(iz) + v.wz) * g.dz;
-                img[c] += cg.amp * (fsqrt(cg.r * cg.r + zcm * zcm) * inv_r) * s;
+                for (int cz = 0; cz < NZL; ++cz) {
+                    const int iz = cz * kWave + lane;
+                    if (iz < iz0 || iz >= iz1) continue;
+                    const T vmin = vstart + T(iz) * zinc;
+                    const T a = vmin + T(0.5), bb = vmin + zinc + T(0.5);
+                    T s;
+                    if (zu <= T(1)) {
+                        s = gather_rows<T, 2>(sw, rw, rwe, a, bb);
+                    } else if (zu <= T(2)) {
+                        s = gather_rows<T, 3>(sw, rw, rwe, a, bb);
+                    } else if (zu <= T(3)) {
+                        s = gather_rows<T, 4>(sw, rw, rwe, a, bb);
+                    } else {
+                        T clo, chi;
+                        int vlo, vhi;
+                        s = T(0);
+                        if (clamp_cells(vmin, vmin + zinc, g.nv, 16, clo, chi, vlo, vhi)) {
+                            const int ra = max(vlo, rw), rb = min(vhi, rwe);
+                            for (int iv = ra; iv <= rb; ++iv)
+                                s += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * sw[iv - rw];
+                        }
+                    }
+                    const T zcm = (T(iz) + wz) * g.dz;
+                    img[cz] += amp * (fsqrt(r * r + zcm * zcm) * inv_r) * s;
+                }
+                __builtin_amdgcn_wave_barrier();
             }
-            __builtin_amdgcn_wave_barrier();
         }
     }
 #pragma unroll
@@ -595,6 +737,7 @@ template <class Tin, class Tout, class T>
 static hipError_t proj3(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* vol,
                         const uint8_t* map, void* proj, int foot, int exhaustive, hipStream_t st) {
     if (nviews <= 0) return hipSuccess;
+    if (g.nu >= 32768) return hipErrorNotSupported;  // footprint cells are packed in 16 bits
     if (foot == FOOT_DD)
         proj3_foot<Tin, Tout, T, FOOT_DD>(g, views, vstride, nviews, vol, map, proj, exhaustive, st);
     else

@@ -214,6 +214,18 @@ __device__ __forceinline__ T dd_overlap16(int i, T lo, T hi, int n) {
     return clo > chi ? T(0) : w;
 }
 
+// dd_overlap16 for footprints at most 14 cells wide: cell i in [0, n) lies inside the
+// clamp range [0, n], so clamping cannot change the overlap, and a footprint narrower
+// than 15 cells never reaches the cap (its last cell, floor(chi), is at most
+// floor(clo) + 15, where the capped rule chi - i equals the plain overlap).
+template <class T>
+__device__ __forceinline__ T dd_overlap(int i, T lo, T hi) {
+    const T a = lo + T(0.5), b = hi + T(0.5);
+    const T fi = T(i);
+    const T w = (b < fi + T(1) ? b : fi + T(1)) - (a > fi ? a : fi);
+    return w > T(0) ? w : T(0);
+}
+
 // v window of a column (cbct_proj.c:569-580): magnified slice pitch zinc,
 // first voxel edge vstart and the voxel range [iz0, iz1).
 template <class T>
@@ -304,6 +316,17 @@ __device__ __forceinline__ double rdl_d(double x, int lane) {
 }
 __device__ __forceinline__ float rdl(float x, int lane) { return rdl_f(x, lane); }
 __device__ __forceinline__ double rdl(double x, int lane) { return rdl_d(x, lane); }
+
+// a value every lane of the wave holds (e.g. read from one LDS address), moved to a
+// scalar register so that comparisons on it become uniform branches
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uni(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
+__device__ __forceinline__ double uni(double x) {
+    long long b = __double_as_longlong(x);
+    int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+    int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 // inclusive prefix sum across the 64 lanes
 __device__ __forceinline__ int wave_incl_scan(int x) {

@@ -77,5 +77,12 @@ hipError_t add_i32_launch(int* p, int inc, hipStream_t st);
 hipError_t sqs_launch(const float* x, float* x_out, const float* g, const float* precom, int nx, int ny, int nz,
                       double beta, double delta, hipStream_t st);
 hipError_t sqdiff_launch(const float* a, const float* b, size_t n, double* out, hipStream_t st);
+hipError_t sqdiff_launch(const double* a, const double* b, size_t n, double* out, hipStream_t st);
+hipError_t huber_launch(const double* x, int nx, int ny, int nz, double delta, int mode, double* out, double* cost,
+                        hipStream_t st);
+hipError_t nesterov_launch(const double* x, const double* z, double* s, const double* g, const double* precom,
+                           double* x_out, double* z_out, double* zcopy, int nx, int ny, int nz, double beta,
+                           double delta, double t, double t_next, const double* tseq, const int* base, int k,
+                           hipStream_t st);
 hipError_t mul_launch(const void* a, int adt, const void* b, int bdt, void* o, int odt, size_t n, hipStream_t st);
 }  // namespace ctsim

@@ -137,10 +137,14 @@ template <class P, class Y, class W, class R>
 __global__ __launch_bounds__(kRB) void residual_kernel(const P* __restrict__ p, const Y* __restrict__ y,
                                                        const W* __restrict__ w, R* __restrict__ r, size_t n,
                                                        double wscale, RedScratch rs, double* __restrict__ phi) {
+    // FP32 arithmetic for FP32 / half storage, FP64 when every operand is FP64
+    using C = typename std::conditional<std::is_same<P, double>::value && std::is_same<Y, double>::value &&
+                                            std::is_same<W, double>::value,
+                                        double, float>::type;
     double acc = 0.0;
     for (size_t i = (size_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (size_t)gridDim.x * kRB) {
-        const float d = ld_f(p + i) - ld_f(y + i);
-        const float wi = ld_f(w + i);
+        const C d = ld<C>(p + i) - ld<C>(y + i);
+        const C wi = ld<C>(w + i);
         r[i] = cvt_store<R>(wi * d);
         acc += (double)wi * (double)d * (double)d;
     }
@@ -161,33 +165,35 @@ struct Nbr {
     float w;
 };
 
-__device__ __forceinline__ float dhuber(float d, float delta) { return d > delta ? delta : (d < -delta ? -delta : d); }
-__device__ __forceinline__ float chuber(float d, float delta) { return fabsf(d) <= delta ? 1.f : 0.f; }
+template <class V>
+__device__ __forceinline__ V dhuber(V d, V delta) { return d > delta ? delta : (d < -delta ? -delta : d); }
+template <class V>
+__device__ __forceinline__ V chuber(V d, V delta) { return fabs_t(d) <= delta ? V(1) : V(0); }
 __device__ __forceinline__ double hcost(double d, double delta) {
     d = fabs(d);
     return d > delta ? delta * d - delta * delta / 2 : d * d / 2;
 }
 
 // S and T of voxel (ix, iy, iz) of volume v (ZXY), neighbours in the reference's order
-template <int MODE>
-__device__ __forceinline__ float huber_point(const float* __restrict__ v, int nx, int ny, int nz, int ix, int iy, int iz,
-                                             float delta) {
-    const float axy = 0.70710678118654752f;  // 1/sqrt(2)
+template <int MODE, class V>
+__device__ __forceinline__ V huber_point(const V* __restrict__ v, int nx, int ny, int nz, int ix, int iy, int iz,
+                                         V delta) {
+    const V axy = V(0.70710678118654752440);  // 1/sqrt(2)
     const size_t sx = nz, sy = (size_t)nx * nz;
     const size_t i = (size_t)iy * sy + (size_t)ix * sx + iz;
-    const float xi = v[i];
-    float acc = 0.f;
-    auto term = [&](bool inside, size_t j, float w) {
+    const V xi = v[i];
+    V acc = V(0);
+    auto term = [&](bool inside, size_t j, V w) {
         if (!inside) return;
-        const float d = xi - v[j];
+        const V d = xi - v[j];
         acc += MODE == 1 ? w * dhuber(d, delta) : w * chuber(d, delta);
     };
     // x neighbours
-    term(ix + 1 < nx, i + sx, 1.f);
-    term(ix > 0, i - sx, 1.f);
+    term(ix + 1 < nx, i + sx, V(1));
+    term(ix > 0, i - sx, V(1));
     // y neighbours
-    term(iy + 1 < ny, i + sy, 1.f);
-    term(iy > 0, i - sy, 1.f);
+    term(iy + 1 < ny, i + sy, V(1));
+    term(iy > 0, i - sy, V(1));
     // diagonal (+1,+1) / (-1,-1)
     term(ix + 1 < nx && iy + 1 < ny, i + sx + sy, axy);
     term(ix > 0 && iy > 0, i - sx - sy, axy);
@@ -195,14 +201,14 @@ __device__ __forceinline__ float huber_point(const float* __restrict__ v, int nx
     term(ix > 0 && iy + 1 < ny, i - sx + sy, axy);
     term(ix + 1 < nx && iy > 0, i + sx - sy, axy);
     // z neighbours
-    term(iz + 1 < nz, i + 1, 0.5f);
-    term(iz > 0, i - 1, 0.5f);
+    term(iz + 1 < nz, i + 1, V(0.5));
+    term(iz > 0, i - 1, V(0.5));
     return acc;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void huber_map_kernel(const float* __restrict__ x, int nx, int ny, int nz, float delta,
-                                                        float* __restrict__ out) {
+template <int MODE, class V>
+__global__ __launch_bounds__(256) void huber_map_kernel(const V* __restrict__ x, int nx, int ny, int nz, V delta,
+                                                        V* __restrict__ out) {
     const size_t n = (size_t)nx * ny * nz;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int iz = (int)(i % nz);
@@ -213,7 +219,8 @@ __global__ __launch_bounds__(256) void huber_map_kernel(const float* __restrict_
 }
 
 // penalty cost: each pair counted once (forward neighbours only)
-__global__ __launch_bounds__(kRB) void huber_cost_kernel(const float* __restrict__ x, int nx, int ny, int nz,
+template <class V>
+__global__ __launch_bounds__(kRB) void huber_cost_kernel(const V* __restrict__ x, int nx, int ny, int nz,
                                                          double delta, RedScratch rs, double* __restrict__ out) {
     const size_t n = (size_t)nx * ny * nz;
     const size_t sx = nz, sy = (size_t)nx * nz;
@@ -234,18 +241,19 @@ __global__ __launch_bounds__(kRB) void huber_cost_kernel(const float* __restrict
     finish_reduction(block_sum(acc), rs, out, 0);
 }
 
-__device__ __forceinline__ float scrub(float v) { return (v < 0.f || !isfinite(v)) ? 0.f : v; }
+template <class V>
+__device__ __forceinline__ V scrub(V v) { return (v < V(0) || !isfinite(v)) ? V(0) : v; }
 
 // fused Nesterov-SQS subset step; reads x, z (stencils) and writes x_out, z_out
 // t and t_next come either from the arguments or, when tseq is not null, from
 // tseq[*base + k] and tseq[*base + k + 1] (device memory, so a captured graph can be
 // replayed across iterations while the momentum sequence advances)
-template <class ZC>
-__global__ __launch_bounds__(256) void nesterov_kernel(const float* __restrict__ x, const float* __restrict__ z,
-                                                       float* __restrict__ s, const float* __restrict__ g,
-                                                       const float* __restrict__ precom, float* __restrict__ x_out,
-                                                       float* __restrict__ z_out, ZC* __restrict__ zcopy, int nx, int ny,
-                                                       int nz, double beta, float delta, double t, double t_next,
+template <class V, class ZC>
+__global__ __launch_bounds__(256) void nesterov_kernel(const V* __restrict__ x, const V* __restrict__ z,
+                                                       V* __restrict__ s, const V* __restrict__ g,
+                                                       const V* __restrict__ precom, V* __restrict__ x_out,
+                                                       V* __restrict__ z_out, ZC* __restrict__ zcopy, int nx, int ny,
+                                                       int nz, double beta, V delta, double t, double t_next,
                                                        const double* __restrict__ tseq, const int* __restrict__ base,
                                                        int k) {
     const size_t n = (size_t)nx * ny * nz;
@@ -259,17 +267,17 @@ __global__ __launch_bounds__(256) void nesterov_kernel(const float* __restrict__
         const int iz = (int)(i % nz);
         const size_t c = i / nz;
         const int ix = (int)(c % nx), iy = (int)(c / nx);
-        const float sz = huber_point<1>(z, nx, ny, nz, ix, iy, iz, delta);
-        const float tx = huber_point<2>(x, nx, ny, nz, ix, iy, iz, delta);
-        const float grad = g[i] + (float)(beta * (double)sz);
+        const V sz = huber_point<1>(z, nx, ny, nz, ix, iy, iz, delta);
+        const V tx = huber_point<2>(x, nx, ny, nz, ix, iy, iz, delta);
+        const V grad = g[i] + (V)(beta * (double)sz);
         const double den = (double)precom[i] + beta * (double)tx;
-        const float du = (float)((double)grad / den);
-        const float xn = z[i] - du;
-        const float sn = s[i] - (float)(t * (double)du);
-        const float zn = (float)(a * (double)xn + b * (double)sn);
+        const V du = (V)((double)grad / den);
+        const V xn = z[i] - du;
+        const V sn = s[i] - (V)(t * (double)du);
+        const V zn = (V)(a * (double)xn + b * (double)sn);
         x_out[i] = scrub(xn);
         s[i] = scrub(sn);
-        const float zs = scrub(zn);
+        const V zs = scrub(zn);
         z_out[i] = zs;
         if (zcopy) zcopy[i] = cvt_store<ZC>(zs);
     }
@@ -291,7 +299,8 @@ __global__ __launch_bounds__(256) void sqs_kernel(const float* __restrict__ x, f
     }
 }
 
-__global__ __launch_bounds__(kRB) void sqdiff_kernel(const float* __restrict__ a, const float* __restrict__ b, size_t n,
+template <class V>
+__global__ __launch_bounds__(kRB) void sqdiff_kernel(const V* __restrict__ a, const V* __restrict__ b, size_t n,
                                                      RedScratch rs, double* __restrict__ out) {
     double acc = 0.0;
     for (size_t i = (size_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (size_t)gridDim.x * kRB) {
@@ -305,6 +314,12 @@ template <class A, class B, class O>
 __global__ void mul_kernel(const A* __restrict__ a, const B* __restrict__ b, O* __restrict__ o, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         o[i] = cvt_store<O>(ld_f(a + i) * ld_f(b + i));
+}
+
+__global__ void mul_f64_kernel(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ o,
+                               size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        o[i] = a[i] * b[i];
 }
 
 // ---------------------------------------------------------------------------------
@@ -384,6 +399,12 @@ hipError_t residual_launch(const void* p, int pdt, const void* y, int ydt, const
     RedScratch rs;
     hipError_t e = scratch(rs);
     if (e != hipSuccess) return e;
+    if (pdt == DT_F64 || ydt == DT_F64 || wdt == DT_F64 || rdt == DT_F64) {
+        if (pdt != DT_F64 || ydt != DT_F64 || wdt != DT_F64 || rdt != DT_F64) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((residual_kernel<double, double, double, double>), dim3(red_grid(n)), dim3(kRB), 0, st,
+                           (const double*)p, (const double*)y, (const double*)w, (double*)r, n, ws, rs, phi);
+        return hipGetLastError();
+    }
     switch (pdt) {
         case DT_F32: return residual_y<float>(p, y, ydt, w, wdt, r, rdt, n, ws, rs, phi, st);
         case DT_F16: return residual_y<__half>(p, y, ydt, w, wdt, r, rdt, n, ws, rs, phi, st);
@@ -392,22 +413,33 @@ hipError_t residual_launch(const void* p, int pdt, const void* y, int ydt, const
     }
 }
 
-hipError_t huber_launch(const float* x, int nx, int ny, int nz, double delta, int mode, float* out, double* cost,
-                        hipStream_t st) {
+template <class V>
+static hipError_t huber_launch_t(const V* x, int nx, int ny, int nz, double delta, int mode, V* out, double* cost,
+                                 hipStream_t st) {
     const size_t n = (size_t)nx * ny * nz;
     if (mode == 1) {
-        hipLaunchKernelGGL(huber_map_kernel<1>, dim3(grid_for(n)), dim3(256), 0, st, x, nx, ny, nz, (float)delta, out);
+        hipLaunchKernelGGL((huber_map_kernel<1, V>), dim3(grid_for(n)), dim3(256), 0, st, x, nx, ny, nz, (V)delta, out);
     } else if (mode == 2) {
-        hipLaunchKernelGGL(huber_map_kernel<2>, dim3(grid_for(n)), dim3(256), 0, st, x, nx, ny, nz, (float)delta, out);
+        hipLaunchKernelGGL((huber_map_kernel<2, V>), dim3(grid_for(n)), dim3(256), 0, st, x, nx, ny, nz, (V)delta, out);
     } else if (mode == 0) {
         RedScratch rs;
         hipError_t e = scratch(rs);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(huber_cost_kernel, dim3(red_grid(n)), dim3(kRB), 0, st, x, nx, ny, nz, delta, rs, cost);
+        hipLaunchKernelGGL(huber_cost_kernel<V>, dim3(red_grid(n)), dim3(kRB), 0, st, x, nx, ny, nz, delta, rs, cost);
     } else {
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t huber_launch(const float* x, int nx, int ny, int nz, double delta, int mode, float* out, double* cost,
+                        hipStream_t st) {
+    return huber_launch_t<float>(x, nx, ny, nz, delta, mode, out, cost, st);
+}
+
+hipError_t huber_launch(const double* x, int nx, int ny, int nz, double delta, int mode, double* out, double* cost,
+                        hipStream_t st) {
+    return huber_launch_t<double>(x, nx, ny, nz, delta, mode, out, cost, st);
 }
 
 hipError_t nesterov_launch(const float* x, const float* z, float* s, const float* g, const float* precom, float* x_out,
@@ -416,11 +448,21 @@ hipError_t nesterov_launch(const float* x, const float* z, float* s, const float
     const size_t n = (size_t)nx * ny * nz;
     const dim3 gr(grid_for(n)), bl(256);
     switch (zcopy ? zcdt : DT_F32) {
-        case DT_F32: hipLaunchKernelGGL(nesterov_kernel<float>, gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (float*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
-        case DT_F16: hipLaunchKernelGGL(nesterov_kernel<__half>, gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__half*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
-        case DT_BF16: hipLaunchKernelGGL(nesterov_kernel<__hip_bfloat16>, gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__hip_bfloat16*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
+        case DT_F32: hipLaunchKernelGGL((nesterov_kernel<float, float>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (float*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
+        case DT_F16: hipLaunchKernelGGL((nesterov_kernel<float, __half>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__half*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
+        case DT_BF16: hipLaunchKernelGGL((nesterov_kernel<float, __hip_bfloat16>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__hip_bfloat16*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t nesterov_launch(const double* x, const double* z, double* s, const double* g, const double* precom,
+                           double* x_out, double* z_out, double* zcopy, int nx, int ny, int nz, double beta,
+                           double delta, double t, double t_next, const double* tseq, const int* base, int k,
+                           hipStream_t st) {
+    const size_t n = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL((nesterov_kernel<double, double>), dim3(grid_for(n)), dim3(256), 0, st, x, z, s, g, precom, x_out,
+                       z_out, zcopy, nx, ny, nz, beta, delta, t, t_next, tseq, base, k);
     return hipGetLastError();
 }
 
@@ -440,12 +482,21 @@ hipError_t sqs_launch(const float* x, float* x_out, const float* g, const float*
     return hipGetLastError();
 }
 
-hipError_t sqdiff_launch(const float* a, const float* b, size_t n, double* out, hipStream_t st) {
+template <class V>
+static hipError_t sqdiff_launch_t(const V* a, const V* b, size_t n, double* out, hipStream_t st) {
     RedScratch rs;
     hipError_t e = scratch(rs);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(sqdiff_kernel, dim3(red_grid(n)), dim3(kRB), 0, st, a, b, n, rs, out);
+    hipLaunchKernelGGL(sqdiff_kernel<V>, dim3(red_grid(n)), dim3(kRB), 0, st, a, b, n, rs, out);
     return hipGetLastError();
+}
+
+hipError_t sqdiff_launch(const float* a, const float* b, size_t n, double* out, hipStream_t st) {
+    return sqdiff_launch_t<float>(a, b, n, out, st);
+}
+
+hipError_t sqdiff_launch(const double* a, const double* b, size_t n, double* out, hipStream_t st) {
+    return sqdiff_launch_t<double>(a, b, n, out, st);
 }
 
 template <class A, class B>
@@ -471,6 +522,12 @@ static hipError_t mul_b(const void* a, const void* b, int bdt, void* o, int odt,
 
 hipError_t mul_launch(const void* a, int adt, const void* b, int bdt, void* o, int odt, size_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
+    if (adt == DT_F64 || bdt == DT_F64 || odt == DT_F64) {
+        if (adt != DT_F64 || bdt != DT_F64 || odt != DT_F64) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(mul_f64_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const double*)a, (const double*)b,
+                           (double*)o, n);
+        return hipGetLastError();
+    }
     switch (adt) {
         case DT_F32: return mul_b<float>(a, b, bdt, o, odt, n, st);
         case DT_F16: return mul_b<__half>(a, b, bdt, o, odt, n, st);

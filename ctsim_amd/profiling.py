@@ -1,0 +1,71 @@
+"""Short, fixed GPU workloads for profiler runs (rocprofv3 kernel traces and counter
+passes), so that a profile names only the kernels it is about.
+
+    python -m ctsim_amd.profiling copy [reps]          # 256 MiB FP32 copy kernel (ctsim_convert)
+    python -m ctsim_amd.profiling pwls-subset [reps]   # proj,tf + back,tf of one 41-view subset,
+                                                       # 512x512x128 volume, 1024x64 detector, FP16
+    python -m ctsim_amd.profiling pair-slab [reps]     # proj,tf + back,tf of 24 of 720 views,
+                                                       # 512x512x256 volume, 1024x384 detector, FP32
+
+The copy workload calibrates the FETCH_SIZE / WRITE_SIZE counters: it reads and writes
+exactly 256 MiB per launch."""
+from __future__ import annotations
+
+import sys
+
+from . import workloads
+from ._lib import F32, call
+from .device import DeviceArray, synchronize
+from .projector import Projector
+
+COPY_ELEMS = 64 * 1024 * 1024  # 256 MiB of float32
+
+
+def copy(reps: int) -> None:
+    a = DeviceArray(COPY_ELEMS, "f32").fill(1.0)
+    b = DeviceArray(COPY_ELEMS, "f32")
+    for _ in range(reps):
+        call("ctsim_convert", a.vp, F32, b.vp, F32, COPY_ELEMS, None)
+    synchronize()
+
+
+def _pair(g, vol, nsub, vstep, dtype, reps):
+    P = Projector(g)
+    x = P.upload_volume(vol, dtype=dtype)
+    s = DeviceArray(P.n_sino(nsub), dtype)
+    b = DeviceArray(P.n_img, "f32")
+    for _ in range(reps):
+        P.proj(x, s, op="proj,tf", view0=0, vstep=vstep, nsub=nsub)
+        P.back(s, b, op="back,tf", view0=0, vstep=vstep, nsub=nsub)
+    synchronize()
+
+
+def pwls_subset(reps: int) -> None:
+    g = workloads.pwls_512x128(with_map=True)
+    vol = workloads.body_phantom(g.reconSize, g.reconSpacing, 10, 7)
+    _pair(g, vol, 41, 24, "f16", reps)
+
+
+def pair_slab(reps: int) -> None:
+    g = workloads.sf_pair_512()
+    vol = workloads.ellipsoid_phantom(g.reconSize, g.reconSpacing, 20, 2)
+    _pair(g, vol, 24, 30, "f32", reps)
+
+
+WORKLOADS = {"copy": copy, "pwls-subset": pwls_subset, "pair-slab": pair_slab}
+
+
+def main(argv=None) -> int:
+    argv = sys.argv[1:] if argv is None else argv
+    if not argv or argv[0] not in WORKLOADS:
+        print("usage: python -m ctsim_amd.profiling {%s} [reps]" % "|".join(WORKLOADS), file=sys.stderr)
+        return 2
+    reps = int(argv[1]) if len(argv) > 1 else 3
+    print("profiling workload", argv[0], "reps", reps, flush=True)
+    WORKLOADS[argv[0]](reps)
+    print("done", argv[0], flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -13,12 +13,14 @@ operators of geom/shell/loadPojectors.m and the Huber penalty of penalty/huberPe
 Everything stays in HBM.  Sinogram-side data (y, w, the residual) is stored in
 subset-major order, so Os(., isub) is a contiguous slice.
 
-Two forms:
+Three forms:
   * plain:  FP32 storage everywhere, one kernel launch per operator call;
   * mixed:  y, w and the residual in FP16 or BF16, the forward projector reads an
             FP16 copy of z written by the update kernel, FP32 footprint sums, FP64
             step denominators and reductions, and each iteration replayed as one
-            captured hipGraph.
+            captured hipGraph;
+  * fp64:   every array, footprint and sum in double (precision="f64"), the accuracy
+            reference the other two are measured against.
 """
 from __future__ import annotations
 
@@ -27,7 +29,7 @@ import math
 
 import numpy as np
 
-from ._lib import F32, call
+from ._lib import F32, F64, call
 from .device import DeviceArray, Graph, Stream, dtype_code
 from .geometry import Geometry
 from .projector import Projector
@@ -46,23 +48,31 @@ def nesterov_t_sequence(n_steps: int, restart_every: int | None = None, steps_pe
 class PwlsOS:
     def __init__(self, geom: Geometry, y: np.ndarray, w: np.ndarray, numos: int, beta: float, delta: float,
                  mixed: bool = True, sino_dtype: str = "f16", zcopy_dtype: str = "f16", graph: bool | None = None,
-                 op: str = "tf", stream: Stream | None = None):
+                 op: str = "tf", stream: Stream | None = None, precision: str | None = None):
         if geom.ndim != 3:
             raise ValueError("PwlsOS drives the 3D cone-beam operators")
+        self.f64 = precision == "f64"
+        if precision not in (None, "f64", "f32", "mixed"):
+            raise ValueError("precision is 'mixed', 'f32' or 'f64'")
+        if precision is not None:
+            mixed = precision == "mixed"
         self.geom = geom
-        self.P = Projector(geom)
+        self.P = Projector(geom, compute="f64" if self.f64 else "f32")
+        self.vdt = F64 if self.f64 else F32  # volume dtype
         self.M = int(numos)
         self.beta = float(beta)
         self.delta = float(delta)
         self.mixed = bool(mixed)
         self.graph_mode = self.mixed if graph is None else bool(graph)
+        if self.f64 and self.graph_mode:
+            raise ValueError("the FP64 form runs without graph capture")
         self.kind = op
         self.stream = stream or Stream()
         nx, ny, nz = geom.reconSize
         self.nxyz = (nx, ny, nz)
         nv, nu, nviews = geom.sino_shape()
         self.cell = nu * nv
-        self.sdt = dtype_code(sino_dtype) if self.mixed else F32
+        self.sdt = dtype_code(sino_dtype) if self.mixed else self.vdt
         self.zdt = dtype_code(zcopy_dtype) if self.mixed else None
         # subset-major permutation of the views
         self.subsets = []
@@ -74,25 +84,27 @@ class PwlsOS:
             order.append(subset_views(nviews, self.M, isub))
             off += ns
         self.order = np.concatenate(order)
-        y = np.asarray(y, np.float32).reshape((nv, nu, nviews), order="F")[:, :, self.order]
-        w = np.asarray(w, np.float32).reshape((nv, nu, nviews), order="F")[:, :, self.order]
+        hdt = np.float64 if self.f64 else np.float32
+        y = np.asarray(y, hdt).reshape((nv, nu, nviews), order="F")[:, :, self.order]
+        w = np.asarray(w, hdt).reshape((nv, nu, nviews), order="F")[:, :, self.order]
         # half-precision storage keeps w / max(w) (counts reach 2e4; w .* d would overflow
         # FP16); the scale returns in the backprojection factor and in the FP64 cost
         self.wscale = float(w.max()) if (self.mixed and w.size and w.max() > 0) else 1.0
-        w = w / np.float32(self.wscale) if self.wscale != 1.0 else w
+        w = w / hdt(self.wscale) if self.wscale != 1.0 else w
         st = self.stream
         self.y = DeviceArray.from_numpy(y.ravel(order="F"), self.sdt, stream=st)
         self.w = DeviceArray.from_numpy(w.ravel(order="F"), self.sdt, stream=st)
         nmax = max(s[2] for s in self.subsets)
         nvox = nx * ny * nz
-        self.p = DeviceArray(nmax * self.cell, F32)  # Aos(z), FP32 so that d = p - y keeps its bits
+        vdt = self.vdt
+        self.p = DeviceArray(nmax * self.cell, vdt)  # Aos(z), FP32 (or FP64) so that d = p - y keeps its bits
         self.r = DeviceArray(nmax * self.cell, self.sdt)  # Os(w) .* d
-        self.x = [DeviceArray(nvox, F32), DeviceArray(nvox, F32)]
-        self.z = [DeviceArray(nvox, F32), DeviceArray(nvox, F32)]
-        self.s = DeviceArray(nvox, F32)
-        self.g = DeviceArray(nvox, F32)
-        self.precom = DeviceArray(nvox, F32)
-        self.xold = DeviceArray(nvox, F32)
+        self.x = [DeviceArray(nvox, vdt), DeviceArray(nvox, vdt)]
+        self.z = [DeviceArray(nvox, vdt), DeviceArray(nvox, vdt)]
+        self.s = DeviceArray(nvox, vdt)
+        self.g = DeviceArray(nvox, vdt)
+        self.precom = DeviceArray(nvox, vdt)
+        self.xold = DeviceArray(nvox, vdt)
         self.zc = DeviceArray(nvox, self.zdt) if self.mixed else None
         self.scal = DeviceArray(4, "f64")  # phi, R, rmsd numerator, spare
         self.cur = 0  # which of the x / z double buffers is current
@@ -126,20 +138,21 @@ class PwlsOS:
     def setup(self, x0: np.ndarray) -> None:
         """x = z = s = x0 (MATLAB order), precom = At(w .* A(ones))."""
         st = self.stream
-        x0d = self.P.upload_volume(np.asarray(x0, np.float32), "f32", stream=st)
+        vdt = self.vdt
+        x0d = self.P.upload_volume(np.asarray(x0, np.float64 if self.f64 else np.float32), vdt, stream=st)
         for a in (self.x[0], self.z[0], self.s):
             a.copy_from(x0d, stream=st)
         if self.zc is not None:
             self.zc.copy_from(self.z[0], stream=st)
-        ones = DeviceArray(self.P.n_img, F32).fill(1.0, stream=st)
-        a = DeviceArray(max(s[2] for s in self.subsets) * self.cell, F32)
-        wa = DeviceArray(a.count, F32)
+        ones = DeviceArray(self.P.n_img, vdt).fill(1.0, stream=st)
+        a = DeviceArray(max(s[2] for s in self.subsets) * self.cell, vdt)
+        wa = DeviceArray(a.count, vdt)
         # At(w .* A(ones)) summed subset by subset: every view is used once, scale 1
         for k, (v0, vs, ns, off) in enumerate(self.subsets):
             self.P.proj(ones, a, f"proj,{self.kind}", v0, vs, ns, stream=st)
-            call("ctsim_mul", a.vp, F32, self._sub_ptr(self.w, off), self.w.dtype, wa.vp, F32, ns * self.cell,
+            call("ctsim_mul", a.vp, vdt, self._sub_ptr(self.w, off), self.w.dtype, wa.vp, vdt, ns * self.cell,
                  self._sv())
-            self.P.back(self._View(wa.ptr, ns * self.cell, F32), self.precom, f"back,{self.kind}", v0, vs, ns,
+            self.P.back(self._View(wa.ptr, ns * self.cell, vdt), self.precom, f"back,{self.kind}", v0, vs, ns,
                         scale=self.wscale, accumulate=k > 0, stream=st)
         st.synchronize()
         self.cur = 0
@@ -162,7 +175,11 @@ class PwlsOS:
         a, b = self.cur, 1 - self.cur
         zc = self.zc.vp if self.zc is not None else None
         zcdt = self.zc.dtype if self.zc is not None else 0
-        if t is None:
+        if self.f64:
+            call("ctsim_pwls_nesterov_update_f64", self.x[a].vp, self.z[a].vp, self.s.vp, self.g.vp,
+                 self.precom.vp, self.x[b].vp, self.z[b].vp, nx, ny, nz, self.beta, self.delta, t, t_next,
+                 self._sv())
+        elif t is None:
             call("ctsim_pwls_nesterov_update_seq", self.x[a].vp, self.z[a].vp, self.s.vp, self.g.vp, self.precom.vp,
                  self.x[b].vp, self.z[b].vp, zc, zcdt, nx, ny, nz, self.beta, self.delta, self._tseq.vp,
                  self._base.vp, k, self._sv())
@@ -184,8 +201,9 @@ class PwlsOS:
                 self._subset_step(k, self.t, t_next)
                 self.t = t_next
         xc = self.x[self.cur]
-        call("ctsim_huber", xc.vp, nx, ny, nz, self.delta, 0, None, C.c_void_p(self.scal.ptr + 8), self._sv())
-        call("ctsim_sqdiff_sum", xc.vp, self.xold.vp, xc.count, C.c_void_p(self.scal.ptr + 16), self._sv())
+        sfx = "_f64" if self.f64 else ""
+        call("ctsim_huber" + sfx, xc.vp, nx, ny, nz, self.delta, 0, None, C.c_void_p(self.scal.ptr + 8), self._sv())
+        call("ctsim_sqdiff_sum" + sfx, xc.vp, self.xold.vp, xc.count, C.c_void_p(self.scal.ptr + 16), self._sv())
         if device_t:
             call("ctsim_add_i32", self._base.vp, self.M, self._sv())
 

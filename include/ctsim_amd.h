@@ -238,7 +238,9 @@ int ctsim_device_count(int32_t* count);
 
 /* r = w .* (p - y) written as dtype r_dtype, and phi += wscale * sum(w .* (p - y).^2) / 2
  * in FP64 at d_phi (statsrecon/pwls/reconPwlsSeNesterovSqs.m:65,95-96).  wscale lets
- * half-precision storage keep w / max(w): the true weights are wscale * w. */
+ * half-precision storage keep w / max(w): the true weights are wscale * w.  FP32
+ * arithmetic for F32/F16/BF16 operands; FP64 arithmetic when all four are F64 (F64
+ * mixed with other dtypes is rejected). */
 int ctsim_pwls_residual(const void* d_p, int32_t p_dtype, const void* d_y, int32_t y_dtype, const void* d_w,
                         int32_t w_dtype, void* d_r, int32_t r_dtype, size_t n, double wscale, double* d_phi,
                         void* stream);
@@ -284,6 +286,16 @@ int ctsim_pwls_sqs_update(const float* d_x, float* d_xnew, const float* d_g, con
 
 /* sum((a - b).^2) in FP64 into d_out (RMSD numerator, reconPwlsSeNesterovSqs.m:103). */
 int ctsim_sqdiff_sum(const float* d_a, const float* d_b, size_t n, double* d_out, void* stream);
+
+/* FP64 forms of the Huber map / cost, the Nesterov-SQS update and the squared-difference
+ * sum, for a fully double-precision loop (accuracy reference); same semantics as the
+ * float32 entry points above, without the z copy. */
+int ctsim_huber_f64(const double* d_x, int32_t nx, int32_t ny, int32_t nz, double delta, int32_t mode, double* d_out,
+                    double* d_cost, void* stream);
+int ctsim_pwls_nesterov_update_f64(const double* d_x, const double* d_z, double* d_s, const double* d_g,
+                                   const double* d_precom, double* d_x_out, double* d_z_out, int32_t nx, int32_t ny,
+                                   int32_t nz, double beta, double delta, double t, double t_next, void* stream);
+int ctsim_sqdiff_sum_f64(const double* d_a, const double* d_b, size_t n, double* d_out, void* stream);
 
 /* out = a .* b elementwise with dtypes (precom = At(w .* A(ones)), reconPwlsSeNesterovSqs.m:42-43) */
 int ctsim_mul(const void* d_a, int32_t a_dtype, const void* d_b, int32_t b_dtype, void* d_out, int32_t out_dtype,

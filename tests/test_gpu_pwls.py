@@ -98,3 +98,19 @@ def test_pwls_mixed_graph_equals_stream_and_is_close_to_plain():
     xc, _, _ = c.run(x0.astype(np.float32), its)
     print("mixed vs plain:", rel_l2(xa, xc))
     assert rel_l2(xa, xc) < 2e-3
+
+
+def test_pwls_fp64_matches_fp64_oracle():
+    """the all-double form (FP64 plan, FP64 elementwise kernels) is the accuracy reference
+    of the mixed form, so it must reproduce the FP64 oracle loop to rounding"""
+    from oracle import pwls_ref as R
+    from ctsim_amd.pwls import PwlsOS
+    g, y, w, x0 = _problem(seed=6)
+    beta, delta, M, its = 50.0, 1e-3, 4, 3
+    want, phis, rmsds = R.nesterov_sqs(y, w, g, beta, delta, its, M, x0)
+    sol = PwlsOS(_geometry(g), y, w, M, beta, delta, precision="f64", graph=False)
+    x, ph, rm = sol.run(x0, its)
+    print("fp64 vs fp64 oracle:", rel_l2(x, want), ph, phis)
+    assert rel_l2(x, want) < 1e-9
+    np.testing.assert_allclose(ph, phis, rtol=1e-9)
+    np.testing.assert_allclose(rm, rmsds, rtol=1e-7)
