@@ -124,7 +124,7 @@ def kernel_times(solver, reps=1):
                 call("ctsim_pwls_nesterov_update", solver.x[a].vp, solver.z[a].vp, solver.s.vp, solver.g.vp,
                      solver.precom.vp, solver.x[b].vp, solver.z[b].vp,
                      solver.zc.vp if solver.zc is not None else None, solver.zc.dtype if solver.zc is not None else 0,
-                     nx, ny, nz, solver.beta, solver.delta, 1.0, 1.0, solver._sv())
+                     nx, ny, nz, solver.beta, solver.delta, solver.pfun, 1.0, 1.0, solver._sv())
             t["update"] += tm.ms
     n = reps * len(solver.subsets)
     return {k: v / n for k, v in t.items()}
@@ -197,6 +197,139 @@ def pair_gups(kinds=("dd", "tf"), reps=2):
     return out
 
 
+def fan_pair_bench(reps=5):
+    """2D DD forward + matched back projection, FP32: 256^2 at 0.8 mm, 10 random ellipses,
+    360 views over 360 degrees, SAD 1000 / SDD 1500 mm, 512 flat bins at 1.0 mm"""
+    from ctsim_amd import workloads as W
+    from ctsim_amd.device import DeviceArray, Stream
+    from ctsim_amd.projector import Projector
+    geom = W.fan_256()
+    P = Projector(geom)
+    st = Stream()
+    img = DeviceArray.from_numpy(W.ellipse_image(256, 0.8, 10, 1).ravel(order="F"), "f32", stream=st)
+    sino = DeviceArray(P.n_sino(), "f32")
+    back = DeviceArray(P.n_img, "f32")
+    P.proj(img, sino, "proj,dd", stream=st)
+    P.back(sino, back, "back,dd", stream=st)
+    tp = tb = 0.0
+    for _ in range(reps):
+        with Timer(st) as a:
+            P.proj(img, sino, "proj,dd", stream=st)
+        with Timer(st) as b:
+            P.back(sino, back, "back,dd", stream=st)
+        tp += a.ms
+        tb += b.ms
+    tp /= reps
+    tb /= reps
+    U = 256 * 256 * 360
+    return dict(workload="fan_dd_256x256_360views_512bins", proj_ms=round(tp, 4), back_ms=round(tb, 4),
+                gups_pair=round(2 * U / ((tp + tb) * 1e-3) / 1e9, 2))
+
+
+def batch2_bench(reps=3):
+    """64 uneven 2D slices (128^2..512^2, 180..984 views, 256..1024 bins) projected and
+    backprojected (DD) in one launch each, FP16 images and sinograms, FP32 accumulation,
+    launch schedule chosen by the autotuner"""
+    from ctsim_amd import workloads as W
+    from ctsim_amd.batch2 import Batch2
+    from ctsim_amd.device import DeviceArray, Stream
+    geoms, imgs = W.batch2_uneven()
+    B = Batch2(geoms)
+    st = Stream()
+    d_img = DeviceArray.from_numpy(B.pack_images(imgs), "f16", stream=st)
+    d_sino = DeviceArray(B.sino_elems, "f16")
+    d_back = DeviceArray(B.img_elems, "f16")
+    default = B.schedule
+    tuned = B.autotune(d_img, d_sino, d_back, "proj,dd", "back,dd", reps=2, stream=st)
+    tp = tb = 0.0
+    for _ in range(reps):
+        with Timer(st) as a:
+            B.proj(d_img, d_sino, "proj,dd", stream=st)
+        with Timer(st) as b:
+            B.back(d_sino, d_back, "back,dd", stream=st)
+        tp += a.ms
+        tb += b.ms
+    tp /= reps
+    tb /= reps
+    U = float(sum(g.reconSize[0] * g.reconSize[1] * g.noViews for g in geoms))
+    log(f"batch2: proj {tp:.2f} ms back {tb:.2f} ms, schedule {B.schedule} (default {default})")
+    return dict(workload="batch2_64_uneven_slices_dd_fp16", slices=len(geoms), pixel_views=U, proj_ms=round(tp, 3),
+                back_ms=round(tb, 3), gups_pair=round(2 * U / ((tp + tb) * 1e-3) / 1e9, 2),
+                schedule={"proj_split": tuned["proj_split"], "back_split": tuned["back_split"],
+                          "default": list(default)})
+
+
+def fdk_bench(reps=2):
+    """FDK at 512^3 / 0.5 mm from 720 views on a flat 1024x768 detector at 0.388 mm:
+    ellipsoid phantom projected on the GPU, Poisson noise at I0 = 1e5, cosine weight and
+    Hann ramp filter on the GPU, filtered views stored in FP32 / FP16 / BF16, FP32
+    back,hpd accumulation.  Each storage form is compared with the all-FP64 FDK (FP64
+    filter and backprojector on the GPU)."""
+    from ctsim_amd import workloads as W
+    from ctsim_amd.device import DeviceArray, Stream
+    from ctsim_amd.fdk import FdkFilter, fdk_scale
+    from ctsim_amd.projector import Projector
+    geom = W.fdk_512()
+    P = Projector(geom)
+    st = Stream()
+    vol = W.ellipsoid_phantom(geom.reconSize, geom.reconSpacing, 20, 3)
+    d_p = DeviceArray(P.n_sino(), "f32")
+    P.proj(P.upload_volume(vol, stream=st), d_p, "proj,tf", stream=st)
+    p = d_p.to_numpy(st)
+    del d_p
+    rng = np.random.default_rng(12)
+    I0 = 1.0e5
+    y = np.empty(p.size, np.float32)
+    step = 1 << 24
+    for a in range(0, p.size, step):
+        c = rng.poisson(I0 * np.exp(-p[a:a + step].astype(np.float64)))
+        y[a:a + step] = -np.log(np.maximum(c, 10) / I0)
+    del p
+    d_y = DeviceArray.from_numpy(y, "f32", stream=st)
+    scale = fdk_scale(geom)
+    U = float(np.prod(geom.reconSize)) * geom.noViews
+    out = {}
+    imgs = {}
+    filt = FdkFilter(geom, "hann")
+    for storage in ("f16", "bf16", "f32"):
+        d_f = DeviceArray(d_y.count, storage)
+        d_v = DeviceArray(P.n_img, "f32")
+        filt(d_y, d_f, stream=st)
+        P.back(d_f, d_v, "back,hpd", scale=scale, stream=st)
+        tf = tb = 0.0
+        for _ in range(reps):
+            with Timer(st) as a:
+                filt(d_y, d_f, stream=st)
+            with Timer(st) as b:
+                P.back(d_f, d_v, "back,hpd", scale=scale, stream=st)
+            tf += a.ms
+            tb += b.ms
+        tf /= reps
+        tb /= reps
+        imgs[storage] = P.download_volume(d_v, stream=st)
+        out[storage] = dict(filter_ms=round(tf, 2), back_ms=round(tb, 2), total_ms=round(tf + tb, 2),
+                            back_gups=round(U / (tb * 1e-3) / 1e9, 1),
+                            back_tflops=round(19.0 * U / (tb * 1e-3) / 1e12, 2))
+        log(f"fdk {storage}: filter {tf:.1f} ms  back {tb:.1f} ms")
+        del d_f, d_v
+    del filt
+    P64 = Projector(geom, compute="f64")
+    d_y64 = DeviceArray(d_y.count, "f64")
+    from ctsim_amd._lib import call
+    call("ctsim_convert", d_y.vp, d_y.dtype, d_y64.vp, d_y64.dtype, d_y.count, C.c_void_p(st.handle))
+    del d_y
+    f64 = FdkFilter(geom, "hann", compute="f64")
+    d_f64 = f64(d_y64, DeviceArray(d_y64.count, "f64"), stream=st)
+    del d_y64
+    d_v64 = DeviceArray(P.n_img, "f64")
+    P64.back(d_f64, d_v64, "back,hpd", scale=scale, stream=st)
+    ref = P64.download_volume(d_v64, stream=st)
+    for storage, img in imgs.items():
+        out[storage]["rel_l2_vs_fp64"] = rel_rmse(img, ref)
+    log("fdk accuracy:", {k: v["rel_l2_vs_fp64"] for k, v in out.items()})
+    return dict(workload="fdk_512x512x512_720views_1024x768det_hann", voxel_views=U, **out)
+
+
 def cpu_baseline(geom, nsample=8):
     """reference CPU projector (oracle/_ref, else the oracle port), one core, on nsample
     evenly spaced views of the PWLS geometry; extrapolated to one iteration (every view
@@ -234,6 +367,8 @@ def main():
     ap.add_argument("--no-pair", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fdk", action="store_true")
+    ap.add_argument("--no-2d", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -324,6 +459,11 @@ def main():
         res["kernel_ms_per_subset_plain"] = {k: round(v, 4) for k, v in kt_plain.items()}
     if not args.no_pair:
         res["pair_512x512x256"] = pair_gups()
+    if not args.no_fdk:
+        res["fdk_512"] = fdk_bench()
+    if not args.no_2d:
+        res["fan_256"] = fan_pair_bench()
+        res["batch2_uneven"] = batch2_bench()
     if not args.no_cpu:
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(geom)

@@ -77,6 +77,9 @@ SIGNATURES = {
     "ctsim_batch2_sizes": (C.c_int, [vp, P_i64, P_i64]),
     "ctsim_batch2_proj": (C.c_int, [vp, cp, vp, i32, vp, i32, vp]),
     "ctsim_batch2_back": (C.c_int, [vp, cp, vp, i32, vp, i32, vp]),
+    "ctsim_batch2_set_schedule": (C.c_int, [vp, i32, i32]),
+    "ctsim_batch2_get_schedule": (C.c_int, [vp, P_i32, P_i32]),
+    "ctsim_batch2_autotune": (C.c_int, [vp, cp, cp, vp, i32, vp, i32, vp, i32, P_f32, vp]),
     "ctsim_malloc": (C.c_int, [C.POINTER(vp), sz]),
     "ctsim_free": (C.c_int, [vp]),
     "ctsim_memcpy_h2d": (C.c_int, [vp, vp, sz, vp]),
@@ -99,16 +102,24 @@ SIGNATURES = {
     "ctsim_device_count": (C.c_int, [P_i32]),
     "ctsim_pwls_residual": (C.c_int, [vp, i32, vp, i32, vp, i32, vp, i32, sz, f64, vp, vp]),
     "ctsim_huber": (C.c_int, [vp, i32, i32, i32, f64, i32, vp, vp, vp]),
-    "ctsim_pwls_nesterov_update": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, f64, f64, f64, f64,
-                                             vp]),
-    "ctsim_pwls_nesterov_update_seq": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, f64, f64, vp,
-                                                 vp, i32, vp]),
+    "ctsim_pwls_nesterov_update": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, f64, f64, i32, f64, f64, vp]),
+    "ctsim_pwls_nesterov_update_seq": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, f64, f64, i32, vp, vp, i32, vp]),
     "ctsim_add_i32": (C.c_int, [vp, i32, vp]),
-    "ctsim_pwls_sqs_update": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, f64, f64, vp]),
+    "ctsim_pwls_sqs_update": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, f64, f64, i32, vp]),
     "ctsim_sqdiff_sum": (C.c_int, [vp, vp, sz, vp, vp]),
     "ctsim_huber_f64": (C.c_int, [vp, i32, i32, i32, f64, i32, vp, vp, vp]),
-    "ctsim_pwls_nesterov_update_f64": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, f64, f64, f64, f64, vp]),
+    "ctsim_pwls_nesterov_update_f64": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, f64, f64, i32, f64, f64, vp]),
     "ctsim_sqdiff_sum_f64": (C.c_int, [vp, vp, sz, vp, vp]),
+    "ctsim_fdk_create": (C.c_int, [C.POINTER(vp), C.POINTER(CtsimGeom), i32, P_f64, i32, P_f32, i32]),
+    "ctsim_fdk_destroy": (C.c_int, [vp]),
+    "ctsim_fdk_filter": (C.c_int, [vp, i32, i32, vp, i32, vp, i32, vp]),
+    "ctsim_poisson_counts": (C.c_int, [vp, i32, sz, f64, C.c_uint64, vp, vp]),
+    "ctsim_log_attenuation": (C.c_int, [vp, sz, f64, f64, vp, i32, vp]),
+    "ctsim_pwls_weights": (C.c_int, [vp, sz, f64, f64, vp, i32, vp]),
+    "ctsim_penalty": (C.c_int, [vp, i32, i32, i32, i32, f64, i32, vp, vp, vp]),
+    "ctsim_penalty_f64": (C.c_int, [vp, i32, i32, i32, i32, f64, i32, vp, vp, vp]),
+    "ctsim_pwls_lalm_sg": (C.c_int, [vp, vp, vp, sz, f64, vp]),
+    "ctsim_pwls_lalm_inner": (C.c_int, [vp, vp, vp, vp, vp, i32, i32, i32, f64, f64, i32, f64, vp]),
     "ctsim_mul": (C.c_int, [vp, i32, vp, i32, vp, i32, sz, vp]),
 }
 

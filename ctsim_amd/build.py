@@ -18,7 +18,7 @@ OUT = os.path.join(HERE, "libctsim_hip.so")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("CTSIM_ARCH", "gfx950")
 
-SOURCES = ["cbct.hip", "fbct.hip", "pwls.hip", "api.cpp"]
+SOURCES = ["cbct.hip", "fbct.hip", "pwls.hip", "fdk.hip", "sim.hip", "api.cpp"]
 FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -44,7 +44,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
-    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lhipfft"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

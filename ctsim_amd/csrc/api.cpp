@@ -529,6 +529,8 @@ struct ctsim_batch2 {
     void* d_slices_proj = nullptr;  // Slice2 table, item0 counted for the forward work list
     void* d_slices_back = nullptr;  // same slices, item0 counted for the back work list
     std::vector<void*> d_views;
+    int proj_split = 1;  // waves per forward work item (strip split)
+    int back_split = 4;  // waves per backprojection work item (view split)
 };
 
 namespace {
@@ -618,9 +620,9 @@ int ctsim_batch2_proj(const ctsim_batch2* b, const char* op, const void* d_img, 
     const int kind = k == OP_PROJ_DD ? 0 : 1;
     hipError_t e = b->compute == CTSIM_F32
                        ? fbct_proj_batch_launch<float>(b->d_slices_proj, b->nslices, b->nitems_proj, d_img, img_dt,
-                                                       d_sino, sino_dt, kind, (hipStream_t)stream)
+                                                       d_sino, sino_dt, kind, b->proj_split, (hipStream_t)stream)
                        : fbct_proj_batch_launch<double>(b->d_slices_proj, b->nslices, b->nitems_proj, d_img, img_dt,
-                                                        d_sino, sino_dt, kind, (hipStream_t)stream);
+                                                        d_sino, sino_dt, kind, b->proj_split, (hipStream_t)stream);
     if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype combination not supported.");
     HIP_TRY(e, "batched forward projection");
     return CTSIM_OK;
@@ -634,11 +636,91 @@ int ctsim_batch2_back(const ctsim_batch2* b, const char* op, const void* d_sino,
     const int kind = k == OP_BACK_DD ? 0 : k == OP_BACK_TF ? 1 : 2;
     hipError_t e = b->compute == CTSIM_F32
                        ? fbct_back_batch_launch<float>(b->d_slices_back, b->nslices, b->nitems_back, d_sino, sino_dt,
-                                                       d_img, img_dt, kind, (hipStream_t)stream)
+                                                       d_img, img_dt, kind, b->back_split, (hipStream_t)stream)
                        : fbct_back_batch_launch<double>(b->d_slices_back, b->nslices, b->nitems_back, d_sino, sino_dt,
-                                                        d_img, img_dt, kind, (hipStream_t)stream);
+                                                        d_img, img_dt, kind, b->back_split, (hipStream_t)stream);
     if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype combination not supported.");
     HIP_TRY(e, "batched backprojection");
+    return CTSIM_OK;
+}
+
+static bool valid_split(int s) { return s == 1 || s == 2 || s == 4; }
+
+int ctsim_batch2_set_schedule(ctsim_batch2* b, int32_t proj_split, int32_t back_split) {
+    if (!b) return fail(CTSIM_EINVAL, "Error: null batch.");
+    if (!valid_split(proj_split) || !valid_split(back_split))
+        return fail(CTSIM_EINVAL, "Error: schedule splits must be 1, 2 or 4.");
+    b->proj_split = proj_split;
+    b->back_split = back_split;
+    return CTSIM_OK;
+}
+
+int ctsim_batch2_get_schedule(const ctsim_batch2* b, int32_t* proj_split, int32_t* back_split) {
+    if (!b) return fail(CTSIM_EINVAL, "Error: null batch.");
+    if (proj_split) *proj_split = b->proj_split;
+    if (back_split) *back_split = b->back_split;
+    return CTSIM_OK;
+}
+
+int ctsim_batch2_autotune(ctsim_batch2* b, const char* proj_op, const char* back_op, const void* d_img,
+                          int32_t img_dt, void* d_sino, int32_t sino_dt, void* d_img_out, int32_t reps, float* ms_out,
+                          void* stream) {
+    if (!b || !d_img || !d_sino || !d_img_out) return fail(CTSIM_EINVAL, "Error: null argument.");
+    if (d_img_out == d_img) return fail(CTSIM_EINVAL, "Error: the tuning output must not alias the input image.");
+    if (reps < 1) reps = 1;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0), "event");
+    hipError_t he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(he, "event");
+    }
+    const int splits[3] = {1, 2, 4};
+    int rc = CTSIM_OK;
+    float best[2] = {1e30f, 1e30f};
+    int pick[2] = {b->proj_split, b->back_split};
+    const int keep[2] = {b->proj_split, b->back_split};
+    for (int pass = 0; pass < 2 && rc == CTSIM_OK; ++pass) {
+        for (int i = 0; i < 3 && rc == CTSIM_OK; ++i) {
+            if (pass == 0) b->proj_split = splits[i];
+            else b->back_split = splits[i];
+            auto run = [&]() {
+                return pass == 0 ? ctsim_batch2_proj(b, proj_op, d_img, img_dt, d_sino, sino_dt, stream)
+                                 : ctsim_batch2_back(b, back_op, d_sino, sino_dt, d_img_out, img_dt, stream);
+            };
+            rc = run();  // warm-up, and the sinogram the back pass reads
+            if (rc) break;
+            if ((he = hipEventRecord(e0, st)) != hipSuccess) { rc = hip_fail(he, "event"); break; }
+            for (int r = 0; r < reps && rc == CTSIM_OK; ++r) rc = run();
+            if (rc) break;
+            float ms = 0.f;
+            if ((he = hipEventRecord(e1, st)) != hipSuccess || (he = hipEventSynchronize(e1)) != hipSuccess ||
+                (he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) {
+                rc = hip_fail(he, "event timing");
+                break;
+            }
+            ms /= reps;
+            if (ms < best[pass]) {
+                best[pass] = ms;
+                pick[pass] = splits[i];
+            }
+        }
+        if (pass == 0) b->proj_split = pick[0];
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        b->proj_split = keep[0];
+        b->back_split = keep[1];
+        return rc;
+    }
+    b->proj_split = pick[0];
+    b->back_split = pick[1];
+    if (ms_out) {
+        ms_out[0] = best[0];
+        ms_out[1] = best[1];
+    }
     return CTSIM_OK;
 }
 
@@ -770,41 +852,63 @@ int ctsim_pwls_residual(const void* p, int32_t pdt, const void* y, int32_t ydt, 
     return CTSIM_OK;
 }
 
-int ctsim_huber(const float* x, int32_t nx, int32_t ny, int32_t nz, double delta, int32_t mode, float* out,
-                double* cost, void* stream) {
+}  // extern "C"
+
+template <class V>
+static int penalty_impl(const V* x, int32_t nx, int32_t ny, int32_t nz, int32_t pfun, double delta, int32_t mode,
+                        V* out, double* cost, void* stream) {
     if (!x || nx <= 0 || ny <= 0 || nz <= 0) return fail(CTSIM_EINVAL, "Error: bad volume.");
     if ((mode == 1 || mode == 2) && !out) return fail(CTSIM_EINVAL, "Error: null output.");
     if (mode == 0 && !cost) return fail(CTSIM_EINVAL, "Error: null cost output.");
-    hipError_t e = huber_launch(x, nx, ny, nz, delta, mode, out, cost, (hipStream_t)stream);
+    if (pfun < 0 || pfun > 2) return fail(CTSIM_EINVAL, "Error: unknow penalty function");
+    hipError_t e = penalty_launch(x, nx, ny, nz, pfun, delta, mode, out, cost, (hipStream_t)stream);
     if (e == hipErrorInvalidValue) return fail(CTSIM_EINVAL, "Error: wrong penalty mode.");
-    HIP_TRY(e, "huber");
+    HIP_TRY(e, "penalty");
     return CTSIM_OK;
+}
+
+extern "C" {
+
+int ctsim_penalty(const float* x, int32_t nx, int32_t ny, int32_t nz, int32_t pfun, double delta, int32_t mode,
+                  float* out, double* cost, void* stream) {
+    return penalty_impl<float>(x, nx, ny, nz, pfun, delta, mode, out, cost, stream);
+}
+
+int ctsim_penalty_f64(const double* x, int32_t nx, int32_t ny, int32_t nz, int32_t pfun, double delta, int32_t mode,
+                      double* out, double* cost, void* stream) {
+    return penalty_impl<double>(x, nx, ny, nz, pfun, delta, mode, out, cost, stream);
+}
+
+int ctsim_huber(const float* x, int32_t nx, int32_t ny, int32_t nz, double delta, int32_t mode, float* out,
+                double* cost, void* stream) {
+    return penalty_impl<float>(x, nx, ny, nz, 0, delta, mode, out, cost, stream);
 }
 
 int ctsim_pwls_nesterov_update(const float* x, const float* z, float* s, const float* g, const float* precom,
                                float* x_out, float* z_out, void* zcopy, int32_t zcopy_dtype, int32_t nx, int32_t ny,
-                               int32_t nz, double beta, double delta, double t, double t_next, void* stream) {
+                               int32_t nz, double beta, double delta, int32_t pfun, double t, double t_next,
+                               void* stream) {
     if (!x || !z || !s || !g || !precom || !x_out || !z_out) return fail(CTSIM_EINVAL, "Error: null argument.");
     if (x_out == x || z_out == z || x_out == z || z_out == x)
         return fail(CTSIM_EINVAL, "Error: outputs must not alias the stencil inputs.");
-    hipError_t e = nesterov_launch(x, z, s, g, precom, x_out, z_out, zcopy, zcopy_dtype, nx, ny, nz, beta, delta, t,
-                                   t_next, nullptr, nullptr, 0, (hipStream_t)stream);
-    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: z copy dtype not supported.");
+    hipError_t e = nesterov_launch(x, z, s, g, precom, x_out, z_out, zcopy, zcopy_dtype, nx, ny, nz, beta, delta,
+                                   pfun, t, t_next, nullptr, nullptr, 0, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: z copy dtype or penalty not supported.");
     HIP_TRY(e, "nesterov update");
     return CTSIM_OK;
 }
 
 int ctsim_pwls_nesterov_update_seq(const float* x, const float* z, float* s, const float* g, const float* precom,
                                    float* x_out, float* z_out, void* zcopy, int32_t zcopy_dtype, int32_t nx,
-                                   int32_t ny, int32_t nz, double beta, double delta, const double* d_tseq,
-                                   const int32_t* d_base, int32_t k, void* stream) {
+                                   int32_t ny, int32_t nz, double beta, double delta, int32_t pfun,
+                                   const double* d_tseq, const int32_t* d_base, int32_t k, void* stream) {
     if (!x || !z || !s || !g || !precom || !x_out || !z_out || !d_tseq || !d_base)
         return fail(CTSIM_EINVAL, "Error: null argument.");
     if (x_out == x || z_out == z || x_out == z || z_out == x)
         return fail(CTSIM_EINVAL, "Error: outputs must not alias the stencil inputs.");
-    hipError_t e = nesterov_launch(x, z, s, g, precom, x_out, z_out, zcopy, zcopy_dtype, nx, ny, nz, beta, delta, 1.0,
-                                   1.0, d_tseq, (const int*)d_base, k, (hipStream_t)stream);
-    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: z copy dtype not supported.");
+    hipError_t e = nesterov_launch(x, z, s, g, precom, x_out, z_out, zcopy, zcopy_dtype, nx, ny, nz, beta, delta,
+                                   pfun, 1.0, 1.0, d_tseq, (const int*)d_base, k, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: z copy dtype or penalty not supported.");
     HIP_TRY(e, "nesterov update");
     return CTSIM_OK;
 }
@@ -816,10 +920,30 @@ int ctsim_add_i32(int32_t* d_counter, int32_t increment, void* stream) {
 }
 
 int ctsim_pwls_sqs_update(const float* x, float* x_out, const float* g, const float* precom, int32_t nx, int32_t ny,
-                          int32_t nz, double beta, double delta, void* stream) {
+                          int32_t nz, double beta, double delta, int32_t pfun, void* stream) {
     if (!x || !x_out || !g || !precom) return fail(CTSIM_EINVAL, "Error: null argument.");
     if (x == x_out) return fail(CTSIM_EINVAL, "Error: output must not alias the input.");
-    HIP_TRY(sqs_launch(x, x_out, g, precom, nx, ny, nz, beta, delta, (hipStream_t)stream), "sqs update");
+    hipError_t e = sqs_launch(x, x_out, g, precom, nx, ny, nz, beta, delta, pfun, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EINVAL, "Error: unknow penalty function");
+    HIP_TRY(e, "sqs update");
+    return CTSIM_OK;
+}
+
+int ctsim_pwls_lalm_sg(const float* d_dl, float* d_g, float* d_s, size_t n, double rho, void* stream) {
+    if ((!d_dl || !d_g || !d_s) && n) return fail(CTSIM_EINVAL, "Error: null argument.");
+    if (n) HIP_TRY(lalm_sg_launch(d_dl, d_g, d_s, n, rho, (hipStream_t)stream), "lalm g/s update");
+    return CTSIM_OK;
+}
+
+int ctsim_pwls_lalm_inner(const float* x, const float* xold, const float* s, const float* precom, float* x_out,
+                          int32_t nx, int32_t ny, int32_t nz, double beta, double delta, int32_t pfun, double rho,
+                          void* stream) {
+    if (!x || !xold || !s || !precom || !x_out) return fail(CTSIM_EINVAL, "Error: null argument.");
+    if (x == x_out) return fail(CTSIM_EINVAL, "Error: output must not alias the input.");
+    if (nx <= 0 || ny <= 0 || nz <= 0) return fail(CTSIM_EINVAL, "Error: bad volume.");
+    hipError_t e = lalm_inner_launch(x, xold, s, precom, x_out, nx, ny, nz, beta, delta, pfun, rho, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EINVAL, "Error: unknow penalty function");
+    HIP_TRY(e, "lalm update");
     return CTSIM_OK;
 }
 
@@ -831,25 +955,21 @@ int ctsim_sqdiff_sum(const float* a, const float* b, size_t n, double* out, void
 
 int ctsim_huber_f64(const double* x, int32_t nx, int32_t ny, int32_t nz, double delta, int32_t mode, double* out,
                     double* cost, void* stream) {
-    if (!x || nx <= 0 || ny <= 0 || nz <= 0) return fail(CTSIM_EINVAL, "Error: bad volume.");
-    if ((mode == 1 || mode == 2) && !out) return fail(CTSIM_EINVAL, "Error: null output.");
-    if (mode == 0 && !cost) return fail(CTSIM_EINVAL, "Error: null cost output.");
-    hipError_t e = huber_launch(x, nx, ny, nz, delta, mode, out, cost, (hipStream_t)stream);
-    if (e == hipErrorInvalidValue) return fail(CTSIM_EINVAL, "Error: wrong penalty mode.");
-    HIP_TRY(e, "huber");
-    return CTSIM_OK;
+    return penalty_impl<double>(x, nx, ny, nz, 0, delta, mode, out, cost, stream);
 }
 
 int ctsim_pwls_nesterov_update_f64(const double* x, const double* z, double* s, const double* g,
                                    const double* precom, double* x_out, double* z_out, int32_t nx, int32_t ny,
-                                   int32_t nz, double beta, double delta, double t, double t_next, void* stream) {
+                                   int32_t nz, double beta, double delta, int32_t pfun, double t, double t_next,
+                                   void* stream) {
     if (!x || !z || !s || !g || !precom || !x_out || !z_out) return fail(CTSIM_EINVAL, "Error: null argument.");
     if (nx <= 0 || ny <= 0 || nz <= 0) return fail(CTSIM_EINVAL, "Error: bad volume.");
     if (x_out == x || z_out == z || x_out == z || z_out == x)
         return fail(CTSIM_EINVAL, "Error: outputs must not alias the stencil inputs.");
-    HIP_TRY(nesterov_launch(x, z, s, g, precom, x_out, z_out, nullptr, nx, ny, nz, beta, delta, t, t_next, nullptr,
-                            nullptr, 0, (hipStream_t)stream),
-            "nesterov update");
+    hipError_t e = nesterov_launch(x, z, s, g, precom, x_out, z_out, nx, ny, nz, beta, delta, pfun, t, t_next,
+                                   (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EINVAL, "Error: unknow penalty function");
+    HIP_TRY(e, "nesterov update");
     return CTSIM_OK;
 }
 
@@ -864,6 +984,100 @@ int ctsim_mul(const void* a, int32_t adt, const void* b, int32_t bdt, void* o, i
     hipError_t e = mul_launch(a, adt, b, bdt, o, odt, n, (hipStream_t)stream);
     if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype combination not supported.");
     HIP_TRY(e, "mul");
+    return CTSIM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// FDK pre-processing
+// ---------------------------------------------------------------------------------
+struct ctsim_fdk {
+    FdkPlanImpl* impl = nullptr;
+    int nviews = 0, f64 = 0;
+};
+
+extern "C" {
+
+int ctsim_fdk_create(ctsim_fdk** out, const ctsim_geom* g, int32_t nviews, const double* H, int32_t order,
+                     const float* view_weights, int32_t compute) {
+    if (!out || !g || !H) return fail(CTSIM_EINVAL, "Error: null FDK argument.");
+    *out = nullptr;
+    if (g->nu <= 0 || g->nv <= 0 || nviews <= 0) return fail(CTSIM_EINVAL, "Error: detector sizes must be positive.");
+    if (order < g->nu || (order & 1)) return fail(CTSIM_EINVAL, "Error: filter length must be even and >= nu.");
+    if (compute != CTSIM_F32 && compute != CTSIM_F64)
+        return fail(CTSIM_EINVAL, "Error: compute precision must be CTSIM_F32 or CTSIM_F64.");
+    FdkGeom fg{g->du, g->dv, g->ou, g->ov, g->sad, g->sdd, g->detector_type == 1 ? 1 : 0};
+    ctsim_fdk* f = new (std::nothrow) ctsim_fdk;
+    if (!f) return fail(CTSIM_ENOMEM, "Error: host allocation failed.");
+    f->nviews = nviews;
+    f->f64 = compute == CTSIM_F64;
+    // workspace of at most 1 GiB: views are filtered in chunks that fit it
+    hipError_t e = fdk_plan_create(&f->impl, fg, g->nu, g->nv, nviews, H, order, view_weights, f->f64, size_t(1) << 30);
+    if (e != hipSuccess) {
+        delete f;
+        return hip_fail(e, "FDK plan (device buffers or hipFFT plans)");
+    }
+    *out = f;
+    return CTSIM_OK;
+}
+
+int ctsim_fdk_destroy(ctsim_fdk* f) {
+    if (!f) return CTSIM_OK;
+    fdk_plan_destroy(f->impl);
+    delete f;
+    return CTSIM_OK;
+}
+
+int ctsim_fdk_filter(const ctsim_fdk* f, int32_t view0, int32_t nsub, const void* d_sino, int32_t sino_dtype,
+                     void* d_out, int32_t out_dtype, void* stream) {
+    if (!f || !d_sino || !d_out) return fail(CTSIM_EINVAL, "Error: null FDK argument.");
+    if (nsub <= 0 || view0 < 0 || (long long)view0 + nsub > f->nviews)
+        return fail(CTSIM_EINVAL, "Error: view range outside the FDK plan.");
+    if (d_sino == d_out) return fail(CTSIM_EINVAL, "Error: the filter does not work in place.");
+    hipError_t e = fdk_filter_launch(f->impl, view0, nsub, d_sino, sino_dtype, d_out, out_dtype, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue)
+        return fail(CTSIM_EUNSUP, "Error: dtype combination (sino %d, out %d) not supported for this plan.", sino_dtype,
+                    out_dtype);
+    HIP_TRY(e, "FDK filter");
+    return CTSIM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// data side: counts, log attenuation, PWLS weights
+// ---------------------------------------------------------------------------------
+extern "C" {
+
+int ctsim_poisson_counts(const void* d_p, int32_t p_dtype, size_t n, double i0, uint64_t seed, float* d_counts,
+                         void* stream) {
+    if ((!d_p || !d_counts) && n) return fail(CTSIM_EINVAL, "Error: null argument.");
+    if (!(i0 > 0)) return fail(CTSIM_EINVAL, "Error: the source intensity must be positive.");
+    if (n == 0) return CTSIM_OK;
+    hipError_t e = poisson_launch(d_p, p_dtype, n, i0, (unsigned long long)seed, d_counts, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: line integrals must be F32 or F64.");
+    HIP_TRY(e, "poisson counts");
+    return CTSIM_OK;
+}
+
+int ctsim_log_attenuation(const float* d_counts, size_t n, double i0, double low_count, void* d_y, int32_t y_dtype,
+                          void* stream) {
+    if ((!d_counts || !d_y) && n) return fail(CTSIM_EINVAL, "Error: null argument.");
+    if (!(i0 > 0) || !(low_count > 0)) return fail(CTSIM_EINVAL, "Error: intensities must be positive.");
+    if (n == 0) return CTSIM_OK;
+    hipError_t e = neglog_launch(d_counts, n, i0, low_count, d_y, y_dtype, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EINVAL, "Error: bad dtype.");
+    HIP_TRY(e, "log attenuation");
+    return CTSIM_OK;
+}
+
+int ctsim_pwls_weights(const float* d_counts, size_t n, double r, double v, void* d_w, int32_t w_dtype, void* stream) {
+    if ((!d_counts || !d_w) && n) return fail(CTSIM_EINVAL, "Error: null argument.");
+    if (n == 0) return CTSIM_OK;
+    hipError_t e = weights_launch(d_counts, n, r, v, d_w, w_dtype, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EINVAL, "Error: bad dtype.");
+    HIP_TRY(e, "pwls weights");
     return CTSIM_OK;
 }
 

@@ -52,32 +52,31 @@ __device__ __forceinline__ T u_weight(int iu, int ilo, int ihi, T p0, T p1, T p2
     return FOOT == FOOT_DD ? dd_weight(iu, p0, p1, ilo, ihi) : sf_weight(iu, p0, p1, p2, p3);
 }
 
-// sum over the K voxels ja .. ja+K-1 of (overlap with the row) * staged value, for
-// voxels inside the staged window [zw0, zw1]; x_j = j zinc - tt is the voxel's lower
-// edge relative to the row's (both in detector rows), so the overlap with the row
-// [0, 1] is min(x_j + zinc, 1) - max(x_j, 0), the DD rule of dd_overlap
+// Staged voxels of a window [zw0, zw1] sit at zwin[1 + j - zw0]; zwin[0] and the
+// kZPad words after the last voxel are zero.  gather_row sums, over the K voxels from
+// ja on, (overlap with the row) * staged value, where voxel j covers detector rows
+// [x_j, x_j + zinc] relative to the row's lower edge, x_j = j zinc - tt, and the row is
+// [0, 1]: min(x_j + zinc, 1) - max(x_j, 0), the DD rule of dd_overlap.  ja is clamped
+// to [zw0 - 1, zw1 + 1] so that voxels outside the window read zero words.
+constexpr int kZPad = 4;
+
 template <class T, int K>
 __device__ __forceinline__ T gather_row(const T* zwin, int zw0, int zw1, int ja, T tt, T zinc) {
+    static_assert(K - 1 <= kZPad, "pad covers the voxels past the window");
+    ja = min(max(ja, zw0 - 1), zw1 + 1);
+    const T* pz = zwin + (ja - zw0 + 1);
     T x = T(ja) * zinc - tt;
     T s = T(0);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const int j = ja + k;
-        const bool in = j >= zw0 && j <= zw1;
-        const T val = zwin[min(max(j, zw0), zw1) - zw0];
-        const T hi = x + zinc < T(1) ? x + zinc : T(1);
+        const T xn = x + zinc;
+        const T hi = xn < T(1) ? xn : T(1);
         const T lo = x > T(0) ? x : T(0);
         const T w = hi - lo > T(0) ? hi - lo : T(0);
-        s = in ? s + w * val : s;
-        x += zinc;
+        s += w * pz[k];
+        x = xn;
     }
     return s;
-}
-
-// x added to an LDS word that only the calling lane touches (ds_add, no return value)
-template <class T>
-__device__ __forceinline__ void lds_add(T* p, T x) {
-    __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
 template <class Tin, class Tout, class T, int FOOT, int TU, int RPL, int SPLIT>
@@ -87,9 +86,9 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                                                            Tout* __restrict__ proj, int exhaustive) {
     static_assert(SPLIT == 1 || SPLIT == kWavesPerBlock, "SPLIT is 1 or the block's wave count");
     static_assert(RPL == 1, "one detector row per lane");
-    __shared__ T zbuf[kWavesPerBlock][kZWin];
+    __shared__ T zbuf[kWavesPerBlock][kZWin + 1 + kZPad];
     __shared__ ColParams<T> cbuf[kWavesPerBlock][kWave];
-    __shared__ T accs[kWavesPerBlock][TU][kWave];  // tile accumulators, one column per lane
+    __shared__ T part[SPLIT == 1 ? 1 : SPLIT - 1][TU][kWave];  // shares of waves 1.. for the combine
     const int rows = RPL * kWave;
     const int nut = (g.nu + TU - 1) / TU;
     const int nvc = (g.nv + rows - 1) / rows;
@@ -116,9 +115,12 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
     T* zw = zbuf[wv];
     ColParams<T>* cw = cbuf[wv];
 
-    T(*accw)[kWave] = accs[wv];
+    // this lane's row of the tile, indexed with wave-uniform cell numbers (VGPR-indexed
+    // moves); three spare cells take the weights past the tile's last cell, so the four
+    // adds of a column need no condition
+    T acc[TU + 3];
 #pragma unroll
-    for (int t = 0; t < TU; ++t) accw[t][lane] = T(0);
+    for (int t = 0; t < TU + 3; ++t) acc[t] = T(0);
 
     for (int m0 = m_begin; m0 < m_end; m0 += kWave) {
         const int m = m0 + lane;
@@ -231,15 +233,16 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                                 const int zz = za + i * kWave + lane;
                                 if (zz <= zw1) {
                                     const T zc = (T(zz) + v.wz) * g.dz;
-                                    zw[zz - zw0] = fsqrt(r * r + zc * zc) * inv_r * T(to_f(pv[i]));
+                                    zw[zz - zw0 + 1] = fsqrt(r * r + zc * zc) * inv_r * T(to_f(pv[i]));
                                 }
                             }
                             zfirst = za + kPre * kWave + lane;
                         }
                         for (int zz = zfirst; zz <= zw1; zz += kWave) {
                             const T zc = (T(zz) + v.wz) * g.dz;
-                            zw[zz - zw0] = fsqrt(r * r + zc * zc) * inv_r * ld<T>(colp + zz);
+                            zw[zz - zw0 + 1] = fsqrt(r * r + zc * zc) * inv_r * ld<T>(colp + zz);
                         }
+                        if (lane <= kZPad) zw[lane == 0 ? 0 : zw1 - zw0 + 1 + lane] = T(0);  // zero pads
                         __builtin_amdgcn_wave_barrier();
                         if (zw0 == za) {
                             // the next column's voxel loads fly while this one is gathered
@@ -267,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                                 const int jb = min((int)ceil((T(iv) + T(0.5) - vstart) * rz) - 1, zw1);
                                 for (int jj = max(ja, zw0); jj <= jb; ++jj) {
                                     const T vmin = vstart + T(jj) * zinc;
-                                    q += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * zw[jj - zw0];
+                                    q += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * zw[jj - zw0 + 1];
                                 }
                             } else if (zu >= T(1)) {
                                 q += gather_row<T, 2>(zw, zw0, zw1, ja, tt, zinc);
@@ -279,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                                 const int jb = min((int)ceil((T(iv) + T(0.5) - vstart) * rz) - 1, zw1);
                                 for (int jj = max(ja, zw0); jj <= jb; ++jj) {
                                     const T vmin = vstart + T(jj) * zinc;
-                                    q += dd_overlap(iv, vmin, vmin + zinc) * zw[jj - zw0];
+                                    q += dd_overlap(iv, vmin, vmin + zinc) * zw[jj - zw0 + 1];
                                 }
                             }
                         }
@@ -292,15 +295,14 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                     const T aq = amp * q;
                     if (c1 - c0 < 4) {
                         const T w0 = uni(c.w0), w1 = uni(c.w1), w2 = uni(c.w2), w3 = uni(c.w3);
-                        T* a = &accw[c0][lane];
-                        lds_add(a, w0 * aq);
-                        if (c0 + 1 < TU) lds_add(a + kWave, w1 * aq);
-                        if (c0 + 2 < TU) lds_add(a + 2 * kWave, w2 * aq);
-                        if (c0 + 3 < TU) lds_add(a + 3 * kWave, w3 * aq);
+                        acc[c0] += w0 * aq;
+                        acc[c0 + 1] += w1 * aq;
+                        acc[c0 + 2] += w2 * aq;
+                        acc[c0 + 3] += w3 * aq;
                     } else {  // footprint wider than four tile cells
                         const T p0 = c.p0, p1 = c.p1, p2 = c.p2, p3 = c.p3;
                         for (int t = c0; t <= c1; ++t)
-                            lds_add(&accw[t][lane], u_weight<T, FOOT>(iu_a + t, ilo, ihi, p0, p1, p2, p3) * aq);
+                            acc[t] += u_weight<T, FOOT>(iu_a + t, ilo, ihi, p0, p1, p2, p3) * aq;
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
@@ -309,8 +311,16 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
     }
     if (SPLIT > 1) {
         // fixed-order combine of the shares: ((s0 + s1) + s2) + s3
+        if (wv > 0) {
+#pragma unroll
+            for (int t = 0; t < TU; ++t) part[wv - 1][t][lane] = acc[t];
+        }
         __syncthreads();
         if (wv > 0) return;
+#pragma unroll
+        for (int t = 0; t < TU; ++t)
+#pragma unroll
+            for (int w = 0; w < SPLIT - 1; ++w) acc[t] += part[w][t][lane];
     }
     Tout* out = proj + (size_t)b * g.nu * g.nv;
     const int iv = iv_a + lane;
@@ -318,11 +328,7 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
 #pragma unroll
         for (int t = 0; t < TU; ++t) {
             const int iu = iu_a + t;
-            // SPLIT = 1: the wave's own tile; otherwise wave 0 combines the block's shares
-            T sum = SPLIT == 1 ? accw[t][lane] : accs[0][t][lane];
-#pragma unroll
-            for (int w = 1; w < SPLIT; ++w) sum += accs[w][t][lane];
-            if (iu < g.nu) out[(size_t)iu * g.nv + iv] = cvt_store<Tout>(sum);
+            if (iu < g.nu) out[(size_t)iu * g.nv + iv] = cvt_store<Tout>(acc[t]);
         }
     }
 }
@@ -335,7 +341,8 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
 // (the next view's rows are already loading), and every voxel gathers its v footprint
 // from LDS.  All views accumulate in registers; one store per voxel.
 // ---------------------------------------------------------------------------------
-constexpr int kRowWin = 1024;  // detector rows staged per wave (nv larger: windowed)
+constexpr int kRowWin = 1019;  // detector rows staged per wave (nv larger: windowed); with the
+                                // zero pads one wave's row buffer is 1024 words
 
 template <class T>
 struct ViewCol {          // one view's footprint on one image column
@@ -344,22 +351,28 @@ struct ViewCol {          // one view's footprint on one image column
     T amp, r, zinc, vstart, wz, w0, w1, w2, w3, p0, p1, p2, p3, padt[3];
 };
 
-// sum over the rows ra .. ra+K-1 inside the staged window [rw, rwe] of the row's
-// overlap with the voxel's edge interval [a, b] times the staged row sum
+// Staged rows of a window [rw, rwe] sit at sw[1 + row - rw]; sw[0] and the kRowPad
+// words after the last row are zero.  gather_rows sums, over the K rows from
+// ra = floor(a) on, the overlap of the voxel's edge interval [a, b] with the row times
+// the staged value.  ra is clamped to [rw - 1, rwe + 1]: rows outside the window then
+// read the zero words (they belong to another window or lie off the detector), and the
+// overlaps follow the DD rule: row ra takes min(b, ra + 1) - a, row f > ra takes
+// max(min(b, f + 1) - f, 0).  A voxel meets at most K = floor(zinc) + 2 rows.
+constexpr int kRowPad = 4;
+
 template <class T, int K>
 __device__ __forceinline__ T gather_rows(const T* sw, int rw, int rwe, T a, T b) {
-    const int ra = (int)floor(a);
-    T s = T(0);
+    static_assert(K - 1 <= kRowPad, "pad covers the rows past the window");
+    const int ra = min(max((int)floor(a), rw - 1), rwe + 1);
+    const T* pr = sw + (ra - rw + 1);
+    T f = T(ra) + T(1);
+    T s = ((b < f ? b : f) - a) * pr[0];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int row = ra + k;
-        const bool in = row >= rw && row <= rwe;
-        const T val = sw[min(max(row, rw), rwe) - rw];
-        const T fr = T(row);
-        const T hi = b < fr + T(1) ? b : fr + T(1);
-        const T lo = a > fr ? a : fr;
-        const T w = hi - lo > T(0) ? hi - lo : T(0);
-        s = in ? s + w * val : s;
+    for (int k = 1; k < K; ++k) {
+        const T hi = b < f + T(1) ? b : f + T(1);
+        const T w = hi - f > T(0) ? hi - f : T(0);
+        s += w * pr[k];
+        f += T(1);
     }
     return s;
 }
@@ -369,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                                                            int nviews, const Tin* __restrict__ proj,
                                                            const uint8_t* __restrict__ map,
                                                            Tout* __restrict__ vol, T scale, int accumulate) {
-    __shared__ T srow[kWavesPerBlock][kRowWin];
+    __shared__ T srow[kWavesPerBlock][kRowWin + 1 + kRowPad];
     __shared__ ViewCol<T> vbuf[kWavesPerBlock][kWave];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int wv = (int)(threadIdx.x / kWave);
@@ -475,10 +488,11 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                         for (int k = 4; k < nw; ++k)
                             s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) *
                                  ld<T>(pv + (size_t)k * g.nv + rfirst);
-                        sw[rfirst - rw] = s;
+                        sw[rfirst - rw + 1] = s;
                     }
                     rfirst += kWave;
                 }
+                if (lane <= kRowPad) sw[lane == 0 ? 0 : rwe - rw + 1 + lane] = T(0);  // zero pads
                 for (int row = rfirst; row <= rwe; row += kWave) {
                     const Tin* pr = pv + row;
                     T s = w0 * ld<T>(pr);
@@ -488,7 +502,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                     for (int k = 4; k < nw; ++k)
                         s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) *
                              ld<T>(pr + (size_t)k * g.nv);
-                    sw[row - rw] = s;
+                    sw[row - rw + 1] = s;
                 }
                 __builtin_amdgcn_wave_barrier();
                 if (rw == r0) {
@@ -527,7 +541,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                         if (clamp_cells(vmin, vmin + zinc, g.nv, 16, clo, chi, vlo, vhi)) {
                             const int ra = max(vlo, rw), rb = min(vhi, rwe);
                             for (int iv = ra; iv <= rb; ++iv)
-                                s += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * sw[iv - rw];
+                                s += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * sw[iv - rw + 1];
                         }
                     }
                     const T zcm = (T(iz) + wz) * g.dz;

@@ -21,22 +21,25 @@ enum Foot2 { F2_DD = 0, F2_TF = 1 };
 // pixels of the strip are evaluated 64 at a time (one per lane); each surviving pixel
 // is broadcast and every lane adds its cell's share.
 // ---------------------------------------------------------------------------------
-template <class Tin, class Tout, class T, int FOOT>
-__device__ __forceinline__ void fbct_proj_wave(const Geom<T>& g, const View<T>& v, int ut, const Tin* __restrict__ img,
-                                               Tout* __restrict__ out_row, int exhaustive) {
+// share `share` of `nshares` of the strip (its rows or columns along the major axis):
+// returns this lane's cell sum
+template <class Tin, class T, int FOOT>
+__device__ __forceinline__ T fbct_proj_acc(const Geom<T>& g, const View<T>& v, int ut, const Tin* __restrict__ img,
+                                           int exhaustive, int share, int nshares) {
     const int lane = lane_id();
     const int iu_a = ut * kW2;
     const int iu_b = min(iu_a + kW2, g.nu) - 1;
     const int iu = iu_a + lane;
     const bool row_major = fabs_t(v.c) >= fabs_t(v.s);
     const int nmajor = row_major ? g.ny : g.nx;
+    const int m_begin = nmajor * share / nshares, m_end = nmajor * (share + 1) / nshares;
     const T kL = u_to_slope(g, T(iu_a) - T(0.5));
     const T kR = u_to_slope(g, T(iu_b) + T(0.5));
     T acc = T(0);
-    for (int m0 = 0; m0 < nmajor; m0 += kW2) {
+    for (int m0 = m_begin; m0 < m_end; m0 += kW2) {
         const int m = m0 + lane;
         int lo = 0, hi = -1;
-        if (m < nmajor) {
+        if (m < m_end) {
             if (!strip_range(g, v, row_major, m, kL, kR, exhaustive != 0, lo, hi)) hi = lo - 1;
         }
         const int cnt = hi - lo + 1 > 0 ? hi - lo + 1 : 0;
@@ -97,6 +100,14 @@ __device__ __forceinline__ void fbct_proj_wave(const Geom<T>& g, const View<T>& 
             }
         }
     }
+    return acc;
+}
+
+template <class Tin, class Tout, class T, int FOOT>
+__device__ __forceinline__ void fbct_proj_wave(const Geom<T>& g, const View<T>& v, int ut, const Tin* __restrict__ img,
+                                               Tout* __restrict__ out_row, int exhaustive) {
+    const T acc = fbct_proj_acc<Tin, T, FOOT>(g, v, ut, img, exhaustive, 0, 1);
+    const int iu = ut * kW2 + lane_id();
     if (iu < g.nu) out_row[iu] = cvt_store<Tout>(acc);
 }
 
@@ -207,45 +218,82 @@ __device__ __forceinline__ int find_slice(const Slice2<T>* __restrict__ sl, int 
     return lo;
 }
 
-template <class Tin, class Tout, class T, int FOOT>
+// Schedules: the forward pass splits each (slice, view, tile) item's strip over S waves
+// of a block (S = 1, 2, 4; the shares are added in a fixed order); the back pass splits
+// each (slice, row, 64-pixel) item's views over VS waves (VS = 1, 2, 4).  A block is
+// always 4 waves and holds 4/S (4/VS) items.
+template <class T, int S>
+__device__ __forceinline__ T combine_shares(T (*part)[kW2], int wv, int lane) {
+    const int w0 = wv - wv % S;
+    if (S == 1) return part[w0][lane];
+    if (S == 2) return part[w0][lane] + part[w0 + 1][lane];
+    return (part[w0][lane] + part[w0 + 1][lane]) + (part[w0 + 2][lane] + part[w0 + 3][lane]);
+}
+
+template <class Tin, class Tout, class T, int FOOT, int S>
 __global__ __launch_bounds__(256) void fbct_proj_batch_kernel(const Slice2<T>* __restrict__ sl, int nslices,
                                                               int nitems, const Tin* __restrict__ img,
                                                               Tout* __restrict__ proj) {
+    __shared__ T part[4][kW2];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int item = bid * 4 + (int)(threadIdx.x / kW2);
-    if (item >= nitems) return;
-    const int si = find_slice(sl, nslices, item);
+    const int wv = (int)(threadIdx.x / kW2), lane = lane_id();
+    const int item = bid * (4 / S) + wv / S;
+    const bool valid = item < nitems;
+    T acc = T(0);
+    int si = 0, ut = 0, b = 0;
+    if (valid) {
+        si = find_slice(sl, nslices, item);
+        const Slice2<T>& s = sl[si];
+        const int local = item - s.item0;
+        const int nut = (s.g.nu + kW2 - 1) / kW2;
+        ut = local % nut;
+        b = local / nut;
+        acc = fbct_proj_acc<Tin, T, FOOT>(s.g, s.views[b], ut, img + s.img_off, 0, wv % S, S);
+    }
+    if (S > 1) {
+        part[wv][lane] = acc;
+        __syncthreads();
+        if (wv % S) return;
+        acc = combine_shares<T, S>(part, wv, lane);
+    }
+    if (!valid) return;
     const Slice2<T>& s = sl[si];
-    const int local = item - s.item0;
-    const int nut = (s.g.nu + kW2 - 1) / kW2;
-    const int ut = local % nut, b = local / nut;
-    fbct_proj_wave<Tin, Tout, T, FOOT>(s.g, s.views[b], ut, img + s.img_off, proj + s.sino_off + (size_t)b * s.g.nu,
-                                       0);
+    const int iu = ut * kW2 + lane;
+    if (iu < s.g.nu) proj[s.sino_off + (size_t)b * s.g.nu + iu] = cvt_store<Tout>(acc);
 }
 
-template <class Tin, class Tout, class T, int KIND>
+template <class Tin, class Tout, class T, int KIND, int VS>
 __global__ __launch_bounds__(256) void fbct_back_batch_kernel(const Slice2<T>* __restrict__ sl, int nslices,
                                                               int nitems, const Tin* __restrict__ proj,
                                                               Tout* __restrict__ img) {
-    __shared__ T part[3][kW2];
+    __shared__ T part[4][kW2];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    if (bid >= nitems) return;  // uniform per block
-    const int si = find_slice(sl, nslices, bid);
-    const Slice2<T>& s = sl[si];
-    const int local = bid - s.item0;
-    const int nxb = (s.g.nx + kW2 - 1) / kW2;
-    const int iy = local / nxb;
-    const int ix = (local % nxb) * kW2 + lane_id();
-    const int w = threadIdx.x / kW2;
-    const int per = (s.nviews + 3) / 4;
-    const int b0 = min(w * per, s.nviews), b1 = min(b0 + per, s.nviews);
-    const T v = fbct_back_pixel<Tin, T, KIND>(s.g, s.views, 1, b0, b1, ix, iy, proj + s.sino_off);
-    if (w > 0) part[w - 1][lane_id()] = v;
-    __syncthreads();
-    if (w == 0 && ix < s.g.nx) {
-        const T tot = (v + part[0][lane_id()]) + (part[1][lane_id()] + part[2][lane_id()]);
-        img[s.img_off + (size_t)iy * s.g.nx + ix] = cvt_store<Tout>(tot);
+    const int wv = (int)(threadIdx.x / kW2), lane = lane_id();
+    const int item = bid * (4 / VS) + wv / VS;
+    const bool valid = item < nitems;
+    T v = T(0);
+    int si = 0, ix = 0, iy = 0;
+    if (valid) {
+        si = find_slice(sl, nslices, item);
+        const Slice2<T>& s = sl[si];
+        const int local = item - s.item0;
+        const int nxb = (s.g.nx + kW2 - 1) / kW2;
+        iy = local / nxb;
+        ix = (local % nxb) * kW2 + lane;
+        const int share = wv % VS;
+        const int per = (s.nviews + VS - 1) / VS;
+        const int b0 = min(share * per, s.nviews), b1 = min(b0 + per, s.nviews);
+        v = fbct_back_pixel<Tin, T, KIND>(s.g, s.views, 1, b0, b1, ix, iy, proj + s.sino_off);
     }
+    if (VS > 1) {
+        part[wv][lane] = v;
+        __syncthreads();
+        if (wv % VS) return;
+        v = combine_shares<T, VS>(part, wv, lane);
+    }
+    if (!valid) return;
+    const Slice2<T>& s = sl[si];
+    if (ix < s.g.nx) img[s.img_off + (size_t)iy * s.g.nx + ix] = cvt_store<Tout>(v);
 }
 
 }  // namespace ctsim
@@ -341,43 +389,59 @@ hipError_t fbct_back_launch<double>(const Geom<double>& g, const View<double>* v
 
 size_t slice2_size(bool dbl) { return dbl ? sizeof(Slice2<double>) : sizeof(Slice2<float>); }
 
-template <class Tin, class Tout, class T>
-static hipError_t proj2b(const void* sl, int nslices, int nitems, const void* img, void* proj, int kind,
-                         hipStream_t st) {
-    const unsigned blocks = (unsigned)((nitems + 3) / 4);
-    if (blocks == 0) return hipSuccess;
-    if (kind == F2_DD)
-        hipLaunchKernelGGL((fbct_proj_batch_kernel<Tin, Tout, T, F2_DD>), dim3(blocks), dim3(256), 0, st,
-                           (const Slice2<T>*)sl, nslices, nitems, (const Tin*)img, (Tout*)proj);
-    else
-        hipLaunchKernelGGL((fbct_proj_batch_kernel<Tin, Tout, T, F2_TF>), dim3(blocks), dim3(256), 0, st,
-                           (const Slice2<T>*)sl, nslices, nitems, (const Tin*)img, (Tout*)proj);
-    return hipGetLastError();
+template <class Tin, class Tout, class T, int FOOT, int S>
+static void proj2b_s(const void* sl, int nslices, int nitems, const void* img, void* proj, hipStream_t st) {
+    const unsigned blocks = (unsigned)(((long long)nitems * S + 3) / 4);
+    hipLaunchKernelGGL((fbct_proj_batch_kernel<Tin, Tout, T, FOOT, S>), dim3(blocks), dim3(256), 0, st,
+                       (const Slice2<T>*)sl, nslices, nitems, (const Tin*)img, (Tout*)proj);
 }
 
 template <class Tin, class Tout, class T>
-static hipError_t back2b(const void* sl, int nslices, int nitems, const void* proj, void* img, int kind,
+static hipError_t proj2b(const void* sl, int nslices, int nitems, const void* img, void* proj, int kind, int split,
                          hipStream_t st) {
-    const unsigned blocks = (unsigned)nitems;
-    if (blocks == 0) return hipSuccess;
-    if (kind == 0)
-        hipLaunchKernelGGL((fbct_back_batch_kernel<Tin, Tout, T, 0>), dim3(blocks), dim3(256), 0, st,
-                           (const Slice2<T>*)sl, nslices, nitems, (const Tin*)proj, (Tout*)img);
-    else if (kind == 1)
-        hipLaunchKernelGGL((fbct_back_batch_kernel<Tin, Tout, T, 1>), dim3(blocks), dim3(256), 0, st,
-                           (const Slice2<T>*)sl, nslices, nitems, (const Tin*)proj, (Tout*)img);
-    else
-        hipLaunchKernelGGL((fbct_back_batch_kernel<Tin, Tout, T, 2>), dim3(blocks), dim3(256), 0, st,
-                           (const Slice2<T>*)sl, nslices, nitems, (const Tin*)proj, (Tout*)img);
+    if (nitems == 0) return hipSuccess;
+    if (split != 1 && split != 2 && split != 4) return hipErrorInvalidValue;
+#define CT_P2B(F, S) proj2b_s<Tin, Tout, T, F, S>(sl, nslices, nitems, img, proj, st)
+    if (kind == F2_DD) {
+        if (split == 1) CT_P2B(F2_DD, 1); else if (split == 2) CT_P2B(F2_DD, 2); else CT_P2B(F2_DD, 4);
+    } else {
+        if (split == 1) CT_P2B(F2_TF, 1); else if (split == 2) CT_P2B(F2_TF, 2); else CT_P2B(F2_TF, 4);
+    }
+#undef CT_P2B
+    return hipGetLastError();
+}
+
+template <class Tin, class Tout, class T, int KIND, int VS>
+static void back2b_s(const void* sl, int nslices, int nitems, const void* proj, void* img, hipStream_t st) {
+    const unsigned blocks = (unsigned)(((long long)nitems * VS + 3) / 4);
+    hipLaunchKernelGGL((fbct_back_batch_kernel<Tin, Tout, T, KIND, VS>), dim3(blocks), dim3(256), 0, st,
+                       (const Slice2<T>*)sl, nslices, nitems, (const Tin*)proj, (Tout*)img);
+}
+
+template <class Tin, class Tout, class T>
+static hipError_t back2b(const void* sl, int nslices, int nitems, const void* proj, void* img, int kind, int split,
+                         hipStream_t st) {
+    if (nitems == 0) return hipSuccess;
+    if (split != 1 && split != 2 && split != 4) return hipErrorInvalidValue;
+#define CT_B2B(K)                                                                         \
+    do {                                                                                  \
+        if (split == 1) back2b_s<Tin, Tout, T, K, 1>(sl, nslices, nitems, proj, img, st); \
+        else if (split == 2) back2b_s<Tin, Tout, T, K, 2>(sl, nslices, nitems, proj, img, st); \
+        else back2b_s<Tin, Tout, T, K, 4>(sl, nslices, nitems, proj, img, st);            \
+    } while (0)
+    if (kind == 0) CT_B2B(0);
+    else if (kind == 1) CT_B2B(1);
+    else CT_B2B(2);
+#undef CT_B2B
     return hipGetLastError();
 }
 
 template <>
 hipError_t fbct_proj_batch_launch<float>(const void* sl, int nslices, int nitems, const void* img, int img_dt,
-                                         void* proj, int proj_dt, int kind, hipStream_t st) {
-#define OUT_F32(TI) return proj2b<TI, float, float>(sl, nslices, nitems, img, proj, kind, st)
-#define OUT_F16(TI) return proj2b<TI, __half, float>(sl, nslices, nitems, img, proj, kind, st)
-#define OUT_BF16(TI) return proj2b<TI, __hip_bfloat16, float>(sl, nslices, nitems, img, proj, kind, st)
+                                         void* proj, int proj_dt, int kind, int split, hipStream_t st) {
+#define OUT_F32(TI) return proj2b<TI, float, float>(sl, nslices, nitems, img, proj, kind, split, st)
+#define OUT_F16(TI) return proj2b<TI, __half, float>(sl, nslices, nitems, img, proj, kind, split, st)
+#define OUT_BF16(TI) return proj2b<TI, __hip_bfloat16, float>(sl, nslices, nitems, img, proj, kind, split, st)
     switch (proj_dt) {
         case DT_F32: CT_DISPATCH_IN(img_dt, OUT_F32);
         case DT_F16: CT_DISPATCH_IN(img_dt, OUT_F16);
@@ -391,16 +455,16 @@ hipError_t fbct_proj_batch_launch<float>(const void* sl, int nslices, int nitems
 
 template <>
 hipError_t fbct_proj_batch_launch<double>(const void* sl, int nslices, int nitems, const void* img, int img_dt,
-                                          void* proj, int proj_dt, int kind, hipStream_t st) {
+                                          void* proj, int proj_dt, int kind, int split, hipStream_t st) {
     if (img_dt != DT_F64 || proj_dt != DT_F64) return hipErrorInvalidValue;
-    return proj2b<double, double, double>(sl, nslices, nitems, img, proj, kind, st);
+    return proj2b<double, double, double>(sl, nslices, nitems, img, proj, kind, split, st);
 }
 
 template <>
 hipError_t fbct_back_batch_launch<float>(const void* sl, int nslices, int nitems, const void* proj, int proj_dt,
-                                         void* img, int img_dt, int kind, hipStream_t st) {
-#define OUT_F32(TI) return back2b<TI, float, float>(sl, nslices, nitems, proj, img, kind, st)
-#define OUT_F16(TI) return back2b<TI, __half, float>(sl, nslices, nitems, proj, img, kind, st)
+                                         void* img, int img_dt, int kind, int split, hipStream_t st) {
+#define OUT_F32(TI) return back2b<TI, float, float>(sl, nslices, nitems, proj, img, kind, split, st)
+#define OUT_F16(TI) return back2b<TI, __half, float>(sl, nslices, nitems, proj, img, kind, split, st)
     switch (img_dt) {
         case DT_F32: CT_DISPATCH_IN(proj_dt, OUT_F32);
         case DT_F16: CT_DISPATCH_IN(proj_dt, OUT_F16);
@@ -412,9 +476,9 @@ hipError_t fbct_back_batch_launch<float>(const void* sl, int nslices, int nitems
 
 template <>
 hipError_t fbct_back_batch_launch<double>(const void* sl, int nslices, int nitems, const void* proj, int proj_dt,
-                                          void* img, int img_dt, int kind, hipStream_t st) {
+                                          void* img, int img_dt, int kind, int split, hipStream_t st) {
     if (img_dt != DT_F64 || proj_dt != DT_F64) return hipErrorInvalidValue;
-    return back2b<double, double, double>(sl, nslices, nitems, proj, img, kind, st);
+    return back2b<double, double, double>(sl, nslices, nitems, proj, img, kind, split, st);
 }
 
 }  // namespace ctsim

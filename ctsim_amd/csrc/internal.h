@@ -51,10 +51,10 @@ struct Slice2 {
 
 template <class T>
 hipError_t fbct_proj_batch_launch(const void* slices, int nslices, int nitems, const void* img, int img_dt,
-                                  void* proj, int proj_dt, int kind, hipStream_t st);
+                                  void* proj, int proj_dt, int kind, int split, hipStream_t st);
 template <class T>
 hipError_t fbct_back_batch_launch(const void* slices, int nslices, int nitems, const void* proj, int proj_dt,
-                                  void* img, int img_dt, int kind, hipStream_t st);
+                                  void* img, int img_dt, int kind, int split, hipStream_t st);
 size_t slice2_size(bool dbl);
 
 // utility kernels
@@ -68,21 +68,44 @@ namespace ctsim {
 hipError_t reduction_scratch_init();
 hipError_t residual_launch(const void* p, int pdt, const void* y, int ydt, const void* w, int wdt, void* r, int rdt,
                            size_t n, double wscale, double* phi, hipStream_t st);
-hipError_t huber_launch(const float* x, int nx, int ny, int nz, double delta, int mode, float* out, double* cost,
-                        hipStream_t st);
-hipError_t nesterov_launch(const float* x, const float* z, float* s, const float* g, const float* precom, float* x_out,
-                           float* z_out, void* zcopy, int zcdt, int nx, int ny, int nz, double beta, double delta,
-                           double t, double t_next, const double* tseq, const int* base, int k, hipStream_t st);
 hipError_t add_i32_launch(int* p, int inc, hipStream_t st);
-hipError_t sqs_launch(const float* x, float* x_out, const float* g, const float* precom, int nx, int ny, int nz,
-                      double beta, double delta, hipStream_t st);
 hipError_t sqdiff_launch(const float* a, const float* b, size_t n, double* out, hipStream_t st);
 hipError_t sqdiff_launch(const double* a, const double* b, size_t n, double* out, hipStream_t st);
-hipError_t huber_launch(const double* x, int nx, int ny, int nz, double delta, int mode, double* out, double* cost,
-                        hipStream_t st);
-hipError_t nesterov_launch(const double* x, const double* z, double* s, const double* g, const double* precom,
-                           double* x_out, double* z_out, double* zcopy, int nx, int ny, int nz, double beta,
-                           double delta, double t, double t_next, const double* tseq, const int* base, int k,
-                           hipStream_t st);
 hipError_t mul_launch(const void* a, int adt, const void* b, int bdt, void* o, int odt, size_t n, hipStream_t st);
+// FDK pre-processing (fdk.hip)
+struct FdkGeom {
+    float du, dv, ou, ov, sad, sdd;
+    int flat;
+};
+struct FdkPlanImpl;
+hipError_t fdk_plan_create(FdkPlanImpl** out, const FdkGeom& g, int nu, int nv, int nviews, const double* H, int L,
+                           const float* view_weights, int f64, size_t max_workspace);
+void fdk_plan_destroy(FdkPlanImpl* p);
+hipError_t fdk_filter_launch(FdkPlanImpl* p, int view0, int nsub, const void* sino, int sdt, void* out, int odt,
+                             hipStream_t st);
+
+// data side (sim.hip)
+hipError_t poisson_launch(const void* p, int pdt, size_t n, double i0, unsigned long long seed, float* counts,
+                          hipStream_t st);
+hipError_t neglog_launch(const float* counts, size_t n, double i0, double low, void* y, int ydt, hipStream_t st);
+hipError_t weights_launch(const float* s, size_t n, double r, double v, void* w, int wdt, hipStream_t st);
+
+// penalties (pfun 0 Huber, 1 quadratic, 2 hyperbola) and the PWLS updates (pwls.hip)
+hipError_t penalty_launch(const float* x, int nx, int ny, int nz, int pfun, double delta, int mode, float* out,
+                          double* cost, hipStream_t st);
+hipError_t penalty_launch(const double* x, int nx, int ny, int nz, int pfun, double delta, int mode, double* out,
+                          double* cost, hipStream_t st);
+hipError_t nesterov_launch(const float* x, const float* z, float* s, const float* g, const float* precom, float* x_out,
+                           float* z_out, void* zcopy, int zcdt, int nx, int ny, int nz, double beta, double delta,
+                           int pfun, double t, double t_next, const double* tseq, const int* base, int k,
+                           hipStream_t st);
+hipError_t nesterov_launch(const double* x, const double* z, double* s, const double* g, const double* precom,
+                           double* x_out, double* z_out, int nx, int ny, int nz, double beta, double delta, int pfun,
+                           double t, double t_next, hipStream_t st);
+hipError_t sqs_launch(const float* x, float* x_out, const float* g, const float* precom, int nx, int ny, int nz,
+                      double beta, double delta, int pfun, hipStream_t st);
+hipError_t lalm_sg_launch(const float* dl, float* g, float* s, size_t n, double rho, hipStream_t st);
+hipError_t lalm_inner_launch(const float* x, const float* xold, const float* s, const float* precom, float* x_out,
+                             int nx, int ny, int nz, double beta, double delta, int pfun, double rho, hipStream_t st);
+
 }  // namespace ctsim

@@ -153,32 +153,52 @@ __global__ __launch_bounds__(kRB) void residual_kernel(const P* __restrict__ p, 
 }
 
 // ---------------------------------------------------------------------------------
-// Huber penalty on a ZXY volume.  For voxel i with neighbours k (weights 1 along x
-// and y, 1/sqrt(2) on the two xy diagonals, 1/2 along z):
-//   S(x)_i = sum_k w_k clip(x_i - x_k, +-delta)      (mode 1, dHuber)
-//   T(x)_i = sum_k w_k [|x_i - x_k| <= delta]        (mode 2, cHuber)
-//   R(x)   = sum over pairs w_k H(x_i - x_k)          (mode 0, sumHuber)
-// which is what huberPenalty.m builds by adding the pair terms into both voxels.
+// Edge-preserving penalties on a ZXY volume (penalty/huberPenalty.m:101-161,
+// quadPenalty.m, hyperbolaPenalty.m).  For voxel i with neighbours k (weights 1 along x
+// and y, 1/sqrt(2) on the two xy diagonals, 1/2 along z) and potential psi:
+//   S(x)_i = sum_k w_k psi'(x_i - x_k)                (mode 1)
+//   T(x)_i = sum_k w_k c(x_i - x_k)                   (mode 2, the curvature bound)
+//   R(x)   = sum over pairs w_k psi(x_i - x_k)        (mode 0)
+// which is what the reference builds by adding the pair terms into both voxels.
+//   Huber:     psi' = clip(d, +-delta), c = [|d| <= delta], psi = d^2/2 or delta|d| - delta^2/2
+//   quadratic: psi' = d, psi = d^2/2, and T is the constant 4 + 4/sqrt2 (+ 2 * 1/2 in 3D)
+//              of quadPenalty.m for every voxel, boundary voxels included
+//   hyperbola: psi' = d / sqrt(1 + (d/delta)^2), c = (1 + (d/delta)^2)^(-3/2),
+//              psi = delta^2 (sqrt(1 + (d/delta)^2) - 1)
 // ---------------------------------------------------------------------------------
-struct Nbr {
-    int dx, dy, dz;
-    float w;
-};
+enum Pot { POT_HUBER = 0, POT_QUAD = 1, POT_HYPER = 2 };
 
-template <class V>
-__device__ __forceinline__ V dhuber(V d, V delta) { return d > delta ? delta : (d < -delta ? -delta : d); }
-template <class V>
-__device__ __forceinline__ V chuber(V d, V delta) { return fabs_t(d) <= delta ? V(1) : V(0); }
-__device__ __forceinline__ double hcost(double d, double delta) {
-    d = fabs(d);
-    return d > delta ? delta * d - delta * delta / 2 : d * d / 2;
+template <int POT, class V>
+__device__ __forceinline__ V dpot(V d, V delta) {
+    if (POT == POT_HUBER) return d > delta ? delta : (d < -delta ? -delta : d);
+    if (POT == POT_QUAD) return d;
+    const V q = d / delta;
+    return d / sqrt(V(1) + q * q);
+}
+template <int POT, class V>
+__device__ __forceinline__ V cpot(V d, V delta) {
+    if (POT == POT_HUBER) return fabs_t(d) <= delta ? V(1) : V(0);
+    const V q = d / delta;
+    const V r = sqrt(V(1) + q * q);
+    return V(1) / (r * r * r);
+}
+template <int POT>
+__device__ __forceinline__ double cost_pot(double d, double delta) {
+    if (POT == POT_HUBER) {
+        d = fabs(d);
+        return d > delta ? delta * d - delta * delta / 2 : d * d / 2;
+    }
+    if (POT == POT_QUAD) return d * d / 2;
+    const double q = d / delta;
+    return delta * delta * (sqrt(1.0 + q * q) - 1.0);
 }
 
-// S and T of voxel (ix, iy, iz) of volume v (ZXY), neighbours in the reference's order
-template <int MODE, class V>
-__device__ __forceinline__ V huber_point(const V* __restrict__ v, int nx, int ny, int nz, int ix, int iy, int iz,
-                                         V delta) {
+// S (MODE 1) or T (MODE 2) of voxel (ix, iy, iz) of volume v (ZXY)
+template <int MODE, int POT, class V>
+__device__ __forceinline__ V pen_point(const V* __restrict__ v, int nx, int ny, int nz, int ix, int iy, int iz,
+                                       V delta) {
     const V axy = V(0.70710678118654752440);  // 1/sqrt(2)
+    if (MODE == 2 && POT == POT_QUAD) return nz > 1 ? V(4) + V(4) * axy + V(1) : V(4) + V(4) * axy;
     const size_t sx = nz, sy = (size_t)nx * nz;
     const size_t i = (size_t)iy * sy + (size_t)ix * sx + iz;
     const V xi = v[i];
@@ -186,7 +206,7 @@ __device__ __forceinline__ V huber_point(const V* __restrict__ v, int nx, int ny
     auto term = [&](bool inside, size_t j, V w) {
         if (!inside) return;
         const V d = xi - v[j];
-        acc += MODE == 1 ? w * dhuber(d, delta) : w * chuber(d, delta);
+        acc += MODE == 1 ? w * dpot<POT>(d, delta) : w * cpot<POT>(d, delta);
     };
     // x neighbours
     term(ix + 1 < nx, i + sx, V(1));
@@ -206,22 +226,22 @@ __device__ __forceinline__ V huber_point(const V* __restrict__ v, int nx, int ny
     return acc;
 }
 
-template <int MODE, class V>
-__global__ __launch_bounds__(256) void huber_map_kernel(const V* __restrict__ x, int nx, int ny, int nz, V delta,
-                                                        V* __restrict__ out) {
+template <int MODE, int POT, class V>
+__global__ __launch_bounds__(256) void pen_map_kernel(const V* __restrict__ x, int nx, int ny, int nz, V delta,
+                                                      V* __restrict__ out) {
     const size_t n = (size_t)nx * ny * nz;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int iz = (int)(i % nz);
         const size_t c = i / nz;
         const int ix = (int)(c % nx), iy = (int)(c / nx);
-        out[i] = huber_point<MODE>(x, nx, ny, nz, ix, iy, iz, delta);
+        out[i] = pen_point<MODE, POT>(x, nx, ny, nz, ix, iy, iz, delta);
     }
 }
 
 // penalty cost: each pair counted once (forward neighbours only)
-template <class V>
-__global__ __launch_bounds__(kRB) void huber_cost_kernel(const V* __restrict__ x, int nx, int ny, int nz,
-                                                         double delta, RedScratch rs, double* __restrict__ out) {
+template <int POT, class V>
+__global__ __launch_bounds__(kRB) void pen_cost_kernel(const V* __restrict__ x, int nx, int ny, int nz, double delta,
+                                                       RedScratch rs, double* __restrict__ out) {
     const size_t n = (size_t)nx * ny * nz;
     const size_t sx = nz, sy = (size_t)nx * nz;
     const double axy = 0.70710678118654752440;
@@ -231,12 +251,12 @@ __global__ __launch_bounds__(kRB) void huber_cost_kernel(const V* __restrict__ x
         const size_t c = i / nz;
         const int ix = (int)(c % nx), iy = (int)(c / nx);
         const double xi = x[i];
-        if (ix + 1 < nx) acc += hcost(xi - x[i + sx], delta);
-        if (iy + 1 < ny) acc += hcost(xi - x[i + sy], delta);
-        if (ix + 1 < nx && iy + 1 < ny) acc += axy * hcost(xi - x[i + sx + sy], delta);
+        if (ix + 1 < nx) acc += cost_pot<POT>(xi - x[i + sx], delta);
+        if (iy + 1 < ny) acc += cost_pot<POT>(xi - x[i + sy], delta);
+        if (ix + 1 < nx && iy + 1 < ny) acc += axy * cost_pot<POT>(xi - x[i + sx + sy], delta);
         // anti-diagonal pair {(ix, iy), (ix-1, iy+1)}, counted at its (ix, iy) member
-        if (ix > 0 && iy + 1 < ny) acc += axy * hcost(xi - x[i - sx + sy], delta);
-        if (iz + 1 < nz) acc += 0.5 * hcost(xi - x[i + 1], delta);
+        if (ix > 0 && iy + 1 < ny) acc += axy * cost_pot<POT>(xi - x[i - sx + sy], delta);
+        if (iz + 1 < nz) acc += 0.5 * cost_pot<POT>(xi - x[i + 1], delta);
     }
     finish_reduction(block_sum(acc), rs, out, 0);
 }
@@ -248,7 +268,7 @@ __device__ __forceinline__ V scrub(V v) { return (v < V(0) || !isfinite(v)) ? V(
 // t and t_next come either from the arguments or, when tseq is not null, from
 // tseq[*base + k] and tseq[*base + k + 1] (device memory, so a captured graph can be
 // replayed across iterations while the momentum sequence advances)
-template <class V, class ZC>
+template <class V, class ZC, int POT>
 __global__ __launch_bounds__(256) void nesterov_kernel(const V* __restrict__ x, const V* __restrict__ z,
                                                        V* __restrict__ s, const V* __restrict__ g,
                                                        const V* __restrict__ precom, V* __restrict__ x_out,
@@ -267,8 +287,8 @@ __global__ __launch_bounds__(256) void nesterov_kernel(const V* __restrict__ x, 
         const int iz = (int)(i % nz);
         const size_t c = i / nz;
         const int ix = (int)(c % nx), iy = (int)(c / nx);
-        const V sz = huber_point<1>(z, nx, ny, nz, ix, iy, iz, delta);
-        const V tx = huber_point<2>(x, nx, ny, nz, ix, iy, iz, delta);
+        const V sz = pen_point<1, POT>(z, nx, ny, nz, ix, iy, iz, delta);
+        const V tx = pen_point<2, POT>(x, nx, ny, nz, ix, iy, iz, delta);
         const V grad = g[i] + (V)(beta * (double)sz);
         const double den = (double)precom[i] + beta * (double)tx;
         const V du = (V)((double)grad / den);
@@ -283,6 +303,7 @@ __global__ __launch_bounds__(256) void nesterov_kernel(const V* __restrict__ x, 
     }
 }
 
+template <int POT>
 __global__ __launch_bounds__(256) void sqs_kernel(const float* __restrict__ x, float* __restrict__ x_out,
                                                   const float* __restrict__ g, const float* __restrict__ precom, int nx,
                                                   int ny, int nz, double beta, float delta) {
@@ -291,11 +312,45 @@ __global__ __launch_bounds__(256) void sqs_kernel(const float* __restrict__ x, f
         const int iz = (int)(i % nz);
         const size_t c = i / nz;
         const int ix = (int)(c % nx), iy = (int)(c / nx);
-        const float sx = huber_point<1>(x, nx, ny, nz, ix, iy, iz, delta);
-        const float tx = huber_point<2>(x, nx, ny, nz, ix, iy, iz, delta);
+        const float sx = pen_point<1, POT>(x, nx, ny, nz, ix, iy, iz, delta);
+        const float tx = pen_point<2, POT>(x, nx, ny, nz, ix, iy, iz, delta);
         const float num = g[i] + (float)(beta * (double)sx);
         const double den = (double)precom[i] + beta * (double)tx;
         x_out[i] = scrub((float)((double)x[i] - (double)num / den));
+    }
+}
+
+// LALM-OS (statsrecon/pwls/reconPwlsLALMOs.m:66-90): after the subset gradient dL,
+//   g = rho/(rho+1) dL + 1/(rho+1) g;   s = rho dL + (1 - rho) g   (NaN/Inf in s -> 0)
+__global__ __launch_bounds__(256) void lalm_sg_kernel(const float* __restrict__ dl, float* __restrict__ g,
+                                                      float* __restrict__ s, size_t n, double rho) {
+    const float a = (float)(rho / (rho + 1.0)), b = (float)(1.0 / (rho + 1.0)), r = (float)rho,
+                c = (float)(1.0 - rho);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float gi = a * dl[i] + b * g[i];
+        g[i] = gi;
+        const float si = r * dl[i] + c * gi;
+        s[i] = isfinite(si) ? si : 0.f;
+    }
+}
+
+// one inner step: x' = x - (beta S(x) + rho precom (x - xold) + s) / (beta T(x) + rho precom),
+// clamped to >= 0 with NaN/Inf -> 0 (the denominator in FP64)
+template <int POT>
+__global__ __launch_bounds__(256) void lalm_inner_kernel(const float* __restrict__ x, const float* __restrict__ xold,
+                                                         const float* __restrict__ s,
+                                                         const float* __restrict__ precom, float* __restrict__ x_out,
+                                                         int nx, int ny, int nz, double beta, float delta, double rho) {
+    const size_t n = (size_t)nx * ny * nz;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(i % nz);
+        const size_t c = i / nz;
+        const int ix = (int)(c % nx), iy = (int)(c / nx);
+        const float sx = pen_point<1, POT>(x, nx, ny, nz, ix, iy, iz, delta);
+        const float tx = pen_point<2, POT>(x, nx, ny, nz, ix, iy, iz, delta);
+        const double num = beta * (double)sx + rho * (double)precom[i] * ((double)x[i] - (double)xold[i]) + (double)s[i];
+        const double den = beta * (double)tx + rho * (double)precom[i];
+        x_out[i] = scrub((float)((double)x[i] - num / den));
     }
 }
 
@@ -413,57 +468,93 @@ hipError_t residual_launch(const void* p, int pdt, const void* y, int ydt, const
     }
 }
 
-template <class V>
-static hipError_t huber_launch_t(const V* x, int nx, int ny, int nz, double delta, int mode, V* out, double* cost,
-                                 hipStream_t st) {
+template <int POT, class V>
+static hipError_t pen_launch_p(const V* x, int nx, int ny, int nz, double delta, int mode, V* out, double* cost,
+                               hipStream_t st) {
     const size_t n = (size_t)nx * ny * nz;
     if (mode == 1) {
-        hipLaunchKernelGGL((huber_map_kernel<1, V>), dim3(grid_for(n)), dim3(256), 0, st, x, nx, ny, nz, (V)delta, out);
+        hipLaunchKernelGGL((pen_map_kernel<1, POT, V>), dim3(grid_for(n)), dim3(256), 0, st, x, nx, ny, nz, (V)delta,
+                           out);
     } else if (mode == 2) {
-        hipLaunchKernelGGL((huber_map_kernel<2, V>), dim3(grid_for(n)), dim3(256), 0, st, x, nx, ny, nz, (V)delta, out);
+        hipLaunchKernelGGL((pen_map_kernel<2, POT, V>), dim3(grid_for(n)), dim3(256), 0, st, x, nx, ny, nz, (V)delta,
+                           out);
     } else if (mode == 0) {
         RedScratch rs;
         hipError_t e = scratch(rs);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(huber_cost_kernel<V>, dim3(red_grid(n)), dim3(kRB), 0, st, x, nx, ny, nz, delta, rs, cost);
+        hipLaunchKernelGGL((pen_cost_kernel<POT, V>), dim3(red_grid(n)), dim3(kRB), 0, st, x, nx, ny, nz, delta, rs,
+                           cost);
     } else {
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t huber_launch(const float* x, int nx, int ny, int nz, double delta, int mode, float* out, double* cost,
-                        hipStream_t st) {
-    return huber_launch_t<float>(x, nx, ny, nz, delta, mode, out, cost, st);
+// pfun: POT_HUBER, POT_QUAD or POT_HYPER
+#define CT_POT_DISPATCH(pfun, CALL)                     \
+    switch (pfun) {                                     \
+        case POT_HUBER: return CALL(POT_HUBER);          \
+        case POT_QUAD: return CALL(POT_QUAD);            \
+        case POT_HYPER: return CALL(POT_HYPER);          \
+        default: return hipErrorInvalidValue;           \
+    }
+
+hipError_t penalty_launch(const float* x, int nx, int ny, int nz, int pfun, double delta, int mode, float* out,
+                          double* cost, hipStream_t st) {
+#define CALL(P) pen_launch_p<P, float>(x, nx, ny, nz, delta, mode, out, cost, st)
+    CT_POT_DISPATCH(pfun, CALL)
+#undef CALL
 }
 
-hipError_t huber_launch(const double* x, int nx, int ny, int nz, double delta, int mode, double* out, double* cost,
-                        hipStream_t st) {
-    return huber_launch_t<double>(x, nx, ny, nz, delta, mode, out, cost, st);
+hipError_t penalty_launch(const double* x, int nx, int ny, int nz, int pfun, double delta, int mode, double* out,
+                          double* cost, hipStream_t st) {
+#define CALL(P) pen_launch_p<P, double>(x, nx, ny, nz, delta, mode, out, cost, st)
+    CT_POT_DISPATCH(pfun, CALL)
+#undef CALL
 }
 
-hipError_t nesterov_launch(const float* x, const float* z, float* s, const float* g, const float* precom, float* x_out,
-                           float* z_out, void* zcopy, int zcdt, int nx, int ny, int nz, double beta, double delta,
-                           double t, double t_next, const double* tseq, const int* base, int k, hipStream_t st) {
+template <int POT>
+static hipError_t nesterov_p(const float* x, const float* z, float* s, const float* g, const float* precom,
+                             float* x_out, float* z_out, void* zcopy, int zcdt, int nx, int ny, int nz, double beta,
+                             double delta, double t, double t_next, const double* tseq, const int* base, int k,
+                             hipStream_t st) {
     const size_t n = (size_t)nx * ny * nz;
     const dim3 gr(grid_for(n)), bl(256);
     switch (zcopy ? zcdt : DT_F32) {
-        case DT_F32: hipLaunchKernelGGL((nesterov_kernel<float, float>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (float*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
-        case DT_F16: hipLaunchKernelGGL((nesterov_kernel<float, __half>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__half*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
-        case DT_BF16: hipLaunchKernelGGL((nesterov_kernel<float, __hip_bfloat16>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__hip_bfloat16*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
+        case DT_F32: hipLaunchKernelGGL((nesterov_kernel<float, float, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (float*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
+        case DT_F16: hipLaunchKernelGGL((nesterov_kernel<float, __half, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__half*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
+        case DT_BF16: hipLaunchKernelGGL((nesterov_kernel<float, __hip_bfloat16, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__hip_bfloat16*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t nesterov_launch(const double* x, const double* z, double* s, const double* g, const double* precom,
-                           double* x_out, double* z_out, double* zcopy, int nx, int ny, int nz, double beta,
-                           double delta, double t, double t_next, const double* tseq, const int* base, int k,
+hipError_t nesterov_launch(const float* x, const float* z, float* s, const float* g, const float* precom, float* x_out,
+                           float* z_out, void* zcopy, int zcdt, int nx, int ny, int nz, double beta, double delta,
+                           int pfun, double t, double t_next, const double* tseq, const int* base, int k,
                            hipStream_t st) {
+#define CALL(P) nesterov_p<P>(x, z, s, g, precom, x_out, z_out, zcopy, zcdt, nx, ny, nz, beta, delta, t, t_next, tseq, base, k, st)
+    CT_POT_DISPATCH(pfun, CALL)
+#undef CALL
+}
+
+template <int POT>
+static hipError_t nesterov64_p(const double* x, const double* z, double* s, const double* g, const double* precom,
+                               double* x_out, double* z_out, int nx, int ny, int nz, double beta, double delta,
+                               double t, double t_next, hipStream_t st) {
     const size_t n = (size_t)nx * ny * nz;
-    hipLaunchKernelGGL((nesterov_kernel<double, double>), dim3(grid_for(n)), dim3(256), 0, st, x, z, s, g, precom, x_out,
-                       z_out, zcopy, nx, ny, nz, beta, delta, t, t_next, tseq, base, k);
+    hipLaunchKernelGGL((nesterov_kernel<double, double, POT>), dim3(grid_for(n)), dim3(256), 0, st, x, z, s, g, precom,
+                       x_out, z_out, (double*)nullptr, nx, ny, nz, beta, delta, t, t_next, (const double*)nullptr,
+                       (const int*)nullptr, 0);
     return hipGetLastError();
+}
+
+hipError_t nesterov_launch(const double* x, const double* z, double* s, const double* g, const double* precom,
+                           double* x_out, double* z_out, int nx, int ny, int nz, double beta, double delta, int pfun,
+                           double t, double t_next, hipStream_t st) {
+#define CALL(P) nesterov64_p<P>(x, z, s, g, precom, x_out, z_out, nx, ny, nz, beta, delta, t, t_next, st)
+    CT_POT_DISPATCH(pfun, CALL)
+#undef CALL
 }
 
 __global__ void add_i32_kernel(int* p, int inc) {
@@ -475,11 +566,41 @@ hipError_t add_i32_launch(int* p, int inc, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t sqs_launch(const float* x, float* x_out, const float* g, const float* precom, int nx, int ny, int nz,
-                      double beta, double delta, hipStream_t st) {
+template <int POT>
+static hipError_t sqs_p(const float* x, float* x_out, const float* g, const float* precom, int nx, int ny, int nz,
+                        double beta, double delta, hipStream_t st) {
     const size_t n = (size_t)nx * ny * nz;
-    hipLaunchKernelGGL(sqs_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, x_out, g, precom, nx, ny, nz, beta, (float)delta);
+    hipLaunchKernelGGL(sqs_kernel<POT>, dim3(grid_for(n)), dim3(256), 0, st, x, x_out, g, precom, nx, ny, nz, beta,
+                       (float)delta);
     return hipGetLastError();
+}
+
+hipError_t sqs_launch(const float* x, float* x_out, const float* g, const float* precom, int nx, int ny, int nz,
+                      double beta, double delta, int pfun, hipStream_t st) {
+#define CALL(P) sqs_p<P>(x, x_out, g, precom, nx, ny, nz, beta, delta, st)
+    CT_POT_DISPATCH(pfun, CALL)
+#undef CALL
+}
+
+hipError_t lalm_sg_launch(const float* dl, float* g, float* s, size_t n, double rho, hipStream_t st) {
+    hipLaunchKernelGGL(lalm_sg_kernel, dim3(grid_for(n)), dim3(256), 0, st, dl, g, s, n, rho);
+    return hipGetLastError();
+}
+
+template <int POT>
+static hipError_t lalm_inner_p(const float* x, const float* xold, const float* s, const float* precom, float* x_out,
+                               int nx, int ny, int nz, double beta, double delta, double rho, hipStream_t st) {
+    const size_t n = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL(lalm_inner_kernel<POT>, dim3(grid_for(n)), dim3(256), 0, st, x, xold, s, precom, x_out, nx, ny,
+                       nz, beta, (float)delta, rho);
+    return hipGetLastError();
+}
+
+hipError_t lalm_inner_launch(const float* x, const float* xold, const float* s, const float* precom, float* x_out,
+                             int nx, int ny, int nz, double beta, double delta, int pfun, double rho, hipStream_t st) {
+#define CALL(P) lalm_inner_p<P>(x, xold, s, precom, x_out, nx, ny, nz, beta, delta, rho, st)
+    CT_POT_DISPATCH(pfun, CALL)
+#undef CALL
 }
 
 template <class V>

@@ -1,19 +1,22 @@
-"""FDK reconstruction (fbp/reconFDK.m:1-85) around the GPU voxel-driven backprojector.
+"""FDK reconstruction (fbp/reconFDK.m:1-85) on the GPU.
 
-The pre-processing restates the reference's MATLAB host code: the flat-panel cosine
-weight (reconFDK.m:39), the ramp filter of fbp/filter/designFilter2.m:18-78 scaled by du,
-per-row frequency-domain filtering (fbp/filter/filterFreq.m:42-51), Parker weights for
-short scans (fbp/weights/shortScanParkerWeight.m), and the final
-backProjectMex(sino, geom, 1, 0, 'back,hpd') * dbeta * 10 (reconFDK.m:78-84), which
-runs on the GPU with the filtered views stored in FP32, FP16 or BF16.
+Host side (as in the reference, computed once per geometry): the ramp filter of
+fbp/filter/designFilter2.m (flat panel) or designEquiangularFilter2.m (arc), scaled by
+du, and the short-scan Parker weights of fbp/weights/shortScanParkerWeight.m.  Device
+side: the cosine weight and the row filtering of reconFDK.m:33-76 (ctsim_fdk_filter:
+weighting, zero padding, batched hipFFT, H, inverse FFT, truncation), the filtered views
+stored in FP32, FP16 or BF16, and backProjectMex(sino, geom, 1, 0, 'back,hpd') * dbeta * 10
+(reconFDK.m:78-84) with the voxel-driven backprojector.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 
 import numpy as np
 
-from .device import DeviceArray
+from ._lib import F32, F64, call, lib
+from .device import DeviceArray, dtype_code
 from .geometry import Geometry
 from .projector import Projector
 
@@ -22,14 +25,18 @@ def nextpow2(n: float) -> int:
     return int(math.ceil(math.log2(abs(n)))) if n else 0
 
 
-def design_filter2(gamma: float, window: str, length: int, crop: float = 1.0) -> np.ndarray:
-    """Ramp filter in the frequency domain, zero padding included (designFilter2.m)."""
+def design_filter2(gamma: float, window: str, length: int, crop: float = 1.0, sdd: float | None = None) -> np.ndarray:
+    """Ramp filter in the frequency domain, zero padding included (designFilter2.m:18-78);
+    with sdd, the equiangular (arc detector) kernel of designEquiangularFilter2.m."""
     order = max(64, 2 ** nextpow2(2 * length - 1))
     index = np.arange(-order // 2 + 1, order // 2)
     h = np.zeros(index.size + 1)  # h(order) = 0 pads to length order
     h[:index.size][index == 0] = 1.0 / (4.0 * gamma ** 2)
     odd = np.mod(index, 2) == 1  # MATLAB mod: negative odd indices included
-    h[:index.size][odd] = -1.0 / (index[odd] * math.pi * gamma) ** 2
+    if sdd is None:
+        h[:index.size][odd] = -1.0 / (index[odd] * math.pi * gamma) ** 2
+    else:
+        h[:index.size][odd] = -1.0 / (sdd * math.pi * np.sin(index[odd] * gamma / sdd)) ** 2
     filt = np.abs(np.fft.fft(h))
     filt = filt[: order // 2 + 1]
     w = 2 * math.pi * np.arange(filt.size) / order
@@ -58,15 +65,6 @@ def design_filter2(gamma: float, window: str, length: int, crop: float = 1.0) ->
     return np.concatenate([f, f[-2:0:-1]])
 
 
-def filter_rows(x: np.ndarray, H: np.ndarray) -> np.ndarray:
-    """filterFreq along the last axis: zero-pad to len(H), multiply in frequency, real part, truncate."""
-    n = x.shape[-1]
-    order = H.size
-    X = np.fft.rfft(x, order, axis=-1)
-    X *= H[: order // 2 + 1]
-    return np.fft.irfft(X, order, axis=-1)[..., :n]
-
-
 def parker_weight(gammas: np.ndarray, beta: float, tau: float) -> np.ndarray:
     """shortScanParkerWeight.m:1-40"""
 
@@ -84,32 +82,6 @@ def parker_weight(gammas: np.ndarray, beta: float, tau: float) -> np.ndarray:
     return out
 
 
-def fdk_filter(sino: np.ndarray, geom: Geometry, window: str = "hamming", crop: float = 1.0,
-               out_dtype=np.float32) -> np.ndarray:
-    """Weighted, ramp-filtered views (MATLAB order [nv nu nviews]) as reconFDK.m:19-76 computes them."""
-    if not geom.flatPanel:
-        raise NotImplementedError("equiangular (arc) FDK filtering is not provided")
-    nu, nv = geom.detSize
-    du, dv = geom.detSpacing
-    ou, ov = geom.detOffset
-    SDD, SAD = geom.SDD, geom.SAD
-    u = (np.arange(nu) - (nu - 1) / 2 + ou) * du
-    v = (np.arange(nv) - (nv - 1) / 2 + ov) * dv
-    uu, vv = np.meshgrid(u, v)  # [nv nu]
-    weight = SAD * np.sqrt(1.0 + uu ** 2 / SDD ** 2) / np.sqrt(uu ** 2 + vv ** 2 + SDD ** 2)
-    H = du * design_filter2(du, window, nu, crop)
-    gamma = np.arctan(-u / SDD)
-    tau = abs(geom.betas[-1] - geom.betas[0]) - math.pi
-    beta_min = geom.betas.min()
-    out = np.empty(sino.shape, out_dtype, order="F")
-    for view in range(geom.noViews):
-        proj = weight * sino[:, :, view]
-        if geom.shortScan:
-            proj = proj * parker_weight(gamma, geom.betas[view] - beta_min, tau)[None, :]
-        out[:, :, view] = filter_rows(proj, H)
-    return out
-
-
 def fdk_scale(geom: Geometry) -> float:
     """dbeta * 10 of reconFDK.m:78-84"""
     dbeta = abs(geom.betas[-1] - geom.betas[0]) / (geom.noViews - 1)
@@ -118,12 +90,80 @@ def fdk_scale(geom: Geometry) -> float:
     return dbeta * 10.0
 
 
-def recon_fdk(sino: np.ndarray, geom: Geometry, window: str = "hamming", crop: float = 1.0,
-              storage: str = "f32", projector: Projector | None = None) -> np.ndarray:
-    """reconFDK: returns the reconstructed volume (MATLAB order [nx ny nz], 1/cm)."""
-    filt = fdk_filter(sino, geom, window, crop)
-    P = projector or Projector(geom)
-    d_s = P.upload_sino(filt, dtype=storage)
-    d_v = DeviceArray(P.n_img, "f32")
-    P.back(d_s, d_v, "back,hpd", scale=fdk_scale(geom))
+def fdk_filter_response(geom: Geometry, window: str = "hamming", crop: float = 1.0) -> np.ndarray:
+    """H of reconFDK.m:38-46: du * designFilter2 (flat) or du * designEquiangularFilter2 (arc)."""
+    nu = geom.detSize[0]
+    du = geom.detSpacing[0]
+    return du * design_filter2(du, window, nu, crop, sdd=None if geom.flatPanel else geom.SDD)
+
+
+def fdk_view_weights(geom: Geometry) -> np.ndarray | None:
+    """short-scan weights [nviews][nu] (reconFDK.m:40-41,45-46,55-59), or None for full scans"""
+    if not geom.shortScan:
+        return None
+    nu = geom.detSize[0]
+    du, ou = geom.detSpacing[0], geom.detOffset[0]
+    u = (np.arange(nu) - (nu - 1) / 2 + ou) * du
+    if geom.flatPanel:
+        gamma = np.arctan(-u / geom.SDD)
+        tau = abs(geom.betas[-1] - geom.betas[0]) - math.pi
+    else:
+        gamma = -u / geom.SDD
+        tau = abs(geom.betas.max() - geom.betas.min()) - math.pi
+    bmin = geom.betas.min()
+    return np.stack([parker_weight(gamma, b - bmin, tau) for b in geom.betas]).astype(np.float32)
+
+
+class FdkFilter:
+    """Device FDK pre-processing for one geometry (ctsim_fdk_create / ctsim_fdk_filter)."""
+
+    def __init__(self, geom: Geometry, window: str = "hamming", crop: float = 1.0, compute: str = "f32"):
+        if geom.ndim != 3:
+            raise ValueError("FDK needs a cone-beam geometry")
+        self.geom = geom
+        self.compute = dtype_code(compute)
+        H = np.ascontiguousarray(fdk_filter_response(geom, window, crop), np.float64)
+        vw = fdk_view_weights(geom)
+        self._vw = None if vw is None else np.ascontiguousarray(vw)
+        g = geom.to_c()
+        p = C.c_void_p()
+        call("ctsim_fdk_create", C.byref(p), C.byref(g), geom.noViews, H.ctypes.data_as(C.POINTER(C.c_double)),
+             H.size, None if self._vw is None else self._vw.ctypes.data_as(C.POINTER(C.c_float)), self.compute)
+        self.handle = p.value
+
+    def __call__(self, d_sino: DeviceArray, d_out: DeviceArray, view0: int = 0, nsub: int | None = None,
+                 stream=None) -> DeviceArray:
+        nsub = self.geom.noViews if nsub is None else nsub
+        nu, nv = self.geom.detSize
+        need = nsub * nu * nv
+        if d_sino.count < need or d_out.count < need:
+            raise ValueError(f"FDK buffers need {need} elements")
+        call("ctsim_fdk_filter", C.c_void_p(self.handle), view0, nsub, d_sino.vp, d_sino.dtype, d_out.vp, d_out.dtype,
+             C.c_void_p(stream.handle if stream else 0))
+        return d_out
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().ctsim_fdk_destroy(C.c_void_p(self.handle))
+                self.handle = None
+        except Exception:
+            pass
+
+
+def recon_fdk(sino, geom: Geometry, window: str = "hamming", crop: float = 1.0, storage: str = "f32",
+              projector: Projector | None = None, compute: str = "f32") -> np.ndarray:
+    """reconFDK: log sinogram (MATLAB [nv nu nviews], numpy or a device array in that
+    order) -> volume (MATLAB order [nx ny nz], 1/cm).  The filtered views are kept on the
+    device in `storage` (f32, f16, bf16; f64 with compute='f64')."""
+    f64 = dtype_code(compute) == F64
+    P = projector or Projector(geom, compute="f64" if f64 else "f32")
+    if isinstance(sino, DeviceArray):
+        d_in = sino
+    else:
+        d_in = P.upload_sino(np.asarray(sino, np.float64 if f64 else np.float32), dtype=F64 if f64 else F32)
+    filt = FdkFilter(geom, window, crop, compute="f64" if f64 else "f32")
+    d_f = filt(d_in, DeviceArray(d_in.count, storage))
+    d_v = DeviceArray(P.n_img, F64 if f64 else F32)
+    P.back(d_f, d_v, "back,hpd", scale=fdk_scale(geom))
     return P.download_volume(d_v)

@@ -36,6 +36,19 @@ from .projector import Projector
 from .shell import subset_views
 
 
+PENALTIES = {"huber": 0, "quad": 1, "hyper": 2}
+
+
+def penalty_code(pfun) -> int:
+    """penalty name of penalty/loadPenaltyOperator.m ('huber', 'quad', 'hyper') -> CTSIM_PEN_*"""
+    if isinstance(pfun, int):
+        return pfun
+    try:
+        return PENALTIES[pfun.lower()]
+    except KeyError:
+        raise ValueError("unknow penalty function") from None
+
+
 def nesterov_t_sequence(n_steps: int, restart_every: int | None = None, steps_per_iter: int = 1) -> np.ndarray:
     """t_0 = 1, t_{k+1} = (1 + sqrt(1 + 4 t_k^2)) / 2 (reconPwlsSeNesterovSqs.m:55,74); restarts to 1."""
     t = np.empty(n_steps + 1, np.float64)
@@ -48,7 +61,7 @@ def nesterov_t_sequence(n_steps: int, restart_every: int | None = None, steps_pe
 class PwlsOS:
     def __init__(self, geom: Geometry, y: np.ndarray, w: np.ndarray, numos: int, beta: float, delta: float,
                  mixed: bool = True, sino_dtype: str = "f16", zcopy_dtype: str = "f16", graph: bool | None = None,
-                 op: str = "tf", stream: Stream | None = None, precision: str | None = None):
+                 op: str = "tf", stream: Stream | None = None, precision: str | None = None, pfun: str = "huber"):
         if geom.ndim != 3:
             raise ValueError("PwlsOS drives the 3D cone-beam operators")
         self.f64 = precision == "f64"
@@ -62,6 +75,7 @@ class PwlsOS:
         self.M = int(numos)
         self.beta = float(beta)
         self.delta = float(delta)
+        self.pfun = penalty_code(pfun)
         self.mixed = bool(mixed)
         self.graph_mode = self.mixed if graph is None else bool(graph)
         if self.f64 and self.graph_mode:
@@ -177,15 +191,16 @@ class PwlsOS:
         zcdt = self.zc.dtype if self.zc is not None else 0
         if self.f64:
             call("ctsim_pwls_nesterov_update_f64", self.x[a].vp, self.z[a].vp, self.s.vp, self.g.vp,
-                 self.precom.vp, self.x[b].vp, self.z[b].vp, nx, ny, nz, self.beta, self.delta, t, t_next,
+                 self.precom.vp, self.x[b].vp, self.z[b].vp, nx, ny, nz, self.beta, self.delta, self.pfun, t, t_next,
                  self._sv())
         elif t is None:
             call("ctsim_pwls_nesterov_update_seq", self.x[a].vp, self.z[a].vp, self.s.vp, self.g.vp, self.precom.vp,
-                 self.x[b].vp, self.z[b].vp, zc, zcdt, nx, ny, nz, self.beta, self.delta, self._tseq.vp,
+                 self.x[b].vp, self.z[b].vp, zc, zcdt, nx, ny, nz, self.beta, self.delta, self.pfun, self._tseq.vp,
                  self._base.vp, k, self._sv())
         else:
             call("ctsim_pwls_nesterov_update", self.x[a].vp, self.z[a].vp, self.s.vp, self.g.vp, self.precom.vp,
-                 self.x[b].vp, self.z[b].vp, zc, zcdt, nx, ny, nz, self.beta, self.delta, t, t_next, self._sv())
+                 self.x[b].vp, self.z[b].vp, zc, zcdt, nx, ny, nz, self.beta, self.delta, self.pfun, t, t_next,
+                 self._sv())
         self.cur = b
 
     def _iteration_body(self, device_t: bool) -> None:
@@ -202,7 +217,8 @@ class PwlsOS:
                 self.t = t_next
         xc = self.x[self.cur]
         sfx = "_f64" if self.f64 else ""
-        call("ctsim_huber" + sfx, xc.vp, nx, ny, nz, self.delta, 0, None, C.c_void_p(self.scal.ptr + 8), self._sv())
+        call("ctsim_penalty" + sfx, xc.vp, nx, ny, nz, self.pfun, self.delta, 0, None, C.c_void_p(self.scal.ptr + 8),
+             self._sv())
         call("ctsim_sqdiff_sum" + sfx, xc.vp, self.xold.vp, xc.count, C.c_void_p(self.scal.ptr + 16), self._sv())
         if device_t:
             call("ctsim_add_i32", self._base.vp, self.M, self._sv())
@@ -247,6 +263,78 @@ class PwlsOS:
         """reconPwlsSeNesterovSqs main loop (restart every 2*itnlim never triggers within itnlim)"""
         self.setup(x0)
         self.prepare(itnlim)
+        phis, rmsds = [], []
+        for itn in range(1, itnlim + 1):
+            out = self.iterate()
+            phis.append(out["phi"])
+            rmsds.append(out["rmsd"])
+            if itn > 5 and out["rmsd"] < stop_crit:
+                break
+        return self.image(), np.array(phis), np.array(rmsds)
+
+
+class PwlsLALM:
+    """LALM-OS on the device (statsrecon/pwls/reconPwlsLALMOs.m:37-110), FP32 storage:
+
+        precom = At(w .* A(ones));  g = 0
+        per subset:  xold = x;  r = Aos(x) - Os(y);  dL = Atos(Os(w) .* r)   (x M)
+                     g = rho/(rho+1) dL + g/(rho+1);  s = rho dL + (1-rho) g
+                     10 x:  x = max(x - (beta S(x) + rho precom (x - xold) + s) /
+                                        (beta T(x) + rho precom), 0)
+        per iteration: phi = sum_sub sum(w r^2)/2 + beta R(x);  rmsd = rms(x - xold)
+
+    y and w sit in subset-major order as in PwlsOS; the inner steps ping-pong between two
+    volume buffers."""
+
+    def __init__(self, geom: Geometry, y: np.ndarray, w: np.ndarray, numos: int, beta: float, delta: float,
+                 rho: float = 0.1, pfun: str = "huber", op: str = "tf", inner: int = 10, stream: Stream | None = None):
+        self.base = PwlsOS(geom, y, w, numos, beta, delta, precision="f32", graph=False, op=op, stream=stream,
+                           pfun=pfun)
+        self.rho = float(rho)
+        self.inner = int(inner)
+        nvox = self.base.P.n_img
+        self.gacc = DeviceArray(nvox, F32)
+        self.sdir = DeviceArray(nvox, F32)
+
+    def setup(self, x0: np.ndarray) -> None:
+        self.base.setup(x0)
+        self.gacc.fill(0.0, stream=self.base.stream)
+        self.itn = 0
+
+    def iterate(self) -> dict:
+        b = self.base
+        st = b.stream
+        nx, ny, nz = b.nxyz
+        call("ctsim_fill", C.c_void_p(b.scal.ptr), 3, 1, 0.0, b._sv())
+        for k, (v0, vs, ns, off) in enumerate(b.subsets):
+            x = b.x[b.cur]
+            b.xold.copy_from(x, stream=st)
+            pw = b._window(b.p, 0, ns)
+            b.P.proj(x, pw, f"proj,{b.kind}", v0, vs, ns, stream=st)
+            call("ctsim_pwls_residual", pw.vp, b.p.dtype, b._sub_ptr(b.y, off), b.y.dtype, b._sub_ptr(b.w, off),
+                 b.w.dtype, b.r.vp, b.r.dtype, ns * b.cell, b.wscale, C.c_void_p(b.scal.ptr), b._sv())
+            b.P.back(b._window(b.r, 0, ns), b.g, f"back,{b.kind}", v0, vs, ns, scale=float(b.M) * b.wscale, stream=st)
+            call("ctsim_pwls_lalm_sg", b.g.vp, self.gacc.vp, self.sdir.vp, b.g.count, self.rho, b._sv())
+            for _ in range(self.inner):
+                src, dst = b.x[b.cur], b.x[1 - b.cur]
+                call("ctsim_pwls_lalm_inner", src.vp, b.xold.vp, self.sdir.vp, b.precom.vp, dst.vp, nx, ny, nz,
+                     b.beta, b.delta, b.pfun, self.rho, b._sv())
+                b.cur = 1 - b.cur
+        xc = b.x[b.cur]
+        call("ctsim_penalty", xc.vp, nx, ny, nz, b.pfun, b.delta, 0, None, C.c_void_p(b.scal.ptr + 8), b._sv())
+        call("ctsim_sqdiff_sum", xc.vp, b.xold.vp, xc.count, C.c_void_p(b.scal.ptr + 16), b._sv())
+        st.synchronize()
+        self.itn += 1
+        sc = b.scal.to_numpy()
+        return dict(phi=float(sc[0] + b.beta * sc[1]), penalty=float(b.beta * sc[1]),
+                    rmsd=float(math.sqrt(sc[2] / b.P.n_img)))
+
+    def image(self) -> np.ndarray:
+        return self.base.image()
+
+    def run(self, x0: np.ndarray, itnlim: int, stop_crit: float = 1e-5):
+        """reconPwlsLALMOs main loop (stops after iteration 5 once rmsd < stop_crit)"""
+        self.setup(x0)
         phis, rmsds = [], []
         for itn in range(1, itnlim + 1):
             out = self.iterate()

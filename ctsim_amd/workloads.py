@@ -9,6 +9,8 @@ Choices the configurations leave open are fixed here and recorded:
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 from .geometry import Geometry, circular_betas, fov_map
@@ -115,3 +117,46 @@ def body_phantom(shape, spacing, n_inserts, seed, water=0.02, mu=(0.015, 0.06), 
             m = inplane + ((z[k] - cz) / c) ** 2 <= 1.0
             vol[:, :, k][m] = val
     return vol.astype(dtype)
+
+
+def batch2_uneven(n_slices=64, seed=5):
+    """64 independent fan-beam slices: image sizes from {128, 256, 384, 512}^2 at 0.8 mm,
+    view counts from {180, 360, 720, 984} over 360 degrees, SAD 1000 / SDD 1500 mm, flat
+    detector at 1.0 mm with 256-1024 bins covering the field of view (the image diagonal
+    magnified by SDD/SAD, rounded up to a multiple of 64).  Each slice holds 10 random
+    ellipses from its own seed (ellipse_image).  Returns (geometries, images)."""
+    rng = np.random.default_rng(seed)
+    geoms, imgs = [], []
+    for i in range(n_slices):
+        n = int(rng.choice([128, 256, 384, 512]))
+        nviews = int(rng.choice([180, 360, 720, 984]))
+        width = n * 0.8 * math.sqrt(2.0) * 1.5
+        nu = int(min(1024, max(256, 64 * math.ceil(width / 64.0))))
+        g = Geometry(reconSize=(n, n), reconSpacing=(0.8, 0.8), reconOffset=(0, 0), detSize=(nu,),
+                     detSpacing=(1.0,), detOffset=(0,), SAD=1000.0, ADD=500.0, betas=circular_betas(nviews),
+                     flatPanel=1)
+        geoms.append(g)
+        imgs.append(ellipse_image(n, 0.8, 10, seed * 1000 + i))
+    return geoms, imgs
+
+
+def ellipse_image(n, dx, n_ell, seed, semi=(5.0, 90.0), mu=(0.01, 0.05)):
+    """n x n image of n_ell random ellipses (centres inside the field, semi-axes in mm,
+    random rotation, attenuation in 1/mm converted to the reference's 1/cm), additive
+    values as in phantom/util/ellipses.m:35-36."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(n) - (n - 1) / 2) * dx
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    img = np.zeros((n, n), np.float64)
+    rmax = 0.45 * n * dx
+    for _ in range(n_ell):
+        r = rmax * math.sqrt(rng.uniform())
+        t = rng.uniform(0, 2 * math.pi)
+        cx, cy = r * math.cos(t), r * math.sin(t)
+        a, b = rng.uniform(*semi, size=2)
+        ang = rng.uniform(0, math.pi)
+        ca, sa = math.cos(ang), math.sin(ang)
+        xr = (X - cx) * ca + (Y - cy) * sa
+        yr = -(X - cx) * sa + (Y - cy) * ca
+        img[(xr / a) ** 2 + (yr / b) ** 2 <= 1.0] += rng.uniform(*mu) * 10.0
+    return img.astype(np.float32)

@@ -202,6 +202,19 @@ int ctsim_batch2_proj(const ctsim_batch2* batch, const char* op, const void* d_i
                       int32_t sino_dtype, void* stream);
 int ctsim_batch2_back(const ctsim_batch2* batch, const char* op, const void* d_sino, int32_t sino_dtype,
                       void* d_img, int32_t img_dtype, void* stream);
+/* Launch schedule of the batch: each forward work item (slice, view, 64-cell tile)
+ * splits its pixel strip over proj_split wavefronts, each backprojection item (slice,
+ * row, 64-pixel block) splits its views over back_split wavefronts (1, 2 or 4; shares
+ * are added in a fixed order).  Default 1 and 4. */
+int ctsim_batch2_set_schedule(ctsim_batch2* batch, int32_t proj_split, int32_t back_split);
+int ctsim_batch2_get_schedule(const ctsim_batch2* batch, int32_t* proj_split, int32_t* back_split);
+/* Autotuner: times every schedule of the forward pass (d_img -> d_sino) and then of the
+ * backprojection (d_sino -> d_img_out, which must not alias d_img) with HIP events,
+ * reps launches each after a warm-up, keeps the fastest of each in the batch and
+ * returns their times in ms_out[2] (may be NULL).  Overwrites d_sino and d_img_out. */
+int ctsim_batch2_autotune(ctsim_batch2* batch, const char* proj_op, const char* back_op, const void* d_img,
+                          int32_t img_dtype, void* d_sino, int32_t sino_dtype, void* d_img_out, int32_t reps,
+                          float* ms_out, void* stream);
 
 /* ------------------------------------------------------------------------------
  * Device memory, streams, events and graphs (thin HIP wrappers so that host code in
@@ -245,10 +258,20 @@ int ctsim_pwls_residual(const void* d_p, int32_t p_dtype, const void* d_y, int32
                         int32_t w_dtype, void* d_r, int32_t r_dtype, size_t n, double wscale, double* d_phi,
                         void* stream);
 
-/* Huber penalty of a ZXY volume: mode 1 gradient S(x), mode 2 curvature T(x), written
- * to d_out (float32); mode 0 cost R(x) accumulated in FP64 into d_cost
- * (penalty/huberPenalty.m:101-161: neighbours +-x, +-y weight 1, xy diagonals 1/sqrt2,
- * +-z weight 1/2). */
+/* Roughness penalties of a ZXY volume (penalty/loadPenaltyOperator.m:22-53): pfun
+ * CTSIM_PEN_HUBER (huberPenalty.m:101-161), CTSIM_PEN_QUAD (quadPenalty.m; T is the
+ * constant 4 + 4/sqrt2 (+1 in 3D) of the reference), CTSIM_PEN_HYPER (hyperbolaPenalty.m,
+ * psi = delta^2 (sqrt(1 + (d/delta)^2) - 1)).  Neighbours +-x, +-y weight 1, xy diagonals
+ * 1/sqrt2, +-z weight 1/2.  mode 1 gradient S(x), mode 2 curvature T(x), written to
+ * d_out; mode 0 cost R(x) accumulated in FP64 into d_cost. */
+#define CTSIM_PEN_HUBER 0
+#define CTSIM_PEN_QUAD 1
+#define CTSIM_PEN_HYPER 2
+int ctsim_penalty(const float* d_x, int32_t nx, int32_t ny, int32_t nz, int32_t pfun, double delta, int32_t mode,
+                  float* d_out, double* d_cost, void* stream);
+int ctsim_penalty_f64(const double* d_x, int32_t nx, int32_t ny, int32_t nz, int32_t pfun, double delta, int32_t mode,
+                      double* d_out, double* d_cost, void* stream);
+/* ctsim_penalty with pfun = CTSIM_PEN_HUBER */
 int ctsim_huber(const float* d_x, int32_t nx, int32_t ny, int32_t nz, double delta, int32_t mode, float* d_out,
                 double* d_cost, void* stream);
 
@@ -264,7 +287,7 @@ int ctsim_huber(const float* d_x, int32_t nx, int32_t ny, int32_t nz, double del
 int ctsim_pwls_nesterov_update(const float* d_x, const float* d_z, float* d_s, const float* d_g,
                                const float* d_precom, float* d_x_out, float* d_z_out, void* d_zcopy,
                                int32_t zcopy_dtype, int32_t nx, int32_t ny, int32_t nz, double beta, double delta,
-                               double t, double t_next, void* stream);
+                               int32_t pfun, double t, double t_next, void* stream);
 
 /* Same update with the momentum scalars read from device memory:
  * t = d_tseq[*d_base + k], t_next = d_tseq[*d_base + k + 1].  Lets one captured graph
@@ -273,7 +296,8 @@ int ctsim_pwls_nesterov_update(const float* d_x, const float* d_z, float* d_s, c
 int ctsim_pwls_nesterov_update_seq(const float* d_x, const float* d_z, float* d_s, const float* d_g,
                                    const float* d_precom, float* d_x_out, float* d_z_out, void* d_zcopy,
                                    int32_t zcopy_dtype, int32_t nx, int32_t ny, int32_t nz, double beta, double delta,
-                                   const double* d_tseq, const int32_t* d_base, int32_t k, void* stream);
+                                   int32_t pfun, const double* d_tseq, const int32_t* d_base, int32_t k,
+                                   void* stream);
 
 /* *d_counter += increment, as a kernel (graph-capturable). */
 int ctsim_add_i32(int32_t* d_counter, int32_t increment, void* stream);
@@ -282,7 +306,17 @@ int ctsim_add_i32(int32_t* d_counter, int32_t increment, void* stream);
  * (precom + beta T(x)), 0) with NaN/Inf -> 0; the denominator in FP64.  Writes
  * d_xnew (must not alias d_x). */
 int ctsim_pwls_sqs_update(const float* d_x, float* d_xnew, const float* d_g, const float* d_precom, int32_t nx,
-                          int32_t ny, int32_t nz, double beta, double delta, void* stream);
+                          int32_t ny, int32_t nz, double beta, double delta, int32_t pfun, void* stream);
+
+/* LALM-OS (statsrecon/pwls/reconPwlsLALMOs.m:66-90).  After the subset gradient
+ * dL = Atos(Os(w) .* (Aos(x) - Os(y))):  g = rho/(rho+1) dL + g/(rho+1),
+ * s = rho dL + (1 - rho) g with NaN/Inf -> 0 (g and s updated in place). */
+int ctsim_pwls_lalm_sg(const float* d_dl, float* d_g, float* d_s, size_t n, double rho, void* stream);
+/* One of the inner steps: x_out = max(x - (beta S(x) + rho precom (x - xold) + s) /
+ * (beta T(x) + rho precom), 0), NaN/Inf -> 0, denominator in FP64.  x_out must not alias x. */
+int ctsim_pwls_lalm_inner(const float* d_x, const float* d_xold, const float* d_s, const float* d_precom,
+                          float* d_x_out, int32_t nx, int32_t ny, int32_t nz, double beta, double delta, int32_t pfun,
+                          double rho, void* stream);
 
 /* sum((a - b).^2) in FP64 into d_out (RMSD numerator, reconPwlsSeNesterovSqs.m:103). */
 int ctsim_sqdiff_sum(const float* d_a, const float* d_b, size_t n, double* d_out, void* stream);
@@ -294,12 +328,50 @@ int ctsim_huber_f64(const double* d_x, int32_t nx, int32_t ny, int32_t nz, doubl
                     double* d_cost, void* stream);
 int ctsim_pwls_nesterov_update_f64(const double* d_x, const double* d_z, double* d_s, const double* d_g,
                                    const double* d_precom, double* d_x_out, double* d_z_out, int32_t nx, int32_t ny,
-                                   int32_t nz, double beta, double delta, double t, double t_next, void* stream);
+                                   int32_t nz, double beta, double delta, int32_t pfun, double t, double t_next,
+                                   void* stream);
 int ctsim_sqdiff_sum_f64(const double* d_a, const double* d_b, size_t n, double* d_out, void* stream);
 
 /* out = a .* b elementwise with dtypes (precom = At(w .* A(ones)), reconPwlsSeNesterovSqs.m:42-43) */
 int ctsim_mul(const void* d_a, int32_t a_dtype, const void* d_b, int32_t b_dtype, void* d_out, int32_t out_dtype,
               size_t n, void* stream);
+
+/* ------------------------------------------------------------------------------
+ * FDK pre-processing on the device (fbp/reconFDK.m:33-76): every detector row of every
+ * view is multiplied by the cosine weight (flat panel: SAD sqrt(1 + u^2/SDD^2) /
+ * sqrt(u^2 + v^2 + SDD^2), reconFDK.m:39; arc: SAD/SDD cos(u / sqrt(SDD^2 + v^2)),
+ * reconFDK.m:44) and by view_weights[view][u] when given (the short-scan weight of
+ * shortScanParkerWeight.m), zero padded to `order` cells and filtered as
+ * fbp/filter/filterFreq.m:42-51 does: real(ifft(fft(row) .* H)), truncated to nu.
+ * H is the full length-`order` frequency response (designFilter2 output times du);
+ * only H[0 .. order/2] is used (the response is real and even).  compute CTSIM_F32 or
+ * CTSIM_F64 (FFT and weights in that precision).
+ * ---------------------------------------------------------------------------- */
+typedef struct ctsim_fdk ctsim_fdk;
+int ctsim_fdk_create(ctsim_fdk** fdk, const ctsim_geom* g, int32_t nviews, const double* H, int32_t order,
+                     const float* view_weights, int32_t compute);
+int ctsim_fdk_destroy(ctsim_fdk* fdk);
+/* views view0 .. view0+nsub-1: d_sino and d_out are device [nsub][nu][nv] (v fastest,
+ * MATLAB's [nv nu nviews] order); d_out in out_dtype (F32, F16, BF16; F64 for F64 plans).
+ * The sinogram dtype is F32/F16/BF16 for F32 plans and F64 for F64 plans. */
+int ctsim_fdk_filter(const ctsim_fdk* fdk, int32_t view0, int32_t nsub, const void* d_sino, int32_t sino_dtype,
+                     void* d_out, int32_t out_dtype, void* stream);
+
+/* ------------------------------------------------------------------------------
+ * Data side of the reconstruction loop on the device.
+ * ---------------------------------------------------------------------------- */
+/* d_counts[i] ~ Poisson(i0 * exp(-p[i])), p in F32 or F64.  Philox-4x32-10 with key =
+ * seed and counter = (i, attempt): the draws depend on (seed, i) only.  The reference
+ * draws counts with MATLAB's poissrnd before computeSimplePoissonSinogramAttunation.m;
+ * the two generators differ, the distribution is the same. */
+int ctsim_poisson_counts(const void* d_p, int32_t p_dtype, size_t n, double i0, uint64_t seed, float* d_counts,
+                         void* stream);
+/* y = -log(max(counts, low_count) / i0) in y_dtype (computeSimplePoissonSinogramAttunation.m:14-16,
+ * monochromatic source, low_count = dummyLowPhotonCount = 10 there). */
+int ctsim_log_attenuation(const float* d_counts, size_t n, double i0, double low_count, void* d_y, int32_t y_dtype,
+                          void* stream);
+/* w = (s - r)^2 / (s + v) in single precision, 0 where s <= r (statsrecon/pwls/computeWeightsPwls.m:8-9). */
+int ctsim_pwls_weights(const float* d_counts, size_t n, double r, double v, void* d_w, int32_t w_dtype, void* stream);
 
 #ifdef __cplusplus
 }

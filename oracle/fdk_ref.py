@@ -1,0 +1,42 @@
+"""FP64 numpy restatement of CTSim's FDK pre-processing, the checker of the device FDK
+filter (test infrastructure; the product filters on the GPU).
+
+fdk_filter follows fbp/reconFDK.m:19-76 view by view: the cosine weight (flat :39, arc
+:44), the short-scan weight (:55-59), and filterFreq.m:42-51 per detector row
+(zero pad to length(H), multiply the FFT by H, real part of the inverse FFT, truncate)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def filter_rows(x: np.ndarray, H: np.ndarray) -> np.ndarray:
+    """filterFreq along the last axis: zero-pad to len(H), multiply in frequency, real part, truncate."""
+    n = x.shape[-1]
+    order = H.size
+    X = np.fft.fft(x, order, axis=-1)
+    X = X * H
+    return np.real(np.fft.ifft(X, axis=-1))[..., :n]
+
+
+def fdk_filter(sino: np.ndarray, geom, H: np.ndarray, view_weights=None) -> np.ndarray:
+    """sino MATLAB [nv nu nviews] -> weighted, filtered views, same order, float64"""
+    nu, nv = geom.detSize
+    du, dv = geom.detSpacing
+    ou, ov = geom.detOffset
+    SDD, SAD = float(geom.SDD), float(geom.SAD)
+    u = (np.arange(nu) - (nu - 1) / 2 + ou) * du
+    v = (np.arange(nv) - (nv - 1) / 2 + ov) * dv
+    uu, vv = np.meshgrid(u, v)  # [nv nu]
+    if geom.flatPanel:
+        weight = SAD * np.sqrt(1.0 + uu ** 2 / SDD ** 2) / np.sqrt(uu ** 2 + vv ** 2 + SDD ** 2)
+    else:
+        weight = SAD / SDD * np.cos(uu / np.sqrt(SDD ** 2 + vv ** 2))
+    out = np.empty(sino.shape, np.float64, order="F")
+    for view in range(sino.shape[2]):
+        proj = weight * np.asarray(sino[:, :, view], np.float64)
+        if view_weights is not None:
+            proj = proj * np.asarray(view_weights[view], np.float64)[None, :]
+        out[:, :, view] = filter_rows(proj, H)
+    return out

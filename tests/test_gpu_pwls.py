@@ -114,3 +114,53 @@ def test_pwls_fp64_matches_fp64_oracle():
     assert rel_l2(x, want) < 1e-9
     np.testing.assert_allclose(ph, phis, rtol=1e-9)
     np.testing.assert_allclose(rm, rmsds, rtol=1e-7)
+
+
+@pytest.mark.parametrize("pfun", ["quad", "hyper"])
+def test_penalty_family_kernels_match_oracle(pfun):
+    from oracle import pwls_ref as R
+    from ctsim_amd._lib import call
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    from ctsim_amd.pwls import penalty_code
+    g, y, w, x0 = _problem()
+    P = Projector(_geometry(g))
+    rng = np.random.default_rng(10)
+    x = rng.uniform(0, 0.01, g.img_shape()).astype(np.float32)
+    dx = P.upload_volume(x)
+    nx, ny, nz = g.img_shape()
+    delta = 0.003
+    for mode in (1, 2):
+        out = DeviceArray(P.n_img, "f32")
+        call("ctsim_penalty", dx.vp, nx, ny, nz, penalty_code(pfun), delta, mode, out.vp, None, None)
+        want = R.penalty(pfun, x.astype(np.float64), mode, delta)
+        assert rel_l2(P.download_volume(out), want) < 1e-6, (pfun, mode)
+    cost = DeviceArray(1, "f64")
+    call("ctsim_penalty", dx.vp, nx, ny, nz, penalty_code(pfun), delta, 0, None, cost.vp, None)
+    assert cost.to_numpy()[0] == pytest.approx(R.penalty(pfun, x.astype(np.float64), 0, delta), rel=1e-6)
+
+
+def test_pwls_nesterov_hyperbola_matches_fp64_oracle():
+    from oracle import pwls_ref as R
+    from ctsim_amd.pwls import PwlsOS
+    g, y, w, x0 = _problem(seed=8)
+    beta, delta, M, its = 50.0, 1e-2, 4, 3
+    want, phis, _ = R.nesterov_sqs(y, w, g, beta, delta, its, M, x0, pfun="hyper")
+    sol = PwlsOS(_geometry(g), y, w, M, beta, delta, mixed=False, pfun="hyper")
+    x, ph, _ = sol.run(x0.astype(np.float32), its)
+    assert rel_l2(x, want) < 1e-4
+    np.testing.assert_allclose(ph, phis, rtol=1e-4)
+
+
+def test_lalm_os_matches_fp64_oracle():
+    from oracle import pwls_ref as R
+    from ctsim_amd.pwls import PwlsLALM
+    g, y, w, x0 = _problem(seed=9)
+    beta, delta, M, its = 50.0, 1e-3, 4, 2
+    want, phis, rmsds = R.lalm_os(y, w, g, beta, delta, its, M, x0, rho=0.1)
+    sol = PwlsLALM(_geometry(g), y, w, M, beta, delta, rho=0.1)
+    x, ph, rm = sol.run(x0.astype(np.float32), its)
+    print("lalm vs fp64 oracle:", rel_l2(x, want), ph, phis)
+    assert rel_l2(x, want) < 1e-4
+    np.testing.assert_allclose(ph, phis, rtol=1e-4)
+    np.testing.assert_allclose(rm, rmsds, rtol=1e-3)

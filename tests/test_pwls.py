@@ -54,3 +54,24 @@ def test_nesterov_sqs_decreases_cost_on_noiseless_data():
     x, phis, rmsds = R.nesterov_sqs(y, w, g, 1e-2, 1e-3, 6, 3, np.zeros(n))
     assert phis[-1] < phis[0]
     assert np.linalg.norm(x - xt) < np.linalg.norm(xt)
+
+
+@pytest.mark.parametrize("pfun", ["quad", "hyper"])
+def test_penalty_gradient_and_curvature_by_finite_differences(pfun):
+    """S = dR/dx and, for these smooth potentials, T = the diagonal of the Hessian of R
+    (quadPenalty.m's constant T equals the interior voxels' diagonal)"""
+    from oracle import pwls_ref as R
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0, 0.02, (6, 5, 4))
+    delta = 0.01
+    S = R.penalty(pfun, x, 1, delta)
+    T = R.penalty(pfun, x, 2, delta)
+    h = 1e-6
+    for idx in [(0, 0, 0), (2, 3, 1), (5, 4, 3), (3, 2, 2)]:
+        e = np.zeros_like(x)
+        e[idx] = h
+        num = (R.penalty(pfun, x + e, 0, delta) - R.penalty(pfun, x - e, 0, delta)) / (2 * h)
+        assert S[idx] == pytest.approx(num, rel=1e-6, abs=1e-10)
+        curv = (R.penalty(pfun, x + e, 1, delta)[idx] - R.penalty(pfun, x - e, 1, delta)[idx]) / (2 * h)
+        if pfun == "hyper" or idx == (3, 2, 2):  # interior voxel for the quadratic constant
+            assert T[idx] == pytest.approx(curv, rel=1e-5)
