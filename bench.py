@@ -1,5 +1,6 @@
 """Benchmark: PWLS-OS iterations per second at 512x512x128 (984 views, 1024x64 detector,
-24 ordered subsets, Huber beta 1e4 delta 1e-3), mixed-precision graph form vs plain FP32,
+24 ordered subsets, Huber beta 1e4 and delta 0.001 /mm = 0.01 in the image's 1/cm units),
+mixed-precision graph form vs plain FP32,
 plus DD / SF forward+back projection GUPS at 512x512x256 (720 views, 1024x384).
 
 A step is one PWLS-OS iteration: 24 x (forward projection of the subset, residual,
@@ -130,7 +131,7 @@ def kernel_times(solver, reps=1):
     return {k: v / n for k, v in t.items()}
 
 
-def run_pwls(form, geom, y, w, x0, steps, warmup, sino_dtype="bf16", numos=24, beta=1e4, delta=1e-3,
+def run_pwls(form, geom, y, w, x0, steps, warmup, sino_dtype="bf16", numos=24, beta=1e4, delta=1e-2,
              total_iters=0):
     """warmup + steps iterations (steps timed) of the form 'mixed', 'plain' or 'f64', then
     untimed iterations up to total_iters"""
@@ -443,7 +444,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (water cylinder + 10 random ellipsoid inserts, Poisson I0=2e4, FDK init)",
         "config": {"workload": "pwls_os_sf_512x512x128_984views_1024x64det_24subsets", "volume": [512, 512, 128],
-                   "views": 984, "detector": [1024, 64], "subsets": 24, "beta": 1e4, "delta": 1e-3,
+                   "views": 984, "detector": [1024, 64], "subsets": 24, "beta": 1e4, "delta_per_mm": 1e-3,
                    "storage": f"{args.sino_dtype} sinogram/weights/residual, fp16 z-copy for the projector, "
                               "fp32 x/z/s and footprint sums, fp64 denominators and reductions",
                    "graph": True},

@@ -40,8 +40,12 @@ static_assert(kPre == 3, "the pipeline below holds three prefetch registers");
 
 template <class T>
 struct ColParams {
-    int col, cells, za, zb;  // cells = ilo | ihi << 16 (detectors narrower than 32768 cells)
-    T amp, r, zinc, vstart, w0, w1, w2, w3, p0, p1, p2, p3;
+    int col;    // image column iy * nx + ix
+    int cells;  // footprint cells ilo | ihi << 16 (detectors narrower than 32768 cells)
+    int zab;    // voxel range za | zb << 16 needed by the tile's rows (nz <= 512)
+    T rz;       // 1 / zinc
+    T ampr, r2;  // amp / r and r^2: the staged value of voxel z is sqrt(r^2 + zc^2) * ampr * img
+    T zinc, vstart, w0, w1, w2, w3, p0, p1, p2, p3;
 };
 
 // weight of tile cell iu for a column whose u footprint is (ilo, ihi, p*): DD uses
@@ -50,6 +54,19 @@ template <class T, int FOOT>
 __device__ __forceinline__ T u_weight(int iu, int ilo, int ihi, T p0, T p1, T p2, T p3) {
     if (iu < ilo || iu > ihi) return T(0);
     return FOOT == FOOT_DD ? dd_weight(iu, p0, p1, ilo, ihi) : sf_weight(iu, p0, p1, p2, p3);
+}
+
+// Tile of rank r in the order c, c+1, c-1, c+2, c-2, ... (then the rest of the longer side)
+// for nut tiles around the tile c of the central ray.
+__device__ __forceinline__ int tile_by_rank(int r, int c, int nut) {
+    const int left = c, right = nut - 1 - c, m = min(left, right);
+    if (r == 0) return c;
+    if (r <= 2 * m) {
+        const int k = (r + 1) >> 1;
+        return (r & 1) ? c + k : c - k;
+    }
+    const int extra = r - 2 * m;
+    return right > left ? c + m + extra : c - m - extra;
 }
 
 // Staged voxels of a window [zw0, zw1] sit at zwin[1 + j - zw0]; zwin[0] and the
@@ -93,14 +110,17 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
     const int nut = (g.nu + TU - 1) / TU;
     const int nvc = (g.nv + rows - 1) / rows;
     const int items = nviews * nut * nvc;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    // longest work first: the dispatcher starts blocks in index order, and the tiles near
+    // the central ray have the longest pixel strips, so items run in order of the tile's
+    // distance from that ray (all views and row chunks of a tile together)
+    const int bid = blockIdx.x;
     const int wv = (int)(threadIdx.x / kWave);
     const int item = SPLIT == 1 ? bid * kWavesPerBlock + wv : bid;
     const int share = SPLIT == 1 ? 0 : wv;
     if (item >= items) return;  // SPLIT > 1: the whole block leaves together
     const int vc = item % nvc;
-    const int ut = (item / nvc) % nut;
-    const int b = item / (nvc * nut);
+    const int b = (item / nvc) % nviews;
+    const int ut = tile_by_rank(item / (nvc * nviews), min(max((int)floor(-g.wu / T(TU)), 0), nut - 1), nut);
     const int lane = lane_id();
     const int iv_a = vc * rows;
     const int iu_a = ut * TU;
@@ -108,11 +128,15 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
     const View<T> v = views[(size_t)b * vstride];
     const bool row_major = fabs_t(v.c) >= fabs_t(v.s);
     const int nmajor = row_major ? g.ny : g.nx;
-    const int m_begin = (nmajor * share) / SPLIT, m_end = (nmajor * (share + 1)) / SPLIT;
+    // the SPLIT waves of a block take interleaved 64-row batches of the strip, so that the
+    // long middle of a chord is shared among them
+    const int m_end = nmajor;
     const T kL = u_to_slope(g, T(iu_a) - T(0.5));
     const T kR = u_to_slope(g, T(iu_b) + T(0.5));
     const int iv_last = min(iv_a + rows, g.nv) - 1;
     T* zw = zbuf[wv];
+    const T wzdz = v.wz * g.dz;                       // zc = z dz + wz dz
+    const T fiv_lo = T(iv_a + lane) - T(0.5);          // lower edge of this lane's row
     ColParams<T>* cw = cbuf[wv];
 
     // this lane's row of the tile, indexed with wave-uniform cell numbers (VGPR-indexed
@@ -122,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
 #pragma unroll
     for (int t = 0; t < TU + 3; ++t) acc[t] = T(0);
 
-    for (int m0 = m_begin; m0 < m_end; m0 += kWave) {
+    for (int m0 = share * kWave; m0 < m_end; m0 += SPLIT * kWave) {
         const int m = m0 + lane;
         int lo = 0, hi = -1;
         if (m < m_end) {
@@ -151,8 +175,9 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                 // exact footprint of the candidate column and its weights on this tile
                 bool ok = k < total;
                 ColParams<T> cp{};
-                cp.r = T(1);
+                cp.r2 = T(1);
                 cp.zinc = T(1);
+                cp.rz = T(1);
                 if (ok && map != nullptr) ok = map[iy * g.nx + ix] != 0;
                 if (ok) {
                     const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
@@ -175,16 +200,18 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                     cp.cells = ilo | (ihi << 16);
                     if (ok) {
                         ok = v_window(g, cg.d, v.wz, cp.zinc, cp.vstart, iz0, iz1) && iz0 < iz1 && cp.zinc > T(0);
-                        cp.amp = cg.amp;
-                        cp.r = cg.r;
+                        cp.ampr = cg.amp * frcp(cg.r);
+                        cp.r2 = cg.r * cg.r;
                     }
                     if (ok) {
                         // voxel range needed by this wave's rows (exact ranges; a voxel left
                         // out at the rounding edge has a weight of the order of the rounding)
                         const T rz = frcp(cp.zinc);
-                        cp.za = max((int)ceil((T(iv_a) - T(0.5) - cp.vstart) * rz - T(1)), iz0);
-                        cp.zb = min((int)ceil((T(iv_last) + T(0.5) - cp.vstart) * rz) - 1, iz1 - 1);
-                        ok = cp.za <= cp.zb;
+                        const int za = max((int)ceil((T(iv_a) - T(0.5) - cp.vstart) * rz - T(1)), iz0);
+                        const int zb = min((int)ceil((T(iv_last) + T(0.5) - cp.vstart) * rz) - 1, iz1 - 1);
+                        cp.rz = rz;
+                        cp.zab = za | (zb << 16);
+                        ok = za <= zb;
                     }
                     if (ok) {
                         // tile weights of the first four footprint cells inside the tile
@@ -205,20 +232,21 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                 if (cur >= 0) mask &= mask - 1;
                 Tin pre0{}, pre1{}, pre2{};
                 if (cur >= 0) {
+                    // wave-uniform column pointer and voxel range: scalar address arithmetic
                     const ColParams<T>& c = cw[cur];
-                    const Tin* colp = vol + (size_t)c.col * g.nz;
-                    const int zz = c.za + lane;
-                    pre0 = colp[zz <= c.zb ? zz : c.za];
-                    pre1 = colp[zz + kWave <= c.zb ? zz + kWave : c.za];
-                    pre2 = colp[zz + 2 * kWave <= c.zb ? zz + 2 * kWave : c.za];
+                    const int zab = uni(c.zab), za = zab & 0xffff, zb = zab >> 16;
+                    const Tin* colp = vol + (size_t)uni(c.col) * g.nz;
+                    const int zz = za + lane;
+                    pre0 = colp[min(zz, zb)];
+                    pre1 = colp[min(zz + kWave, zb)];
+                    pre2 = colp[min(zz + 2 * kWave, zb)];
                 }
                 while (cur >= 0) {
                     const ColParams<T>& c = cw[cur];
-                    const int za = c.za, zb = c.zb, ilo = c.cells & 0xffff, ihi = c.cells >> 16;
-                    const T amp = c.amp, r = c.r, zinc = c.zinc, vstart = c.vstart;
-                    const T inv_r = frcp(r);
-                    const T rz = frcp(zinc);
-                    const Tin* colp = vol + (size_t)c.col * g.nz;
+                    const int zab = uni(c.zab), cells = uni(c.cells);
+                    const int za = zab & 0xffff, zb = zab >> 16, ilo = cells & 0xffff, ihi = cells >> 16;
+                    const T ampr = c.ampr, r2 = c.r2, zinc = c.zinc, vstart = c.vstart, rz = c.rz;
+                    const Tin* colp = vol + (size_t)uni(c.col) * g.nz;
                     const int iv = iv_a + lane;
                     T q = T(0);
                     for (int zw0 = za; zw0 <= zb; zw0 += kZWin) {
@@ -232,15 +260,15 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                             for (int i = 0; i < kPre; ++i) {
                                 const int zz = za + i * kWave + lane;
                                 if (zz <= zw1) {
-                                    const T zc = (T(zz) + v.wz) * g.dz;
-                                    zw[zz - zw0 + 1] = fsqrt(r * r + zc * zc) * inv_r * T(to_f(pv[i]));
+                                    const T zc = T(zz) * g.dz + wzdz;
+                                    zw[zz - zw0 + 1] = fsqrt(r2 + zc * zc) * ampr * T(to_f(pv[i]));
                                 }
                             }
                             zfirst = za + kPre * kWave + lane;
                         }
                         for (int zz = zfirst; zz <= zw1; zz += kWave) {
-                            const T zc = (T(zz) + v.wz) * g.dz;
-                            zw[zz - zw0 + 1] = fsqrt(r * r + zc * zc) * inv_r * ld<T>(colp + zz);
+                            const T zc = T(zz) * g.dz + wzdz;
+                            zw[zz - zw0 + 1] = fsqrt(r2 + zc * zc) * ampr * ld<T>(colp + zz);
                         }
                         if (lane <= kZPad) zw[lane == 0 ? 0 : zw1 - zw0 + 1 + lane] = T(0);  // zero pads
                         __builtin_amdgcn_wave_barrier();
@@ -250,11 +278,12 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                             if (nxt >= 0) {
                                 mask &= mask - 1;
                                 const ColParams<T>& cn = cw[nxt];
-                                const Tin* coln = vol + (size_t)cn.col * g.nz;
-                                const int zz = cn.za + lane;
-                                pre0 = coln[zz <= cn.zb ? zz : cn.za];
-                                pre1 = coln[zz + kWave <= cn.zb ? zz + kWave : cn.za];
-                                pre2 = coln[zz + 2 * kWave <= cn.zb ? zz + 2 * kWave : cn.za];
+                                const int zabn = uni(cn.zab), zan = zabn & 0xffff, zbn = zabn >> 16;
+                                const Tin* coln = vol + (size_t)uni(cn.col) * g.nz;
+                                const int zz = zan + lane;
+                                pre0 = coln[min(zz, zbn)];
+                                pre1 = coln[min(zz + kWave, zbn)];
+                                pre2 = coln[min(zz + 2 * kWave, zbn)];
                             }
                             cur = nxt;
                         }
@@ -263,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                             // [x_j, x_j + zinc] relative to the row's lower edge, with
                             // x_j = j zinc - (iv - 1/2 - vstart); at most ceil(1/zinc) + 1 of
                             // them overlap the row, so thick voxels take a fixed unrolled count
-                            const T tt = T(iv) - T(0.5) - vstart;
+                            const T tt = fiv_lo - vstart;
                             const int ja = (int)ceil(tt * rz - T(1));
                             const T zu = uni(zinc);
                             if (zu > T(14)) {  // footprints that may reach the 16-cell cap
@@ -292,17 +321,23 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                     // the four weighted adds go to registers picked by a uniform branch
                     const int c0 = uni(max(ilo, iu_a) - iu_a);
                     const int c1 = uni(min(ihi, iu_b) - iu_a);
-                    const T aq = amp * q;
-                    if (c1 - c0 < 4) {
-                        const T w0 = uni(c.w0), w1 = uni(c.w1), w2 = uni(c.w2), w3 = uni(c.w3);
-                        acc[c0] += w0 * aq;
-                        acc[c0 + 1] += w1 * aq;
-                        acc[c0 + 2] += w2 * aq;
-                        acc[c0 + 3] += w3 * aq;
-                    } else {  // footprint wider than four tile cells
-                        const T p0 = c.p0, p1 = c.p1, p2 = c.p2, p3 = c.p3;
-                        for (int t = c0; t <= c1; ++t)
-                            acc[t] += u_weight<T, FOOT>(iu_a + t, ilo, ihi, p0, p1, p2, p3) * aq;
+                    const T aq = q;  // amp is folded into the staged voxel values
+                    // cells c0 .. c1 in chunks of four: the first chunk's weights were made
+                    // with the column's parameters, footprints wider than four tile cells
+                    // (rare) evaluate the further ones here
+                    T w0 = uni(c.w0), w1 = uni(c.w1), w2 = uni(c.w2), w3 = uni(c.w3);
+                    for (int cc = c0; cc <= c1; cc += 4) {
+                        if (cc > c0) {
+                            const T p0 = uni(c.p0), p1 = uni(c.p1), p2 = uni(c.p2), p3 = uni(c.p3);
+                            w0 = u_weight<T, FOOT>(iu_a + cc, ilo, ihi, p0, p1, p2, p3);
+                            w1 = cc + 1 <= c1 ? u_weight<T, FOOT>(iu_a + cc + 1, ilo, ihi, p0, p1, p2, p3) : T(0);
+                            w2 = cc + 2 <= c1 ? u_weight<T, FOOT>(iu_a + cc + 2, ilo, ihi, p0, p1, p2, p3) : T(0);
+                            w3 = cc + 3 <= c1 ? u_weight<T, FOOT>(iu_a + cc + 3, ilo, ihi, p0, p1, p2, p3) : T(0);
+                        }
+                        acc[cc] += w0 * aq;
+                        acc[cc + 1] += w1 * aq;
+                        acc[cc + 2] += w2 * aq;
+                        acc[cc + 3] += w3 * aq;
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
@@ -472,8 +507,9 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             const int ilo = uni(c.ilo), nw = uni(c.nw), r0 = uni(c.r0), r1 = uni(c.r1);
             const int iz0 = uni(c.iz0), iz1 = uni(c.iz1), vb = uni(c.pad0);
             const T w0 = uni(c.w0), w1 = uni(c.w1), w2 = uni(c.w2), w3 = uni(c.w3);
-            const T amp = c.amp, r = c.r, zinc = c.zinc, vstart = c.vstart, wz = c.wz;
-            const T inv_r = frcp(r);
+            const T zinc = uni(c.zinc), vstart = uni(c.vstart), r = uni(c.r);
+            const T ampr = uni(c.amp) * frcp(r), r2 = r * r;  // ampv * amp = sqrt(r^2 + zc^2) * amp / r
+            const T wzdz = uni(c.wz) * g.dz;                  // zc = z dz + wz dz
             const Tin* pv = proj + ((size_t)vb * g.nu + ilo) * g.nv;
             for (int rw = r0; rw <= r1; rw += kRowWin) {
                 const int rwe = min(r1, rw + kRowWin - 1);
@@ -544,8 +580,8 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                                 s += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * sw[iv - rw + 1];
                         }
                     }
-                    const T zcm = (T(iz) + wz) * g.dz;
-                    img[cz] += amp * (fsqrt(r * r + zcm * zcm) * inv_r) * s;
+                    const T zcm = T(iz) * g.dz + wzdz;
+                    img[cz] += ampr * fsqrt(r2 + zcm * zcm) * s;
                 }
                 __builtin_amdgcn_wave_barrier();
             }
