@@ -379,11 +379,12 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
 constexpr int kRowWin = 1019;  // detector rows staged per wave (nv larger: windowed); with the
                                 // zero pads one wave's row buffer is 1024 words
 
+
 template <class T>
-struct ViewCol {          // one view's footprint on one image column
-    int ilo, nw, r0, r1;  // first footprint cell, cell count, rows covered
-    int iz0, iz1, pad0, pad1;
-    T amp, r, zinc, vstart, wz, w0, w1, w2, w3, p0, p1, p2, p3, padt[3];
+struct __align__(16) ViewCol {  // one view's footprint on one image column (80 bytes in FP32)
+    int ilo, nw, r0, r1;        // first footprint cell, cell count, rows covered
+    int iz0, iz1, pad0;         // voxel range; pad0 = view index within the selection
+    T amp, r, zinc, vstart, wz, w0, w1, w2, w3, p0, p1, p2, p3;
 };
 
 // Staged rows of a window [rw, rwe] sit at sw[1 + row - rw]; sw[0] and the kRowPad
@@ -394,6 +395,13 @@ struct ViewCol {          // one view's footprint on one image column
 // overlaps follow the DD rule: row ra takes min(b, ra + 1) - a, row f > ra takes
 // max(min(b, f + 1) - f, 0).  A voxel meets at most K = floor(zinc) + 2 rows.
 constexpr int kRowPad = 4;
+
+// rows staged per wave for a detector of nv rows, and the dynamic LDS of a block
+inline int back_rowwin(int nv) { return nv < kRowWin ? nv : kRowWin; }
+template <class T>
+inline size_t back_lds_bytes(int rowwin) {
+    return (size_t)kWavesPerBlock * (size_t)(rowwin + 1 + kRowPad) * sizeof(T);
+}
 
 template <class T, int K>
 __device__ __forceinline__ T gather_rows(const T* sw, int rw, int rwe, T a, T b) {
@@ -416,8 +424,12 @@ template <class Tin, class Tout, class T, int FOOT, int NZL>
 __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View<T>* __restrict__ views, int vstride,
                                                            int nviews, const Tin* __restrict__ proj,
                                                            const uint8_t* __restrict__ map,
-                                                           Tout* __restrict__ vol, T scale, int accumulate) {
-    __shared__ T srow[kWavesPerBlock][kRowWin + 1 + kRowPad];
+                                                           Tout* __restrict__ vol, T scale, int accumulate,
+                                                           int rowwin) {
+    // row buffers (rowwin + 1 + kRowPad words per wave) in dynamic LDS, sized at launch
+    // for the detector, so short detectors leave room for more resident waves
+    extern __shared__ __align__(16) unsigned char back_dyn_lds[];
+    T* srow = reinterpret_cast<T*>(back_dyn_lds);
     __shared__ ViewCol<T> vbuf[kWavesPerBlock][kWave];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int wv = (int)(threadIdx.x / kWave);
@@ -431,7 +443,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             for (int iz = lane; iz < g.nz; iz += kWave) ocol[iz] = cvt_store<Tout>(T(0));
         return;
     }
-    T* sw = srow[wv];
+    T* sw = srow + (size_t)wv * (rowwin + 1 + kRowPad);
     ViewCol<T>* vw = vbuf[wv];
     const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
     T img[NZL];
@@ -511,8 +523,8 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             const T ampr = uni(c.amp) * frcp(r), r2 = r * r;  // ampv * amp = sqrt(r^2 + zc^2) * amp / r
             const T wzdz = uni(c.wz) * g.dz;                  // zc = z dz + wz dz
             const Tin* pv = proj + ((size_t)vb * g.nu + ilo) * g.nv;
-            for (int rw = r0; rw <= r1; rw += kRowWin) {
-                const int rwe = min(r1, rw + kRowWin - 1);
+            for (int rw = r0; rw <= r1; rw += rowwin) {
+                const int rwe = min(r1, rw + rowwin - 1);
                 // u combination of the footprint's detector columns, row by row
                 int rfirst = rw + lane;
                 if (rw == r0) {
@@ -799,15 +811,18 @@ template <class Tin, class Tout, class T, int NZL>
 static void launch_back(unsigned blocks, hipStream_t st, const Geom<T>& g, const View<T>* views, int vstride,
                         int nviews, const void* proj, const uint8_t* map, void* vol, int foot, int pd_mode, T scale,
                         int accumulate) {
+    const int rowwin = back_rowwin(g.nv);
     if (foot == 2)
         hipLaunchKernelGGL((cbct_back_pd_kernel<Tin, Tout, T, NZL>), dim3(blocks), dim3(kBlock), 0, st, g, views,
                            vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate, pd_mode);
     else if (foot == FOOT_DD)
-        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_DD, NZL>), dim3(blocks), dim3(kBlock), 0, st, g,
-                           views, vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate);
+        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_DD, NZL>), dim3(blocks), dim3(kBlock),
+                           back_lds_bytes<T>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
+                           (Tout*)vol, scale, accumulate, rowwin);
     else
-        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_TF, NZL>), dim3(blocks), dim3(kBlock), 0, st, g,
-                           views, vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate);
+        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_TF, NZL>), dim3(blocks), dim3(kBlock),
+                           back_lds_bytes<T>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
+                           (Tout*)vol, scale, accumulate, rowwin);
 }
 
 template <class Tin, class Tout, class T>
