@@ -6,6 +6,8 @@ passes), so that a profile names only the kernels it is about.
                                                        # 512x512x128 volume, 1024x64 detector, FP16
     python -m ctsim_amd.profiling pair-slab [reps]     # proj,tf + back,tf of 24 of 720 views,
                                                        # 512x512x256 volume, 1024x384 detector, FP32
+    python -m ctsim_amd.profiling fdk-back [reps]      # back,hpd of 720 filtered views (FP16) into a
+                                                       # 512^3 FP32 volume, 1024x768 detector
 
 The copy workload calibrates the FETCH_SIZE / WRITE_SIZE counters: it reads and writes
 exactly 256 MiB per launch."""
@@ -52,7 +54,17 @@ def pair_slab(reps: int) -> None:
     _pair(g, vol, 24, 30, "f32", reps)
 
 
-WORKLOADS = {"copy": copy, "pwls-subset": pwls_subset, "pair-slab": pair_slab}
+def fdk_back(reps: int) -> None:
+    g = workloads.fdk_512()
+    P = Projector(g)
+    s = DeviceArray(P.n_sino(g.noViews), "f16").fill(1.0)
+    b = DeviceArray(P.n_img, "f32")
+    for _ in range(reps):
+        P.back(s, b, op="back,hpd", scale=1.0)
+    synchronize()
+
+
+WORKLOADS = {"copy": copy, "pwls-subset": pwls_subset, "pair-slab": pair_slab, "fdk-back": fdk_back}
 
 
 def main(argv=None) -> int:
