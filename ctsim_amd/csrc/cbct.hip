@@ -73,8 +73,11 @@ __device__ __forceinline__ int tile_by_rank(int r, int c, int nut) {
 // kZPad words after the last voxel are zero.  gather_row sums, over the K voxels from
 // ja on, (overlap with the row) * staged value, where voxel j covers detector rows
 // [x_j, x_j + zinc] relative to the row's lower edge, x_j = j zinc - tt, and the row is
-// [0, 1]: min(x_j + zinc, 1) - max(x_j, 0), the DD rule of dd_overlap.  ja is clamped
-// to [zw0 - 1, zw1 + 1] so that voxels outside the window read zero words.
+// [0, 1].  The DD rule of dd_overlap, max(min(x_j + zinc, 1) - max(x_j, 0), 0), equals
+// clamp01(x_{j+1}) - clamp01(x_j) bit for bit when x_j < x_{j+1}: where the overlap is
+// positive the clamps reduce to the same min and max, elsewhere both ends clamp to the
+// same bound.  ja is clamped to [zw0 - 1, zw1 + 1] so that voxels outside the window
+// read zero words.
 constexpr int kZPad = 4;
 
 template <class T, int K>
@@ -83,15 +86,14 @@ __device__ __forceinline__ T gather_row(const T* zwin, int zw0, int zw1, int ja,
     ja = min(max(ja, zw0 - 1), zw1 + 1);
     const T* pz = zwin + (ja - zw0 + 1);
     T x = T(ja) * zinc - tt;
+    T cl = clamp01(x);
     T s = T(0);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const T xn = x + zinc;
-        const T hi = xn < T(1) ? xn : T(1);
-        const T lo = x > T(0) ? x : T(0);
-        const T w = hi - lo > T(0) ? hi - lo : T(0);
-        s += w * pz[k];
-        x = xn;
+        x = x + zinc;
+        const T ch = clamp01(x);
+        s += (ch - cl) * pz[k];
+        cl = ch;
     }
     return s;
 }
@@ -105,7 +107,10 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
     static_assert(RPL == 1, "one detector row per lane");
     __shared__ T zbuf[kWavesPerBlock][kZWin + 1 + kZPad];
     __shared__ ColParams<T> cbuf[kWavesPerBlock][kWave];
-    __shared__ T part[SPLIT == 1 ? 1 : SPLIT - 1][TU][kWave];  // shares of waves 1.. for the combine
+    // each lane's row of the tile, [cell][lane] so that a wave's access is one word per
+    // bank; three spare cells take the weights past the tile's last cell, so the four
+    // adds of a column need no condition.  Every word has one writer (its lane).
+    __shared__ T accs[kWavesPerBlock][TU + 3][kWave];
     const int rows = RPL * kWave;
     const int nut = (g.nu + TU - 1) / TU;
     const int nvc = (g.nv + rows - 1) / rows;
@@ -122,6 +127,7 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
     const int b = (item / nvc) % nviews;
     const int ut = tile_by_rank(item / (nvc * nviews), min(max((int)floor(-g.wu / T(TU)), 0), nut - 1), nut);
     const int lane = lane_id();
+    const int loff = lane * (int)sizeof(Tin);  // byte offset of this lane's voxel in a prefetch slot
     const int iv_a = vc * rows;
     const int iu_a = ut * TU;
     const int iu_b = min(iu_a + TU, g.nu) - 1;
@@ -139,12 +145,9 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
     const T fiv_lo = T(iv_a + lane) - T(0.5);          // lower edge of this lane's row
     ColParams<T>* cw = cbuf[wv];
 
-    // this lane's row of the tile, indexed with wave-uniform cell numbers (VGPR-indexed
-    // moves); three spare cells take the weights past the tile's last cell, so the four
-    // adds of a column need no condition
-    T acc[TU + 3];
+    T* acc = &accs[wv][0][lane];
 #pragma unroll
-    for (int t = 0; t < TU + 3; ++t) acc[t] = T(0);
+    for (int t = 0; t < TU + 3; ++t) acc[t * kWave] = T(0);
 
     for (int m0 = share * kWave; m0 < m_end; m0 += SPLIT * kWave) {
         const int m = m0 + lane;
@@ -232,14 +235,14 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                 if (cur >= 0) mask &= mask - 1;
                 Tin pre0{}, pre1{}, pre2{};
                 if (cur >= 0) {
-                    // wave-uniform column pointer and voxel range: scalar address arithmetic
+                    // wave-uniform column base and voxel range: a buffer resource over
+                    // voxels [za, zb] of the column, so lanes past zb read zero
                     const ColParams<T>& c = cw[cur];
                     const int zab = uni(c.zab), za = zab & 0xffff, zb = zab >> 16;
-                    const Tin* colp = vol + (size_t)uni(c.col) * g.nz;
-                    const int zz = za + lane;
-                    pre0 = colp[min(zz, zb)];
-                    pre1 = colp[min(zz + kWave, zb)];
-                    pre2 = colp[min(zz + 2 * kWave, zb)];
+                    const auto rs = buf_rsrc(vol + (size_t)uni(c.col) * g.nz + za, (zb - za + 1) * (int)sizeof(Tin));
+                    pre0 = buf_ld<Tin>(rs, loff);
+                    pre1 = buf_ld<Tin>(rs, loff + kWave * (int)sizeof(Tin));
+                    if (zb - za >= 2 * kWave) pre2 = buf_ld<Tin>(rs, loff + 2 * kWave * (int)sizeof(Tin));
                 }
                 while (cur >= 0) {
                     const ColParams<T>& c = cw[cur];
@@ -255,11 +258,14 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                         // the registers loaded one column earlier, the rest directly
                         int zfirst = zw0 + lane;
                         if (zw0 == za) {
+                            // slots wholly past the window are skipped (uniform branch);
+                            // lanes past zb hold zeros from the range-checked loads and
+                            // write them inside the padded window (kPre * kWave <= kZWin)
                             const Tin pv[kPre] = {pre0, pre1, pre2};
 #pragma unroll
                             for (int i = 0; i < kPre; ++i) {
-                                const int zz = za + i * kWave + lane;
-                                if (zz <= zw1) {
+                                if (i * kWave <= zw1 - za) {
+                                    const int zz = za + i * kWave + lane;
                                     const T zc = T(zz) * g.dz + wzdz;
                                     zw[zz - zw0 + 1] = fsqrt(r2 + zc * zc) * ampr * T(to_f(pv[i]));
                                 }
@@ -279,11 +285,11 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                                 mask &= mask - 1;
                                 const ColParams<T>& cn = cw[nxt];
                                 const int zabn = uni(cn.zab), zan = zabn & 0xffff, zbn = zabn >> 16;
-                                const Tin* coln = vol + (size_t)uni(cn.col) * g.nz;
-                                const int zz = zan + lane;
-                                pre0 = coln[min(zz, zbn)];
-                                pre1 = coln[min(zz + kWave, zbn)];
-                                pre2 = coln[min(zz + 2 * kWave, zbn)];
+                                const auto rs = buf_rsrc(vol + (size_t)uni(cn.col) * g.nz + zan,
+                                                         (zbn - zan + 1) * (int)sizeof(Tin));
+                                pre0 = buf_ld<Tin>(rs, loff);
+                                pre1 = buf_ld<Tin>(rs, loff + kWave * (int)sizeof(Tin));
+                                if (zbn - zan >= 2 * kWave) pre2 = buf_ld<Tin>(rs, loff + 2 * kWave * (int)sizeof(Tin));
                             }
                             cur = nxt;
                         }
@@ -318,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                         __builtin_amdgcn_wave_barrier();
                     }
                     // first and last tile cell of the footprint; the same on every lane, so
-                    // the four weighted adds go to registers picked by a uniform branch
+                    // the four weighted adds touch one LDS word per lane and bank
                     const int c0 = uni(max(ilo, iu_a) - iu_a);
                     const int c1 = uni(min(ihi, iu_b) - iu_a);
                     const T aq = q;  // amp is folded into the staged voxel values
@@ -334,10 +340,11 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                             w2 = cc + 2 <= c1 ? u_weight<T, FOOT>(iu_a + cc + 2, ilo, ihi, p0, p1, p2, p3) : T(0);
                             w3 = cc + 3 <= c1 ? u_weight<T, FOOT>(iu_a + cc + 3, ilo, ihi, p0, p1, p2, p3) : T(0);
                         }
-                        acc[cc] += w0 * aq;
-                        acc[cc + 1] += w1 * aq;
-                        acc[cc + 2] += w2 * aq;
-                        acc[cc + 3] += w3 * aq;
+                        T* pa = acc + cc * kWave;
+                        pa[0] += w0 * aq;
+                        pa[kWave] += w1 * aq;
+                        pa[2 * kWave] += w2 * aq;
+                        pa[3 * kWave] += w3 * aq;
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
@@ -345,17 +352,9 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
         }
     }
     if (SPLIT > 1) {
-        // fixed-order combine of the shares: ((s0 + s1) + s2) + s3
-        if (wv > 0) {
-#pragma unroll
-            for (int t = 0; t < TU; ++t) part[wv - 1][t][lane] = acc[t];
-        }
+        // fixed-order combine of the shares by wave 0: ((s0 + s1) + s2) + s3
         __syncthreads();
         if (wv > 0) return;
-#pragma unroll
-        for (int t = 0; t < TU; ++t)
-#pragma unroll
-            for (int w = 0; w < SPLIT - 1; ++w) acc[t] += part[w][t][lane];
     }
     Tout* out = proj + (size_t)b * g.nu * g.nv;
     const int iv = iv_a + lane;
@@ -363,7 +362,10 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
 #pragma unroll
         for (int t = 0; t < TU; ++t) {
             const int iu = iu_a + t;
-            if (iu < g.nu) out[(size_t)iu * g.nv + iv] = cvt_store<Tout>(acc[t]);
+            T sum = acc[t * kWave];
+#pragma unroll
+            for (int w = 1; w < SPLIT; ++w) sum += accs[w][t][lane];
+            if (iu < g.nu) out[(size_t)iu * g.nv + iv] = cvt_store<Tout>(sum);
         }
     }
 }

@@ -35,6 +35,35 @@ __device__ __forceinline__ double to_f(double x) { return x; }
 template <>
 __device__ __forceinline__ double ld<double, double>(const double* p) { return *p; }
 
+// Range-checked loads through a buffer resource (base in scalar registers, byte offsets
+// per lane): an access at or past nbytes from the base reads zero instead of memory.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, int nbytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, nbytes, 0x00020000);
+}
+template <class S>
+__device__ __forceinline__ S buf_ld(__amdgpu_buffer_rsrc_t r, int off);
+template <>
+__device__ __forceinline__ float buf_ld<float>(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ double buf_ld<double>(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ __half buf_ld<__half>(__amdgpu_buffer_rsrc_t r, int off) {
+    return __ushort_as_half(__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ __hip_bfloat16 buf_ld<__hip_bfloat16>(__amdgpu_buffer_rsrc_t r, int off) {
+    const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+    return __builtin_bit_cast(__hip_bfloat16, b);
+}
+
+// clamp to [0, 1] (one v_med3 in FP32)
+__device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.f, 1.f); }
+__device__ __forceinline__ double clamp01(double x) { return x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x); }
+
 template <class S, class C>
 __device__ __forceinline__ S cvt_store(C x) { return (S)x; }
 template <>
