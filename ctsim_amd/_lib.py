@@ -121,6 +121,11 @@ SIGNATURES = {
     "ctsim_pwls_lalm_sg": (C.c_int, [vp, vp, vp, sz, f64, vp]),
     "ctsim_pwls_lalm_inner": (C.c_int, [vp, vp, vp, vp, vp, i32, i32, i32, f64, f64, i32, f64, vp]),
     "ctsim_mul": (C.c_int, [vp, i32, vp, i32, vp, i32, sz, vp]),
+    "ctsim_nusqs_seed": (C.c_int, [vp, vp, i32, i32, i32, f64, vp]),
+    "ctsim_div": (C.c_int, [vp, vp, vp, sz, vp]),
+    "ctsim_absdiff_floor": (C.c_int, [vp, vp, vp, sz, f64, vp]),
+    "ctsim_dynamic_range": (C.c_int, [vp, sz, f64, f64, vp, vp]),
+    "ctsim_pwls_nusqs_update": (C.c_int, [vp, vp, vp, vp, vp, i32, i32, i32, f64, f64, i32, vp]),
 }
 
 

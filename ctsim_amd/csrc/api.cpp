@@ -858,9 +858,10 @@ template <class V>
 static int penalty_impl(const V* x, int32_t nx, int32_t ny, int32_t nz, int32_t pfun, double delta, int32_t mode,
                         V* out, double* cost, void* stream) {
     if (!x || nx <= 0 || ny <= 0 || nz <= 0) return fail(CTSIM_EINVAL, "Error: bad volume.");
-    if ((mode == 1 || mode == 2) && !out) return fail(CTSIM_EINVAL, "Error: null output.");
+    if ((mode == 1 || mode == 2 || mode == 3) && !out) return fail(CTSIM_EINVAL, "Error: null output.");
     if (mode == 0 && !cost) return fail(CTSIM_EINVAL, "Error: null cost output.");
-    if (pfun < 0 || pfun > 2) return fail(CTSIM_EINVAL, "Error: unknow penalty function");
+    if (pfun < 0 || pfun > 6) return fail(CTSIM_EINVAL, "Error: unknow penalty function");
+    if (mode == 3 && pfun != 0) return fail(CTSIM_EINVAL, "Error: wrong penalty mode.");
     hipError_t e = penalty_launch(x, nx, ny, nz, pfun, delta, mode, out, cost, (hipStream_t)stream);
     if (e == hipErrorInvalidValue) return fail(CTSIM_EINVAL, "Error: wrong penalty mode.");
     HIP_TRY(e, "penalty");
@@ -944,6 +945,42 @@ int ctsim_pwls_lalm_inner(const float* x, const float* xold, const float* s, con
     hipError_t e = lalm_inner_launch(x, xold, s, precom, x_out, nx, ny, nz, beta, delta, pfun, rho, (hipStream_t)stream);
     if (e == hipErrorInvalidValue) return fail(CTSIM_EINVAL, "Error: unknow penalty function");
     HIP_TRY(e, "lalm update");
+    return CTSIM_OK;
+}
+
+int ctsim_nusqs_seed(const float* x, float* v, int32_t nx, int32_t ny, int32_t nz, double eps, void* stream) {
+    if (!x || !v || nx <= 0 || ny <= 0 || nz <= 0) return fail(CTSIM_EINVAL, "Error: bad volume.");
+    if (x == v) return fail(CTSIM_EINVAL, "Error: d_v must not alias d_x.");
+    HIP_TRY(nusqs_seed_launch(x, v, nx, ny, nz, eps, (hipStream_t)stream), "nusqs seed");
+    return CTSIM_OK;
+}
+
+int ctsim_div(const float* a, const float* b, float* out, size_t n, void* stream) {
+    if (!a || !b || !out) return fail(CTSIM_EINVAL, "Error: null buffer.");
+    if (n == 0) return CTSIM_OK;
+    HIP_TRY(div_launch(a, b, out, n, (hipStream_t)stream), "div");
+    return CTSIM_OK;
+}
+
+int ctsim_absdiff_floor(const float* x, const float* xref, float* out, size_t n, double eps, void* stream) {
+    if (!x || !xref || !out) return fail(CTSIM_EINVAL, "Error: null buffer.");
+    if (n == 0) return CTSIM_OK;
+    HIP_TRY(absdiff_floor_launch(x, xref, out, n, eps, (hipStream_t)stream), "absdiff floor");
+    return CTSIM_OK;
+}
+
+int ctsim_dynamic_range(const float* x, size_t n, double t, double eta, float* out, void* stream) {
+    if (!x || !out || n == 0) return fail(CTSIM_EINVAL, "Error: bad input.");
+    HIP_TRY(dynamic_range_launch(x, n, t, eta, out, (hipStream_t)stream), "dynamic range");
+    return CTSIM_OK;
+}
+
+int ctsim_pwls_nusqs_update(const float* x, float* x_out, const float* g, const float* dl, const float* h, int32_t nx,
+                            int32_t ny, int32_t nz, double beta, double delta, int32_t pfun, void* stream) {
+    if (!x || !x_out || !g || !dl || !h || nx <= 0 || ny <= 0 || nz <= 0) return fail(CTSIM_EINVAL, "Error: bad volume.");
+    if (x == x_out) return fail(CTSIM_EINVAL, "Error: d_x_out must not alias d_x.");
+    if (pfun < 0 || pfun > 6) return fail(CTSIM_EINVAL, "Error: unknow penalty function");
+    HIP_TRY(nusqs_launch(x, x_out, g, dl, h, nx, ny, nz, beta, delta, pfun, (hipStream_t)stream), "nusqs update");
     return CTSIM_OK;
 }
 

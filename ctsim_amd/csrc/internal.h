@@ -107,5 +107,11 @@ hipError_t sqs_launch(const float* x, float* x_out, const float* g, const float*
 hipError_t lalm_sg_launch(const float* dl, float* g, float* s, size_t n, double rho, hipStream_t st);
 hipError_t lalm_inner_launch(const float* x, const float* xold, const float* s, const float* precom, float* x_out,
                              int nx, int ny, int nz, double beta, double delta, int pfun, double rho, hipStream_t st);
+hipError_t nusqs_seed_launch(const float* x, float* v, int nx, int ny, int nz, double eps, hipStream_t st);
+hipError_t div_launch(const float* a, const float* b, float* o, size_t n, hipStream_t st);
+hipError_t absdiff_floor_launch(const float* x, const float* xref, float* o, size_t n, double eps, hipStream_t st);
+hipError_t dynamic_range_launch(const float* x, size_t n, double t, double eta, float* out, hipStream_t st);
+hipError_t nusqs_launch(const float* x, float* x_out, const float* g, const float* dl, const float* h, int nx, int ny,
+                        int nz, double beta, double delta, int pfun, hipStream_t st);
 
 }  // namespace ctsim

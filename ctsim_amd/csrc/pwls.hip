@@ -165,38 +165,99 @@ __global__ __launch_bounds__(kRB) void residual_kernel(const P* __restrict__ p, 
 //              of quadPenalty.m for every voxel, boundary voxels included
 //   hyperbola: psi' = d / sqrt(1 + (d/delta)^2), c = (1 + (d/delta)^2)^(-3/2),
 //              psi = delta^2 (sqrt(1 + (d/delta)^2) - 1)
+//   aniso:     c = exp(-d^2/delta^2), psi' = d c, psi = d^2 c / 2 (anisotropicPenalty.m
+//              :144-155; its S is d c as written, not the derivative of its R)
+//   isohuber:  Huber over the axis neighbours only, every weight 1 (huberIsoPenalty.m:73-104)
+// Mode 3 of the Huber penalty is the curvature H of reconPwlsSeNusqs.m
+// (huberPenalty.m:163-185): H(x)_i = sum_k w_k [|x_i - x_k| <= delta] (x_i + x_k) / x_i.
+//
+// Total variation (totalVariationPenalty.m:53-92, totalVariationIsoPenalty.m:53-92) is not a
+// pair sum: on the cells k of the grid cropped by one in every dimension (x and y only for
+// a 2D volume, nz == 1), c_k = 1 / sqrt(sum_e d_e(k)^2 + 1e-8) with d_e(k) = x_k - x_{k+e}
+// over e = x, y (and z for the isotropic form of a 3D volume).  S adds sum_e d_e c at k and
+// -d_e c at k + e; T adds c at k once per direction and at every k + e; R = sum_k 1 / c_k.
 // ---------------------------------------------------------------------------------
-enum Pot { POT_HUBER = 0, POT_QUAD = 1, POT_HYPER = 2 };
+enum Pot { POT_HUBER = 0, POT_QUAD = 1, POT_HYPER = 2, POT_ANISO = 3, POT_ISOHUBER = 4, POT_TV = 5, POT_ISOTV = 6 };
 
 template <int POT, class V>
 __device__ __forceinline__ V dpot(V d, V delta) {
-    if (POT == POT_HUBER) return d > delta ? delta : (d < -delta ? -delta : d);
+    if (POT == POT_HUBER || POT == POT_ISOHUBER) return d > delta ? delta : (d < -delta ? -delta : d);
     if (POT == POT_QUAD) return d;
+    if (POT == POT_ANISO) return d * exp(-(d * d) / (delta * delta));
     const V q = d / delta;
     return d / sqrt(V(1) + q * q);
 }
 template <int POT, class V>
 __device__ __forceinline__ V cpot(V d, V delta) {
-    if (POT == POT_HUBER) return fabs_t(d) <= delta ? V(1) : V(0);
+    if (POT == POT_HUBER || POT == POT_ISOHUBER) return fabs_t(d) <= delta ? V(1) : V(0);
+    if (POT == POT_ANISO) return exp(-(d * d) / (delta * delta));
     const V q = d / delta;
     const V r = sqrt(V(1) + q * q);
     return V(1) / (r * r * r);
 }
 template <int POT>
 __device__ __forceinline__ double cost_pot(double d, double delta) {
-    if (POT == POT_HUBER) {
+    if (POT == POT_HUBER || POT == POT_ISOHUBER) {
         d = fabs(d);
         return d > delta ? delta * d - delta * delta / 2 : d * d / 2;
     }
     if (POT == POT_QUAD) return d * d / 2;
+    if (POT == POT_ANISO) return d * d * exp(-(d * d) / (delta * delta)) / 2;
     const double q = d / delta;
     return delta * delta * (sqrt(1.0 + q * q) - 1.0);
 }
 
-// S (MODE 1) or T (MODE 2) of voxel (ix, iy, iz) of volume v (ZXY)
+constexpr double kTvEps = 1e-8;
+
+// total-variation cell k = (a, b, c) of the cropped grid: differences and c_k
+template <bool ISO, class V>
+__device__ __forceinline__ V tv_cell(const V* __restrict__ v, int nx, int nz, int a, int b, int c, V& d1, V& d2,
+                                     V& d3) {
+    const size_t sx = nz, sy = (size_t)nx * nz;
+    const size_t k = (size_t)b * sy + (size_t)a * sx + c;
+    const V xk = v[k];
+    d1 = xk - v[k + sx];
+    d2 = xk - v[k + sy];
+    d3 = (ISO && nz > 1) ? xk - v[k + 1] : V(0);
+    return V(1) / sqrt(d1 * d1 + d2 * d2 + d3 * d3 + V(kTvEps));
+}
+
+// S (MODE 1) or T (MODE 2) of the total variation at voxel (ix, iy, iz): the voxel's own
+// cell and the cells one step below it along each direction
+template <int MODE, bool ISO, class V>
+__device__ __forceinline__ V tv_point(const V* __restrict__ v, int nx, int ny, int nz, int ix, int iy, int iz) {
+    const bool three = nz > 1;
+    const bool zdir = ISO && three;
+    auto inside = [&](int a, int b, int c) {
+        return a >= 0 && b >= 0 && c >= 0 && a <= nx - 2 && b <= ny - 2 && (!three || c <= nz - 2);
+    };
+    V acc = V(0), d1, d2, d3;
+    if (inside(ix, iy, iz)) {
+        const V c = tv_cell<ISO>(v, nx, nz, ix, iy, iz, d1, d2, d3);
+        acc += MODE == 1 ? d1 * c + d2 * c + d3 * c : V(zdir ? 3 : 2) * c;
+    }
+    if (inside(ix - 1, iy, iz)) {
+        const V c = tv_cell<ISO>(v, nx, nz, ix - 1, iy, iz, d1, d2, d3);
+        acc += MODE == 1 ? -d1 * c : c;
+    }
+    if (inside(ix, iy - 1, iz)) {
+        const V c = tv_cell<ISO>(v, nx, nz, ix, iy - 1, iz, d1, d2, d3);
+        acc += MODE == 1 ? -d2 * c : c;
+    }
+    if (zdir && inside(ix, iy, iz - 1)) {
+        const V c = tv_cell<ISO>(v, nx, nz, ix, iy, iz - 1, d1, d2, d3);
+        acc += MODE == 1 ? -d3 * c : c;
+    }
+    return acc;
+}
+
+// S (MODE 1), T (MODE 2) or, for the Huber penalty, H (MODE 3) of voxel (ix, iy, iz) of
+// volume v (ZXY)
 template <int MODE, int POT, class V>
 __device__ __forceinline__ V pen_point(const V* __restrict__ v, int nx, int ny, int nz, int ix, int iy, int iz,
                                        V delta) {
+    static_assert(MODE != 3 || POT == POT_HUBER, "mode 3 is the Huber curvature H");
+    if (POT == POT_TV || POT == POT_ISOTV) return tv_point<MODE == 1 ? 1 : 2, POT == POT_ISOTV>(v, nx, ny, nz, ix, iy, iz);
     const V axy = V(0.70710678118654752440);  // 1/sqrt(2)
     if (MODE == 2 && POT == POT_QUAD) return nz > 1 ? V(4) + V(4) * axy + V(1) : V(4) + V(4) * axy;
     const size_t sx = nz, sy = (size_t)nx * nz;
@@ -205,9 +266,21 @@ __device__ __forceinline__ V pen_point(const V* __restrict__ v, int nx, int ny, 
     V acc = V(0);
     auto term = [&](bool inside, size_t j, V w) {
         if (!inside) return;
-        const V d = xi - v[j];
-        acc += MODE == 1 ? w * dpot<POT>(d, delta) : w * cpot<POT>(d, delta);
+        const V xj = v[j];
+        const V d = xi - xj;
+        if (MODE == 1) acc += w * dpot<POT>(d, delta);
+        else if (MODE == 2) acc += w * cpot<POT>(d, delta);
+        else acc += w * cpot<POT>(d, delta) * (xi + xj);
     };
+    if (POT == POT_ISOHUBER) {
+        term(ix + 1 < nx, i + sx, V(1));
+        term(ix > 0, i - sx, V(1));
+        term(iy + 1 < ny, i + sy, V(1));
+        term(iy > 0, i - sy, V(1));
+        term(iz + 1 < nz, i + 1, V(1));
+        term(iz > 0, i - 1, V(1));
+        return acc;
+    }
     // x neighbours
     term(ix + 1 < nx, i + sx, V(1));
     term(ix > 0, i - sx, V(1));
@@ -223,7 +296,7 @@ __device__ __forceinline__ V pen_point(const V* __restrict__ v, int nx, int ny, 
     // z neighbours
     term(iz + 1 < nz, i + 1, V(0.5));
     term(iz > 0, i - 1, V(0.5));
-    return acc;
+    return MODE == 3 ? acc / xi : acc;
 }
 
 template <int MODE, int POT, class V>
@@ -251,6 +324,21 @@ __global__ __launch_bounds__(kRB) void pen_cost_kernel(const V* __restrict__ x, 
         const size_t c = i / nz;
         const int ix = (int)(c % nx), iy = (int)(c / nx);
         const double xi = x[i];
+        if (POT == POT_TV || POT == POT_ISOTV) {
+            const bool three = nz > 1;
+            if (ix <= nx - 2 && iy <= ny - 2 && (!three || iz <= nz - 2)) {
+                const double d1 = xi - (double)x[i + sx], d2 = xi - (double)x[i + sy];
+                const double d3 = (POT == POT_ISOTV && three) ? xi - (double)x[i + 1] : 0.0;
+                acc += sqrt(d1 * d1 + d2 * d2 + d3 * d3 + kTvEps);
+            }
+            continue;
+        }
+        if (POT == POT_ISOHUBER) {
+            if (ix + 1 < nx) acc += cost_pot<POT>(xi - x[i + sx], delta);
+            if (iy + 1 < ny) acc += cost_pot<POT>(xi - x[i + sy], delta);
+            if (iz + 1 < nz) acc += cost_pot<POT>(xi - x[i + 1], delta);
+            continue;
+        }
         if (ix + 1 < nx) acc += cost_pot<POT>(xi - x[i + sx], delta);
         if (iy + 1 < ny) acc += cost_pot<POT>(xi - x[i + sy], delta);
         if (ix + 1 < nx && iy + 1 < ny) acc += axy * cost_pot<POT>(xi - x[i + sx + sy], delta);
@@ -351,6 +439,159 @@ __global__ __launch_bounds__(256) void lalm_inner_kernel(const float* __restrict
         const double num = beta * (double)sx + rho * (double)precom[i] * ((double)x[i] - (double)xold[i]) + (double)s[i];
         const double den = beta * (double)tx + rho * (double)precom[i];
         x_out[i] = scrub((float)((double)x[i] - num / den));
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// NU-SQS (statsrecon/pwls/reconPwlsSeNusqs.m:47-90)
+// ---------------------------------------------------------------------------------
+// v = 2 (|G1 * x| + |G2 * x|) + |x|, at least eps (:47-51).  G1 = [1 2 1; 0 0 0; -1 -2 1] and
+// G2 = [1 0 -1; 2 0 -2; 1 0 -1] correlate each z slice (filter/imageFilter2D.m: imfilter,
+// 'replicate' edges); MATLAB's rows are x, its columns y.
+__global__ __launch_bounds__(256) void nusqs_seed_kernel(const float* __restrict__ x, float* __restrict__ v, int nx,
+                                                         int ny, int nz, float eps) {
+    const size_t n = (size_t)nx * ny * nz;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(i % nz);
+        const size_t c = i / nz;
+        const int ix = (int)(c % nx), iy = (int)(c / nx);
+        auto at = [&](int a, int b) {
+            a = a < 0 ? 0 : (a >= nx ? nx - 1 : a);
+            b = b < 0 ? 0 : (b >= ny ? ny - 1 : b);
+            return x[((size_t)b * nx + a) * nz + iz];
+        };
+        const float g1 = (at(ix - 1, iy - 1) + 2.f * at(ix - 1, iy) + at(ix - 1, iy + 1)) -
+                         (at(ix + 1, iy - 1) + 2.f * at(ix + 1, iy) + at(ix + 1, iy + 1));
+        const float g2 = (at(ix - 1, iy - 1) + 2.f * at(ix, iy - 1) + at(ix + 1, iy - 1)) -
+                         (at(ix - 1, iy + 1) + 2.f * at(ix, iy + 1) + at(ix + 1, iy + 1));
+        const float val = (fabsf(g1) + fabsf(g2)) * 2.f + fabsf(x[i]);
+        v[i] = val < eps ? eps : val;
+    }
+}
+
+// out = a / b
+__global__ void div_kernel(const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ o, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        o[i] = a[i] / b[i];
+}
+
+// v = |x - xref|, at least eps (:72-73)
+__global__ void absdiff_floor_kernel(const float* __restrict__ x, const float* __restrict__ xref,
+                                     float* __restrict__ o, size_t n, float eps) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float d = fabsf(x[i] - xref[i]);
+        o[i] = d < eps ? eps : d;
+    }
+}
+
+// x' = x - (g + beta S(x)) / (dL + beta H), NaN/Inf and negatives -> 0 (:81-90); the
+// denominator in FP64
+template <int POT>
+__global__ __launch_bounds__(256) void nusqs_kernel(const float* __restrict__ x, float* __restrict__ x_out,
+                                                    const float* __restrict__ g, const float* __restrict__ dl,
+                                                    const float* __restrict__ h, int nx, int ny, int nz, double beta,
+                                                    float delta) {
+    const size_t n = (size_t)nx * ny * nz;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(i % nz);
+        const size_t c = i / nz;
+        const int ix = (int)(c % nx), iy = (int)(c / nx);
+        const float sx = pen_point<1, POT>(x, nx, ny, nz, ix, iy, iz, delta);
+        const float num = g[i] + (float)(beta * (double)sx);
+        const double den = (double)dl[i] + beta * (double)h[i];
+        x_out[i] = scrub((float)((double)x[i] - (double)num / den));
+    }
+}
+
+// dynamicRangeAdjustment (statsrecon/pwls/dynamicRangeAdjustment.m): z = max(F(x)^t, eta)
+// with F the piecewise-linear empirical CDF through min, (min+mean)/2, mean, (mean+max)/2,
+// max.  Three passes: min / max / sum, the three counts, the map.  The input is
+// non-negative (|x - xref| floored at eps), so the order of the float bit patterns is the
+// order of the values and min / max are exact integer atomics; the sum is the
+// deterministic FP64 reduction and the counts are exact.
+struct DraStats {
+    unsigned int minbits, maxbits;
+    unsigned long long cnt[3];
+    double sum;
+};
+
+__global__ void dra_init_kernel(DraStats* s) {
+    s->minbits = 0x7f800000u;  // +inf
+    s->maxbits = 0u;
+    s->cnt[0] = s->cnt[1] = s->cnt[2] = 0ull;
+    s->sum = 0.0;
+}
+
+__global__ __launch_bounds__(kRB) void dra_stats_kernel(const float* __restrict__ x, size_t n, DraStats* s,
+                                                        RedScratch rs) {
+    unsigned int mn = 0x7f800000u, mx = 0u;
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (size_t)gridDim.x * kRB) {
+        const float xi = x[i];
+        const unsigned int b = __float_as_uint(xi);
+        mn = b < mn ? b : mn;
+        mx = b > mx ? b : mx;
+        acc += (double)xi;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned int a = __shfl_down(mn, off, 64), c = __shfl_down(mx, off, 64);
+        mn = a < mn ? a : mn;
+        mx = c > mx ? c : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&s->minbits, mn);
+        atomicMax(&s->maxbits, mx);
+    }
+    finish_reduction(block_sum(acc), rs, &s->sum, 0);
+}
+
+__device__ __forceinline__ void dra_thresholds(const DraStats* s, size_t n, float& a, float& t1, float& t2, float& t3,
+                                               float& c) {
+    a = __uint_as_float(s->minbits);
+    c = __uint_as_float(s->maxbits);
+    const float b = (float)(s->sum / (double)n);
+    t1 = (a + b) / 2.f;
+    t2 = b;
+    t3 = (b + c) / 2.f;
+}
+
+__global__ __launch_bounds__(kRB) void dra_count_kernel(const float* __restrict__ x, size_t n, DraStats* s) {
+    float a, t1, t2, t3, c;
+    dra_thresholds(s, n, a, t1, t2, t3, c);
+    unsigned long long k1 = 0, k2 = 0, k3 = 0;
+    for (size_t i = (size_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (size_t)gridDim.x * kRB) {
+        const float xi = x[i];
+        k1 += xi <= t1;
+        k2 += xi <= t2;
+        k3 += xi <= t3;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        k1 += __shfl_down(k1, off, 64);
+        k2 += __shfl_down(k2, off, 64);
+        k3 += __shfl_down(k3, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&s->cnt[0], k1);
+        atomicAdd(&s->cnt[1], k2);
+        atomicAdd(&s->cnt[2], k3);
+    }
+}
+
+__global__ void dra_map_kernel(const float* __restrict__ x, size_t n, const DraStats* s, float t, float eta,
+                               float* __restrict__ out) {
+    float a, t1, t2, t3, c;
+    dra_thresholds(s, n, a, t1, t2, t3, c);
+    const float p1 = (float)((double)s->cnt[0] / (double)n), p2 = (float)((double)s->cnt[1] / (double)n),
+                p3 = (float)((double)s->cnt[2] / (double)n);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float xi = x[i];
+        float y;
+        if (xi <= t1) y = p1 * (xi - a) / (t1 - a);
+        else if (xi <= t2) y = (p2 - p1) * (xi - t1) / (t2 - t1) + p1;
+        else if (xi <= t3) y = (p3 - p2) * (xi - t2) / (t3 - t2) + p2;
+        else y = (1.f - p3) * (xi - t3) / (c - t3) + p3;
+        const float z = powf(y, t);
+        out[i] = z > eta ? z : eta;  // MATLAB max: a NaN loses to eta
     }
 }
 
@@ -478,6 +719,13 @@ static hipError_t pen_launch_p(const V* x, int nx, int ny, int nz, double delta,
     } else if (mode == 2) {
         hipLaunchKernelGGL((pen_map_kernel<2, POT, V>), dim3(grid_for(n)), dim3(256), 0, st, x, nx, ny, nz, (V)delta,
                            out);
+    } else if (mode == 3) {
+        if constexpr (POT == POT_HUBER) {
+            hipLaunchKernelGGL((pen_map_kernel<3, POT, V>), dim3(grid_for(n)), dim3(256), 0, st, x, nx, ny, nz,
+                               (V)delta, out);
+        } else {
+            return hipErrorInvalidValue;  // mode 3 is defined for the Huber penalty only
+        }
     } else if (mode == 0) {
         RedScratch rs;
         hipError_t e = scratch(rs);
@@ -490,12 +738,16 @@ static hipError_t pen_launch_p(const V* x, int nx, int ny, int nz, double delta,
     return hipGetLastError();
 }
 
-// pfun: POT_HUBER, POT_QUAD or POT_HYPER
+// pfun: one of Pot
 #define CT_POT_DISPATCH(pfun, CALL)                     \
     switch (pfun) {                                     \
         case POT_HUBER: return CALL(POT_HUBER);          \
         case POT_QUAD: return CALL(POT_QUAD);            \
         case POT_HYPER: return CALL(POT_HYPER);          \
+        case POT_ANISO: return CALL(POT_ANISO);          \
+        case POT_ISOHUBER: return CALL(POT_ISOHUBER);    \
+        case POT_TV: return CALL(POT_TV);                \
+        case POT_ISOTV: return CALL(POT_ISOTV);          \
         default: return hipErrorInvalidValue;           \
     }
 
@@ -555,6 +807,64 @@ hipError_t nesterov_launch(const double* x, const double* z, double* s, const do
 #define CALL(P) nesterov64_p<P>(x, z, s, g, precom, x_out, z_out, nx, ny, nz, beta, delta, t, t_next, st)
     CT_POT_DISPATCH(pfun, CALL)
 #undef CALL
+}
+
+// ---- NU-SQS launchers ---------------------------------------------------------------
+hipError_t nusqs_seed_launch(const float* x, float* v, int nx, int ny, int nz, double eps, hipStream_t st) {
+    const size_t n = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL(nusqs_seed_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, v, nx, ny, nz, (float)eps);
+    return hipGetLastError();
+}
+
+hipError_t div_launch(const float* a, const float* b, float* o, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(div_kernel, dim3(grid_for(n)), dim3(256), 0, st, a, b, o, n);
+    return hipGetLastError();
+}
+
+hipError_t absdiff_floor_launch(const float* x, const float* xref, float* o, size_t n, double eps, hipStream_t st) {
+    hipLaunchKernelGGL(absdiff_floor_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, xref, o, n, (float)eps);
+    return hipGetLastError();
+}
+
+template <int POT>
+static hipError_t nusqs_p(const float* x, float* x_out, const float* g, const float* dl, const float* h, int nx,
+                          int ny, int nz, double beta, double delta, hipStream_t st) {
+    const size_t n = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL((nusqs_kernel<POT>), dim3(grid_for(n)), dim3(256), 0, st, x, x_out, g, dl, h, nx, ny, nz, beta,
+                       (float)delta);
+    return hipGetLastError();
+}
+
+hipError_t nusqs_launch(const float* x, float* x_out, const float* g, const float* dl, const float* h, int nx, int ny,
+                        int nz, double beta, double delta, int pfun, hipStream_t st) {
+#define CALL(P) nusqs_p<P>(x, x_out, g, dl, h, nx, ny, nz, beta, delta, st)
+    CT_POT_DISPATCH(pfun, CALL)
+#undef CALL
+}
+
+static DraStats* g_dra[16];
+
+hipError_t dynamic_range_launch(const float* x, size_t n, double t, double eta, float* out, hipStream_t st) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 16) return hipErrorInvalidDevice;
+    if (!g_dra[dev]) {  // allocated once per device, kept until the process ends
+        void* p = nullptr;
+        e = hipMalloc(&p, sizeof(DraStats));
+        if (e != hipSuccess) return e;
+        g_dra[dev] = (DraStats*)p;
+    }
+    RedScratch rs;
+    e = scratch(rs);
+    if (e != hipSuccess) return e;
+    DraStats* s = g_dra[dev];
+    hipLaunchKernelGGL(dra_init_kernel, dim3(1), dim3(1), 0, st, s);
+    hipLaunchKernelGGL(dra_stats_kernel, dim3(red_grid(n)), dim3(kRB), 0, st, x, n, s, rs);
+    hipLaunchKernelGGL(dra_count_kernel, dim3(red_grid(n)), dim3(kRB), 0, st, x, n, s);
+    hipLaunchKernelGGL(dra_map_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, n, (const DraStats*)s, (float)t,
+                       (float)eta, out);
+    return hipGetLastError();
 }
 
 __global__ void add_i32_kernel(int* p, int inc) {

@@ -36,11 +36,12 @@ from .projector import Projector
 from .shell import subset_views
 
 
-PENALTIES = {"huber": 0, "quad": 1, "hyper": 2}
+PENALTIES = {"huber": 0, "quad": 1, "hyper": 2, "aniso": 3, "isohuber": 4, "tv": 5, "isotv": 6}
 
 
 def penalty_code(pfun) -> int:
-    """penalty name of penalty/loadPenaltyOperator.m ('huber', 'quad', 'hyper') -> CTSIM_PEN_*"""
+    """penalty name of penalty/loadPenaltyOperator.m:22-50 ('huber', 'quad', 'hyper', 'aniso',
+    'isohuber', 'tv', 'isotv') -> CTSIM_PEN_*"""
     if isinstance(pfun, int):
         return pfun
     try:
@@ -334,6 +335,107 @@ class PwlsLALM:
 
     def run(self, x0: np.ndarray, itnlim: int, stop_crit: float = 1e-5):
         """reconPwlsLALMOs main loop (stops after iteration 5 once rmsd < stop_crit)"""
+        self.setup(x0)
+        phis, rmsds = [], []
+        for itn in range(1, itnlim + 1):
+            out = self.iterate()
+            phis.append(out["phi"])
+            rmsds.append(out["rmsd"])
+            if itn > 5 and out["rmsd"] < stop_crit:
+                break
+        return self.image(), np.array(phis), np.array(rmsds)
+
+
+class PwlsNUSQS:
+    """Spatially non-uniform SQS on the device (statsrecon/pwls/reconPwlsSeNusqs.m:32-121), FP32:
+
+        v = max(2 (|G1 x0| + |G2 x0|) + |x0|, 1e-3)            Sobel seed (:47-51)
+        dL = At(w .* A(v)) ./ v                                (:54)
+        itn = 1, 4, 7:  dLt = dL;  dRt = beta H(v)             (:66-68)
+        itn = 2, 5, 8:  xref = x                               (:69-70)
+        itn = 3, 6, 9:  v = DRA(max(|x - xref|, 1e-3), 40, 0.05);  dLt = At(w .* A(v)) ./ v   (:71-75)
+        per subset:     x = max(x - (Atos(Os(w) .* (Aos(x) - Os(y))) + beta S(x)) ./ (dLt + dRt), 0)
+
+    At(w .* A(v)) is summed subset by subset over the subset-major y/w of PwlsOS (every view
+    once, scale 1).  H is the Huber curvature of huberPenalty.m mode 3 whatever pfun is, as
+    loadPenaltyOperator.m:54 defines it."""
+
+    EPS = 1e-3
+    NLOOP, NFIX = 3, 10
+
+    def __init__(self, geom: Geometry, y: np.ndarray, w: np.ndarray, numos: int, beta: float, delta: float,
+                 pfun: str = "huber", op: str = "tf", stream: Stream | None = None):
+        self.base = PwlsOS(geom, y, w, numos, beta, delta, precision="f32", graph=False, op=op, stream=stream,
+                           pfun=pfun)
+        nvox = self.base.P.n_img
+        self.v = DeviceArray(nvox, F32)
+        self.dl = DeviceArray(nvox, F32)    # dL of the seed image
+        self.dlt = DeviceArray(nvox, F32)   # dLtilde in use
+        self.h = DeviceArray(nvox, F32)     # H(v); the update scales it by beta
+        self.xref = DeviceArray(nvox, F32)
+        self.tmp = DeviceArray(nvox, F32)
+
+    def _data_curvature(self, out: DeviceArray) -> None:
+        """out = At(w .* A(v)) ./ v"""
+        b = self.base
+        st = b.stream
+        for k, (v0, vs, ns, off) in enumerate(b.subsets):
+            pw = b._window(b.p, 0, ns)
+            b.P.proj(self.v, pw, f"proj,{b.kind}", v0, vs, ns, stream=st)
+            call("ctsim_mul", pw.vp, F32, b._sub_ptr(b.w, off), b.w.dtype, pw.vp, F32, ns * b.cell, b._sv())
+            b.P.back(pw, self.tmp, f"back,{b.kind}", v0, vs, ns, scale=b.wscale, accumulate=k > 0, stream=st)
+        call("ctsim_div", self.tmp.vp, self.v.vp, out.vp, out.count, b._sv())
+
+    def setup(self, x0: np.ndarray) -> None:
+        b = self.base
+        b.setup(x0)
+        nx, ny, nz = b.nxyz
+        call("ctsim_nusqs_seed", b.x[b.cur].vp, self.v.vp, nx, ny, nz, self.EPS, b._sv())
+        self._data_curvature(self.dl)
+        b.stream.synchronize()
+        self.itn = 0
+
+    def iterate(self) -> dict:
+        b = self.base
+        st = b.stream
+        nx, ny, nz = b.nxyz
+        itn = self.itn + 1
+        L, F = self.NLOOP, self.NFIX
+        if itn % L == 1 and itn < F:
+            self.dlt.copy_from(self.dl, stream=st)
+            call("ctsim_penalty", self.v.vp, nx, ny, nz, 0, b.delta, 3, self.h.vp, None, b._sv())
+        elif itn % L == L - 1 and itn <= F - 2:
+            self.xref.copy_from(b.x[b.cur], stream=st)
+        elif itn % L == 0 and itn <= F - 1:
+            call("ctsim_absdiff_floor", b.x[b.cur].vp, self.xref.vp, self.tmp.vp, self.tmp.count, self.EPS, b._sv())
+            call("ctsim_dynamic_range", self.tmp.vp, self.tmp.count, 40.0, 0.05, self.v.vp, b._sv())
+            self._data_curvature(self.dlt)
+        b.xold.copy_from(b.x[b.cur], stream=st)
+        call("ctsim_fill", C.c_void_p(b.scal.ptr), 3, 1, 0.0, b._sv())
+        for k, (v0, vs, ns, off) in enumerate(b.subsets):
+            x = b.x[b.cur]
+            pw = b._window(b.p, 0, ns)
+            b.P.proj(x, pw, f"proj,{b.kind}", v0, vs, ns, stream=st)
+            call("ctsim_pwls_residual", pw.vp, b.p.dtype, b._sub_ptr(b.y, off), b.y.dtype, b._sub_ptr(b.w, off),
+                 b.w.dtype, b.r.vp, b.r.dtype, ns * b.cell, b.wscale, C.c_void_p(b.scal.ptr), b._sv())
+            b.P.back(b._window(b.r, 0, ns), b.g, f"back,{b.kind}", v0, vs, ns, scale=float(b.M) * b.wscale, stream=st)
+            call("ctsim_pwls_nusqs_update", x.vp, b.x[1 - b.cur].vp, b.g.vp, self.dlt.vp, self.h.vp, nx, ny, nz,
+                 b.beta, b.delta, b.pfun, b._sv())
+            b.cur = 1 - b.cur
+        xc = b.x[b.cur]
+        call("ctsim_penalty", xc.vp, nx, ny, nz, b.pfun, b.delta, 0, None, C.c_void_p(b.scal.ptr + 8), b._sv())
+        call("ctsim_sqdiff_sum", xc.vp, b.xold.vp, xc.count, C.c_void_p(b.scal.ptr + 16), b._sv())
+        st.synchronize()
+        self.itn = itn
+        sc = b.scal.to_numpy()
+        return dict(phi=float(sc[0] + b.beta * sc[1]), penalty=float(b.beta * sc[1]),
+                    rmsd=float(math.sqrt(sc[2] / b.P.n_img)))
+
+    def image(self) -> np.ndarray:
+        return self.base.image()
+
+    def run(self, x0: np.ndarray, itnlim: int, stop_crit: float = 1e-5):
+        """reconPwlsSeNusqs main loop (stops after iteration 5 once rmsd < stop_crit)"""
         self.setup(x0)
         phis, rmsds = [], []
         for itn in range(1, itnlim + 1):

@@ -261,12 +261,22 @@ int ctsim_pwls_residual(const void* d_p, int32_t p_dtype, const void* d_y, int32
 /* Roughness penalties of a ZXY volume (penalty/loadPenaltyOperator.m:22-53): pfun
  * CTSIM_PEN_HUBER (huberPenalty.m:101-161), CTSIM_PEN_QUAD (quadPenalty.m; T is the
  * constant 4 + 4/sqrt2 (+1 in 3D) of the reference), CTSIM_PEN_HYPER (hyperbolaPenalty.m,
- * psi = delta^2 (sqrt(1 + (d/delta)^2) - 1)).  Neighbours +-x, +-y weight 1, xy diagonals
- * 1/sqrt2, +-z weight 1/2.  mode 1 gradient S(x), mode 2 curvature T(x), written to
- * d_out; mode 0 cost R(x) accumulated in FP64 into d_cost. */
+ * psi = delta^2 (sqrt(1 + (d/delta)^2) - 1)), CTSIM_PEN_ANISO (anisotropicPenalty.m:79-142,
+ * c = exp(-d^2/delta^2), S adds d c), all over the neighbours +-x, +-y weight 1, xy
+ * diagonals 1/sqrt2, +-z weight 1/2; CTSIM_PEN_ISOHUBER (huberIsoPenalty.m:73-104, axis
+ * neighbours, weight 1); CTSIM_PEN_TV and CTSIM_PEN_ISOTV (totalVariationPenalty.m:53-92,
+ * totalVariationIsoPenalty.m:53-92: c = 1/sqrt(sum d^2 + 1e-8) on the grid cropped by one,
+ * x/y differences, plus z for the isotropic form of a 3D volume; delta unused).
+ * mode 1 gradient S(x), mode 2 curvature T(x), written to d_out; mode 0 cost R(x)
+ * accumulated in FP64 into d_cost; mode 3 (CTSIM_PEN_HUBER only) the curvature H of
+ * huberPenalty.m:163-185, sum_k w_k [|x_i - x_k| <= delta] (x_i + x_k) / x_i, used by NU-SQS. */
 #define CTSIM_PEN_HUBER 0
 #define CTSIM_PEN_QUAD 1
 #define CTSIM_PEN_HYPER 2
+#define CTSIM_PEN_ANISO 3
+#define CTSIM_PEN_ISOHUBER 4
+#define CTSIM_PEN_TV 5
+#define CTSIM_PEN_ISOTV 6
 int ctsim_penalty(const float* d_x, int32_t nx, int32_t ny, int32_t nz, int32_t pfun, double delta, int32_t mode,
                   float* d_out, double* d_cost, void* stream);
 int ctsim_penalty_f64(const double* d_x, int32_t nx, int32_t ny, int32_t nz, int32_t pfun, double delta, int32_t mode,
@@ -317,6 +327,26 @@ int ctsim_pwls_lalm_sg(const float* d_dl, float* d_g, float* d_s, size_t n, doub
 int ctsim_pwls_lalm_inner(const float* d_x, const float* d_xold, const float* d_s, const float* d_precom,
                           float* d_x_out, int32_t nx, int32_t ny, int32_t nz, double beta, double delta, int32_t pfun,
                           double rho, void* stream);
+
+/* NU-SQS (statsrecon/pwls/reconPwlsSeNusqs.m:47-90).
+ * ctsim_nusqs_seed: v = max(2 (|G1*x| + |G2*x|) + |x|, eps), G1 = [1 2 1; 0 0 0; -1 -2 1],
+ *   G2 = [1 0 -1; 2 0 -2; 1 0 -1] correlated per z slice with replicated edges
+ *   (filter/imageFilter2D.m; :47-51).  d_v must not alias d_x.
+ * ctsim_div: out = a ./ b (dL = At(w .* A(v)) ./ v, :54 and :75).
+ * ctsim_absdiff_floor: out = max(|x - xref|, eps) (:72-73).
+ * ctsim_dynamic_range: statsrecon/pwls/dynamicRangeAdjustment.m, out = max(F(x)^t, eta)
+ *   with F the piecewise-linear empirical CDF through min, (min+mean)/2, mean,
+ *   (mean+max)/2, max; x must be non-negative (the NU-SQS input is floored at eps).
+ *   Uses a small per-device statistics block owned by the library.
+ * ctsim_pwls_nusqs_update: x_out = max(x - (g + beta S(x)) ./ (dl + beta h), 0), NaN/Inf
+ *   -> 0, the denominator in FP64 (:81-90), with h = H(v) (ctsim_penalty mode 3); d_x_out
+ *   must not alias d_x. */
+int ctsim_nusqs_seed(const float* d_x, float* d_v, int32_t nx, int32_t ny, int32_t nz, double eps, void* stream);
+int ctsim_div(const float* d_a, const float* d_b, float* d_out, size_t n, void* stream);
+int ctsim_absdiff_floor(const float* d_x, const float* d_xref, float* d_out, size_t n, double eps, void* stream);
+int ctsim_dynamic_range(const float* d_x, size_t n, double t, double eta, float* d_out, void* stream);
+int ctsim_pwls_nusqs_update(const float* d_x, float* d_x_out, const float* d_g, const float* d_dl, const float* d_h,
+                            int32_t nx, int32_t ny, int32_t nz, double beta, double delta, int32_t pfun, void* stream);
 
 /* sum((a - b).^2) in FP64 into d_out (RMSD numerator, reconPwlsSeNesterovSqs.m:103). */
 int ctsim_sqdiff_sum(const float* d_a, const float* d_b, size_t n, double* d_out, void* stream);

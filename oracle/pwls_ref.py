@@ -5,6 +5,14 @@ TEST INFRASTRUCTURE ONLY (imported by tests/): the checker of ctsim_amd.pwls.
   huber(x, mode, delta)          penalty/huberPenalty.m:101-161 (3D), helpers :222-238
   quad(x, mode)                  penalty/quadPenalty.m (mode 2 is the constant 4 + 4/sqrt2 + 1)
   hyper(x, mode, delta)          penalty/hyperbolaPenalty.m (sumHyper/dHyper/cHyper :147-157)
+  aniso(x, mode, delta)          penalty/anisotropicPenalty.m (sumAniso/dAniso/cAniso :144-155)
+  isohuber(x, mode, delta)       penalty/huberIsoPenalty.m:70-105 (axis pairs, weight 1)
+  tv(x, mode), isotv(x, mode)    penalty/totalVariationPenalty.m:55-95, totalVariationIsoPenalty.m:55-92
+  huber(x, 3, delta)             penalty/huberPenalty.m mode 3 (the NU-SQS curvature H)
+  sobel_edges(x)                 filter/imageFilter2D.m with the two Sobel kernels of
+                                 statsrecon/pwls/reconPwlsSeNusqs.m:48-49 (imfilter 'replicate')
+  dynamic_range(x, t, eta)       statsrecon/pwls/dynamicRangeAdjustment.m
+  nusqs(...)                     statsrecon/pwls/reconPwlsSeNusqs.m:32-121
   nesterov_sqs(...)              statsrecon/pwls/reconPwlsSeNesterovSqs.m:37-137
   lalm_os(...)                   statsrecon/pwls/reconPwlsLALMOs.m:37-110
 
@@ -37,6 +45,15 @@ def _pairs(x):
 
 def huber(x, mode, delta):
     x = np.asarray(x, np.float64)
+    if mode == 3:
+        # H(x) = sum over pairs w [|d| <= delta] (x_i + x_k), added to both, divided by x
+        y = np.zeros_like(x)
+        for a, b, wt in _pairs(x):
+            c = wt * (np.abs(x[a] - x[b]) <= delta) * (x[a] + x[b])
+            y[a] += c
+            y[b] += c
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return y / x
     if mode == 0:
         y = 0.0
         for a, b, wt in _pairs(x):
@@ -93,15 +110,136 @@ def hyper(x, mode, delta):
                          lambda d: delta ** 2 * (np.sqrt(1 + (d / delta) ** 2) - 1))
 
 
+def aniso(x, mode, delta):
+    """anisotropic quadratic with Gaussian weighting: c = exp(-d^2/delta^2), S adds d c, T adds
+    c, R sums d^2 c / 2 (the reference's S is d c, not the derivative of its R)"""
+    c = lambda d: np.exp(-d ** 2 / delta ** 2)  # noqa: E731
+    return _pair_penalty(x, mode, lambda d: d * c(d), c, lambda d: d ** 2 * c(d) / 2)
+
+
+def _axis_pairs(x):
+    """huberIsoPenalty.m: x, y (and z when the volume is 3D) neighbours, weight 1 each"""
+    P = [((slice(0, -1), slice(None), slice(None)), (slice(1, None), slice(None), slice(None)), 1.0),
+         ((slice(None), slice(0, -1), slice(None)), (slice(None), slice(1, None), slice(None)), 1.0)]
+    if x.shape[2] > 1:
+        P.append(((slice(None), slice(None), slice(0, -1)), (slice(None), slice(None), slice(1, None)), 1.0))
+    return P
+
+
+def isohuber(x, mode, delta):
+    x = np.asarray(x, np.float64)
+    if mode == 0:
+        y = 0.0
+        for a, b, wt in _axis_pairs(x):
+            d = np.abs(x[a] - x[b])
+            y += np.where(d > delta, delta * d - delta ** 2 / 2, d ** 2 / 2).sum()
+        return y
+    y = np.zeros_like(x)
+    for a, b, wt in _axis_pairs(x):
+        d = x[a] - x[b]
+        if mode == 1:
+            v = np.clip(d, -delta, delta)
+            y[a] += v
+            y[b] -= v
+        elif mode == 2:
+            v = (np.abs(d) <= delta).astype(np.float64)
+            y[a] += v
+            y[b] += v
+        else:
+            raise ValueError("wrong penalty mode")
+    return y
+
+
+TV_EPS = 1e-8
+
+
+def _tv(x, mode, iso):
+    """total variation on the grid cropped by one in every dimension: c = 1/sqrt(sum d_k^2 + eps)
+    with d_k = x(i) - x(i + e_k) over x, y (and z for the isotropic 3D form); S adds
+    sum_k d_k c at i and -d_k c at i + e_k, T adds c at i once per direction and at each i + e_k"""
+    x = np.asarray(x, np.float64)
+    three = x.shape[2] > 1
+    crop = (slice(0, -1), slice(0, -1), slice(0, -1) if three else slice(None))
+    shifts = [(1, 0, 0), (0, 1, 0)] + ([(0, 0, 1)] if (iso and three) else [])
+    nx, ny, nz = x.shape
+    cx, cy, cz = nx - 1, ny - 1, (nz - 1 if three else nz)
+
+    def at(e):
+        return (slice(e[0], e[0] + cx), slice(e[1], e[1] + cy), slice(e[2], e[2] + cz))
+
+    ds = [x[crop] - x[at(e)] for e in shifts]
+    c = 1.0 / np.sqrt(sum(d * d for d in ds) + TV_EPS)
+    if mode == 0:
+        return float(np.sum(1.0 / c))
+    y = np.zeros_like(x)
+    for e, d in zip(shifts, ds):
+        if mode == 1:
+            y[crop] += d * c
+            y[at(e)] -= d * c
+        elif mode in (2, 3):
+            y[crop] += c
+            y[at(e)] += c
+        else:
+            raise ValueError("wrong penalty mode")
+    return y
+
+
+def tv(x, mode):
+    return _tv(x, mode, iso=False)
+
+
+def isotv(x, mode):
+    return _tv(x, mode, iso=True)
+
+
 def penalty(pfun, x, mode, delta):
-    """loadPenaltyOperator.m:22-53 for 'huber', 'quad', 'hyper'"""
+    """loadPenaltyOperator.m:22-53"""
     if pfun == "huber":
         return huber(x, mode, delta)
     if pfun == "quad":
         return quad(x, mode)
     if pfun == "hyper":
         return hyper(x, mode, delta)
+    if pfun == "aniso":
+        return aniso(x, mode, delta)
+    if pfun == "isohuber":
+        return isohuber(x, mode, delta)
+    if pfun == "isotv":
+        return isotv(x, mode)
+    if pfun == "tv":
+        return tv(x, mode)
     raise ValueError("unknow penalty function")
+
+
+def sobel_edges(x):
+    """|imfilter(x, [1 2 1; 0 0 0; -1 -2 1])| + |imfilter(x, [1 0 -1; 2 0 -2; 1 0 -1])| per z
+    slice; imfilter correlates, 'replicate' repeats the edge voxels (MATLAB rows = x)"""
+    x = np.asarray(x, np.float64)
+    p = np.pad(x, ((1, 1), (1, 1), (0, 0)), mode="edge")
+    P = lambda a, b: p[1 + a:1 + a + x.shape[0], 1 + b:1 + b + x.shape[1], :]  # noqa: E731  offset (a, b)
+    g1 = (P(-1, -1) + 2 * P(-1, 0) + P(-1, 1)) - (P(1, -1) + 2 * P(1, 0) + P(1, 1))
+    g2 = (P(-1, -1) + 2 * P(0, -1) + P(1, -1)) - (P(-1, 1) + 2 * P(0, 1) + P(1, 1))
+    return np.abs(g1) + np.abs(g2)
+
+
+def dynamic_range(x, t, eta):
+    """max(F(x)^t, eta), F the piecewise-linear empirical CDF through (min, 0), (t1, P1),
+    (mean, P2), (t3, P3), (max, 1) with t1 = (min + mean)/2, t3 = (mean + max)/2"""
+    x = np.asarray(x, np.float64)
+    a, b, c = x.min(), x.mean(), x.max()
+    t1, t2, t3 = (a + b) / 2, b, (b + c) / 2
+    n = x.size
+    p1, p2, p3 = (x <= t1).sum() / n, (x <= t2).sum() / n, (x <= t3).sum() / n
+    y = np.zeros_like(x)
+    m = x <= t1
+    y[m] = p1 * (x[m] - a) / (t1 - a)
+    m = (x > t1) & (x <= t2)
+    y[m] = (p2 - p1) * (x[m] - t1) / (t2 - t1) + p1
+    m = (x > t2) & (x <= t3)
+    y[m] = (p3 - p2) * (x[m] - t2) / (t3 - t2) + p2
+    m = x > t3
+    y[m] = (1 - p3) * (x[m] - t3) / (c - t3) + p3
+    return np.maximum(y ** t, eta)
 
 
 def subset_geom(g: O.G3, M: int, isub: int):
@@ -169,6 +307,51 @@ def lalm_os(y, w, g: O.G3, beta, delta, itnlim, numos, x0, rho=0.1, pfun="huber"
                     beta * penalty(pfun, x, 2, delta) + rho * precom)
                 x[(x < 0) | ~np.isfinite(x)] = 0
             phi += np.sum(w[:, :, idx] * r ** 2) / 2
+        phi += beta * penalty(pfun, x, 0, delta)
+        phis.append(phi)
+        rmsds.append(math.sqrt(np.mean((x - xold) ** 2)))
+    return x, np.array(phis), np.array(rmsds)
+
+
+NUSQS_EPS = 1e-3
+
+
+def nusqs(y, w, g: O.G3, beta, delta, itnlim, numos, x0, pfun="huber", map_=None, kind="tf"):
+    """reconPwlsSeNusqs.m:32-121 (epsilon 1e-3, nloop 3, nfix 10); returns x, phis, rmsds (FP64)"""
+    A = lambda v: O.cbct(f"proj,{kind}", g, v, map_, prec=64)  # noqa: E731
+    At = lambda s: O.cbct(f"back,{kind}", g, s, map_, prec=64)  # noqa: E731
+    subs = [subset_geom(g, numos, isub) for isub in range(1, numos + 1)]
+    x = np.asarray(x0, np.float64).copy()
+    v = 2 * sobel_edges(x) + np.abs(x)
+    v[v < NUSQS_EPS] = NUSQS_EPS
+    with np.errstate(divide="ignore", invalid="ignore"):
+        dL = At(w * A(v)) / v
+    nloop, nfix = 3, 10
+    dLt = dRt = None
+    xref = None
+    phis, rmsds = [], []
+    for itn in range(1, itnlim + 1):
+        if itn % nloop == 1 and itn < nfix:
+            dLt = dL
+            dRt = beta * huber(v, 3, delta)
+        elif itn % nloop == nloop - 1 and itn <= nfix - 2:
+            xref = x.copy()
+        elif itn % nloop == 0 and itn <= nfix - 1:
+            v = np.abs(x - xref)
+            v[v < NUSQS_EPS] = NUSQS_EPS
+            v = dynamic_range(v, 40, 0.05)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                dLt = At(w * A(v)) / v
+        xold = x.copy()
+        phi = 0.0
+        for isub in range(1, numos + 1):
+            gs, idx = subs[isub - 1]
+            d = O.cbct(f"proj,{kind}", gs, x, map_, prec=64) - y[:, :, idx]
+            grad = numos * O.cbct(f"back,{kind}", gs, w[:, :, idx] * d, map_, prec=64) + beta * penalty(
+                pfun, x, 1, delta)
+            x = x - grad / (dLt + dRt)
+            x[(x < 0) | ~np.isfinite(x)] = 0
+            phi += np.sum(w[:, :, idx] * d ** 2) / 2
         phi += beta * penalty(pfun, x, 0, delta)
         phis.append(phi)
         rmsds.append(math.sqrt(np.mean((x - xold) ** 2)))

@@ -116,7 +116,7 @@ def test_pwls_fp64_matches_fp64_oracle():
     np.testing.assert_allclose(rm, rmsds, rtol=1e-7)
 
 
-@pytest.mark.parametrize("pfun", ["quad", "hyper"])
+@pytest.mark.parametrize("pfun", ["quad", "hyper", "aniso", "isohuber", "tv", "isotv"])
 def test_penalty_family_kernels_match_oracle(pfun):
     from oracle import pwls_ref as R
     from ctsim_amd._lib import call
@@ -164,3 +164,79 @@ def test_lalm_os_matches_fp64_oracle():
     assert rel_l2(x, want) < 1e-4
     np.testing.assert_allclose(ph, phis, rtol=1e-4)
     np.testing.assert_allclose(rm, rmsds, rtol=1e-3)
+
+
+@pytest.mark.parametrize("pfun", ["huber", "quad", "hyper", "aniso", "isohuber", "tv", "isotv"])
+@pytest.mark.parametrize("shape", [(9, 7, 5), (9, 7, 1)])
+def test_penalty_family_fp32_fp64_small_volumes(pfun, shape):
+    """every penalty and mode on a 3D and a 2D (nz = 1) volume, FP32 and FP64 kernels"""
+    from oracle import pwls_ref as R
+    from ctsim_amd._lib import call
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.pwls import penalty_code
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0.001, 0.011, shape)
+    nx, ny, nz = shape
+    zxy = np.ascontiguousarray(np.transpose(x, (1, 0, 2)))  # device order: z fastest, then x, then y
+    delta = 0.003
+    modes = (1, 2, 3) if pfun == "huber" else (1, 2)
+    for dt, fn, tol in (("f32", "ctsim_penalty", 2e-6), ("f64", "ctsim_penalty_f64", 1e-12)):
+        dx = DeviceArray.from_numpy(zxy.ravel(), dt)
+        for mode in modes:
+            out = DeviceArray(x.size, dt)
+            call(fn, dx.vp, nx, ny, nz, penalty_code(pfun), delta, mode, out.vp, None, None)
+            got = np.transpose(out.to_numpy().reshape(ny, nx, nz), (1, 0, 2))
+            want = R.penalty(pfun, x, mode, delta)
+            assert rel_l2(got, want) < tol, (pfun, shape, dt, mode, rel_l2(got, want))
+        cost = DeviceArray(1, "f64")
+        call(fn, dx.vp, nx, ny, nz, penalty_code(pfun), delta, 0, None, cost.vp, None)
+        assert cost.to_numpy()[0] == pytest.approx(R.penalty(pfun, x, 0, delta), rel=max(tol, 1e-9))
+
+
+def test_nusqs_seed_and_dynamic_range_match_oracle():
+    from oracle import pwls_ref as R
+    from ctsim_amd._lib import call
+    from ctsim_amd.device import DeviceArray
+    rng = np.random.default_rng(12)
+    shape = (23, 17, 6)
+    nx, ny, nz = shape
+    x = rng.uniform(0, 0.05, shape)
+    x[4:9, 3:7, :] += 0.2
+    to_dev = lambda a: DeviceArray.from_numpy(np.ascontiguousarray(np.transpose(a, (1, 0, 2))).ravel(), "f32")  # noqa: E731
+    from_dev = lambda d: np.transpose(d.to_numpy().reshape(ny, nx, nz), (1, 0, 2))  # noqa: E731
+    dx = to_dev(x)
+    v = DeviceArray(x.size, "f32")
+    call("ctsim_nusqs_seed", dx.vp, v.vp, nx, ny, nz, 1e-3, None)
+    want = np.maximum(2 * R.sobel_edges(x) + np.abs(x), 1e-3)
+    assert rel_l2(from_dev(v), want) < 1e-6
+    xr = np.clip(x + rng.normal(0, 0.01, shape), 0, None)
+    dxr = to_dev(xr)
+    d = DeviceArray(x.size, "f32")
+    call("ctsim_absdiff_floor", dx.vp, dxr.vp, d.vp, x.size, 1e-3, None)
+    # the same FP32 operands and operations: bit-exact
+    x32, xr32 = x.astype(np.float32), xr.astype(np.float32)
+    dref = np.maximum(np.abs(x32 - xr32), np.float32(1e-3))
+    np.testing.assert_array_equal(from_dev(d), dref)
+    z = DeviceArray(x.size, "f32")
+    for _ in range(2):  # the statistics block is reused: a second call must give the same map
+        call("ctsim_dynamic_range", d.vp, x.size, 40.0, 0.05, z.vp, None)
+        np.testing.assert_allclose(from_dev(z), R.dynamic_range(dref.astype(np.float64), 40, 0.05), rtol=2e-4,
+                                   atol=1e-6)
+    q = DeviceArray(x.size, "f32")
+    call("ctsim_div", dx.vp, d.vp, q.vp, x.size, None)
+    np.testing.assert_array_equal(from_dev(q), x32 / dref)
+
+
+def test_nusqs_matches_fp64_oracle():
+    """four iterations cover every branch of the NU-SQS schedule (seed curvature, xref, the
+    dynamic-range update of v and dLtilde, and the return to the seed dL with H(v_new))"""
+    from oracle import pwls_ref as R
+    from ctsim_amd.pwls import PwlsNUSQS
+    g, y, w, x0 = _problem(seed=13)
+    beta, delta, M, its = 50.0, 1e-3, 4, 4
+    want, phis, rmsds = R.nusqs(y, w, g, beta, delta, its, M, x0)
+    sol = PwlsNUSQS(_geometry(g), y, w, M, beta, delta)
+    x, ph, rm = sol.run(x0.astype(np.float32), its)
+    print("nusqs vs fp64 oracle:", rel_l2(x, want), ph, phis)
+    assert rel_l2(x, want) < 1e-4
+    np.testing.assert_allclose(ph, phis, rtol=1e-4)
