@@ -615,7 +615,19 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
 // FDK voxel-driven bilinear backprojection (cbct_back1_pd).  off_boundary = 1 is
 // 'back,pd' (skip below the detector, stop above it), 0 is 'back,hpd' (clamp v).
 // One wavefront per image column, NZL voxels per lane.
+//
+// Every wave walks all views, so what stays in an XCD's L2 is the detector footprint of
+// the columns resident on that XCD over the few views they are apart.  Blocks therefore
+// take 2 x 2 column patches in super-tiles of 32 x 32 columns (256 blocks), and each XCD
+// takes a contiguous range of super-tiles: the resident columns form a compact patch
+// whose per-view footprint is about a tenth of the detector, not two full image rows.
 // ---------------------------------------------------------------------------------
+constexpr int kPdTile = 32;  // super-tile edge in columns
+static_assert(kWavesPerBlock == 4, "a block's waves form a 2 x 2 column patch");
+
+__host__ __device__ inline int pd_blocks(int nx, int ny) {
+    return ((nx + kPdTile - 1) / kPdTile) * ((ny + kPdTile - 1) / kPdTile) * (kPdTile * kPdTile / kWavesPerBlock);
+}
 template <class Tin, class Tout, class T, int NZL>
 __global__ __launch_bounds__(kBlock) void cbct_back_pd_kernel(Geom<T> g, const View<T>* __restrict__ views,
                                                               int vstride, int nviews, const Tin* __restrict__ proj,
@@ -623,9 +635,14 @@ __global__ __launch_bounds__(kBlock) void cbct_back_pd_kernel(Geom<T> g, const V
                                                               Tout* __restrict__ vol, T scale, int accumulate,
                                                               int off_boundary) {
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int colid = bid * kWavesPerBlock + (int)(threadIdx.x / kWave);
-    if (colid >= g.nx * g.ny) return;
-    const int ix = colid % g.nx, iy = colid / g.nx;
+    const int wv = (int)(threadIdx.x / kWave);
+    constexpr int kHalf = kPdTile / 2;  // 2 x 2 patches per super-tile edge
+    const int st = bid / (kHalf * kHalf), r = bid % (kHalf * kHalf);
+    const int stx = (g.nx + kPdTile - 1) / kPdTile;
+    const int ix = (st % stx) * kPdTile + (r % kHalf) * 2 + (wv & 1);
+    const int iy = (st / stx) * kPdTile + (r / kHalf) * 2 + (wv >> 1);
+    if (ix >= g.nx || iy >= g.ny) return;
+    const int colid = iy * g.nx + ix;
     const int lane = lane_id();
     Tout* ocol = vol + (size_t)colid * g.nz;
     if (map != nullptr && map[iy * g.nx + ix] == 0) {
@@ -832,8 +849,9 @@ static hipError_t back3(const Geom<T>& g, const View<T>* views, int vstride, int
                         const uint8_t* map, void* vol, int foot, int pd_mode, T scale, int accumulate,
                         hipStream_t st) {
     const long long cols = (long long)g.nx * g.ny;
-    const unsigned blocks = (unsigned)((cols + kWavesPerBlock - 1) / kWavesPerBlock);
-    if (blocks == 0) return hipSuccess;
+    const unsigned blocks = foot == 2 ? (unsigned)pd_blocks(g.nx, g.ny)
+                                      : (unsigned)((cols + kWavesPerBlock - 1) / kWavesPerBlock);
+    if (cols == 0 || blocks == 0) return hipSuccess;
     if (foot == 2 && (g.nu < 2 || g.nv < 2)) return hipErrorInvalidValue;  // bilinear needs 2x2 cells
     if (g.nz > 8 * kWave) return hipErrorNotSupported;                      // voxel columns up to 512
     const int nzl = (g.nz + kWave - 1) / kWave;

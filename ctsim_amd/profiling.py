@@ -8,6 +8,8 @@ passes), so that a profile names only the kernels it is about.
                                                        # 512x512x256 volume, 1024x384 detector, FP32
     python -m ctsim_amd.profiling fdk-back [reps]      # back,hpd of 720 filtered views (FP16) into a
                                                        # 512^3 FP32 volume, 1024x768 detector
+    python -m ctsim_amd.profiling pwls-update [reps]   # fused Nesterov-SQS update (Huber, FP16 z copy)
+                                                       # on 512x512x128 FP32 volumes
 
 The copy workload calibrates the FETCH_SIZE / WRITE_SIZE counters: it reads and writes
 exactly 256 MiB per launch."""
@@ -64,7 +66,24 @@ def fdk_back(reps: int) -> None:
     synchronize()
 
 
-WORKLOADS = {"copy": copy, "pwls-subset": pwls_subset, "pair-slab": pair_slab, "fdk-back": fdk_back}
+def pwls_update(reps: int) -> None:
+    nx, ny, nz = 512, 512, 128
+    n = nx * ny * nz
+    x = [DeviceArray(n, "f32").fill(0.02), DeviceArray(n, "f32")]
+    z = [DeviceArray(n, "f32").fill(0.02), DeviceArray(n, "f32")]
+    s = DeviceArray(n, "f32").fill(0.02)
+    g = DeviceArray(n, "f32").fill(1e-3)
+    pre = DeviceArray(n, "f32").fill(10.0)
+    zc = DeviceArray(n, "f16")
+    for k in range(reps):
+        a, b = k % 2, 1 - k % 2
+        call("ctsim_pwls_nesterov_update", x[a].vp, z[a].vp, s.vp, g.vp, pre.vp, x[b].vp, z[b].vp, zc.vp, zc.dtype,
+             nx, ny, nz, 1e4, 1e-2, 0, 1.0 + k, 2.0 + k, None)
+    synchronize()
+
+
+WORKLOADS = {"copy": copy, "pwls-subset": pwls_subset, "pair-slab": pair_slab, "fdk-back": fdk_back,
+             "pwls-update": pwls_update}
 
 
 def main(argv=None) -> int:

@@ -240,3 +240,39 @@ def test_nusqs_matches_fp64_oracle():
     print("nusqs vs fp64 oracle:", rel_l2(x, want), ph, phis)
     assert rel_l2(x, want) < 1e-4
     np.testing.assert_allclose(ph, phis, rtol=1e-4)
+
+
+@pytest.mark.parametrize("nz", [12, 13])
+@pytest.mark.parametrize("pfun", ["huber", "hyper", "isohuber"])
+def test_nesterov_update_step_both_kernel_forms(nz, pfun):
+    """one fused update against an FP64 numpy step: nz = 12 runs the four-voxel form (for the
+    13-neighbour penalties), nz = 13 and the iso-Huber penalty the one-voxel form"""
+    from oracle import pwls_ref as R
+    from ctsim_amd._lib import call
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.pwls import penalty_code
+    rng = np.random.default_rng(14)
+    nx, ny = 9, 7
+    shp = (nx, ny, nz)
+    x, z, s0 = (rng.uniform(0.0, 0.03, shp) for _ in range(3))
+    g = rng.normal(0, 1e-3, shp)
+    pre = rng.uniform(5, 20, shp)
+    beta, delta, t, tn = 40.0, 4e-3, 1.7, 2.3
+    dev = lambda a: DeviceArray.from_numpy(np.ascontiguousarray(np.transpose(a, (1, 0, 2))).ravel(), "f32")  # noqa: E731
+    back = lambda d: np.transpose(d.to_numpy().reshape(ny, nx, nz), (1, 0, 2)).astype(np.float64)  # noqa: E731
+    dx, dz, ds, dg, dp = dev(x), dev(z), dev(s0), dev(g), dev(pre)
+    xo, zo = DeviceArray(x.size, "f32"), DeviceArray(x.size, "f32")
+    zc = DeviceArray(x.size, "f16")
+    call("ctsim_pwls_nesterov_update", dx.vp, dz.vp, ds.vp, dg.vp, dp.vp, xo.vp, zo.vp, zc.vp, zc.dtype, nx, ny, nz,
+         beta, delta, penalty_code(pfun), t, tn, None)
+    f = lambda a: a.astype(np.float32).astype(np.float64)  # noqa: E731  the device's FP32 inputs
+    x, z, s0, g, pre = f(x), f(z), f(s0), f(g), f(pre)
+    du = (g + beta * R.penalty(pfun, z, 1, delta)) / (pre + beta * R.penalty(pfun, x, 2, delta))
+    # z' is formed from the unclamped x' and s' and clamped after (reconPwlsSeNesterovSqs.m:72-84)
+    xn = np.maximum(z - du, 0)
+    sn = np.maximum(s0 - t * du, 0)
+    zn = np.maximum((1 - 1 / tn) * (z - du) + (1 / tn) * (s0 - t * du), 0)
+    np.testing.assert_allclose(back(xo), xn, rtol=1e-5, atol=1e-8)
+    np.testing.assert_allclose(back(ds), sn, rtol=1e-5, atol=1e-8)
+    np.testing.assert_allclose(back(zo), zn, rtol=1e-5, atol=1e-8)
+    np.testing.assert_allclose(back(zc), zn, rtol=1e-3, atol=1e-6)
