@@ -623,17 +623,26 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
 // whose per-view footprint is about a tenth of the detector, not two full image rows.
 // ---------------------------------------------------------------------------------
 constexpr int kPdTile = 32;  // super-tile edge in columns
+
+// The reference tests u and v in double against 1e-6 and n - 1 - 1e-6.  In T these are
+// the same tests against lo = the smallest T whose double value is >= 1e-6 and hi = the
+// largest T whose double value is <= n - 1 - 1e-6 (computed on the host).
+template <class T>
+struct PdLimits {
+    T lo, u_hi, v_hi;
+};
 static_assert(kWavesPerBlock == 4, "a block's waves form a 2 x 2 column patch");
 
 __host__ __device__ inline int pd_blocks(int nx, int ny) {
     return ((nx + kPdTile - 1) / kPdTile) * ((ny + kPdTile - 1) / kPdTile) * (kPdTile * kPdTile / kWavesPerBlock);
 }
-template <class Tin, class Tout, class T, int NZL>
+template <class Tin, class Tout, class T, int NZL, int OFF_BOUNDARY>
 __global__ __launch_bounds__(kBlock) void cbct_back_pd_kernel(Geom<T> g, const View<T>* __restrict__ views,
                                                               int vstride, int nviews, const Tin* __restrict__ proj,
                                                               const uint8_t* __restrict__ map,
                                                               Tout* __restrict__ vol, T scale, int accumulate,
-                                                              int off_boundary) {
+                                                              PdLimits<T> lim) {
+    constexpr bool off_boundary = OFF_BOUNDARY != 0;
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int wv = (int)(threadIdx.x / kWave);
     constexpr int kHalf = kPdTile / 2;  // 2 x 2 patches per super-tile edge
@@ -651,8 +660,9 @@ __global__ __launch_bounds__(kBlock) void cbct_back_pd_kernel(Geom<T> g, const V
         return;
     }
     const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
-    const double lim_u = (double)(g.nu - 1) - 1e-6;
-    const double lim_v = (double)(g.nv - 1) - 1e-6;
+    T izf[NZL];  // voxel indices as T, once
+#pragma unroll
+    for (int c = 0; c < NZL; ++c) izf[c] = T(c * kWave + lane);
     T img[NZL];
 #pragma unroll
     for (int c = 0; c < NZL; ++c) img[c] = T(0);
@@ -671,33 +681,33 @@ __global__ __launch_bounds__(kBlock) void cbct_back_pd_kernel(Geom<T> g, const V
             w2s = g.sdd * g.sdd / ((g.sad + yr) * (g.sad + yr) + (xr - roff) * (xr - roff));
             w2 = g.sdd / (g.sad + yr);
         }
-        if (!((double)u >= 1e-6 && (double)u <= lim_u)) continue;  // also skips NaN
+        if (!(u >= lim.lo && u <= lim.u_hi)) continue;  // also skips NaN
         const int iu = (int)floor(u);
         const T wul = u - T(iu);
+        // v of voxel iz: (iz + wz) dz w2 / dv - wv, evaluated as iz A + B with the view's
+        // A and B (one FMA per voxel; within an ulp or two of the reference's order)
+        const T A = g.dz * w2 / g.dv;
+        const T B = v.wz * A - g.wv;
         // 'back,pd' stops at the first voxel above the detector; v is monotone in iz, so
         // that is a per-voxel test when v grows and "nothing if voxel 0 is already above"
         // when it shrinks
-        bool none = false;
-        if (off_boundary && !(g.dz * w2 > T(0))) {
-            const T v0 = v.wz * g.dz * w2 / g.dv - g.wv;
-            none = (double)v0 > lim_v;
-        }
-        if (none) continue;
+        if (off_boundary && !(A > T(0)) && B > lim.v_hi) continue;
         const Tin* p = proj + ((size_t)b * g.nu + iu) * g.nv;
 #pragma unroll
         for (int c = 0; c < NZL; ++c) {
             const int iz = c * kWave + lane;
             if (iz >= g.nz) continue;
-            T vv = (T(iz) + v.wz) * g.dz * w2 / g.dv - g.wv;
+            T vv = fma_t(izf[c], A, B);
             if (off_boundary) {
-                if (!((double)vv >= 1e-6 && (double)vv <= lim_v)) continue;
+                if (!(vv >= lim.lo && vv <= lim.v_hi)) continue;
             } else {
-                if ((double)vv < 1e-6) vv = T(0.5);
-                if ((double)vv > lim_v) vv = T(g.nv) - T(1.5);
+                if (vv < lim.lo) vv = T(0.5);
+                if (vv > lim.v_hi) vv = T(g.nv) - T(1.5);
                 if (vv != vv) continue;  // NaN
             }
-            int iv = (int)floor(vv);
-            iv = iv < 0 ? 0 : (iv > g.nv - 2 ? g.nv - 2 : iv);  // no-op for finite v; keeps reads in range
+            // the tests above leave v in [1e-6, nv - 1 - 1e-6] or at 0.5 / nv - 1.5, so the
+            // clamp is a no-op that keeps the reads in range whatever the arithmetic
+            const int iv = max(0, min((int)floor(vv), g.nv - 2));
             const T wvl = vv - T(iv);
             const T a0 = ld<T>(p + iv), a1 = ld<T>(p + iv + 1), a2 = ld<T>(p + g.nv + iv),
                     a3 = ld<T>(p + g.nv + iv + 1);
@@ -826,14 +836,43 @@ static hipError_t proj3(const Geom<T>& g, const View<T>* views, int vstride, int
     return hipGetLastError();
 }
 
+// PdLimits for the T arithmetic of the device (float: the nearest floats on the safe side)
+static PdLimits<double> pd_limits_d(int nu, int nv) {
+    return PdLimits<double>{1e-6, (double)(nu - 1) - 1e-6, (double)(nv - 1) - 1e-6};
+}
+static float float_at_least(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = nextafterf(f, INFINITY);
+    return f;
+}
+static float float_at_most(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = nextafterf(f, -INFINITY);
+    return f;
+}
+template <class T>
+static PdLimits<T> pd_limits(int nu, int nv);
+template <>
+PdLimits<double> pd_limits<double>(int nu, int nv) { return pd_limits_d(nu, nv); }
+template <>
+PdLimits<float> pd_limits<float>(int nu, int nv) {
+    const PdLimits<double> d = pd_limits_d(nu, nv);
+    return PdLimits<float>{float_at_least(d.lo), float_at_most(d.u_hi), float_at_most(d.v_hi)};
+}
+
 template <class Tin, class Tout, class T, int NZL>
 static void launch_back(unsigned blocks, hipStream_t st, const Geom<T>& g, const View<T>* views, int vstride,
                         int nviews, const void* proj, const uint8_t* map, void* vol, int foot, int pd_mode, T scale,
                         int accumulate) {
     const int rowwin = back_rowwin(g.nv);
-    if (foot == 2)
-        hipLaunchKernelGGL((cbct_back_pd_kernel<Tin, Tout, T, NZL>), dim3(blocks), dim3(kBlock), 0, st, g, views,
-                           vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate, pd_mode);
+    if (foot == 2 && pd_mode)
+        hipLaunchKernelGGL((cbct_back_pd_kernel<Tin, Tout, T, NZL, 1>), dim3(blocks), dim3(kBlock), 0, st, g, views,
+                           vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate,
+                           pd_limits<T>(g.nu, g.nv));
+    else if (foot == 2)
+        hipLaunchKernelGGL((cbct_back_pd_kernel<Tin, Tout, T, NZL, 0>), dim3(blocks), dim3(kBlock), 0, st, g, views,
+                           vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate,
+                           pd_limits<T>(g.nu, g.nv));
     else if (foot == FOOT_DD)
         hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_DD, NZL>), dim3(blocks), dim3(kBlock),
                            back_lds_bytes<T>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,

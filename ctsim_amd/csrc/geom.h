@@ -108,6 +108,8 @@ struct Column {
 
 template <class T>
 __device__ __forceinline__ T fabs_t(T x) { return x < T(0) ? -x : x; }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
 
 // Division and square root of the footprint arithmetic.  The FP32 path uses the
 // hardware's 1-ulp v_rcp_f32 / v_sqrt_f32: every weight is a continuous function of
