@@ -384,9 +384,15 @@ constexpr int kRowWin = 1019;  // detector rows staged per wave (nv larger: wind
 
 template <class T>
 struct __align__(16) ViewCol {  // one view's footprint on one image column (80 bytes in FP32)
-    int ilo, nw, r0, r1;        // first footprint cell, cell count, rows covered
-    int iz0, iz1, pad0;         // voxel range; pad0 = view index within the selection
-    T amp, r, zinc, vstart, wz, w0, w1, w2, w3, p0, p1, p2, p3;
+    int cells;                  // first footprint cell ilo | cell count nw << 16
+    int rows;                   // rows covered r0 | r1 << 16
+    int zr;                     // voxel range iz0 | iz1 << 16
+    int vb;                     // view index within the selection
+    T ampr, r2;                 // amp / r and r^2: the voxel factor is sqrt(r^2 + zc^2) amp / r
+    T zinc, vs5, wzdz;          // v of voxel z spans [vs5 + z zinc, vs5 + (z + 1) zinc] (+1/2 folded);
+                                // zc = z dz + wzdz
+    T vstart;                   // the unshifted start, for footprints that may reach the cap
+    T w0, w1, w2, w3, p0, p1, p2, p3;
 };
 
 // Staged rows of a window [rw, rwe] sit at sw[1 + row - rw]; sw[0] and the kRowPad
@@ -395,7 +401,9 @@ struct __align__(16) ViewCol {  // one view's footprint on one image column (80 
 // the staged value.  ra is clamped to [rw - 1, rwe + 1]: rows outside the window then
 // read the zero words (they belong to another window or lie off the detector), and the
 // overlaps follow the DD rule: row ra takes min(b, ra + 1) - a, row f > ra takes
-// max(min(b, f + 1) - f, 0).  A voxel meets at most K = floor(zinc) + 2 rows.
+// max(min(b, f + 1) - f, 0), which equals clamp01(b - f) bit for bit (f is an integer:
+// b - f is exact wherever the result lies inside (0, 1)).  A voxel meets at most
+// K = floor(zinc) + 2 rows.
 constexpr int kRowPad = 4;
 
 // rows staged per wave for a detector of nv rows, and the dynamic LDS of a block
@@ -414,9 +422,7 @@ __device__ __forceinline__ T gather_rows(const T* sw, int rw, int rwe, T a, T b)
     T s = ((b < f ? b : f) - a) * pr[0];
 #pragma unroll
     for (int k = 1; k < K; ++k) {
-        const T hi = b < f + T(1) ? b : f + T(1);
-        const T w = hi - f > T(0) ? hi - f : T(0);
-        s += w * pr[k];
+        s += clamp01(b - f) * pr[k];
         f += T(1);
     }
     return s;
@@ -455,8 +461,9 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
         // footprint of view b0 + lane on this column
         bool ok = b0 + lane < nviews;
         ViewCol<T> vc{};
-        vc.r = T(1);
         vc.zinc = T(1);
+        int iz0 = 0, iz1 = 0, r0 = 0, r1 = 0;
+        T vstart = T(0);
         if (ok) {
             const View<T> v = views[(size_t)(b0 + lane) * vstride];
             const Column<T> cg = column_geom(xc, yc, v, g.sad, g.dx);
@@ -475,29 +482,31 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 vc.p0 = tau[0]; vc.p1 = tau[1]; vc.p2 = tau[2]; vc.p3 = tau[3];
             }
             if (ok)
-                ok = v_window(g, cg.d, v.wz, vc.zinc, vc.vstart, vc.iz0, vc.iz1) && vc.iz0 < vc.iz1 &&
-                     vc.zinc > T(0);
+                ok = v_window(g, cg.d, v.wz, vc.zinc, vstart, iz0, iz1) && iz0 < iz1 && vc.zinc > T(0);
             if (ok) {
                 // rows touched by voxels iz0 .. iz1-1
-                T a0 = vc.vstart + T(vc.iz0) * vc.zinc + T(0.5);
-                T a1 = vc.vstart + T(vc.iz1 - 1) * vc.zinc + vc.zinc + T(0.5);
+                T a0 = vstart + T(iz0) * vc.zinc + T(0.5);
+                T a1 = vstart + T(iz1 - 1) * vc.zinc + vc.zinc + T(0.5);
                 a0 = a0 > T(0) ? a0 : T(0);
                 a1 = a1 < T(g.nv) ? a1 : T(g.nv);
                 ok = a0 <= a1;
-                vc.r0 = min((int)floor(a0), g.nv - 1);
-                vc.r1 = min((int)floor(a1), g.nv - 1);
+                r0 = min((int)floor(a0), g.nv - 1);
+                r1 = min((int)floor(a1), g.nv - 1);
             }
             if (ok) {
-                vc.ilo = ilo;
-                vc.nw = ihi - ilo + 1;
+                vc.cells = ilo | ((ihi - ilo + 1) << 16);
+                vc.rows = r0 | (r1 << 16);
+                vc.zr = iz0 | (iz1 << 16);
                 vc.w0 = u_weight<T, FOOT>(ilo + 0, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3);
                 vc.w1 = u_weight<T, FOOT>(ilo + 1, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3);
                 vc.w2 = u_weight<T, FOOT>(ilo + 2, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3);
                 vc.w3 = u_weight<T, FOOT>(ilo + 3, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3);
-                vc.amp = cg.amp;
-                vc.r = cg.r;
-                vc.wz = v.wz;
-                vc.pad0 = b0 + lane;
+                vc.ampr = cg.amp * frcp(cg.r);
+                vc.r2 = cg.r * cg.r;
+                vc.vs5 = vstart + T(0.5);
+                vc.vstart = vstart;
+                vc.wzdz = v.wz * g.dz;
+                vc.vb = b0 + lane;
             }
         }
         vw[lane] = vc;
@@ -509,8 +518,9 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
         Tin pre0{}, pre1{}, pre2{}, pre3{};
         if (cur >= 0) {
             const ViewCol<T>& c = vw[cur];
-            const int nw = uni(c.nw), r0 = uni(c.r0), r1 = uni(c.r1);
-            const Tin* pr = proj + ((size_t)uni(c.pad0) * g.nu + uni(c.ilo)) * g.nv + min(r0 + lane, r1);
+            const int cells = uni(c.cells), rows = uni(c.rows);
+            const int nw = cells >> 16, r0 = rows & 0xffff, r1 = rows >> 16;
+            const Tin* pr = proj + ((size_t)uni(c.vb) * g.nu + (cells & 0xffff)) * g.nv + min(r0 + lane, r1);
             pre0 = pr[0];
             if (nw > 1) pre1 = pr[g.nv];
             if (nw > 2) pre2 = pr[2 * g.nv];
@@ -518,12 +528,13 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
         }
         while (cur >= 0) {
             const ViewCol<T>& c = vw[cur];
-            const int ilo = uni(c.ilo), nw = uni(c.nw), r0 = uni(c.r0), r1 = uni(c.r1);
-            const int iz0 = uni(c.iz0), iz1 = uni(c.iz1), vb = uni(c.pad0);
+            const int cells = uni(c.cells), rows = uni(c.rows), zr = uni(c.zr), vb = uni(c.vb);
+            const int ilo = cells & 0xffff, nw = cells >> 16, r0 = rows & 0xffff, r1 = rows >> 16;
+            const int iz0 = zr & 0xffff, iz1 = zr >> 16;
             const T w0 = uni(c.w0), w1 = uni(c.w1), w2 = uni(c.w2), w3 = uni(c.w3);
-            const T zinc = uni(c.zinc), vstart = uni(c.vstart), r = uni(c.r);
-            const T ampr = uni(c.amp) * frcp(r), r2 = r * r;  // ampv * amp = sqrt(r^2 + zc^2) * amp / r
-            const T wzdz = uni(c.wz) * g.dz;                  // zc = z dz + wz dz
+            const T zinc = uni(c.zinc), vs5 = uni(c.vs5);
+            const T ampr = uni(c.ampr), r2 = uni(c.r2);  // ampv * amp = sqrt(r^2 + zc^2) * amp / r
+            const T wzdz = uni(c.wzdz);                  // zc = z dz + wz dz
             const Tin* pv = proj + ((size_t)vb * g.nu + ilo) * g.nv;
             for (int rw = r0; rw <= r1; rw += rowwin) {
                 const int rwe = min(r1, rw + rowwin - 1);
@@ -560,9 +571,10 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                     if (nxt >= 0) {
                         mask &= mask - 1;
                         const ViewCol<T>& cn = vw[nxt];
-                        const int nwn = uni(cn.nw), r0n = uni(cn.r0), r1n = uni(cn.r1);
+                        const int cellsn = uni(cn.cells), rowsn = uni(cn.rows);
+                        const int nwn = cellsn >> 16, r0n = rowsn & 0xffff, r1n = rowsn >> 16;
                         const Tin* pr =
-                            proj + ((size_t)uni(cn.pad0) * g.nu + uni(cn.ilo)) * g.nv + min(r0n + lane, r1n);
+                            proj + ((size_t)uni(cn.vb) * g.nu + (cellsn & 0xffff)) * g.nv + min(r0n + lane, r1n);
                         pre0 = pr[0];
                         if (nwn > 1) pre1 = pr[g.nv];
                         if (nwn > 2) pre2 = pr[2 * g.nv];
@@ -575,8 +587,9 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 for (int cz = 0; cz < NZL; ++cz) {
                     const int iz = cz * kWave + lane;
                     if (iz < iz0 || iz >= iz1) continue;
-                    const T vmin = vstart + T(iz) * zinc;
-                    const T a = vmin + T(0.5), bb = vmin + zinc + T(0.5);
+                    // voxel edges in row units, +1/2 folded into vs5 (within an ulp of the
+                    // reference's vstart + iz zinc + 1/2)
+                    const T a = fma_t(T(iz), zinc, vs5), bb = fma_t(T(iz + 1), zinc, vs5);
                     T s;
                     if (zu <= T(1)) {
                         s = gather_rows<T, 2>(sw, rw, rwe, a, bb);
@@ -585,6 +598,8 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                     } else if (zu <= T(3)) {
                         s = gather_rows<T, 4>(sw, rw, rwe, a, bb);
                     } else {
+                        // footprints that may reach the 16-cell cap: the reference's edges
+                        const T vmin = uni(c.vstart) + T(iz) * zinc;
                         T clo, chi;
                         int vlo, vhi;
                         s = T(0);
