@@ -125,7 +125,8 @@ def kernel_times(solver, reps=1):
                 call("ctsim_pwls_nesterov_update", solver.x[a].vp, solver.z[a].vp, solver.s.vp, solver.g.vp,
                      solver.precom.vp, solver.x[b].vp, solver.z[b].vp,
                      solver.zc.vp if solver.zc is not None else None, solver.zc.dtype if solver.zc is not None else 0,
-                     nx, ny, nz, solver.beta, solver.delta, solver.pfun, 1.0, 1.0, solver._sv())
+                     solver.x0.vp if solver.x0 is not None else None, nx, ny, nz, solver.beta, solver.delta,
+                     solver.pfun, 1.0, 1.0, solver._sv())
             t["update"] += tm.ms
     n = reps * len(solver.subsets)
     return {k: v / n for k, v in t.items()}
@@ -140,7 +141,7 @@ def run_pwls(form, geom, y, w, x0, steps, warmup, sino_dtype="bf16", numos=24, b
     precision = {"mixed": "mixed", "plain": "f32", "f64": "f64"}[form]
     n_all = max(warmup + steps, total_iters)
     solver = PwlsOS(geom, y, w, numos, beta, delta, precision=precision, sino_dtype=sino_dtype,
-                    zcopy_dtype="f16", graph=form == "mixed")
+                    graph=form == "mixed")
     solver.setup(x0)
     solver.prepare(n_all)
     for _ in range(warmup):
@@ -466,8 +467,8 @@ def main():
         "data": "synthetic (water cylinder + 10 random ellipsoid inserts, Poisson I0=2e4, FDK init)",
         "config": {"workload": "pwls_os_sf_512x512x128_984views_1024x64det_24subsets", "volume": [512, 512, 128],
                    "views": 984, "detector": [1024, 64], "subsets": 24, "beta": 1e4, "delta_per_mm": 1e-3,
-                   "storage": f"{args.sino_dtype} sinogram/weights/residual, fp16 z-copy for the projector, "
-                              "fp32 x/z/s and footprint sums, fp64 denominators and reductions",
+                   "storage": f"{args.sino_dtype} data (y - A x0), weights and residual; fp32 volumes, projector "
+                              "input z - x0 and footprint sums; fp64 denominators and reductions",
                    "graph": True},
         "plain_fp32_iter_per_s": round(1e3 / plain, 4) if plain else None,
         "mixed_over_plain": round(plain / ms_mixed, 3) if plain else None,

@@ -102,8 +102,8 @@ SIGNATURES = {
     "ctsim_device_count": (C.c_int, [P_i32]),
     "ctsim_pwls_residual": (C.c_int, [vp, i32, vp, i32, vp, i32, vp, i32, sz, f64, vp, vp]),
     "ctsim_huber": (C.c_int, [vp, i32, i32, i32, f64, i32, vp, vp, vp]),
-    "ctsim_pwls_nesterov_update": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, f64, f64, i32, f64, f64, vp]),
-    "ctsim_pwls_nesterov_update_seq": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, f64, f64, i32, vp, vp, i32, vp]),
+    "ctsim_pwls_nesterov_update": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, i32, i32, i32, f64, f64, i32, f64, f64, vp]),
+    "ctsim_pwls_nesterov_update_seq": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, i32, i32, i32, f64, f64, i32, vp, vp, i32, vp]),
     "ctsim_add_i32": (C.c_int, [vp, i32, vp]),
     "ctsim_pwls_sqs_update": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, f64, f64, i32, vp]),
     "ctsim_sqdiff_sum": (C.c_int, [vp, vp, sz, vp, vp]),
@@ -121,6 +121,7 @@ SIGNATURES = {
     "ctsim_pwls_lalm_sg": (C.c_int, [vp, vp, vp, sz, f64, vp]),
     "ctsim_pwls_lalm_inner": (C.c_int, [vp, vp, vp, vp, vp, i32, i32, i32, f64, f64, i32, f64, vp]),
     "ctsim_mul": (C.c_int, [vp, i32, vp, i32, vp, i32, sz, vp]),
+    "ctsim_sub": (C.c_int, [vp, vp, vp, i32, sz, vp]),
     "ctsim_nusqs_seed": (C.c_int, [vp, vp, i32, i32, i32, f64, vp]),
     "ctsim_div": (C.c_int, [vp, vp, vp, sz, vp]),
     "ctsim_absdiff_floor": (C.c_int, [vp, vp, vp, sz, f64, vp]),

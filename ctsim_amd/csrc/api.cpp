@@ -886,29 +886,29 @@ int ctsim_huber(const float* x, int32_t nx, int32_t ny, int32_t nz, double delta
 }
 
 int ctsim_pwls_nesterov_update(const float* x, const float* z, float* s, const float* g, const float* precom,
-                               float* x_out, float* z_out, void* zcopy, int32_t zcopy_dtype, int32_t nx, int32_t ny,
-                               int32_t nz, double beta, double delta, int32_t pfun, double t, double t_next,
-                               void* stream) {
+                               float* x_out, float* z_out, void* zcopy, int32_t zcopy_dtype, const float* zref,
+                               int32_t nx, int32_t ny, int32_t nz, double beta, double delta, int32_t pfun, double t,
+                               double t_next, void* stream) {
     if (!x || !z || !s || !g || !precom || !x_out || !z_out) return fail(CTSIM_EINVAL, "Error: null argument.");
     if (x_out == x || z_out == z || x_out == z || z_out == x)
         return fail(CTSIM_EINVAL, "Error: outputs must not alias the stencil inputs.");
-    hipError_t e = nesterov_launch(x, z, s, g, precom, x_out, z_out, zcopy, zcopy_dtype, nx, ny, nz, beta, delta,
-                                   pfun, t, t_next, nullptr, nullptr, 0, (hipStream_t)stream);
+    hipError_t e = nesterov_launch(x, z, s, g, precom, x_out, z_out, zcopy, zcopy_dtype, zref, nx, ny, nz, beta,
+                                   delta, pfun, t, t_next, nullptr, nullptr, 0, (hipStream_t)stream);
     if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: z copy dtype or penalty not supported.");
     HIP_TRY(e, "nesterov update");
     return CTSIM_OK;
 }
 
 int ctsim_pwls_nesterov_update_seq(const float* x, const float* z, float* s, const float* g, const float* precom,
-                                   float* x_out, float* z_out, void* zcopy, int32_t zcopy_dtype, int32_t nx,
-                                   int32_t ny, int32_t nz, double beta, double delta, int32_t pfun,
+                                   float* x_out, float* z_out, void* zcopy, int32_t zcopy_dtype, const float* zref,
+                                   int32_t nx, int32_t ny, int32_t nz, double beta, double delta, int32_t pfun,
                                    const double* d_tseq, const int32_t* d_base, int32_t k, void* stream) {
     if (!x || !z || !s || !g || !precom || !x_out || !z_out || !d_tseq || !d_base)
         return fail(CTSIM_EINVAL, "Error: null argument.");
     if (x_out == x || z_out == z || x_out == z || z_out == x)
         return fail(CTSIM_EINVAL, "Error: outputs must not alias the stencil inputs.");
-    hipError_t e = nesterov_launch(x, z, s, g, precom, x_out, z_out, zcopy, zcopy_dtype, nx, ny, nz, beta, delta,
-                                   pfun, 1.0, 1.0, d_tseq, (const int*)d_base, k, (hipStream_t)stream);
+    hipError_t e = nesterov_launch(x, z, s, g, precom, x_out, z_out, zcopy, zcopy_dtype, zref, nx, ny, nz, beta,
+                                   delta, pfun, 1.0, 1.0, d_tseq, (const int*)d_base, k, (hipStream_t)stream);
     if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: z copy dtype or penalty not supported.");
     HIP_TRY(e, "nesterov update");
     return CTSIM_OK;
@@ -1021,6 +1021,14 @@ int ctsim_mul(const void* a, int32_t adt, const void* b, int32_t bdt, void* o, i
     hipError_t e = mul_launch(a, adt, b, bdt, o, odt, n, (hipStream_t)stream);
     if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype combination not supported.");
     HIP_TRY(e, "mul");
+    return CTSIM_OK;
+}
+
+int ctsim_sub(const float* a, const float* b, void* o, int32_t odt, size_t n, void* stream) {
+    if (!a || !b || !o) return fail(CTSIM_EINVAL, "Error: null buffer.");
+    hipError_t e = sub_launch(a, b, o, odt, n, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype not supported.");
+    HIP_TRY(e, "sub");
     return CTSIM_OK;
 }
 

@@ -72,6 +72,7 @@ hipError_t add_i32_launch(int* p, int inc, hipStream_t st);
 hipError_t sqdiff_launch(const float* a, const float* b, size_t n, double* out, hipStream_t st);
 hipError_t sqdiff_launch(const double* a, const double* b, size_t n, double* out, hipStream_t st);
 hipError_t mul_launch(const void* a, int adt, const void* b, int bdt, void* o, int odt, size_t n, hipStream_t st);
+hipError_t sub_launch(const float* a, const float* b, void* o, int odt, size_t n, hipStream_t st);
 // FDK pre-processing (fdk.hip)
 struct FdkGeom {
     float du, dv, ou, ov, sad, sdd;
@@ -96,8 +97,9 @@ hipError_t penalty_launch(const float* x, int nx, int ny, int nz, int pfun, doub
 hipError_t penalty_launch(const double* x, int nx, int ny, int nz, int pfun, double delta, int mode, double* out,
                           double* cost, hipStream_t st);
 hipError_t nesterov_launch(const float* x, const float* z, float* s, const float* g, const float* precom, float* x_out,
-                           float* z_out, void* zcopy, int zcdt, int nx, int ny, int nz, double beta, double delta,
-                           int pfun, double t, double t_next, const double* tseq, const int* base, int k,
+                           float* z_out, void* zcopy, int zcdt, const float* zref, int nx, int ny, int nz,
+                           double beta, double delta, int pfun, double t, double t_next, const double* tseq,
+                           const int* base, int k,
                            hipStream_t st);
 hipError_t nesterov_launch(const double* x, const double* z, double* s, const double* g, const double* precom,
                            double* x_out, double* z_out, int nx, int ny, int nz, double beta, double delta, int pfun,

@@ -387,7 +387,8 @@ template <class V, class ZC, int POT>
 __global__ __launch_bounds__(256) void nesterov_kernel(const V* __restrict__ x, const V* __restrict__ z,
                                                        V* __restrict__ s, const V* __restrict__ g,
                                                        const V* __restrict__ precom, V* __restrict__ x_out,
-                                                       V* __restrict__ z_out, ZC* __restrict__ zcopy, int nx, int ny,
+                                                       V* __restrict__ z_out, ZC* __restrict__ zcopy,
+                                                       const V* __restrict__ zref, int nx, int ny,
                                                        int nz, double beta, V delta, double t, double t_next,
                                                        const double* __restrict__ tseq, const int* __restrict__ base,
                                                        int k, int lbz) {
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(256) void nesterov_kernel(const V* __restrict__ x, 
         s[i] = scrub(sn);
         const V zs = scrub(zn);
         z_out[i] = zs;
-        if (zcopy) zcopy[i] = cvt_store<ZC>(zs);
+        if (zcopy) zcopy[i] = cvt_store<ZC>(zref ? zs - zref[i] : zs);
     }
 }
 
@@ -483,7 +484,8 @@ template <class ZC, int POT>
 __global__ __launch_bounds__(256) void nesterov4_kernel(const float* __restrict__ x, const float* __restrict__ z,
                                                         float* __restrict__ s, const float* __restrict__ g,
                                                         const float* __restrict__ precom, float* __restrict__ x_out,
-                                                        float* __restrict__ z_out, ZC* __restrict__ zcopy, int nx,
+                                                        float* __restrict__ z_out, ZC* __restrict__ zcopy,
+                                                        const float* __restrict__ zref, int nx,
                                                         int ny, int nz, double beta, float delta, double t,
                                                         double t_next, const double* __restrict__ tseq,
                                                         const int* __restrict__ base, int k, int lbz) {
@@ -522,7 +524,15 @@ __global__ __launch_bounds__(256) void nesterov4_kernel(const float* __restrict_
     store4(x_out + i, xo);
     store4(s + i, so);
     store4(z_out + i, zo);
-    if (zcopy) store4(zcopy + i, zo);
+    if (zcopy) {
+        if (zref) {
+            const float4 r4 = *reinterpret_cast<const float4*>(zref + i);
+            const float d[4] = {zo[0] - r4.x, zo[1] - r4.y, zo[2] - r4.z, zo[3] - r4.w};
+            store4(zcopy + i, d);
+        } else {
+            store4(zcopy + i, zo);
+        }
+    }
 }
 
 template <int POT>
@@ -898,7 +908,8 @@ hipError_t penalty_launch(const double* x, int nx, int ny, int nz, int pfun, dou
 
 template <int POT>
 static hipError_t nesterov_p(const float* x, const float* z, float* s, const float* g, const float* precom,
-                             float* x_out, float* z_out, void* zcopy, int zcdt, int nx, int ny, int nz, double beta,
+                             float* x_out, float* z_out, void* zcopy, int zcdt, const float* zref, int nx, int ny,
+                             int nz, double beta,
                              double delta, double t, double t_next, const double* tseq, const int* base, int k,
                              hipStream_t st) {
     if (nz % 4 == 0 && (POT == POT_HUBER || POT == POT_QUAD || POT == POT_HYPER || POT == POT_ANISO)) {
@@ -906,9 +917,9 @@ static hipError_t nesterov_p(const float* x, const float* z, float* s, const flo
         const dim3 gr = vg.grid, bl(256);
         const int lbz = vg.lbz;
         switch (zcopy ? zcdt : DT_F32) {
-            case DT_F32: hipLaunchKernelGGL((nesterov4_kernel<float, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (float*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k, lbz); break;
-            case DT_F16: hipLaunchKernelGGL((nesterov4_kernel<__half, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__half*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k, lbz); break;
-            case DT_BF16: hipLaunchKernelGGL((nesterov4_kernel<__hip_bfloat16, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__hip_bfloat16*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k, lbz); break;
+            case DT_F32: hipLaunchKernelGGL((nesterov4_kernel<float, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (float*)zcopy, zref, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k, lbz); break;
+            case DT_F16: hipLaunchKernelGGL((nesterov4_kernel<__half, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__half*)zcopy, zref, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k, lbz); break;
+            case DT_BF16: hipLaunchKernelGGL((nesterov4_kernel<__hip_bfloat16, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__hip_bfloat16*)zcopy, zref, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k, lbz); break;
             default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
@@ -917,19 +928,20 @@ static hipError_t nesterov_p(const float* x, const float* z, float* s, const flo
     const dim3 gr = vg.grid, bl(256);
     const int lbz = vg.lbz;
     switch (zcopy ? zcdt : DT_F32) {
-        case DT_F32: hipLaunchKernelGGL((nesterov_kernel<float, float, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (float*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k, lbz); break;
-        case DT_F16: hipLaunchKernelGGL((nesterov_kernel<float, __half, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__half*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k, lbz); break;
-        case DT_BF16: hipLaunchKernelGGL((nesterov_kernel<float, __hip_bfloat16, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__hip_bfloat16*)zcopy, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k, lbz); break;
+        case DT_F32: hipLaunchKernelGGL((nesterov_kernel<float, float, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (float*)zcopy, zref, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k, lbz); break;
+        case DT_F16: hipLaunchKernelGGL((nesterov_kernel<float, __half, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__half*)zcopy, zref, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k, lbz); break;
+        case DT_BF16: hipLaunchKernelGGL((nesterov_kernel<float, __hip_bfloat16, POT>), gr, bl, 0, st, x, z, s, g, precom, x_out, z_out, (__hip_bfloat16*)zcopy, zref, nx, ny, nz, beta, (float)delta, t, t_next, tseq, base, k, lbz); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t nesterov_launch(const float* x, const float* z, float* s, const float* g, const float* precom, float* x_out,
-                           float* z_out, void* zcopy, int zcdt, int nx, int ny, int nz, double beta, double delta,
+                           float* z_out, void* zcopy, int zcdt, const float* zref, int nx, int ny, int nz, double beta,
+                           double delta,
                            int pfun, double t, double t_next, const double* tseq, const int* base, int k,
                            hipStream_t st) {
-#define CALL(P) nesterov_p<P>(x, z, s, g, precom, x_out, z_out, zcopy, zcdt, nx, ny, nz, beta, delta, t, t_next, tseq, base, k, st)
+#define CALL(P) nesterov_p<P>(x, z, s, g, precom, x_out, z_out, zcopy, zcdt, zref, nx, ny, nz, beta, delta, t, t_next, tseq, base, k, st)
     CT_POT_DISPATCH(pfun, CALL)
 #undef CALL
 }
@@ -940,7 +952,8 @@ static hipError_t nesterov64_p(const double* x, const double* z, double* s, cons
                                double t, double t_next, hipStream_t st) {
     const VoxGrid vg = vox_grid(nx, ny, nz);
     hipLaunchKernelGGL((nesterov_kernel<double, double, POT>), vg.grid, dim3(256), 0, st, x, z, s, g, precom, x_out,
-                       z_out, (double*)nullptr, nx, ny, nz, beta, delta, t, t_next, (const double*)nullptr,
+                       z_out, (double*)nullptr, (const double*)nullptr, nx, ny, nz, beta, delta, t, t_next,
+                       (const double*)nullptr,
                        (const int*)nullptr, 0, vg.lbz);
     return hipGetLastError();
 }
@@ -1093,6 +1106,23 @@ static hipError_t mul_b(const void* a, const void* b, int bdt, void* o, int odt,
         case DT_BF16: return mul_o<A, __hip_bfloat16>(a, b, o, odt, n, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+template <class O>
+__global__ void sub_kernel(const float* __restrict__ a, const float* __restrict__ b, O* __restrict__ o, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        o[i] = cvt_store<O>(a[i] - b[i]);
+}
+
+hipError_t sub_launch(const float* a, const float* b, void* o, int odt, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    switch (odt) {
+        case DT_F32: hipLaunchKernelGGL((sub_kernel<float>), dim3(grid_for(n)), dim3(256), 0, st, a, b, (float*)o, n); break;
+        case DT_F16: hipLaunchKernelGGL((sub_kernel<__half>), dim3(grid_for(n)), dim3(256), 0, st, a, b, (__half*)o, n); break;
+        case DT_BF16: hipLaunchKernelGGL((sub_kernel<__hip_bfloat16>), dim3(grid_for(n)), dim3(256), 0, st, a, b, (__hip_bfloat16*)o, n); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t mul_launch(const void* a, int adt, const void* b, int bdt, void* o, int odt, size_t n, hipStream_t st) {

@@ -78,7 +78,7 @@ def pwls_update(reps: int) -> None:
     for k in range(reps):
         a, b = k % 2, 1 - k % 2
         call("ctsim_pwls_nesterov_update", x[a].vp, z[a].vp, s.vp, g.vp, pre.vp, x[b].vp, z[b].vp, zc.vp, zc.dtype,
-             nx, ny, nz, 1e4, 1e-2, 0, 1.0 + k, 2.0 + k, None)
+             None, nx, ny, nz, 1e4, 1e-2, 0, 1.0 + k, 2.0 + k, None)
     synchronize()
 
 

@@ -15,10 +15,14 @@ subset-major order, so Os(., isub) is a contiguous slice.
 
 Three forms:
   * plain:  FP32 storage everywhere, one kernel launch per operator call;
-  * mixed:  y, w and the residual in FP16 or BF16, the forward projector reads an
-            FP16 copy of z written by the update kernel, FP32 footprint sums, FP64
-            step denominators and reductions, and each iteration replayed as one
-            captured hipGraph;
+  * mixed:  y, w and the residual in FP16 or BF16, FP32 footprint sums and volumes,
+            FP64 step denominators and reductions, and each iteration replayed as one
+            captured hipGraph.  By default the stored data are y - A x0 (x0 the start
+            image, projected once in FP32) and the forward projector reads z - x0,
+            written by the update kernel: A is linear, so A z - y = A(z - x0) - (y - A x0)
+            exactly, and the half-precision rounding of the data is taken on the much
+            smaller y - A x0 (offset=False stores y itself and projects an FP16 copy
+            of z);
   * fp64:   every array, footprint and sum in double (precision="f64"), the accuracy
             reference the other two are measured against.
 """
@@ -61,8 +65,9 @@ def nesterov_t_sequence(n_steps: int, restart_every: int | None = None, steps_pe
 
 class PwlsOS:
     def __init__(self, geom: Geometry, y: np.ndarray, w: np.ndarray, numos: int, beta: float, delta: float,
-                 mixed: bool = True, sino_dtype: str = "f16", zcopy_dtype: str = "f16", graph: bool | None = None,
-                 op: str = "tf", stream: Stream | None = None, precision: str | None = None, pfun: str = "huber"):
+                 mixed: bool = True, sino_dtype: str = "f16", zcopy_dtype: str | None = None,
+                 graph: bool | None = None, op: str = "tf", stream: Stream | None = None,
+                 precision: str | None = None, pfun: str = "huber", offset: bool | None = None):
         if geom.ndim != 3:
             raise ValueError("PwlsOS drives the 3D cone-beam operators")
         self.f64 = precision == "f64"
@@ -78,6 +83,7 @@ class PwlsOS:
         self.delta = float(delta)
         self.pfun = penalty_code(pfun)
         self.mixed = bool(mixed)
+        self.offset = self.mixed and (True if offset is None else bool(offset))
         self.graph_mode = self.mixed if graph is None else bool(graph)
         if self.f64 and self.graph_mode:
             raise ValueError("the FP64 form runs without graph capture")
@@ -88,6 +94,8 @@ class PwlsOS:
         nv, nu, nviews = geom.sino_shape()
         self.cell = nu * nv
         self.sdt = dtype_code(sino_dtype) if self.mixed else self.vdt
+        if zcopy_dtype is None:
+            zcopy_dtype = "f32" if self.offset else "f16"
         self.zdt = dtype_code(zcopy_dtype) if self.mixed else None
         # subset-major permutation of the views
         self.subsets = []
@@ -107,7 +115,12 @@ class PwlsOS:
         self.wscale = float(w.max()) if (self.mixed and w.size and w.max() > 0) else 1.0
         w = w / hdt(self.wscale) if self.wscale != 1.0 else w
         st = self.stream
-        self.y = DeviceArray.from_numpy(y.ravel(order="F"), self.sdt, stream=st)
+        if self.offset:
+            # y - A x0 is formed in setup(); y stays on the host in FP32 until then
+            self._y_host = np.ascontiguousarray(y.ravel(order="F"), np.float32)
+            self.y = DeviceArray(y.size, self.sdt)
+        else:
+            self.y = DeviceArray.from_numpy(y.ravel(order="F"), self.sdt, stream=st)
         self.w = DeviceArray.from_numpy(w.ravel(order="F"), self.sdt, stream=st)
         nmax = max(s[2] for s in self.subsets)
         nvox = nx * ny * nz
@@ -121,6 +134,7 @@ class PwlsOS:
         self.precom = DeviceArray(nvox, vdt)
         self.xold = DeviceArray(nvox, vdt)
         self.zc = DeviceArray(nvox, self.zdt) if self.mixed else None
+        self.x0 = DeviceArray(nvox, F32) if self.offset else None  # reference image of the offset form
         self.scal = DeviceArray(4, "f64")  # phi, R, rmsd numerator, spare
         self.cur = 0  # which of the x / z double buffers is current
         self.t = 1.0
@@ -157,11 +171,23 @@ class PwlsOS:
         x0d = self.P.upload_volume(np.asarray(x0, np.float64 if self.f64 else np.float32), vdt, stream=st)
         for a in (self.x[0], self.z[0], self.s):
             a.copy_from(x0d, stream=st)
-        if self.zc is not None:
-            self.zc.copy_from(self.z[0], stream=st)
         ones = DeviceArray(self.P.n_img, vdt).fill(1.0, stream=st)
         a = DeviceArray(max(s[2] for s in self.subsets) * self.cell, vdt)
         wa = DeviceArray(a.count, vdt)
+        if self.offset:
+            # stored data y - A x0 (FP32 projection, one rounding to the storage dtype);
+            # the projector input z - x0 starts at zero
+            self.x0.copy_from(x0d, stream=st)
+            self.zc.fill(0.0, stream=st)
+            yf = DeviceArray.from_numpy(self._y_host, F32, stream=st)
+            for v0, vs, ns, off in self.subsets:
+                self.P.proj(x0d, a, f"proj,{self.kind}", v0, vs, ns, stream=st)
+                call("ctsim_sub", self._sub_ptr(yf, off), a.vp, self._sub_ptr(self.y, off), self.y.dtype,
+                     ns * self.cell, self._sv())
+            st.synchronize()
+            del yf
+        elif self.zc is not None:
+            self.zc.copy_from(self.z[0], stream=st)
         # At(w .* A(ones)) summed subset by subset: every view is used once, scale 1
         for k, (v0, vs, ns, off) in enumerate(self.subsets):
             self.P.proj(ones, a, f"proj,{self.kind}", v0, vs, ns, stream=st)
@@ -190,17 +216,18 @@ class PwlsOS:
         a, b = self.cur, 1 - self.cur
         zc = self.zc.vp if self.zc is not None else None
         zcdt = self.zc.dtype if self.zc is not None else 0
+        zref = self.x0.vp if self.x0 is not None else None
         if self.f64:
             call("ctsim_pwls_nesterov_update_f64", self.x[a].vp, self.z[a].vp, self.s.vp, self.g.vp,
                  self.precom.vp, self.x[b].vp, self.z[b].vp, nx, ny, nz, self.beta, self.delta, self.pfun, t, t_next,
                  self._sv())
         elif t is None:
             call("ctsim_pwls_nesterov_update_seq", self.x[a].vp, self.z[a].vp, self.s.vp, self.g.vp, self.precom.vp,
-                 self.x[b].vp, self.z[b].vp, zc, zcdt, nx, ny, nz, self.beta, self.delta, self.pfun, self._tseq.vp,
+                 self.x[b].vp, self.z[b].vp, zc, zcdt, zref, nx, ny, nz, self.beta, self.delta, self.pfun, self._tseq.vp,
                  self._base.vp, k, self._sv())
         else:
             call("ctsim_pwls_nesterov_update", self.x[a].vp, self.z[a].vp, self.s.vp, self.g.vp, self.precom.vp,
-                 self.x[b].vp, self.z[b].vp, zc, zcdt, nx, ny, nz, self.beta, self.delta, self.pfun, t, t_next,
+                 self.x[b].vp, self.z[b].vp, zc, zcdt, zref, nx, ny, nz, self.beta, self.delta, self.pfun, t, t_next,
                  self._sv())
         self.cur = b
 

@@ -292,12 +292,13 @@ int ctsim_huber(const float* d_x, int32_t nx, int32_t ny, int32_t nz, double del
  *   go to separate buffers (d_x_out, d_z_out must not alias d_x or d_z); s is
  *   updated in place.  t and t_next are the momentum scalars of the current and next
  *   step.  The denominator and the step are formed in FP64.  d_zcopy (may be NULL)
- *   receives z' again in zcopy_dtype, e.g. an FP16 copy for the next forward
- *   projection. */
+ *   receives z' again in zcopy_dtype, minus d_zref when d_zref is not NULL: the input of
+ *   the next forward projection, e.g. z' - x0 when the stored data are y - A x0 (A is
+ *   linear, so A(z') - y = A(z' - x0) - (y - A x0)), or an FP16 copy of z'. */
 int ctsim_pwls_nesterov_update(const float* d_x, const float* d_z, float* d_s, const float* d_g,
                                const float* d_precom, float* d_x_out, float* d_z_out, void* d_zcopy,
-                               int32_t zcopy_dtype, int32_t nx, int32_t ny, int32_t nz, double beta, double delta,
-                               int32_t pfun, double t, double t_next, void* stream);
+                               int32_t zcopy_dtype, const float* d_zref, int32_t nx, int32_t ny, int32_t nz,
+                               double beta, double delta, int32_t pfun, double t, double t_next, void* stream);
 
 /* Same update with the momentum scalars read from device memory:
  * t = d_tseq[*d_base + k], t_next = d_tseq[*d_base + k + 1].  Lets one captured graph
@@ -305,9 +306,9 @@ int ctsim_pwls_nesterov_update(const float* d_x, const float* d_z, float* d_s, c
  * base at the end of the graph). */
 int ctsim_pwls_nesterov_update_seq(const float* d_x, const float* d_z, float* d_s, const float* d_g,
                                    const float* d_precom, float* d_x_out, float* d_z_out, void* d_zcopy,
-                                   int32_t zcopy_dtype, int32_t nx, int32_t ny, int32_t nz, double beta, double delta,
-                                   int32_t pfun, const double* d_tseq, const int32_t* d_base, int32_t k,
-                                   void* stream);
+                                   int32_t zcopy_dtype, const float* d_zref, int32_t nx, int32_t ny, int32_t nz,
+                                   double beta, double delta, int32_t pfun, const double* d_tseq,
+                                   const int32_t* d_base, int32_t k, void* stream);
 
 /* *d_counter += increment, as a kernel (graph-capturable). */
 int ctsim_add_i32(int32_t* d_counter, int32_t increment, void* stream);
@@ -365,6 +366,9 @@ int ctsim_sqdiff_sum_f64(const double* d_a, const double* d_b, size_t n, double*
 /* out = a .* b elementwise with dtypes (precom = At(w .* A(ones)), reconPwlsSeNesterovSqs.m:42-43) */
 int ctsim_mul(const void* d_a, int32_t a_dtype, const void* d_b, int32_t b_dtype, void* d_out, int32_t out_dtype,
               size_t n, void* stream);
+/* out = a - b for FP32 a and b, written as out_dtype (F32, F16 or BF16); the mixed PWLS
+ * form stores y - A x0 this way. */
+int ctsim_sub(const float* d_a, const float* d_b, void* d_out, int32_t out_dtype, size_t n, void* stream);
 
 /* ------------------------------------------------------------------------------
  * FDK pre-processing on the device (fbp/reconFDK.m:33-76): every detector row of every

@@ -95,9 +95,15 @@ def test_pwls_mixed_graph_equals_stream_and_is_close_to_plain():
     assert np.array_equal(xa, xb)
     np.testing.assert_array_equal(pa, pb)
     c = PwlsOS(_geometry(g), y, w, M, beta, delta, mixed=False)
-    xc, _, _ = c.run(x0.astype(np.float32), its)
-    print("mixed vs plain:", rel_l2(xa, xc))
-    assert rel_l2(xa, xc) < 2e-3
+    xc, pc, _ = c.run(x0.astype(np.float32), its)
+    # the stored data are y - A x0 and the projector reads z - x0: the FP16 rounding is
+    # taken on the small difference, so the iterate stays close to the plain FP32 one
+    d = PwlsOS(_geometry(g), y, w, M, beta, delta, mixed=True, graph=True, offset=False)
+    xd, _, _ = d.run(x0.astype(np.float32), its)
+    print("mixed vs plain:", rel_l2(xa, xc), "without the offset:", rel_l2(xd, xc))
+    assert rel_l2(xa, xc) < 5e-4
+    assert rel_l2(xd, xc) < 2e-3
+    np.testing.assert_allclose(pa, pc, rtol=1e-3)
 
 
 def test_pwls_fp64_matches_fp64_oracle():
@@ -263,7 +269,7 @@ def test_nesterov_update_step_both_kernel_forms(nz, pfun):
     dx, dz, ds, dg, dp = dev(x), dev(z), dev(s0), dev(g), dev(pre)
     xo, zo = DeviceArray(x.size, "f32"), DeviceArray(x.size, "f32")
     zc = DeviceArray(x.size, "f16")
-    call("ctsim_pwls_nesterov_update", dx.vp, dz.vp, ds.vp, dg.vp, dp.vp, xo.vp, zo.vp, zc.vp, zc.dtype, nx, ny, nz,
+    call("ctsim_pwls_nesterov_update", dx.vp, dz.vp, ds.vp, dg.vp, dp.vp, xo.vp, zo.vp, zc.vp, zc.dtype, None, nx, ny, nz,
          beta, delta, penalty_code(pfun), t, tn, None)
     f = lambda a: a.astype(np.float32).astype(np.float64)  # noqa: E731  the device's FP32 inputs
     x, z, s0, g, pre = f(x), f(z), f(s0), f(g), f(pre)
