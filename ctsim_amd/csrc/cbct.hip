@@ -38,15 +38,45 @@ constexpr int kZWin = 256;  // voxels staged per LDS window and wave
 constexpr int kPre = 3;     // voxel loads per lane issued one column ahead (3 x 64 voxels)
 static_assert(kPre == 3, "the pipeline below holds three prefetch registers");
 
+// 44 bytes in FP32: with the 16-cell accumulators a block stays under 32 KB of LDS, so a
+// CU holds five blocks.  1/zinc is recomputed from zinc (the same v_rcp, the same value);
+// the footprint corners, needed only for footprints wider than four tile cells, are
+// recomputed from the column on that path.
 template <class T>
 struct ColParams {
     int col;    // image column iy * nx + ix
     int cells;  // footprint cells ilo | ihi << 16 (detectors narrower than 32768 cells)
     int zab;    // voxel range za | zb << 16 needed by the tile's rows (nz <= 512)
-    T rz;       // 1 / zinc
     T ampr, r2;  // amp / r and r^2: the staged value of voxel z is sqrt(r^2 + zc^2) * ampr * img
-    T zinc, vstart, w0, w1, w2, w3, p0, p1, p2, p3;
+    T zinc, vstart, w0, w1, w2, w3;
 };
+
+// footprint corners of column (ix, iy) in view v: DD (clo, chi) in p0, p1; SF the sorted taus
+template <class T, int FOOT>
+__device__ __forceinline__ bool col_corners(const Geom<T>& g, const View<T>& v, int ix, int iy, T& p0, T& p1, T& p2,
+                                           T& p3, int& ilo, int& ihi, T& amp, T& r, T& d) {
+    const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
+    const Column<T> cg = column_geom(xc, yc, v, g.sad, g.dx);
+    T clo, chi;
+    bool ok;
+    if (FOOT == FOOT_DD) {
+        T umin, umax;
+        dd_bounds(cg, v, g, umin, umax);
+        ok = clamp_cells(umin, umax, g.nu, 16, clo, chi, ilo, ihi);
+        p0 = clo;
+        p1 = chi;
+        p2 = p3 = T(0);
+    } else {
+        T tau[4];
+        sf_taus(cg, v, g, tau);
+        ok = clamp_cells(tau[0], tau[3], g.nu, 16, clo, chi, ilo, ihi);
+        p0 = tau[0]; p1 = tau[1]; p2 = tau[2]; p3 = tau[3];
+    }
+    amp = cg.amp;
+    r = cg.r;
+    d = cg.d;
+    return ok;
+}
 
 // weight of tile cell iu for a column whose u footprint is (ilo, ihi, p*): DD uses
 // (clo, chi) in p0, p1; SF the sorted corners p0..p3
@@ -99,7 +129,7 @@ __device__ __forceinline__ T gather_row(const T* zwin, int zw0, int zw1, int ja,
 }
 
 template <class Tin, class Tout, class T, int FOOT, int TU, int RPL, int SPLIT>
-__global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View<T>* __restrict__ views,
+__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 5 : 1) void cbct_proj_kernel(Geom<T> g, const View<T>* __restrict__ views,
                                                            int vstride, int nviews, const Tin* __restrict__ vol,
                                                            const uint8_t* __restrict__ map,
                                                            Tout* __restrict__ proj, int exhaustive) {
@@ -108,9 +138,8 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
     __shared__ T zbuf[kWavesPerBlock][kZWin + 1 + kZPad];
     __shared__ ColParams<T> cbuf[kWavesPerBlock][kWave];
     // each lane's row of the tile, [cell][lane] so that a wave's access is one word per
-    // bank; three spare cells take the weights past the tile's last cell, so the four
-    // adds of a column need no condition.  Every word has one writer (its lane).
-    __shared__ T accs[kWavesPerBlock][TU + 3][kWave];
+    // bank.  Every word has one writer (its lane).
+    __shared__ T accs[kWavesPerBlock][TU][kWave];
     const int rows = RPL * kWave;
     const int nut = (g.nu + TU - 1) / TU;
     const int nvc = (g.nv + rows - 1) / rows;
@@ -147,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
 
     T* acc = &accs[wv][0][lane];
 #pragma unroll
-    for (int t = 0; t < TU + 3; ++t) acc[t * kWave] = T(0);
+    for (int t = 0; t < TU; ++t) acc[t * kWave] = T(0);
 
     for (int m0 = share * kWave; m0 < m_end; m0 += SPLIT * kWave) {
         const int m = m0 + lane;
@@ -180,31 +209,17 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                 ColParams<T> cp{};
                 cp.r2 = T(1);
                 cp.zinc = T(1);
-                cp.rz = T(1);
                 if (ok && map != nullptr) ok = map[iy * g.nx + ix] != 0;
                 if (ok) {
-                    const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
-                    const Column<T> cg = column_geom(xc, yc, v, g.sad, g.dx);
-                    T clo, chi;
+                    T p0, p1, p2, p3, amp, r, d;
                     int ilo = 0, ihi = -1, iz0 = 0, iz1 = 0;
-                    if (FOOT == FOOT_DD) {
-                        T umin, umax;
-                        dd_bounds(cg, v, g, umin, umax);
-                        ok = clamp_cells(umin, umax, g.nu, 16, clo, chi, ilo, ihi);
-                        cp.p0 = clo;
-                        cp.p1 = chi;
-                    } else {
-                        T tau[4];
-                        sf_taus(cg, v, g, tau);
-                        ok = clamp_cells(tau[0], tau[3], g.nu, 16, clo, chi, ilo, ihi);
-                        cp.p0 = tau[0]; cp.p1 = tau[1]; cp.p2 = tau[2]; cp.p3 = tau[3];
-                    }
+                    ok = col_corners<T, FOOT>(g, v, ix, iy, p0, p1, p2, p3, ilo, ihi, amp, r, d);
                     ok = ok && ihi >= iu_a && ilo <= iu_b;
                     cp.cells = ilo | (ihi << 16);
                     if (ok) {
-                        ok = v_window(g, cg.d, v.wz, cp.zinc, cp.vstart, iz0, iz1) && iz0 < iz1 && cp.zinc > T(0);
-                        cp.ampr = cg.amp * frcp(cg.r);
-                        cp.r2 = cg.r * cg.r;
+                        ok = v_window(g, d, v.wz, cp.zinc, cp.vstart, iz0, iz1) && iz0 < iz1 && cp.zinc > T(0);
+                        cp.ampr = amp * frcp(r);
+                        cp.r2 = r * r;
                     }
                     if (ok) {
                         // voxel range needed by this wave's rows (exact ranges; a voxel left
@@ -212,17 +227,16 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                         const T rz = frcp(cp.zinc);
                         const int za = max((int)ceil((T(iv_a) - T(0.5) - cp.vstart) * rz - T(1)), iz0);
                         const int zb = min((int)ceil((T(iv_last) + T(0.5) - cp.vstart) * rz) - 1, iz1 - 1);
-                        cp.rz = rz;
                         cp.zab = za | (zb << 16);
                         ok = za <= zb;
                     }
                     if (ok) {
                         // tile weights of the first four footprint cells inside the tile
                         const int c0 = max(ilo, iu_a);
-                        cp.w0 = u_weight<T, FOOT>(c0 + 0, ilo, ihi, cp.p0, cp.p1, cp.p2, cp.p3);
-                        cp.w1 = u_weight<T, FOOT>(c0 + 1, ilo, ihi, cp.p0, cp.p1, cp.p2, cp.p3);
-                        cp.w2 = u_weight<T, FOOT>(c0 + 2, ilo, ihi, cp.p0, cp.p1, cp.p2, cp.p3);
-                        cp.w3 = u_weight<T, FOOT>(c0 + 3, ilo, ihi, cp.p0, cp.p1, cp.p2, cp.p3);
+                        cp.w0 = u_weight<T, FOOT>(c0 + 0, ilo, ihi, p0, p1, p2, p3);
+                        cp.w1 = u_weight<T, FOOT>(c0 + 1, ilo, ihi, p0, p1, p2, p3);
+                        cp.w2 = u_weight<T, FOOT>(c0 + 2, ilo, ihi, p0, p1, p2, p3);
+                        cp.w3 = u_weight<T, FOOT>(c0 + 3, ilo, ihi, p0, p1, p2, p3);
                         cp.col = iy * g.nx + ix;
                     }
                 }
@@ -248,7 +262,7 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                     const ColParams<T>& c = cw[cur];
                     const int zab = uni(c.zab), cells = uni(c.cells);
                     const int za = zab & 0xffff, zb = zab >> 16, ilo = cells & 0xffff, ihi = cells >> 16;
-                    const T ampr = c.ampr, r2 = c.r2, zinc = c.zinc, vstart = c.vstart, rz = c.rz;
+                    const T ampr = c.ampr, r2 = c.r2, zinc = c.zinc, vstart = c.vstart, rz = frcp(zinc);
                     const Tin* colp = vol + (size_t)uni(c.col) * g.nz;
                     const int iv = iv_a + lane;
                     T q = T(0);
@@ -334,17 +348,22 @@ __global__ __launch_bounds__(kBlock) void cbct_proj_kernel(Geom<T> g, const View
                     T w0 = uni(c.w0), w1 = uni(c.w1), w2 = uni(c.w2), w3 = uni(c.w3);
                     for (int cc = c0; cc <= c1; cc += 4) {
                         if (cc > c0) {
-                            const T p0 = uni(c.p0), p1 = uni(c.p1), p2 = uni(c.p2), p3 = uni(c.p3);
+                            // wider than four tile cells (rare): the corners again, from the column
+                            const int col = uni(c.col);
+                            T p0, p1, p2, p3, amp, r, d;
+                            int jlo, jhi;
+                            col_corners<T, FOOT>(g, v, col % g.nx, col / g.nx, p0, p1, p2, p3, jlo, jhi, amp, r, d);
                             w0 = u_weight<T, FOOT>(iu_a + cc, ilo, ihi, p0, p1, p2, p3);
                             w1 = cc + 1 <= c1 ? u_weight<T, FOOT>(iu_a + cc + 1, ilo, ihi, p0, p1, p2, p3) : T(0);
                             w2 = cc + 2 <= c1 ? u_weight<T, FOOT>(iu_a + cc + 2, ilo, ihi, p0, p1, p2, p3) : T(0);
                             w3 = cc + 3 <= c1 ? u_weight<T, FOOT>(iu_a + cc + 3, ilo, ihi, p0, p1, p2, p3) : T(0);
                         }
+                        // the cells past c1 (zero weight) are skipped by uniform branches
                         T* pa = acc + cc * kWave;
                         pa[0] += w0 * aq;
-                        pa[kWave] += w1 * aq;
-                        pa[2 * kWave] += w2 * aq;
-                        pa[3 * kWave] += w3 * aq;
+                        if (cc + 1 <= c1) pa[kWave] += w1 * aq;
+                        if (cc + 2 <= c1) pa[2 * kWave] += w2 * aq;
+                        if (cc + 3 <= c1) pa[3 * kWave] += w3 * aq;
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
