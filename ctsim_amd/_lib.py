@@ -54,6 +54,8 @@ SIGNATURES = {
     "ctsim_cbct_dd_cuda": (C.c_int, _GATEWAY3),
     "ctsim_cbct_sf_cuda": (C.c_int, _GATEWAY3),
     "ctsim_cbct_back_pd_cuda": (C.c_int, _GATEWAY3),
+    "ctsim_fbct_proj_rd": (C.c_int, _PER_OP),
+    "ctsim_cbct_proj_rd": (C.c_int, _PER_OP),
     "ctsim_fbct_proj_dd": (C.c_int, _PER_OP),
     "ctsim_fbct_proj_tf": (C.c_int, _PER_OP),
     "ctsim_fbct_back_dd": (C.c_int, _PER_OP),

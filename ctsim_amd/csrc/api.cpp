@@ -105,10 +105,11 @@ int check_geom(int ndim, const ctsim_geom& g) {
     return CTSIM_OK;
 }
 
-enum OpKind { OP_PROJ_DD, OP_PROJ_TF, OP_BACK_DD, OP_BACK_TF, OP_BACK_PD, OP_BACK_HPD, OP_BAD };
+enum OpKind { OP_PROJ_RD, OP_PROJ_DD, OP_PROJ_TF, OP_BACK_DD, OP_BACK_TF, OP_BACK_PD, OP_BACK_HPD, OP_BAD };
 
 OpKind parse_op(const char* op) {
     if (!op) return OP_BAD;
+    if (!std::strcmp(op, "proj,rd")) return OP_PROJ_RD;
     if (!std::strcmp(op, "proj,dd")) return OP_PROJ_DD;
     if (!std::strcmp(op, "proj,tf")) return OP_PROJ_TF;
     if (!std::strcmp(op, "back,dd")) return OP_BACK_DD;
@@ -116,6 +117,83 @@ OpKind parse_op(const char* op) {
     if (!std::strcmp(op, "back,pd")) return OP_BACK_PD;
     if (!std::strcmp(op, "back,hpd")) return OP_BACK_HPD;
     return OP_BAD;
+}
+
+// The ray table of "proj,rd", in float with the reference's operations and promotions
+// (geom/cbct_os/cbct_proj.c:38-134, geom/fbct_os/fbct_proj.c:22-121): box corners and dx/2 as
+// double expressions rounded to float, sin / cos of beta and atan in double, phi rounded
+// to float and sinf / cosf / sqrtf of floats.  The reference's axis-parallel test
+// `raydir < -1E-12 && raydir > 1E-12` is never true, so both slab tests always divide.
+RdConst rd_const(const ctsim_geom& g, int ndim) {
+    RdConst c{};
+    c.wv = ndim == 3 ? (float)((-(g.nv - 1.0) / 2.0 + g.ov) * g.dv) : 0.f;
+    c.cmin_x = (float)((-(g.nx - 1.0) / 2.0 + g.ox) * g.dx);
+    c.cmin_y = (float)((-(g.ny - 1.0) / 2.0 + g.oy) * g.dy);
+    c.delta = (float)(g.dx / 2.0);
+    c.dx = g.dx;
+    c.dy = g.dy;
+    c.dz = ndim == 3 ? g.dz : 1.f;
+    c.dv = ndim == 3 ? g.dv : 1.f;
+    c.nx = g.nx;
+    c.ny = g.ny;
+    c.nz = ndim == 3 ? g.nz : 1;
+    c.nu = g.nu;
+    c.nv = ndim == 3 ? g.nv : 1;
+    return c;
+}
+
+std::vector<RayRec> rd_table(const ctsim_geom& g, int ndim, int nviews, const double* betas, const float* couchz) {
+    const float wu = (float)((-(g.nu - 1.0) / 2.0 + g.ou) * g.du);
+    const float cmin_x = (float)((-(g.nx - 1.0) / 2.0 + g.ox) * g.dx);
+    const float cmax_x = (float)((+(g.nx - 1.0) / 2.0 + g.ox) * g.dx);
+    const float cmin_y = (float)((-(g.ny - 1.0) / 2.0 + g.oy) * g.dy);
+    const float cmax_y = (float)((+(g.ny - 1.0) / 2.0 + g.oy) * g.dy);
+    std::vector<RayRec> t((size_t)nviews * g.nu);
+    for (int ib = 0; ib < nviews; ++ib) {
+        const double beta = betas[ib];
+        const float sin_a = (float)std::sin(beta), cos_a = (float)std::cos(beta);
+        const float sx = -g.sad * sin_a, sy = -g.sad * cos_a;
+        const float czmin =
+            ndim == 3 ? (float)((-(g.nz - 1.0) / 2.0 + g.oz) * g.dz + (couchz ? couchz[ib] : 0.f)) : 0.f;
+        for (int iu = 0; iu < g.nu; ++iu) {
+            RayRec r{1.f, 0.f, 0.f, 0.f, 1.f, sx, sy, czmin};  // tn > tf: the ray misses the box
+            const float u = iu * g.du + wu;
+            float spd, phi;
+            if (g.detector_type == 1) {
+                spd = sqrtf(g.sdd * g.sdd + u * u);
+                phi = (float)(beta + std::atan((double)(u / g.sdd)));
+            } else {
+                spd = g.sdd;
+                phi = (float)(beta + (double)(u / g.sdd));
+            }
+            const float rdx = sinf(phi), rdy = cosf(phi);
+            float tn = -1E10f, tf = 1E10f;
+            bool hit = true;
+            {
+                float t1 = (cmin_x - sx) / rdx, t2 = (cmax_x - sx) / rdx;
+                if (t1 > t2) std::swap(t1, t2);
+                if (t1 > tn) tn = t1;
+                if (t2 < tf) tf = t2;
+                if (tn > tf || tf < 0) hit = false;
+            }
+            if (hit) {
+                float t1 = (cmin_y - sy) / rdy, t2 = (cmax_y - sy) / rdy;
+                if (t1 > t2) std::swap(t1, t2);
+                if (t1 > tn) tn = t1;
+                if (t2 < tf) tf = t2;
+                if (tn > tf || tf < 0) hit = false;
+            }
+            if (hit) {
+                r.tn = tn;
+                r.tf = tf;
+            }
+            r.rdx = rdx;
+            r.rdy = rdy;
+            r.spd = spd;
+            t[(size_t)ib * g.nu + iu] = r;
+        }
+    }
+    return t;
 }
 
 }  // namespace
@@ -131,6 +209,10 @@ struct ctsim_plan {
     View<float>* d_vf = nullptr;
     View<double>* d_vd = nullptr;
     uint8_t* d_map = nullptr;
+    std::vector<double> betas;  // kept for the ray table, built on the first "proj,rd"
+    std::vector<float> couchz;
+    mutable RayRec* d_rays = nullptr;
+    mutable std::mutex rays_mutex;
 };
 
 extern "C" {
@@ -155,6 +237,8 @@ int ctsim_plan_create(ctsim_plan** out, int32_t ndim, const ctsim_geom* g, int32
     p->compute = compute;
     p->nviews = nviews;
     p->g = *g;
+    p->betas.assign(betas, betas + nviews);
+    if (couchz) p->couchz.assign(couchz, couchz + nviews);
     HIP_TRY(hipGetDevice(&p->device), "hipGetDevice");
     hipError_t e = reduction_scratch_init();
     if (e != hipSuccess) {
@@ -190,6 +274,7 @@ int ctsim_plan_destroy(ctsim_plan* p) {
     if (p->d_vf) (void)hipFree(p->d_vf);
     if (p->d_vd) (void)hipFree(p->d_vd);
     if (p->d_map) (void)hipFree(p->d_map);
+    if (p->d_rays) (void)hipFree(p->d_rays);
     delete p;
     return CTSIM_OK;
 }
@@ -205,11 +290,37 @@ static int plan_proj_impl(const ctsim_plan* p, const char* op, int view0, int vs
                           int img_dt, void* d_sino, int sino_dt, void* stream, int exhaustive) {
     if (!p) return fail(CTSIM_EINVAL, "Error: null plan.");
     const OpKind k = parse_op(op);
-    if (k != OP_PROJ_DD && k != OP_PROJ_TF) return fail(CTSIM_EINVAL, "Error: unknown geom type.");
+    if (k != OP_PROJ_DD && k != OP_PROJ_TF && k != OP_PROJ_RD) return fail(CTSIM_EINVAL, "Error: unknown geom type.");
     if (int rc = check_views(p, view0, vstep, nsub)) return rc;
     if (!d_img || !d_sino) return fail(CTSIM_EINVAL, "Error: null device buffer.");
     if (!valid_dtype(img_dt) || !valid_dtype(sino_dt)) return fail(CTSIM_EINVAL, "Error: bad dtype.");
     hipStream_t st = (hipStream_t)stream;
+    if (k == OP_PROJ_RD) {
+        // the ray march is the reference's float arithmetic; the plan's map is not used
+        // (cbct_geom_mex runs "proj,rd" only in the 15-argument, map-free form)
+        if (p->compute != CTSIM_F32) return fail(CTSIM_EUNSUP, "Error: proj,rd runs in FP32 only.");
+        {
+            std::lock_guard<std::mutex> lock(p->rays_mutex);
+            if (!p->d_rays) {
+                const std::vector<RayRec> t =
+                    rd_table(p->g, p->ndim, p->nviews, p->betas.data(), p->couchz.empty() ? nullptr : p->couchz.data());
+                RayRec* d = nullptr;
+                HIP_TRY(hipMalloc((void**)&d, t.size() * sizeof(RayRec)), "ray table");
+                hipError_t ec = hipMemcpy(d, t.data(), t.size() * sizeof(RayRec), hipMemcpyHostToDevice);
+                if (ec != hipSuccess) {
+                    (void)hipFree(d);
+                    return hip_fail(ec, "ray table upload");
+                }
+                p->d_rays = d;
+            }
+        }
+        const RdConst c = rd_const(p->g, p->ndim);
+        hipError_t er = p->ndim == 3 ? cbct_rd_launch(p->d_rays, view0, vstep, nsub, c, d_img, img_dt, d_sino, sino_dt, st)
+                                     : fbct_rd_launch(p->d_rays, view0, vstep, nsub, c, d_img, img_dt, d_sino, sino_dt, st);
+        if (er == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype combination not supported for proj,rd.");
+        HIP_TRY(er, "ray-driven projection launch");
+        return CTSIM_OK;
+    }
     const int foot = k == OP_PROJ_DD ? 0 : 1;
     hipError_t e;
     if (p->compute == CTSIM_F32) {
@@ -360,10 +471,13 @@ int gateway(int ndim, const char* op, const ctsim_geom& g, int32_t noviews, cons
     if (k == OP_BAD || (ndim == 2 && k == OP_BACK_HPD)) return fail(CTSIM_EINVAL, "Error: unknown geom type.");
     if (!in || !out) return fail(CTSIM_EINVAL, "Error: null data pointer.");
     if (noviews <= 0 || !betas) return fail(CTSIM_EINVAL, "Error: mx_betas must have noviews X 1 double array.");
+    // cbct_geom_mex's 16-argument (map) family has no ray-driven projector
+    // (geom/cbct_os/cbct_geom_mex.c:282-286): the call fails there
+    if (ndim == 3 && k == OP_PROJ_RD && map) return fail(CTSIM_EINVAL, "Error: cbct_geom_mex() failed.");
     std::lock_guard<std::mutex> lock(g_gateway_mutex);
     PlanGuard pg;
     if (int rc = ctsim_plan_create(&pg.p, ndim, &g, noviews, betas, couchz, map, CTSIM_F32)) return rc;
-    const bool proj = k == OP_PROJ_DD || k == OP_PROJ_TF;
+    const bool proj = k == OP_PROJ_DD || k == OP_PROJ_TF || k == OP_PROJ_RD;
     const size_t nimg = (size_t)g.nx * g.ny * g.nz;
     const size_t nsino = (size_t)g.nu * g.nv * noviews;
     void *d_in = nullptr, *d_out = nullptr, *d_tmp = nullptr;
@@ -503,11 +617,13 @@ int ctsim_cbct_back_pd_cuda(const char* op, const int32_t nxyz[3], const float d
         return rc;                                                                                                 \
     }
 
+CTSIM_PER_OP(ctsim_fbct_proj_rd, 2, "proj,rd")
 CTSIM_PER_OP(ctsim_fbct_proj_dd, 2, "proj,dd")
 CTSIM_PER_OP(ctsim_fbct_proj_tf, 2, "proj,tf")
 CTSIM_PER_OP(ctsim_fbct_back_dd, 2, "back,dd")
 CTSIM_PER_OP(ctsim_fbct_back_tf, 2, "back,tf")
 CTSIM_PER_OP(ctsim_fbct_back_pd, 2, "back,pd")
+CTSIM_PER_OP(ctsim_cbct_proj_rd, 3, "proj,rd")
 CTSIM_PER_OP(ctsim_cbct_proj_dd, 3, "proj,dd")
 CTSIM_PER_OP(ctsim_cbct_proj_tf, 3, "proj,tf")
 CTSIM_PER_OP(ctsim_cbct_back_dd, 3, "back,dd")

@@ -28,6 +28,24 @@ hipError_t cbct_back_pd_launch(const Geom<T>& g, const View<T>* views, int vstri
 hipError_t xyz_zxy_launch(const void* src, int src_dt, void* dst, int dst_dt, int nxy, int nz, bool to_zxy,
                           hipStream_t st);
 
+// Ray-driven forward projection (geom/fbct_os/fbct_proj.c:3-153, geom/cbct_os/cbct_proj.c:10-198).
+// One record per (view, detector column), built on the host in float exactly as the
+// reference forms it (double sin/atan, float sinf/sqrtf, box slab tests); a ray that misses
+// the box has tn > tf.  czmin is the view's lower z bound of the box (3D).
+struct RayRec {
+    float tn, tf, rdx, rdy, spd, sx, sy, czmin;
+};
+// Constants of the march, formed on the host like the reference (double expressions
+// rounded to float).
+struct RdConst {
+    float wv, cmin_x, cmin_y, delta, dx, dy, dz, dv;
+    int nx, ny, nz, nu, nv;
+};
+hipError_t cbct_rd_launch(const RayRec* rays, int view0, int vstep, int nviews, const RdConst& c, const void* vol,
+                          int vol_dt, void* proj, int proj_dt, hipStream_t st);
+hipError_t fbct_rd_launch(const RayRec* rays, int view0, int vstep, int nviews, const RdConst& c, const void* img,
+                          int img_dt, void* proj, int proj_dt, hipStream_t st);
+
 // 2D fan beam.  kind: 0 = DD, 1 = SF, 2 = pd (back only).
 template <class T>
 hipError_t fbct_proj_launch(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* img,

@@ -36,9 +36,6 @@ def _check_op(op, ops):
         raise ValueError("The 1st argument is invalid!")
     if op not in ops:
         raise ValueError("Error: unknown geom type.")
-    if op == "proj,rd":
-        # ray-driven projection is outside this library (not on the PWLS / FDK path)
-        raise NotImplementedError("proj,rd is not provided by the MI355X library")
 
 
 def _f32_data(x, shape, msg):

@@ -56,7 +56,7 @@ int ctsim_version(void); /* 100 * major + minor */
 /* Replaces fbct_geom_mex (geom/fbct_os/fbct_geom_mex.c:250-326), the 14-argument 2D
  * fan-beam gateway called by geom/shell/forwardProjectMex.m:52-55 and
  * geom/shell/backProjectMex.m:145-148.
- * op: "proj,dd" "proj,tf" "back,dd" "back,tf" "back,pd".
+ * op: "proj,rd" "proj,dd" "proj,tf" "back,dd" "back,tf" "back,pd".
  * in:  image [nx ny] for proj, sinogram [nu noviews] for back (float32).
  * out: sinogram [nu noviews] for proj, image [nx ny] for back (float32). */
 int ctsim_fbct_geom_mex(const char* op, const int32_t nxy[2], const float dxy[2], const float oxy[2], int32_t nu,
@@ -67,7 +67,8 @@ int ctsim_fbct_geom_mex(const char* op, const int32_t nxy[2], const float dxy[2]
  * gateway called by geom/shell/forwardProjectMex.m:82-85 and
  * geom/shell/backProjectMex.m:181-184.  map == NULL is the 15-argument (no map)
  * family, otherwise the 16-argument family with logical map [nx ny] (one byte per
- * column).  op: "proj,dd" "proj,tf" "back,dd" "back,tf" "back,pd" "back,hpd".
+ * column).  op: "proj,rd" (15-argument family only; with a map the call fails as the
+ * reference's does), "proj,dd" "proj,tf" "back,dd" "back,tf" "back,pd" "back,hpd".
  * in:  image [nx ny nz] for proj, sinogram [nv nu noviews] for back (float32).
  * out: sinogram [nv nu noviews] for proj, image [nx ny nz] for back (float32). */
 int ctsim_cbct_geom_mex(const char* op, const int32_t nxyz[3], const float dxyz[3], const float cxyz[3],
@@ -123,6 +124,15 @@ typedef struct ctsim_geom {
  * (NULL = zeros) and map [nx ny] (NULL = all columns). */
 int ctsim_fbct_proj_dd(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
                        const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
+/* Ray-driven forward projection, op "proj,rd" (geom/fbct_os/fbct_proj.c:3-153,
+ * geom/cbct_os/cbct_proj.c:10-198): rays marched in steps of dx/2 with bilinear / trilinear
+ * interpolation, in the reference's float arithmetic.  The 3D form takes no map: with a
+ * map it fails ("Error: cbct_geom_mex() failed."), as the reference's 16-argument
+ * gateway does (geom/cbct_os/cbct_geom_mex.c:282-286). */
+int ctsim_fbct_proj_rd(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
+                       const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
+int ctsim_cbct_proj_rd(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
+                       const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
 int ctsim_fbct_proj_tf(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
                        const uint8_t* map, const float* in, float* out, char* errbuf, size_t errlen);
 int ctsim_fbct_back_dd(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz,
@@ -159,7 +169,8 @@ int ctsim_plan_destroy(ctsim_plan* plan);
 
 /* Forward projection of the views view0, view0 + vstep, ... (nsub of them), the
  * ordered-subset selection of geom/shell/forwardProjectMex.m:35-39.
- * op "proj,dd" or "proj,tf".  d_img: 3D volume in device ZXY order (z fastest,
+ * op "proj,dd", "proj,tf" or "proj,rd" (FP32 plans only; the plan's map is not used: the
+ * ray table is built on the plan's first "proj,rd").  d_img: 3D volume in device ZXY order (z fastest,
  * vol[(iy*nx+ix)*nz+iz]) or 2D image [nx ny]; d_sino: [nsub][nu][nv] (v fastest),
  * every element written. */
 int ctsim_plan_proj(const ctsim_plan* plan, const char* op, int32_t view0, int32_t vstep, int32_t nsub,

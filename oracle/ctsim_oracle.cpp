@@ -26,6 +26,8 @@
 //   cbct_back1_pd  geom/cbct_os/cbct_back.c:135-262   (cbct_back_pd :9-132; pd/hpd flag)
 //   cbct_back1_dd  geom/cbct_os/cbct_back.c:479-692   (cbct_back_dd :267-475)
 //   cbct_back1_tf  geom/cbct_os/cbct_back.c:945-1197  (cbct_back_tf :695-942)
+//   fbct_proj_rd   geom/fbct_os/fbct_proj.c:3-153
+//   cbct_proj_rd   geom/cbct_os/cbct_proj.c:10-198
 //   op dispatch    geom/fbct_os/fbct_geom_mex.c:271-305, geom/cbct_os/cbct_geom_mex.c:527-583
 //
 // Layouts are MATLAB column-major: image [nx ny (nz)] with x fastest, 2D
@@ -583,6 +585,134 @@ void cbct_back_pd(bool off_boundary, const Geom3& g, const double* betas, const 
     }
 }
 
+// Ray-driven forward projection (geom/fbct_os/fbct_proj.c:3-153, geom/cbct_os/cbct_proj.c:10-198):
+// every detector cell's ray is marched from its entry into the image box in steps of dx/2,
+// t accumulated by repeated addition as the reference does, sampling the image by
+// bilinear (2D) / trilinear (3D) interpolation at interior positions.  The reference's
+// test `raydir_x < -1E-12 && raydir_x > 1E-12` is never true, so the slab bounds are
+// always formed by division (an axis-parallel ray gives +-inf, as there).  In the float
+// form: sin(beta) and atan in double, sin(phi) and sqrt of floats in float (C++
+// overloads), the box corners and dx/2 in double rounded to float.
+template <class T>
+struct RayRow {
+    bool hit;
+    T tn, tf, rdx, rdy, spd, sx, sy;
+};
+
+template <class T>
+RayRow<T> ray_row(double beta, int iu, T du, T wu, T sad, T sdd, int dettype, T cmin_x, T cmax_x, T cmin_y,
+                  T cmax_y) {
+    RayRow<T> r{};
+    const T sin_a = (T)std::sin(beta), cos_a = (T)std::cos(beta);
+    r.sx = -sad * sin_a;
+    r.sy = -sad * cos_a;
+    const T u = iu * du + wu;
+    T phi;
+    if (dettype == 1) {
+        r.spd = std::sqrt(sdd * sdd + u * u);
+        phi = (T)(beta + std::atan((double)(u / sdd)));
+    } else {
+        r.spd = sdd;
+        phi = (T)(beta + (double)(u / sdd));
+    }
+    r.rdx = (T)std::sin(phi);
+    r.rdy = (T)std::cos(phi);
+    T tn = (T)-1E10, tf = (T)1E10;
+    {
+        T t1 = (cmin_x - r.sx) / r.rdx, t2 = (cmax_x - r.sx) / r.rdx;
+        if (t1 > t2) std::swap(t1, t2);
+        if (t1 > tn) tn = t1;
+        if (t2 < tf) tf = t2;
+        if (tn > tf || tf < 0) return r;
+    }
+    {
+        T t1 = (cmin_y - r.sy) / r.rdy, t2 = (cmax_y - r.sy) / r.rdy;
+        if (t1 > t2) std::swap(t1, t2);
+        if (t1 > tn) tn = t1;
+        if (t2 < tf) tf = t2;
+        if (tn > tf || tf < 0) return r;
+    }
+    r.hit = true;
+    r.tn = tn;
+    r.tf = tf;
+    return r;
+}
+
+template <class T>
+void fbct_proj_rd(const Geom2& g, const double* betas, const T* img, T* proj, int nthreads) {
+    const T wu = (T)((-(g.nu - 1.0) / 2.0 + g.ou) * g.du);
+    const T cmin_x = (T)((-(g.nx - 1.0) / 2.0 + g.ox) * g.dx), cmax_x = (T)((+(g.nx - 1.0) / 2.0 + g.ox) * g.dx);
+    const T cmin_y = (T)((-(g.ny - 1.0) / 2.0 + g.oy) * g.dy), cmax_y = (T)((+(g.ny - 1.0) / 2.0 + g.oy) * g.dy);
+    const T delta = (T)(g.dx / 2.0);
+    const T dx = g.dx, dy = g.dy;
+    const int nx = g.nx, ny = g.ny;
+#pragma omp parallel for num_threads(nthreads_or_default(nthreads)) schedule(dynamic)
+    for (int ib = 0; ib < g.nviews; ++ib)
+        for (int iu = 0; iu < g.nu; ++iu) {
+            const RayRow<T> r = ray_row<T>(betas[ib], iu, (T)g.du, wu, (T)g.sad, (T)g.sdd, g.dettype, cmin_x,
+                                           cmax_x, cmin_y, cmax_y);
+            if (!r.hit) continue;
+            T temp = 0;
+            for (T t = r.tn; t < r.tf; t = t + delta) {
+                const T x = (r.sx + t * r.rdx - cmin_x) / dx;
+                const T y = (r.sy + t * r.rdy - cmin_y) / dy;
+                const int ix = (int)std::floor(x), iy = (int)std::floor(y);
+                if (ix > 0 && ix < nx - 1 && iy > 0 && iy < ny - 1) {
+                    const T wxl = x - ix, wyl = y - iy;
+                    const T* p = img + iy * nx + ix;
+                    temp = temp + (1 - wyl) * ((1 - wxl) * p[0] + wxl * p[1]) + wyl * ((1 - wxl) * p[nx] + wxl * p[nx + 1]);
+                }
+            }
+            proj[ib * g.nu + iu] = temp * delta / 10;
+        }
+}
+
+template <class T>
+void cbct_proj_rd(const Geom3& g, const double* betas, const float* couchz, const T* img, T* proj, int nthreads) {
+    const T wu = (T)((-(g.nu - 1.0) / 2.0 + g.ou) * g.du);
+    const T wv = (T)((-(g.nv - 1.0) / 2.0 + g.ov) * g.dv);
+    const T cmin_x = (T)((-(g.nx - 1.0) / 2.0 + g.ox) * g.dx), cmax_x = (T)((+(g.nx - 1.0) / 2.0 + g.ox) * g.dx);
+    const T cmin_y = (T)((-(g.ny - 1.0) / 2.0 + g.oy) * g.dy), cmax_y = (T)((+(g.ny - 1.0) / 2.0 + g.oy) * g.dy);
+    const T delta = (T)(g.dx / 2.0);
+    const T dx = g.dx, dy = g.dy, dz = g.dz, dv = g.dv;
+    const int nx = g.nx, ny = g.ny, nz = g.nz, nv = g.nv;
+    const size_t nxy = (size_t)nx * ny;
+#pragma omp parallel for num_threads(nthreads_or_default(nthreads)) schedule(dynamic)
+    for (int ib = 0; ib < g.nviews; ++ib) {
+        const T cmin_z = (T)((-(nz - 1.0) / 2.0 + g.oz) * g.dz + couchz[ib]);
+        const T source_z = 0;
+        for (int iu = 0; iu < g.nu; ++iu) {
+            const RayRow<T> r = ray_row<T>(betas[ib], iu, (T)g.du, wu, (T)g.sad, (T)g.sdd, g.dettype, cmin_x,
+                                           cmax_x, cmin_y, cmax_y);
+            if (!r.hit) continue;
+            T* out = proj + ((size_t)ib * g.nu + iu) * nv;
+            for (int iv = 0; iv < nv; ++iv) {
+                const T rdz = (iv * dv + wv - source_z) / r.spd;
+                T temp = 0;
+                for (T t = r.tn; t < r.tf; t = t + delta) {
+                    const T z = (source_z + t * rdz - cmin_z) / dz;
+                    if (z < 1) continue;
+                    if ((double)z > nz - 1.1) break;
+                    const T x = (r.sx + t * r.rdx - cmin_x) / dx;
+                    const T y = (r.sy + t * r.rdy - cmin_y) / dy;
+                    const int ix = (int)std::floor(x), iy = (int)std::floor(y), iz = (int)std::floor(z);
+                    const T wzl = z - iz;
+                    if (ix > 0 && ix < nx - 1 && iy > 0 && iy < ny - 1) {
+                        const T wxl = x - ix, wyl = y - iy;
+                        const T* p = img + iz * nxy + (size_t)iy * nx + ix;
+                        temp = temp +
+                               (1 - wzl) * ((1 - wyl) * ((1 - wxl) * p[0] + wxl * p[1]) +
+                                            wyl * ((1 - wxl) * p[nx] + wxl * p[nx + 1])) +
+                               wzl * ((1 - wyl) * ((1 - wxl) * p[nxy] + wxl * p[nxy + 1]) +
+                                      wyl * ((1 - wxl) * p[nxy + nx] + wxl * p[nxy + nx + 1]));
+                    }
+                }
+                out[iv] = temp * delta * std::sqrt(1 + rdz * rdz) / 10;
+            }
+        }
+    }
+}
+
 // Counts of the work a 3D DD/SF operator performs, independent of image values:
 // U = active (voxel, view) pairs inside the iz window with a non-empty v footprint,
 // C = (voxel, detector cell) weight pairs, K = active (column, view) pairs.
@@ -638,6 +768,7 @@ int fbct_dispatch(const char* op, const Geom2& g, const double* betas, const voi
     const T* src = static_cast<const T*>(in);
     T* dst = static_cast<T*>(out);
     if (!std::strcmp(op, "proj,dd")) fbct_proj_dd<T>(g, betas, src, dst, nthreads);
+    else if (!std::strcmp(op, "proj,rd")) fbct_proj_rd<T>(g, betas, src, dst, nthreads);
     else if (!std::strcmp(op, "proj,tf")) fbct_proj_tf<T>(g, betas, src, dst, nthreads);
     else if (!std::strcmp(op, "back,dd")) fbct_back_dd<T>(g, betas, dst, src, nthreads);
     else if (!std::strcmp(op, "back,tf")) fbct_back_tf<T>(g, betas, dst, src, nthreads);
@@ -651,7 +782,8 @@ int cbct_dispatch(const char* op, const Geom3& g, const double* betas, const flo
                   const uint8_t* map, void* out, int nthreads) {
     const T* src = static_cast<const T*>(in);
     T* dst = static_cast<T*>(out);
-    if (!std::strcmp(op, "proj,dd")) cbct_proj<T>(FOOT_DD, g, betas, couchz, src, map, dst, nthreads);
+    if (!std::strcmp(op, "proj,rd")) cbct_proj_rd<T>(g, betas, couchz, src, dst, nthreads);  // no map
+    else if (!std::strcmp(op, "proj,dd")) cbct_proj<T>(FOOT_DD, g, betas, couchz, src, map, dst, nthreads);
     else if (!std::strcmp(op, "proj,tf")) cbct_proj<T>(FOOT_TF, g, betas, couchz, src, map, dst, nthreads);
     else if (!std::strcmp(op, "back,dd")) cbct_back<T>(FOOT_DD, g, betas, couchz, dst, map, src, nthreads);
     else if (!std::strcmp(op, "back,tf")) cbct_back<T>(FOOT_TF, g, betas, couchz, dst, map, src, nthreads);

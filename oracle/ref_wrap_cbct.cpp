@@ -14,7 +14,8 @@ extern "C" int ref_cbct(const char* op, const int* n, const float* d, const floa
 #define GEOM n[0], n[1], n[2], d[0], d[1], d[2], o[0], o[1], o[2], nuv[0], nuv[1], duv[0], duv[1], ouv[0], \
              ouv[1], sad, add, sdd, nviews, betas, couchz, dettype
     bool ok = false;
-    if (!std::strcmp(op, "proj,dd")) ok = m ? cbct_proj1_dd(GEOM, in, out, m) : cbct_proj_dd(GEOM, in, out);
+    if (!std::strcmp(op, "proj,rd")) ok = m ? false : cbct_proj_rd(GEOM, in, out);  // no map form (cbct_geom_mex.c:169)
+    else if (!std::strcmp(op, "proj,dd")) ok = m ? cbct_proj1_dd(GEOM, in, out, m) : cbct_proj_dd(GEOM, in, out);
     else if (!std::strcmp(op, "proj,tf")) ok = m ? cbct_proj1_tf(GEOM, in, out, m) : cbct_proj_tf(GEOM, in, out);
     else if (!std::strcmp(op, "back,dd")) ok = m ? cbct_back1_dd(GEOM, out, in, m) : cbct_back_dd(GEOM, out, in);
     else if (!std::strcmp(op, "back,tf")) ok = m ? cbct_back1_tf(GEOM, out, in, m) : cbct_back_tf(GEOM, out, in);

@@ -10,6 +10,9 @@
 extern "C" int ref_fbct(const char* op, const int* nxy, const float* dxy, const float* oxy, int nu,
                         float du, float ou, float sad, float add, float sdd, int nviews, double* betas,
                         int dettype, float* in, float* out) {
+    if (!std::strcmp(op, "proj,rd"))
+        return !fbct_proj_rd(nxy[0], nxy[1], dxy[0], dxy[1], oxy[0], oxy[1], nu, du, ou, sad, add, sdd,
+                             nviews, betas, dettype, in, out);
     if (!std::strcmp(op, "proj,dd"))
         return !fbct_proj_dd(nxy[0], nxy[1], dxy[0], dxy[1], oxy[0], oxy[1], nu, du, ou, sad, add, sdd,
                              nviews, betas, dettype, in, out);

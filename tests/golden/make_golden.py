@@ -41,7 +41,7 @@ def cases2():
         rec = dict(nxy=g.nxy, dxy=g.dxy, oxy=g.oxy, nu=np.int32(nu), du=np.float32(du), ou=np.float32(ou),
                    sad=np.float32(sad), add=np.float32(0), sdd=np.float32(sdd), betas=betas, dettype=np.int32(dt),
                    img=img, sino=sino)
-        for op in ("proj,dd", "proj,tf"):
+        for op in ("proj,rd", "proj,dd", "proj,tf"):
             rec[op.replace(",", "_")] = O.ref_fbct(op, g, img)
         for op in ("back,dd", "back,tf", "back,pd"):
             rec[op.replace(",", "_")] = O.ref_fbct(op, g, sino)
@@ -80,6 +80,8 @@ def cases3():
             rec[op.replace(",", "_")] = O.ref_cbct(op, g, vol, mp)
         for op in ("back,dd", "back,tf", "back,pd", "back,hpd"):
             rec[op.replace(",", "_")] = O.ref_cbct(op, g, sino, mp)
+        # the ray-driven projector exists only in the 15-argument (no map) family
+        rec["proj_rd"] = O.ref_cbct("proj,rd", g, vol, None)
         out.append((name, rec))
     return out
 

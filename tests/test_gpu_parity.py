@@ -444,3 +444,69 @@ def test_batched_schedules_and_autotune_vs_oracle():
     B.proj(d_img16, d_sino16, "proj,dd")
     for s, g, im in zip(B.unpack_sinos(d_sino16.to_numpy()), gs, imgs):
         assert rel_l2(s, O.fbct("proj,dd", g, im)) <= 2e-3
+
+
+# ---------------------------------------------------------------------------------
+# ray-driven projection ("proj,rd"): the reference's float march, one for one
+# ---------------------------------------------------------------------------------
+RD_TOL = 1e-6  # the march reproduces the reference's float operations; only the device's
+               # libm-free arithmetic order could differ
+
+
+@pytest.mark.parametrize("path", golden_files("fan_") + golden_files("cone_"), ids=os.path.basename)
+def test_proj_rd_gateways_vs_reference_golden(path):
+    from ctsim_amd import mex
+    rec = load_golden(path)
+    say("golden rd", os.path.basename(path))
+    if "nxy" in rec:
+        got = mex.fbct_geom_mex("proj,rd", rec["nxy"], rec["dxy"], rec["oxy"], rec["nu"], rec["du"], rec["ou"],
+                                rec["sad"], rec["add"], rec["sdd"], len(rec["betas"]), rec["betas"], rec["dettype"],
+                                np.asfortranarray(rec["img"]))
+    else:
+        got = mex.cbct_geom_mex("proj,rd", rec["n"], rec["d"], rec["o"], rec["nuv"], rec["duv"], rec["ouv"],
+                                rec["sad"], rec["add"], rec["sdd"], len(rec["betas"]), rec["betas"], rec["couchz"],
+                                rec["dettype"], np.asfortranarray(rec["img"]))
+    want = rec["proj_rd"]
+    assert got.shape == want.shape
+    print("rd rel L2", rel_l2(got, want), "bit-equal", np.array_equal(got, want))
+    assert rel_l2(got, want) <= RD_TOL, rel_l2(got, want)
+
+
+def test_proj_rd_with_map_fails_like_the_reference():
+    from ctsim_amd import mex
+    from ctsim_amd._lib import CtsimError
+    rec = load_golden(golden_files("cone_flat_map")[0])
+    with pytest.raises((CtsimError, ValueError), match="cbct_geom_mex"):
+        mex.cbct_geom_mex("proj,rd", rec["n"], rec["d"], rec["o"], rec["nuv"], rec["duv"], rec["ouv"], rec["sad"],
+                          rec["add"], rec["sdd"], len(rec["betas"]), rec["betas"], rec["couchz"], rec["dettype"],
+                          np.asfortranarray(rec["img"]), rec["map"])
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_proj_rd_random_geometries_vs_oracle(seed):
+    """per-operator entry points (2D and 3D, both detector types, offsets, couch z) against
+    the FP32 oracle, which equals the compiled reference bit for bit"""
+    from oracle import oracle as O
+    from ctsim_amd._lib import CtsimGeom
+    rng = np.random.default_rng(100 + seed)
+    nx, ny, nz = (int(v) for v in rng.integers(10, 30, 3))
+    nu, nv, nviews = int(rng.integers(16, 48)), int(rng.integers(40, 80)), 6
+    dettype = seed % 2
+    betas = rng.uniform(0, 2 * np.pi, nviews)
+    cz = rng.uniform(-2, 2, nviews).astype(np.float32)
+    d = [float(v) for v in rng.uniform(0.6, 1.3, 3)]
+    o = [float(v) for v in rng.uniform(-1, 1, 3)]
+    duv = [float(v) for v in rng.uniform(0.6, 1.4, 2)]
+    ouv = [float(v) for v in rng.uniform(-1, 1, 2)]
+    g3 = O.G3([nx, ny, nz], d, o, [nu, nv], duv, ouv, 600.0, 0, 1000.0, betas, couchz=cz, dettype=dettype)
+    vol = rng.uniform(0, 1, (nx, ny, nz)).astype(np.float32)
+    want = O.cbct("proj,rd", g3, vol)
+    g = CtsimGeom(nx, ny, nz, *d, *o, nu, nv, *duv, *ouv, 600.0, 0.0, 1000.0, dettype)
+    got = _per_op("ctsim_cbct_proj_rd", g, betas, cz, None, vol, (nv, nu, nviews))
+    say("rd 3D seed", seed, rel_l2(got, want), np.array_equal(got, want))
+    assert rel_l2(got, want) <= RD_TOL
+    g2 = O.G2(nx, ny, d[0], d[1], o[0], o[1], nu, duv[0], ouv[0], 600.0, 0, 1000.0, betas, dettype)
+    img = vol[:, :, 0].copy()
+    want2 = O.fbct("proj,rd", g2, img)
+    got2 = _per_op("ctsim_fbct_proj_rd", g, betas, None, None, img, (nu, nviews))
+    assert rel_l2(got2, want2) <= RD_TOL

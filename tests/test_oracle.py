@@ -17,6 +17,9 @@ from helpers import g2_from, g3_from
 
 OPS2 = ("proj,dd", "proj,tf", "back,dd", "back,tf", "back,pd")
 OPS3 = ("proj,dd", "proj,tf", "back,dd", "back,tf", "back,pd", "back,hpd")
+# The ray-driven projector ("proj,rd") is checked separately: its sample positions come
+# from float accumulation of the ray parameter, so its FP64 form samples elsewhere and is
+# no accuracy reference; it is pinned to the compiled reference bit for bit.
 
 
 @pytest.mark.parametrize("path", golden_files("fan_"), ids=os.path.basename)
@@ -28,6 +31,8 @@ def test_fbct_oracle_matches_reference_golden(oracle_mod, path):
         got = oracle_mod.fbct(op, g, x)
         want = rec[op.replace(",", "_")]
         assert np.array_equal(got, want), (op, np.abs(got - want).max())
+    got = oracle_mod.fbct("proj,rd", g, rec["img"])
+    assert np.array_equal(got, rec["proj_rd"]), np.abs(got - rec["proj_rd"]).max()
 
 
 @pytest.mark.parametrize("path", golden_files("cone_"), ids=os.path.basename)
@@ -40,6 +45,8 @@ def test_cbct_oracle_matches_reference_golden(oracle_mod, path):
         got = oracle_mod.cbct(op, g, x, mp)
         want = rec[op.replace(",", "_")]
         assert np.array_equal(got, want), (op, np.abs(got - want).max())
+    got = oracle_mod.cbct("proj,rd", g, rec["img"], None)
+    assert np.array_equal(got, rec["proj_rd"]), np.abs(got - rec["proj_rd"]).max()
 
 
 @pytest.mark.skipif(not os.path.isdir("/root/reference/geom"), reason="reference tree not present")
@@ -55,7 +62,7 @@ def test_oracle_matches_live_reference_random(oracle_mod):
         g2 = oracle_mod.G2(nx, ny, 0.9, 0.7, 0.3, -0.2, nu, du, 0.7, 600, 0, 950, betas, dettype)
         img = rng.uniform(size=(nx, ny)).astype(np.float32)
         sino = rng.uniform(size=(nu, nviews)).astype(np.float32)
-        for op in OPS2:
+        for op in OPS2 + ("proj,rd",):
             x = img if op.startswith("proj") else sino
             assert np.array_equal(oracle_mod.fbct(op, g2, x), oracle_mod.ref_fbct(op, g2, x)), op
         g3 = oracle_mod.G3([nx, ny, nz], [0.9, 0.7, 1.1], [0.3, -0.2, 0.4], [nu, nv], [du, 0.8], [0.7, -0.6], 600, 0,
@@ -66,6 +73,7 @@ def test_oracle_matches_live_reference_random(oracle_mod):
         for op in OPS3:
             x = vol if op.startswith("proj") else s3
             assert np.array_equal(oracle_mod.cbct(op, g3, x, mp), oracle_mod.ref_cbct(op, g3, x, mp)), op
+        assert np.array_equal(oracle_mod.cbct("proj,rd", g3, vol), oracle_mod.ref_cbct("proj,rd", g3, vol))
 
 
 @pytest.mark.parametrize("path", golden_files("fan_") + golden_files("cone_"), ids=os.path.basename)
