@@ -114,6 +114,7 @@ SIGNATURES = {
     "ctsim_sqdiff_sum_f64": (C.c_int, [vp, vp, sz, vp, vp]),
     "ctsim_fdk_create": (C.c_int, [C.POINTER(vp), C.POINTER(CtsimGeom), i32, P_f64, i32, P_f32, i32]),
     "ctsim_fdk_destroy": (C.c_int, [vp]),
+    "ctsim_fbp2d_create": (C.c_int, [C.POINTER(vp), C.POINTER(CtsimGeom), i32, P_f64, i32, P_f32, i32]),
     "ctsim_fdk_filter": (C.c_int, [vp, i32, i32, vp, i32, vp, i32, vp]),
     "ctsim_poisson_counts": (C.c_int, [vp, i32, sz, f64, C.c_uint64, vp, vp]),
     "ctsim_log_attenuation": (C.c_int, [vp, sz, f64, f64, vp, i32, vp]),

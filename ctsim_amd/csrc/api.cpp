@@ -1183,6 +1183,30 @@ int ctsim_fdk_create(ctsim_fdk** out, const ctsim_geom* g, int32_t nviews, const
     return CTSIM_OK;
 }
 
+// the same plan for a fan-beam sinogram [nviews][nu] (one detector row): the cosine weight
+// of reconFBP2d.m:32 (flat) / :38 (arc)
+int ctsim_fbp2d_create(ctsim_fdk** out, const ctsim_geom* g, int32_t nviews, const double* H, int32_t order,
+                       const float* view_weights, int32_t compute) {
+    if (!out || !g || !H) return fail(CTSIM_EINVAL, "Error: null FBP argument.");
+    *out = nullptr;
+    if (g->nu <= 0 || nviews <= 0) return fail(CTSIM_EINVAL, "Error: detector sizes must be positive.");
+    if (order < g->nu || (order & 1)) return fail(CTSIM_EINVAL, "Error: filter length must be even and >= nu.");
+    if (compute != CTSIM_F32 && compute != CTSIM_F64)
+        return fail(CTSIM_EINVAL, "Error: compute precision must be CTSIM_F32 or CTSIM_F64.");
+    FdkGeom fg{g->du, 1.f, g->ou, 0.f, g->sad, g->sdd, g->detector_type == 1 ? 2 : 0};
+    ctsim_fdk* f = new (std::nothrow) ctsim_fdk;
+    if (!f) return fail(CTSIM_ENOMEM, "Error: host allocation failed.");
+    f->nviews = nviews;
+    f->f64 = compute == CTSIM_F64;
+    hipError_t e = fdk_plan_create(&f->impl, fg, g->nu, 1, nviews, H, order, view_weights, f->f64, size_t(1) << 30);
+    if (e != hipSuccess) {
+        delete f;
+        return hip_fail(e, "FBP plan (device buffers or hipFFT plans)");
+    }
+    *out = f;
+    return CTSIM_OK;
+}
+
 int ctsim_fdk_destroy(ctsim_fdk* f) {
     if (!f) return CTSIM_OK;
     fdk_plan_destroy(f->impl);

@@ -19,9 +19,12 @@ namespace ctsim {
 
 constexpr int kT = 64;  // transpose tile (u x v)
 
-// cosine weight of detector cell (u, v) in mm (reconFDK.m:39 flat, :44 arc)
+// cosine weight of detector cell (u, v) in mm: flat 1 = reconFDK.m:39, arc (flat 0) =
+// reconFDK.m:44 (at v = 0 also the fan-beam arc weight of reconFBP2d.m:38), flat 2 = the
+// fan-beam flat-panel weight of reconFBP2d.m:32, SAD / sqrt(u^2 + SDD^2)
 template <class C>
 __device__ __forceinline__ C fdk_weight(C u, C v, C sad, C sdd, int flat) {
+    if (flat == 2) return sad / sqrt(u * u + sdd * sdd);
     if (flat) return sad * sqrt(C(1) + u * u / (sdd * sdd)) / sqrt(u * u + v * v + sdd * sdd);
     return sad / sdd * cos(u / sqrt(sdd * sdd + v * v));
 }

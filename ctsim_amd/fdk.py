@@ -82,8 +82,21 @@ def parker_weight(gammas: np.ndarray, beta: float, tau: float) -> np.ndarray:
     return out
 
 
+def sliver_weight(gammas: np.ndarray, beta: float, tau: float) -> np.ndarray:
+    """shortScanSliverWeight.m:8-21"""
+    out = np.zeros(gammas.size)
+    for i, g in enumerate(gammas):
+        if 0 <= beta < tau + 2 * g:
+            out[i] = math.sin(math.pi / 4 * beta / (tau / 2 + g)) ** 2
+        elif tau + 2 * g <= beta < math.pi + 2 * g:
+            out[i] = 1.0
+        elif math.pi + 2 * g <= beta < math.pi + tau:
+            out[i] = math.sin(math.pi / 4 * (math.pi + tau - beta) / (tau / 2 - g)) ** 2
+    return out
+
+
 def fdk_scale(geom: Geometry) -> float:
-    """dbeta * 10 of reconFDK.m:78-84"""
+    """dbeta * 10 of reconFDK.m:78-84 (and reconFBP2d.m:60-66)"""
     dbeta = abs(geom.betas[-1] - geom.betas[0]) / (geom.noViews - 1)
     if not geom.shortScan:
         dbeta = dbeta / 2
@@ -91,18 +104,26 @@ def fdk_scale(geom: Geometry) -> float:
 
 
 def fdk_filter_response(geom: Geometry, window: str = "hamming", crop: float = 1.0) -> np.ndarray:
-    """H of reconFDK.m:38-46: du * designFilter2 (flat) or du * designEquiangularFilter2 (arc)."""
-    nu = geom.detSize[0]
-    du = geom.detSpacing[0]
+    """H of reconFDK.m:38-46 / reconFBP2d.m:31-40: du * designFilter2 (flat) or
+    du * designEquiangularFilter2 (arc)."""
+    nu = _det(geom.detSize)[0]
+    du = _det(geom.detSpacing)[0]
     return du * design_filter2(du, window, nu, crop, sdd=None if geom.flatPanel else geom.SDD)
 
 
+def _det(x):
+    """detector size / spacing / offset as a sequence (a 2D geometry may hold scalars)"""
+    return tuple(np.atleast_1d(x))
+
+
 def fdk_view_weights(geom: Geometry) -> np.ndarray | None:
-    """short-scan weights [nviews][nu] (reconFDK.m:40-41,45-46,55-59), or None for full scans"""
+    """short-scan weights [nviews][nu], or None for full scans: the Parker weight of
+    reconFDK.m:40-41,45-46,55-59 for a cone beam, the sliver weight of reconFBP2d.m:33-40,47-51
+    for a fan beam"""
     if not geom.shortScan:
         return None
-    nu = geom.detSize[0]
-    du, ou = geom.detSpacing[0], geom.detOffset[0]
+    nu = _det(geom.detSize)[0]
+    du, ou = _det(geom.detSpacing)[0], _det(geom.detOffset)[0]
     u = (np.arange(nu) - (nu - 1) / 2 + ou) * du
     if geom.flatPanel:
         gamma = np.arctan(-u / geom.SDD)
@@ -111,15 +132,17 @@ def fdk_view_weights(geom: Geometry) -> np.ndarray | None:
         gamma = -u / geom.SDD
         tau = abs(geom.betas.max() - geom.betas.min()) - math.pi
     bmin = geom.betas.min()
+    if geom.ndim == 2:
+        tau = abs(geom.betas[-1] - geom.betas[0]) - math.pi
+        return np.stack([sliver_weight(gamma, b - bmin, tau) for b in geom.betas]).astype(np.float32)
     return np.stack([parker_weight(gamma, b - bmin, tau) for b in geom.betas]).astype(np.float32)
 
 
 class FdkFilter:
-    """Device FDK pre-processing for one geometry (ctsim_fdk_create / ctsim_fdk_filter)."""
+    """Device FDK / fan-beam FBP pre-processing for one geometry (ctsim_fdk_create or, for a
+    2D geometry, ctsim_fbp2d_create; ctsim_fdk_filter)."""
 
     def __init__(self, geom: Geometry, window: str = "hamming", crop: float = 1.0, compute: str = "f32"):
-        if geom.ndim != 3:
-            raise ValueError("FDK needs a cone-beam geometry")
         self.geom = geom
         self.compute = dtype_code(compute)
         H = np.ascontiguousarray(fdk_filter_response(geom, window, crop), np.float64)
@@ -127,14 +150,16 @@ class FdkFilter:
         self._vw = None if vw is None else np.ascontiguousarray(vw)
         g = geom.to_c()
         p = C.c_void_p()
-        call("ctsim_fdk_create", C.byref(p), C.byref(g), geom.noViews, H.ctypes.data_as(C.POINTER(C.c_double)),
-             H.size, None if self._vw is None else self._vw.ctypes.data_as(C.POINTER(C.c_float)), self.compute)
+        call("ctsim_fbp2d_create" if geom.ndim == 2 else "ctsim_fdk_create", C.byref(p), C.byref(g), geom.noViews,
+             H.ctypes.data_as(C.POINTER(C.c_double)), H.size,
+             None if self._vw is None else self._vw.ctypes.data_as(C.POINTER(C.c_float)), self.compute)
         self.handle = p.value
 
     def __call__(self, d_sino: DeviceArray, d_out: DeviceArray, view0: int = 0, nsub: int | None = None,
                  stream=None) -> DeviceArray:
         nsub = self.geom.noViews if nsub is None else nsub
-        nu, nv = self.geom.detSize
+        det = _det(self.geom.detSize)
+        nu, nv = det[0], (det[1] if len(det) > 1 else 1)
         need = nsub * nu * nv
         if d_sino.count < need or d_out.count < need:
             raise ValueError(f"FDK buffers need {need} elements")
@@ -167,3 +192,26 @@ def recon_fdk(sino, geom: Geometry, window: str = "hamming", crop: float = 1.0, 
     d_v = DeviceArray(P.n_img, F64 if f64 else F32)
     P.back(d_f, d_v, "back,hpd", scale=fdk_scale(geom))
     return P.download_volume(d_v)
+
+
+def recon_fbp2d(sino, geom: Geometry, window: str = "hamming", crop: float = 1.0, storage: str = "f32",
+                projector: Projector | None = None, compute: str = "f32") -> np.ndarray:
+    """reconFBP2d (fbp/reconFBP2d.m:1-70): fan-beam log sinogram [nu nviews] -> image [nx ny]
+    (1/cm): cosine (and short-scan sliver) weight and ramp filter on the device, back,pd
+    times dbeta * 10, zero outside geom.map."""
+    if geom.ndim != 2:
+        raise ValueError("reconFBP2d needs a fan-beam geometry")
+    f64 = dtype_code(compute) == F64
+    P = projector or Projector(geom, compute="f64" if f64 else "f32")
+    if isinstance(sino, DeviceArray):
+        d_in = sino
+    else:
+        d_in = P.upload_sino(np.asarray(sino, np.float64 if f64 else np.float32), dtype=F64 if f64 else F32)
+    filt = FdkFilter(geom, window, crop, compute="f64" if f64 else "f32")
+    d_f = filt(d_in, DeviceArray(d_in.count, storage))
+    d_v = DeviceArray(P.n_img, F64 if f64 else F32)
+    P.back(d_f, d_v, "back,pd", scale=fdk_scale(geom))
+    img = P.download_volume(d_v)
+    if geom.map is not None:
+        img = np.where(np.asarray(geom.map, bool), img, 0).astype(img.dtype)
+    return img

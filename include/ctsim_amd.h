@@ -395,6 +395,13 @@ int ctsim_sub(const float* d_a, const float* d_b, void* d_out, int32_t out_dtype
 typedef struct ctsim_fdk ctsim_fdk;
 int ctsim_fdk_create(ctsim_fdk** fdk, const ctsim_geom* g, int32_t nviews, const double* H, int32_t order,
                      const float* view_weights, int32_t compute);
+/* The same pre-processing for a fan-beam sinogram [nu nviews] (fbp/reconFBP2d.m:31-55):
+ * cosine weight SAD / sqrt(u^2 + SDD^2) (flat, :32) or SAD/SDD cos(u/SDD) (arc, :38), the
+ * optional short-scan weight view_weights[view][u] (shortScanSliverWeight.m), filterFreq.
+ * Filter with ctsim_fdk_filter (the sinogram is [view][u] with one detector row);
+ * g->nv, dv, ov, nz... are ignored. */
+int ctsim_fbp2d_create(ctsim_fdk** fdk, const ctsim_geom* g, int32_t nviews, const double* H, int32_t order,
+                       const float* view_weights, int32_t compute);
 int ctsim_fdk_destroy(ctsim_fdk* fdk);
 /* views view0 .. view0+nsub-1: d_sino and d_out are device [nsub][nu][nv] (v fastest,
  * MATLAB's [nv nu nviews] order); d_out in out_dtype (F32, F16, BF16; F64 for F64 plans).

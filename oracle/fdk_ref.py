@@ -1,7 +1,7 @@
 """FP64 numpy restatement of CTSim's FDK pre-processing, the checker of the device FDK
 filter (test infrastructure; the product filters on the GPU).
 
-fdk_filter follows fbp/reconFDK.m:19-76 view by view: the cosine weight (flat :39, arc
+fdk_filter follows fbp/reconFDK.m:19-76 (and fbp2d_filter fbp/reconFBP2d.m:26-55) view by view: the cosine weight (flat :39, arc
 :44), the short-scan weight (:55-59), and filterFreq.m:42-51 per detector row
 (zero pad to length(H), multiply the FFT by H, real part of the inverse FFT, truncate)."""
 from __future__ import annotations
@@ -39,4 +39,22 @@ def fdk_filter(sino: np.ndarray, geom, H: np.ndarray, view_weights=None) -> np.n
         if view_weights is not None:
             proj = proj * np.asarray(view_weights[view], np.float64)[None, :]
         out[:, :, view] = filter_rows(proj, H)
+    return out
+
+
+def fbp2d_filter(sino: np.ndarray, geom, H: np.ndarray, view_weights=None) -> np.ndarray:
+    """fbp/reconFBP2d.m:26-55: sino MATLAB [nu nviews] -> cosine (SAD / sqrt(u^2 + SDD^2) flat,
+    SAD/SDD cos(u/SDD) arc) and short-scan weighted, filterFreq per view, float64"""
+    nu = int(np.atleast_1d(geom.detSize)[0])
+    du = float(np.atleast_1d(geom.detSpacing)[0])
+    ou = float(np.atleast_1d(geom.detOffset)[0])
+    SDD, SAD = float(geom.SDD), float(geom.SAD)
+    u = (np.arange(nu) - (nu - 1) / 2 + ou) * du
+    weight = SAD / np.sqrt(u ** 2 + SDD ** 2) if geom.flatPanel else SAD / SDD * np.cos(u / SDD)
+    out = np.empty(sino.shape, np.float64, order="F")
+    for view in range(sino.shape[1]):
+        proj = weight * np.asarray(sino[:, view], np.float64)
+        if view_weights is not None:
+            proj = proj * np.asarray(view_weights[view], np.float64)
+        out[:, view] = filter_rows(proj, H)
     return out

@@ -101,3 +101,17 @@ def test_equiangular_kernel_tends_to_the_flat_one():
     np.testing.assert_allclose(far, flat, rtol=1e-8, atol=1e-12)
     near = fdk.design_filter2(g, "hann", n, sdd=300.0)
     assert np.max(np.abs(near - flat)) > 1e-4 * np.max(np.abs(flat))
+
+
+def test_sliver_weights_of_conjugate_rays_sum_to_one():
+    """shortScanSliverWeight: the ray (beta, gamma) and its conjugate (beta + pi - 2 gamma,
+    -gamma) carry weights that add to one over the short scan"""
+    from ctsim_amd.fdk import sliver_weight
+    rng = np.random.default_rng(3)
+    tau = 0.4
+    for _ in range(200):
+        g = rng.uniform(-0.15, 0.15)
+        beta = rng.uniform(0, math.pi + 2 * g)
+        w1 = sliver_weight(np.array([g]), beta, tau)[0]
+        w2 = sliver_weight(np.array([-g]), beta + math.pi - 2 * g, tau)[0]
+        assert w1 + w2 == pytest.approx(1.0, abs=1e-12), (g, beta, w1, w2)
