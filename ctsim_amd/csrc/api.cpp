@@ -1257,6 +1257,20 @@ int ctsim_log_attenuation(const float* d_counts, size_t n, double i0, double low
     return CTSIM_OK;
 }
 
+int ctsim_log_attenuation_ex(const float* d_counts, size_t n, int32_t ncell, double source_intensity,
+                             double low_count, int32_t compound, const float* d_flat_field,
+                             const float* d_tube_profile, void* d_y, int32_t y_dtype, void* stream) {
+    if ((!d_counts || !d_y) && n) return fail(CTSIM_EINVAL, "Error: null argument.");
+    if (!(source_intensity > 0) || !(low_count > 0)) return fail(CTSIM_EINVAL, "Error: intensities must be positive.");
+    if (ncell <= 0 || n % (size_t)ncell) return fail(CTSIM_EINVAL, "Error: n must be a multiple of the cells per view.");
+    if (n == 0) return CTSIM_OK;
+    hipError_t e = neglog_ex_launch(d_counts, n, ncell, source_intensity, low_count, compound != 0, d_flat_field,
+                                    d_tube_profile, d_y, y_dtype, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EINVAL, "Error: bad dtype.");
+    HIP_TRY(e, "log attenuation");
+    return CTSIM_OK;
+}
+
 int ctsim_pwls_weights(const float* d_counts, size_t n, double r, double v, void* d_w, int32_t w_dtype, void* stream) {
     if ((!d_counts || !d_w) && n) return fail(CTSIM_EINVAL, "Error: null argument.");
     if (n == 0) return CTSIM_OK;

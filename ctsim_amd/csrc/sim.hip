@@ -83,6 +83,33 @@ __global__ void neglog_kernel(const float* __restrict__ counts, size_t n, double
     }
 }
 
+// General form of the raw-data log (sinogram/log/processCTRawData.m:14-33 with
+// computeSimplePoissonSinogramAttunation.m:14-52 or computeCompoundPoissonSinogramAttunation.m
+// :14-38): element i of view i / ncell and detector cell i % ncell,
+//   c = counts / profile[view]                        (tube-current profile, optional)
+//   simple:   y = -log((max(c, low) / i0) / ff[cell])  (flat-field ratio of the bowtie, optional)
+//   compound: y = -log(max((c / i0) / ff[cell], low / i0))
+template <class Y>
+__global__ void neglog_ex_kernel(const float* __restrict__ counts, size_t n, int ncell, double i0, double low,
+                                 int compound, const float* __restrict__ ff, const float* __restrict__ profile,
+                                 Y* __restrict__ y) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double c = (double)counts[i];
+        if (profile) c = c / (double)profile[i / (size_t)ncell];
+        double r;
+        if (compound) {
+            r = c / i0;
+            if (ff) r = r / (double)ff[i % (size_t)ncell];
+            const double rl = low / i0;
+            r = r > rl ? r : rl;
+        } else {
+            r = (c > low ? c : low) / i0;
+            if (ff) r = r / (double)ff[i % (size_t)ncell];
+        }
+        y[i] = cvt_store<Y>(-log(r));
+    }
+}
+
 template <class W>
 __global__ void weights_kernel(const float* __restrict__ s, size_t n, double r, double v, W* __restrict__ w) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -115,6 +142,19 @@ hipError_t neglog_launch(const float* counts, size_t n, double i0, double low, v
         case DT_F16: hipLaunchKernelGGL(neglog_kernel<__half>, dim3(grid_sim(n)), dim3(256), 0, st, counts, n, i0, low, (__half*)y); break;
         case DT_BF16: hipLaunchKernelGGL(neglog_kernel<__hip_bfloat16>, dim3(grid_sim(n)), dim3(256), 0, st, counts, n, i0, low, (__hip_bfloat16*)y); break;
         case DT_F64: hipLaunchKernelGGL(neglog_kernel<double>, dim3(grid_sim(n)), dim3(256), 0, st, counts, n, i0, low, (double*)y); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t neglog_ex_launch(const float* counts, size_t n, int ncell, double i0, double low, int compound,
+                            const float* ff, const float* profile, void* y, int ydt, hipStream_t st) {
+    const dim3 g(grid_sim(n)), b(256);
+    switch (ydt) {
+        case DT_F32: hipLaunchKernelGGL(neglog_ex_kernel<float>, g, b, 0, st, counts, n, ncell, i0, low, compound, ff, profile, (float*)y); break;
+        case DT_F16: hipLaunchKernelGGL(neglog_ex_kernel<__half>, g, b, 0, st, counts, n, ncell, i0, low, compound, ff, profile, (__half*)y); break;
+        case DT_BF16: hipLaunchKernelGGL(neglog_ex_kernel<__hip_bfloat16>, g, b, 0, st, counts, n, ncell, i0, low, compound, ff, profile, (__hip_bfloat16*)y); break;
+        case DT_F64: hipLaunchKernelGGL(neglog_ex_kernel<double>, g, b, 0, st, counts, n, ncell, i0, low, compound, ff, profile, (double*)y); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -422,6 +422,17 @@ int ctsim_poisson_counts(const void* d_p, int32_t p_dtype, size_t n, double i0, 
  * monochromatic source, low_count = dummyLowPhotonCount = 10 there). */
 int ctsim_log_attenuation(const float* d_counts, size_t n, double i0, double low_count, void* d_y, int32_t y_dtype,
                           void* stream);
+/* The raw-data log of sinogram/log/processCTRawData.m:14-33 in full.  Element i belongs to
+ * view i / ncell and detector cell i % ncell (ncell = nu * nv); counts are first divided by
+ * d_tube_profile[view] when given.  compound = 0 (computeSimplePoissonSinogramAttunation.m):
+ * y = -log((max(c, low_count) / source_intensity) / d_flat_field[cell]); compound = 1
+ * (computeCompoundPoissonSinogramAttunation.m): y = -log(max((c / source_intensity) /
+ * d_flat_field[cell], low_count / source_intensity)).  d_flat_field (the bowtie's
+ * flatFieldRatio, [nv nu]) and d_tube_profile ([nviews]) may be NULL.  source_intensity is
+ * the spectrum's expected unattenuated count (ctsim_amd.sim.source_intensity). */
+int ctsim_log_attenuation_ex(const float* d_counts, size_t n, int32_t ncell, double source_intensity,
+                             double low_count, int32_t compound, const float* d_flat_field,
+                             const float* d_tube_profile, void* d_y, int32_t y_dtype, void* stream);
 /* w = (s - r)^2 / (s + v) in single precision, 0 where s <= r (statsrecon/pwls/computeWeightsPwls.m:8-9). */
 int ctsim_pwls_weights(const float* d_counts, size_t n, double r, double v, void* d_w, int32_t w_dtype, void* stream);
 

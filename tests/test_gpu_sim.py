@@ -69,3 +69,26 @@ def test_log_attenuation_and_weights_match_reference_formulas():
     ww = ((s - np.float32(2.0)) ** 2 / (s + np.float32(5.0))).astype(np.float32)
     ww[s <= 2.0] = 0
     np.testing.assert_allclose(w.to_numpy(), ww, rtol=2e-7)
+
+
+@pytest.mark.parametrize("compound", [False, True])
+@pytest.mark.parametrize("with_ff,with_prof", [(False, False), (True, False), (False, True), (True, True)])
+def test_log_attenuation_ex_matches_the_raw_data_restatement(compound, with_ff, with_prof):
+    from oracle import sim_ref
+    from ctsim_amd import sim
+    from ctsim_amd.device import DeviceArray
+    rng = np.random.default_rng(4)
+    nv, nc, i0 = 37, 211, 3e4
+    prof = rng.uniform(0.5, 2.0, nv).astype(np.float32)
+    ff = rng.uniform(0.2, 1.0, nc).astype(np.float32)
+    raw = rng.poisson(i0 * 0.05, (nv, nc)).astype(np.float32)
+    raw[:, :20] = rng.integers(0, 150, (nv, 20)).astype(np.float32)  # the clamp region
+    low = 100.0 if compound else 10.0
+    d_raw = DeviceArray.from_numpy(raw, "f32")
+    d_ff = DeviceArray.from_numpy(ff, "f32") if with_ff else None
+    d_prof = DeviceArray.from_numpy(prof, "f32") if with_prof else None
+    want = sim_ref.process_raw(raw, i0, prof if with_prof else None, ff if with_ff else None, compound, low)
+    y = sim.log_attenuation(d_raw, nc, i0, compound, None, d_ff, d_prof, "f64").to_numpy()
+    np.testing.assert_allclose(y, want, rtol=1e-14, atol=1e-14)
+    y32 = sim.log_attenuation(d_raw, nc, i0, compound, None, d_ff, d_prof, "f32").to_numpy()
+    np.testing.assert_allclose(y32, want.astype(np.float32), rtol=2e-7, atol=1e-7)
