@@ -28,20 +28,28 @@ enum Foot { FOOT_DD = 0, FOOT_TF = 1 };
 //
 // The wave enumerates the image columns whose footprint can reach its tile (the strip,
 // found by inverse-mapping the tile edges line by line), evaluates 64 candidate
-// footprints at a time (one per lane, including the tile's u weights) and parks the
-// survivors' parameters in LDS.  It then walks them in a software pipeline: the voxel
-// loads of the next column are issued before the current column is staged (value times
-// the cone factor ampv, in LDS) and gathered: every row sums the voxels whose v
-// footprint covers it, with the exact clamps of the reference.
+// footprints at a time (one per lane, including the tile's u weights); each lane keeps
+// its candidate's parameters in registers.  The wave then walks the survivors in a
+// software pipeline, reading a column's parameters into scalar registers with readlane:
+// the voxel loads of the next column are issued once the current column is staged
+// (value times the cone factor ampv, in LDS), and fly while it is gathered: every row
+// sums the voxels whose v footprint covers it, with the exact clamps of the reference.
+// Columns of up to kFastSpan voxels (the common case) take one straight-line pass with
+// no window loop; longer ones are staged in windows of kZWin voxels.
 // ---------------------------------------------------------------------------------
 constexpr int kZWin = 256;  // voxels staged per LDS window and wave
 constexpr int kPre = 3;     // voxel loads per lane issued one column ahead (3 x 64 voxels)
 static_assert(kPre == 3, "the pipeline below holds three prefetch registers");
+constexpr int kZPad = 4;    // zero words kept after the staged voxels (the gather reads K - 1 <= 3 past them)
+// columns of fewer voxels than this stage their three prefetched slots whole: the zeros the
+// range-checked loads return past zb are the pads
+constexpr int kFastSpan = kPre * kWave - kZPad;
+static_assert(kPre * kWave <= kZWin, "the prefetched slots fit the staging window");
 
-// 44 bytes in FP32: with the 16-cell accumulators a block stays under 32 KB of LDS, so a
-// CU holds five blocks.  1/zinc is recomputed from zinc (the same v_rcp, the same value);
-// the footprint corners, needed only for footprints wider than four tile cells, are
-// recomputed from the column on that path.
+// Eleven registers per lane (no LDS: a block needs 20.5 KB for its staging windows and
+// 16-cell accumulators, and 80 VGPRs give six waves per SIMD).  1/zinc is recomputed from
+// zinc (the same v_rcp, the same value); the footprint corners, needed only for
+// footprints wider than four tile cells, are recomputed from the column on that path.
 template <class T>
 struct ColParams {
     int col;    // image column iy * nx + ix
@@ -108,7 +116,6 @@ __device__ __forceinline__ int tile_by_rank(int r, int c, int nut) {
 // positive the clamps reduce to the same min and max, elsewhere both ends clamp to the
 // same bound.  ja is clamped to [zw0 - 1, zw1 + 1] so that voxels outside the window
 // read zero words.
-constexpr int kZPad = 4;
 
 template <class T, int K>
 __device__ __forceinline__ T gather_row(const T* zwin, int zw0, int zw1, int ja, T tt, T zinc) {
@@ -129,14 +136,13 @@ __device__ __forceinline__ T gather_row(const T* zwin, int zw0, int zw1, int ja,
 }
 
 template <class Tin, class Tout, class T, int FOOT, int TU, int RPL, int SPLIT>
-__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 5 : 1) void cbct_proj_kernel(Geom<T> g, const View<T>* __restrict__ views,
+__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kernel(Geom<T> g, const View<T>* __restrict__ views,
                                                            int vstride, int nviews, const Tin* __restrict__ vol,
                                                            const uint8_t* __restrict__ map,
                                                            Tout* __restrict__ proj, int exhaustive) {
     static_assert(SPLIT == 1 || SPLIT == kWavesPerBlock, "SPLIT is 1 or the block's wave count");
     static_assert(RPL == 1, "one detector row per lane");
     __shared__ T zbuf[kWavesPerBlock][kZWin + 1 + kZPad];
-    __shared__ ColParams<T> cbuf[kWavesPerBlock][kWave];
     // each lane's row of the tile, [cell][lane] so that a wave's access is one word per
     // bank.  Every word has one writer (its lane).
     __shared__ T accs[kWavesPerBlock][TU][kWave];
@@ -170,9 +176,9 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 5 : 1) void cbct_proj_kern
     const T kR = u_to_slope(g, T(iu_b) + T(0.5));
     const int iv_last = min(iv_a + rows, g.nv) - 1;
     T* zw = zbuf[wv];
+    if (lane == 0) zw[0] = T(0);  // the word before the first staged voxel stays zero
     const T wzdz = v.wz * g.dz;                       // zc = z dz + wz dz
     const T fiv_lo = T(iv_a + lane) - T(0.5);          // lower edge of this lane's row
-    ColParams<T>* cw = cbuf[wv];
 
     T* acc = &accs[wv][0][lane];
 #pragma unroll
@@ -240,102 +246,106 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 5 : 1) void cbct_proj_kern
                         cp.col = iy * g.nx + ix;
                     }
                 }
-                cw[lane] = cp;
-                __builtin_amdgcn_wave_barrier();
+                // each lane keeps its candidate's parameters in registers; the column loop
+                // reads those of the column it walks with readlane (scalar registers)
                 unsigned long long mask = __ballot(ok);
                 // software pipeline over the surviving columns: loads of column i+1 are in
                 // flight while column i is staged and gathered
                 int cur = mask ? __ffsll((long long)mask) - 1 : -1;
                 if (cur >= 0) mask &= mask - 1;
                 Tin pre0{}, pre1{}, pre2{};
-                if (cur >= 0) {
-                    // wave-uniform column base and voxel range: a buffer resource over
-                    // voxels [za, zb] of the column, so lanes past zb read zero
-                    const ColParams<T>& c = cw[cur];
-                    const int zab = uni(c.zab), za = zab & 0xffff, zb = zab >> 16;
-                    const auto rs = buf_rsrc(vol + (size_t)uni(c.col) * g.nz + za, (zb - za + 1) * (int)sizeof(Tin));
+                // the three voxel loads of column c: a buffer resource over voxels [za, zb]
+                // of the column (wave-uniform base and range), so lanes past zb read zero
+                auto prefetch = [&](int c) {
+                    const int zabn = rdl_i(cp.zab, c), zan = zabn & 0xffff, zbn = zabn >> 16;
+                    const auto rs = buf_rsrc(vol + (size_t)rdl_i(cp.col, c) * g.nz + zan,
+                                             (zbn - zan + 1) * (int)sizeof(Tin));
                     pre0 = buf_ld<Tin>(rs, loff);
                     pre1 = buf_ld<Tin>(rs, loff + kWave * (int)sizeof(Tin));
-                    if (zb - za >= 2 * kWave) pre2 = buf_ld<Tin>(rs, loff + 2 * kWave * (int)sizeof(Tin));
-                }
+                    pre2 = buf_ld<Tin>(rs, loff + 2 * kWave * (int)sizeof(Tin));
+                };
+                if (cur >= 0) prefetch(cur);
                 while (cur >= 0) {
-                    const ColParams<T>& c = cw[cur];
-                    const int zab = uni(c.zab), cells = uni(c.cells);
+                    const int me = cur;  // cur moves on to the next column inside the loop
+                    const int zab = rdl_i(cp.zab, me), cells = rdl_i(cp.cells, me);
                     const int za = zab & 0xffff, zb = zab >> 16, ilo = cells & 0xffff, ihi = cells >> 16;
-                    const T ampr = c.ampr, r2 = c.r2, zinc = c.zinc, vstart = c.vstart, rz = frcp(zinc);
-                    const Tin* colp = vol + (size_t)uni(c.col) * g.nz;
+                    const T ampr = rdl(cp.ampr, me), r2 = rdl(cp.r2, me), zinc = rdl(cp.zinc, me),
+                            vstart = rdl(cp.vstart, me), rz = frcp(zinc);
                     const int iv = iv_a + lane;
+                    // row iv against the column's voxels: voxel j covers detector rows
+                    // [x_j, x_j + zinc] relative to the row's lower edge, with
+                    // x_j = j zinc - (iv - 1/2 - vstart); at most ceil(1/zinc) + 1 of them
+                    // overlap the row, so thick voxels take a fixed unrolled count (lanes
+                    // past the last row compute a sum nobody stores; their reads stay in
+                    // the window through the clamp of ja)
+                    const T tt = fiv_lo - vstart;
+                    const int ja = (int)ceil(tt * rz - T(1));
+                    auto gather = [&](int zw0, int zw1) -> T {
+                        if (zinc > T(14) || zinc < T(1) / T(3)) {
+                            // footprints that may reach the 16-cell cap, and thin voxels
+                            T s = T(0);
+                            if (iv <= iv_last) {
+                                const int jb = min((int)ceil((T(iv) + T(0.5) - vstart) * rz) - 1, zw1);
+                                for (int jj = max(ja, zw0); jj <= jb; ++jj) {
+                                    const T vmin = vstart + T(jj) * zinc;
+                                    s += (zinc > T(14) ? dd_overlap16(iv, vmin, vmin + zinc, g.nv)
+                                                       : dd_overlap(iv, vmin, vmin + zinc)) * zw[jj - zw0 + 1];
+                                }
+                            }
+                            return s;
+                        }
+                        if (zinc >= T(1)) return gather_row<T, 2>(zw, zw0, zw1, ja, tt, zinc);
+                        if (zinc >= T(0.5)) return gather_row<T, 3>(zw, zw0, zw1, ja, tt, zinc);
+                        return gather_row<T, 4>(zw, zw0, zw1, ja, tt, zinc);
+                    };
+                    auto next = [&]() {
+                        cur = mask ? __ffsll((long long)mask) - 1 : -1;
+                        if (cur >= 0) {
+                            mask &= mask - 1;
+                            prefetch(cur);
+                        }
+                    };
                     T q = T(0);
-                    for (int zw0 = za; zw0 <= zb; zw0 += kZWin) {
-                        const int zw1 = min(zb, zw0 + kZWin - 1);
-                        // stage ampv * img of voxels [zw0, zw1] in LDS: the first window from
-                        // the registers loaded one column earlier, the rest directly
-                        int zfirst = zw0 + lane;
-                        if (zw0 == za) {
-                            // slots wholly past the window are skipped (uniform branch);
-                            // lanes past zb hold zeros from the range-checked loads and
-                            // write them inside the padded window (kPre * kWave <= kZWin)
-                            const Tin pv[kPre] = {pre0, pre1, pre2};
+                    if (zb - za < kFastSpan) {
+                        // the range and the zero words the gather reads past it fit the three
+                        // prefetched slots: stage all of them (lanes past zb write zeros)
+                        const Tin pv[kPre] = {pre0, pre1, pre2};
 #pragma unroll
-                            for (int i = 0; i < kPre; ++i) {
-                                if (i * kWave <= zw1 - za) {
+                        for (int i = 0; i < kPre; ++i) {
+                            const T zc = T(za + i * kWave + lane) * g.dz + wzdz;
+                            zw[1 + i * kWave + lane] = fsqrt(r2 + zc * zc) * ampr * T(to_f(pv[i]));
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        next();  // the next column's loads fly while this one is gathered
+                        q = gather(za, zb);
+                        __builtin_amdgcn_wave_barrier();
+                    } else {
+                        const Tin* colp = vol + (size_t)rdl_i(cp.col, me) * g.nz;
+                        for (int zw0 = za; zw0 <= zb; zw0 += kZWin) {
+                            const int zw1 = min(zb, zw0 + kZWin - 1);
+                            // stage ampv * img of voxels [zw0, zw1] in LDS: the first window
+                            // from the registers loaded one column earlier, the rest directly
+                            int zfirst = zw0 + lane;
+                            if (zw0 == za) {
+                                const Tin pv[kPre] = {pre0, pre1, pre2};
+#pragma unroll
+                                for (int i = 0; i < kPre; ++i) {
                                     const int zz = za + i * kWave + lane;
                                     const T zc = T(zz) * g.dz + wzdz;
                                     zw[zz - zw0 + 1] = fsqrt(r2 + zc * zc) * ampr * T(to_f(pv[i]));
                                 }
+                                zfirst = za + kPre * kWave + lane;
                             }
-                            zfirst = za + kPre * kWave + lane;
-                        }
-                        for (int zz = zfirst; zz <= zw1; zz += kWave) {
-                            const T zc = T(zz) * g.dz + wzdz;
-                            zw[zz - zw0 + 1] = fsqrt(r2 + zc * zc) * ampr * ld<T>(colp + zz);
-                        }
-                        if (lane <= kZPad) zw[lane == 0 ? 0 : zw1 - zw0 + 1 + lane] = T(0);  // zero pads
-                        __builtin_amdgcn_wave_barrier();
-                        if (zw0 == za) {
-                            // the next column's voxel loads fly while this one is gathered
-                            const int nxt = mask ? __ffsll((long long)mask) - 1 : -1;
-                            if (nxt >= 0) {
-                                mask &= mask - 1;
-                                const ColParams<T>& cn = cw[nxt];
-                                const int zabn = uni(cn.zab), zan = zabn & 0xffff, zbn = zabn >> 16;
-                                const auto rs = buf_rsrc(vol + (size_t)uni(cn.col) * g.nz + zan,
-                                                         (zbn - zan + 1) * (int)sizeof(Tin));
-                                pre0 = buf_ld<Tin>(rs, loff);
-                                pre1 = buf_ld<Tin>(rs, loff + kWave * (int)sizeof(Tin));
-                                if (zbn - zan >= 2 * kWave) pre2 = buf_ld<Tin>(rs, loff + 2 * kWave * (int)sizeof(Tin));
+                            for (int zz = zfirst; zz <= zw1; zz += kWave) {
+                                const T zc = T(zz) * g.dz + wzdz;
+                                zw[zz - zw0 + 1] = fsqrt(r2 + zc * zc) * ampr * ld<T>(colp + zz);
                             }
-                            cur = nxt;
+                            if (lane < kZPad) zw[zw1 - zw0 + 2 + lane] = T(0);  // zero pads
+                            __builtin_amdgcn_wave_barrier();
+                            if (zw0 == za) next();
+                            q += gather(zw0, zw1);
+                            __builtin_amdgcn_wave_barrier();
                         }
-                        if (iv <= iv_last) {
-                            // row iv against the column's voxels: voxel j covers detector rows
-                            // [x_j, x_j + zinc] relative to the row's lower edge, with
-                            // x_j = j zinc - (iv - 1/2 - vstart); at most ceil(1/zinc) + 1 of
-                            // them overlap the row, so thick voxels take a fixed unrolled count
-                            const T tt = fiv_lo - vstart;
-                            const int ja = (int)ceil(tt * rz - T(1));
-                            const T zu = uni(zinc);
-                            if (zu > T(14)) {  // footprints that may reach the 16-cell cap
-                                const int jb = min((int)ceil((T(iv) + T(0.5) - vstart) * rz) - 1, zw1);
-                                for (int jj = max(ja, zw0); jj <= jb; ++jj) {
-                                    const T vmin = vstart + T(jj) * zinc;
-                                    q += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * zw[jj - zw0 + 1];
-                                }
-                            } else if (zu >= T(1)) {
-                                q += gather_row<T, 2>(zw, zw0, zw1, ja, tt, zinc);
-                            } else if (zu >= T(0.5)) {
-                                q += gather_row<T, 3>(zw, zw0, zw1, ja, tt, zinc);
-                            } else if (zu >= T(1) / T(3)) {
-                                q += gather_row<T, 4>(zw, zw0, zw1, ja, tt, zinc);
-                            } else {
-                                const int jb = min((int)ceil((T(iv) + T(0.5) - vstart) * rz) - 1, zw1);
-                                for (int jj = max(ja, zw0); jj <= jb; ++jj) {
-                                    const T vmin = vstart + T(jj) * zinc;
-                                    q += dd_overlap(iv, vmin, vmin + zinc) * zw[jj - zw0 + 1];
-                                }
-                            }
-                        }
-                        __builtin_amdgcn_wave_barrier();
                     }
                     // first and last tile cell of the footprint; the same on every lane, so
                     // the four weighted adds touch one LDS word per lane and bank
@@ -345,11 +355,11 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 5 : 1) void cbct_proj_kern
                     // cells c0 .. c1 in chunks of four: the first chunk's weights were made
                     // with the column's parameters, footprints wider than four tile cells
                     // (rare) evaluate the further ones here
-                    T w0 = uni(c.w0), w1 = uni(c.w1), w2 = uni(c.w2), w3 = uni(c.w3);
+                    T w0 = rdl(cp.w0, me), w1 = rdl(cp.w1, me), w2 = rdl(cp.w2, me), w3 = rdl(cp.w3, me);
                     for (int cc = c0; cc <= c1; cc += 4) {
                         if (cc > c0) {
                             // wider than four tile cells (rare): the corners again, from the column
-                            const int col = uni(c.col);
+                            const int col = rdl_i(cp.col, me);
                             T p0, p1, p2, p3, amp, r, d;
                             int jlo, jhi;
                             col_corners<T, FOOT>(g, v, col % g.nx, col / g.nx, p0, p1, p2, p3, jlo, jhi, amp, r, d);
