@@ -435,11 +435,13 @@ struct __align__(16) ViewCol {  // one view's footprint on one image column (80 
 // K = floor(zinc) + 2 rows.
 constexpr int kRowPad = 4;
 
-// rows staged per wave for a detector of nv rows, and the dynamic LDS of a block
+// rows staged per wave for a detector of nv rows, the words of one wave's row buffer (at
+// least one row per lane plus the zero words on either side) and the dynamic LDS of a block
 inline int back_rowwin(int nv) { return nv < kRowWin ? nv : kRowWin; }
+__host__ __device__ inline int back_rowbuf(int rowwin) { return (rowwin > kWave ? rowwin : kWave) + 1 + kRowPad; }
 template <class T>
 inline size_t back_lds_bytes(int rowwin) {
-    return (size_t)kWavesPerBlock * (size_t)(rowwin + 1 + kRowPad) * sizeof(T);
+    return (size_t)kWavesPerBlock * (size_t)back_rowbuf(rowwin) * sizeof(T);
 }
 
 template <class T, int K>
@@ -480,7 +482,10 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             for (int iz = lane; iz < g.nz; iz += kWave) ocol[iz] = cvt_store<Tout>(T(0));
         return;
     }
-    T* sw = srow + (size_t)wv * (rowwin + 1 + kRowPad);
+    T* sw = srow + (size_t)wv * back_rowbuf(rowwin);
+    // the word before the first staged row and the kRowPad words after the 64 lane slots
+    // start at zero; the one-pass form below never writes them
+    if (lane <= kRowPad) sw[lane == 0 ? 0 : kWave + lane] = T(0);
     ViewCol<T>* vw = vbuf[wv];
     const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
     T img[NZL];
@@ -565,52 +570,25 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             const T ampr = uni(c.ampr), r2 = uni(c.r2);  // ampv * amp = sqrt(r^2 + zc^2) * amp / r
             const T wzdz = uni(c.wzdz);                  // zc = z dz + wz dz
             const Tin* pv = proj + ((size_t)vb * g.nu + ilo) * g.nv;
-            for (int rw = r0; rw <= r1; rw += rowwin) {
-                const int rwe = min(r1, rw + rowwin - 1);
-                // u combination of the footprint's detector columns, row by row
-                int rfirst = rw + lane;
-                if (rw == r0) {
-                    if (rfirst <= rwe) {
-                        T s = w0 * T(to_f(pre0));
-                        if (nw > 1) s += w1 * T(to_f(pre1));
-                        if (nw > 2) s += w2 * T(to_f(pre2));
-                        if (nw > 3) s += w3 * T(to_f(pre3));
-                        for (int k = 4; k < nw; ++k)
-                            s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) *
-                                 ld<T>(pv + (size_t)k * g.nv + rfirst);
-                        sw[rfirst - rw + 1] = s;
-                    }
-                    rfirst += kWave;
+            // the next view's first row chunk starts loading once this one is staged
+            auto next = [&]() {
+                const int nxt = mask ? __ffsll((long long)mask) - 1 : -1;
+                if (nxt >= 0) {
+                    mask &= mask - 1;
+                    const ViewCol<T>& cn = vw[nxt];
+                    const int cellsn = uni(cn.cells), rowsn = uni(cn.rows);
+                    const int nwn = cellsn >> 16, r0n = rowsn & 0xffff, r1n = rowsn >> 16;
+                    const Tin* pr =
+                        proj + ((size_t)uni(cn.vb) * g.nu + (cellsn & 0xffff)) * g.nv + min(r0n + lane, r1n);
+                    pre0 = pr[0];
+                    if (nwn > 1) pre1 = pr[g.nv];
+                    if (nwn > 2) pre2 = pr[2 * g.nv];
+                    if (nwn > 3) pre3 = pr[3 * g.nv];
                 }
-                if (lane <= kRowPad) sw[lane == 0 ? 0 : rwe - rw + 1 + lane] = T(0);  // zero pads
-                for (int row = rfirst; row <= rwe; row += kWave) {
-                    const Tin* pr = pv + row;
-                    T s = w0 * ld<T>(pr);
-                    if (nw > 1) s += w1 * ld<T>(pr + g.nv);
-                    if (nw > 2) s += w2 * ld<T>(pr + 2 * g.nv);
-                    if (nw > 3) s += w3 * ld<T>(pr + 3 * g.nv);
-                    for (int k = 4; k < nw; ++k)
-                        s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) *
-                             ld<T>(pr + (size_t)k * g.nv);
-                    sw[row - rw + 1] = s;
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (rw == r0) {
-                    const int nxt = mask ? __ffsll((long long)mask) - 1 : -1;
-                    if (nxt >= 0) {
-                        mask &= mask - 1;
-                        const ViewCol<T>& cn = vw[nxt];
-                        const int cellsn = uni(cn.cells), rowsn = uni(cn.rows);
-                        const int nwn = cellsn >> 16, r0n = rowsn & 0xffff, r1n = rowsn >> 16;
-                        const Tin* pr =
-                            proj + ((size_t)uni(cn.vb) * g.nu + (cellsn & 0xffff)) * g.nv + min(r0n + lane, r1n);
-                        pre0 = pr[0];
-                        if (nwn > 1) pre1 = pr[g.nv];
-                        if (nwn > 2) pre2 = pr[2 * g.nv];
-                        if (nwn > 3) pre3 = pr[3 * g.nv];
-                    }
-                    cur = nxt;
-                }
+                cur = nxt;
+            };
+            // the column's voxels gather the staged rows [rw, rwe]
+            auto gather = [&](int rw, int rwe) {
                 const T zu = uni(zinc);
 #pragma unroll
                 for (int cz = 0; cz < NZL; ++cz) {
@@ -641,6 +619,58 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                     const T zcm = T(iz) * g.dz + wzdz;
                     img[cz] += ampr * fsqrt(r2 + zcm * zcm) * s;
                 }
+            };
+            // (short columns only: with more voxels per lane the second copy of the gather
+            // costs the longer columns a wave per SIMD)
+            if (NZL <= 2 && (r1 - r0 + 1 + kRowPad <= kWave || g.nv <= kWave)) {
+                // one row per lane: lanes past r1 write the zero words the gather reads past
+                // the rows (past the 64 slots they were zeroed at the start)
+                T s = w0 * T(to_f(pre0));
+                if (nw > 1) s += w1 * T(to_f(pre1));
+                if (nw > 2) s += w2 * T(to_f(pre2));
+                if (nw > 3) s += w3 * T(to_f(pre3));
+                for (int k = 4; k < nw; ++k)
+                    s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) *
+                         ld<T>(pv + (size_t)k * g.nv + min(r0 + lane, r1));
+                sw[1 + lane] = r0 + lane <= r1 ? s : T(0);
+                __builtin_amdgcn_wave_barrier();
+                next();
+                gather(r0, r1);
+                __builtin_amdgcn_wave_barrier();
+                continue;
+            }
+            for (int rw = r0; rw <= r1; rw += rowwin) {
+                const int rwe = min(r1, rw + rowwin - 1);
+                // u combination of the footprint's detector columns, row by row
+                int rfirst = rw + lane;
+                if (rw == r0) {
+                    if (rfirst <= rwe) {
+                        T s = w0 * T(to_f(pre0));
+                        if (nw > 1) s += w1 * T(to_f(pre1));
+                        if (nw > 2) s += w2 * T(to_f(pre2));
+                        if (nw > 3) s += w3 * T(to_f(pre3));
+                        for (int k = 4; k < nw; ++k)
+                            s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) *
+                                 ld<T>(pv + (size_t)k * g.nv + rfirst);
+                        sw[rfirst - rw + 1] = s;
+                    }
+                    rfirst += kWave;
+                }
+                if (lane <= kRowPad) sw[lane == 0 ? 0 : rwe - rw + 1 + lane] = T(0);  // zero pads
+                for (int row = rfirst; row <= rwe; row += kWave) {
+                    const Tin* pr = pv + row;
+                    T s = w0 * ld<T>(pr);
+                    if (nw > 1) s += w1 * ld<T>(pr + g.nv);
+                    if (nw > 2) s += w2 * ld<T>(pr + 2 * g.nv);
+                    if (nw > 3) s += w3 * ld<T>(pr + 3 * g.nv);
+                    for (int k = 4; k < nw; ++k)
+                        s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) *
+                             ld<T>(pr + (size_t)k * g.nv);
+                    sw[row - rw + 1] = s;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (rw == r0) next();
+                gather(rw, rwe);
                 __builtin_amdgcn_wave_barrier();
             }
         }

@@ -147,6 +147,9 @@ CONE_CASES = [
     dict(seed=5, n=(64, 64, 8), nuv=(200, 12), nviews=7),  # thin detector, wide fan
     dict(seed=6, n=(40, 40, 150), nuv=(90, 60), nviews=8, dz=0.3, dv=1.2),  # slices finer than rows: multi-pass
     dict(seed=7, n=(30, 30, 6), nuv=(70, 140), nviews=6, dz=4.0, dv=0.3),  # 20 rows per slice: 16-cell cap
+    dict(seed=9, n=(30, 30, 100), nuv=(60, 64), nviews=6, dz=0.5, dv=0.45),  # 61-64 rows per column, nv = 64
+    dict(seed=10, n=(30, 30, 40), nuv=(60, 62), nviews=6, dz=1.0, dv=0.45),  # nv = 62: rows reach the pads
+    dict(seed=11, n=(24, 24, 300), nuv=(50, 40), nviews=5, dz=0.2, dv=1.5),  # 300 voxels per column: two windows
 ]
 
 
