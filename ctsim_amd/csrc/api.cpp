@@ -681,6 +681,7 @@ int batch2_build(ctsim_batch2* b, int n, const ctsim_geom* geoms, const int32_t*
     }
     b->nitems_proj = ip;
     b->nitems_back = ib;
+    b->proj_split = ip < kFewItems2D ? 4 : 1;  // the single-slice forward's rule
     b->img_elems = img;
     b->sino_elems = sino;
     HIP_TRY(hipMalloc(&b->d_slices_proj, n * sizeof(Slice2<T>)), "batch table");

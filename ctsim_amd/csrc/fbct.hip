@@ -18,14 +18,22 @@ enum Foot2 { F2_DD = 0, F2_TF = 1 };
 
 // ---------------------------------------------------------------------------------
 // forward: one wavefront per (view, 64 detector cells); lanes own cells.  Candidate
-// pixels of the strip are evaluated 64 at a time (one per lane); each surviving pixel
-// is broadcast and every lane adds its cell's share.
+// pixels of the strip are evaluated 64 at a time (one per lane) into LDS entries, and
+// every tile cell notes the first and the last entry whose footprint covers it (LDS
+// min / max, order-free).  Each lane then walks the entries of its own cell range in
+// candidate order, so a cell adds the same products in the same order as a walk over
+// all candidates would, and a pixel costs the few lanes it covers, not the whole wave.
 // ---------------------------------------------------------------------------------
 // share `share` of `nshares` of the strip (its rows or columns along the major axis):
-// returns this lane's cell sum
+// returns this lane's cell sum.  Called by whole 256-thread blocks (four waves).
 template <class Tin, class T, int FOOT>
 __device__ __forceinline__ T fbct_proj_acc(const Geom<T>& g, const View<T>& v, int ut, const Tin* __restrict__ img,
                                            int exhaustive, int share, int nshares) {
+    __shared__ T ep[4][FOOT == F2_DD ? 2 : 4][kW2];  // footprint parameters of the entries
+    __shared__ T eval_[4][kW2];                        // amp * pixel value
+    __shared__ int ecell[4][kW2];                      // ilo | ihi << 16 (empty: ilo 1 > ihi 0)
+    __shared__ int cfirst[4][kW2], clast[4][kW2];      // per tile cell: first and last entry
+    const int wv = (int)(threadIdx.x / kW2);
     const int lane = lane_id();
     const int iu_a = ut * kW2;
     const int iu_b = min(iu_a + kW2, g.nu) - 1;
@@ -81,45 +89,77 @@ __device__ __forceinline__ T fbct_proj_acc(const Geom<T>& g, const View<T>& v, i
                 ok = ok && ihi >= iu_a && ilo <= iu_b;
                 if (ok) val = cg.amp * ld<T>(img + (size_t)iy * g.nx + ix);
             }
-            unsigned long long mask = __ballot(ok);
-            while (mask) {
-                const int src = __ffsll((long long)mask) - 1;
-                mask &= mask - 1;
-                const int cilo = rdl_i(ilo, src), cihi = rdl_i(ihi, src);
-                const T cval = rdl(val, src);
-                const T cp0 = rdl(p0, src), cp1 = rdl(p1, src);
-                T cp2 = T(0), cp3 = T(0);
-                if (FOOT == F2_TF) {
-                    cp2 = rdl(p2, src);
-                    cp3 = rdl(p3, src);
-                }
-                if (iu >= cilo && iu <= cihi) {
-                    const T w = FOOT == F2_DD ? dd_weight(iu, cp0, cp1, cilo, cihi) : sf_weight(iu, cp0, cp1, cp2, cp3);
-                    acc += w * cval;
+            ecell[wv][lane] = ok ? (ilo | (ihi << 16)) : 1;
+            ep[wv][0][lane] = p0;
+            ep[wv][1][lane] = p1;
+            if constexpr (FOOT == F2_TF) {
+                ep[wv][2][lane] = p2;
+                ep[wv][3][lane] = p3;
+            }
+            eval_[wv][lane] = val;
+            cfirst[wv][lane] = kW2;
+            clast[wv][lane] = -1;
+            __builtin_amdgcn_wave_barrier();
+            if (ok) {
+                const int c1 = min(ihi, iu_b) - iu_a;
+                for (int c = max(ilo, iu_a) - iu_a; c <= c1; ++c) {
+                    atomicMin(&cfirst[wv][c], lane);
+                    atomicMax(&clast[wv][c], lane);
                 }
             }
+            __builtin_amdgcn_wave_barrier();
+            const int jl = clast[wv][lane];
+            for (int e = cfirst[wv][lane]; e <= jl; ++e) {
+                const int cl = ecell[wv][e];
+                const int cilo = cl & 0xffff, cihi = cl >> 16;
+                if (iu >= cilo && iu <= cihi) {
+                    const T cp0 = ep[wv][0][e], cp1 = ep[wv][1][e];
+                    T w;
+                    if constexpr (FOOT == F2_DD) {
+                        w = dd_weight(iu, cp0, cp1, cilo, cihi);
+                    } else {
+                        w = sf_weight(iu, cp0, cp1, ep[wv][2][e], ep[wv][3][e]);
+                    }
+                    acc += w * eval_[wv][e];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
     return acc;
 }
 
-template <class Tin, class Tout, class T, int FOOT>
-__device__ __forceinline__ void fbct_proj_wave(const Geom<T>& g, const View<T>& v, int ut, const Tin* __restrict__ img,
-                                               Tout* __restrict__ out_row, int exhaustive) {
-    const T acc = fbct_proj_acc<Tin, T, FOOT>(g, v, ut, img, exhaustive, 0, 1);
-    const int iu = ut * kW2 + lane_id();
-    if (iu < g.nu) out_row[iu] = cvt_store<Tout>(acc);
+template <class T, int S>
+__device__ __forceinline__ T combine_shares(T (*part)[kW2], int wv, int lane) {
+    const int w0 = wv - wv % S;
+    if (S == 1) return part[w0][lane];
+    if (S == 2) return part[w0][lane] + part[w0 + 1][lane];
+    return (part[w0][lane] + part[w0 + 1][lane]) + (part[w0 + 2][lane] + part[w0 + 3][lane]);
 }
 
-template <class Tin, class Tout, class T, int FOOT>
+// S waves share a (view, tile) item, each walking 1/S of the strip's lines; the partial
+// sums are added in a fixed order.  S = 4 when the items alone would leave the chip
+// with few waves.
+template <class Tin, class Tout, class T, int FOOT, int S>
 __global__ __launch_bounds__(256) void fbct_proj_kernel(Geom<T> g, const View<T>* __restrict__ views, int vstride,
                                                         int nviews, const Tin* __restrict__ img,
                                                         Tout* __restrict__ proj, int exhaustive) {
+    __shared__ T part[4][kW2];
     const int nut = (g.nu + kW2 - 1) / kW2;
-    const int wid = blockIdx.x * 4 + (int)(threadIdx.x / kW2);
-    if (wid >= nviews * nut) return;
-    const int ut = wid % nut, b = wid / nut;
-    fbct_proj_wave<Tin, Tout, T, FOOT>(g, views[(size_t)b * vstride], ut, img, proj + (size_t)b * g.nu, exhaustive);
+    const int wv = (int)(threadIdx.x / kW2), lane = lane_id();
+    const int wid = blockIdx.x * (4 / S) + wv / S;
+    const bool valid = wid < nviews * nut;
+    const int ut = valid ? wid % nut : 0, b = valid ? wid / nut : 0;
+    T acc = T(0);
+    if (valid) acc = fbct_proj_acc<Tin, T, FOOT>(g, views[(size_t)b * vstride], ut, img, exhaustive, wv % S, S);
+    if (S > 1) {
+        part[wv][lane] = acc;
+        __syncthreads();
+        if (wv % S) return;
+        acc = combine_shares<T, S>(part, wv, lane);
+    }
+    const int iu = ut * kW2 + lane;
+    if (valid && iu < g.nu) proj[(size_t)b * g.nu + iu] = cvt_store<Tout>(acc);
 }
 
 // ---------------------------------------------------------------------------------
@@ -222,14 +262,6 @@ __device__ __forceinline__ int find_slice(const Slice2<T>* __restrict__ sl, int 
 // of a block (S = 1, 2, 4; the shares are added in a fixed order); the back pass splits
 // each (slice, row, 64-pixel) item's views over VS waves (VS = 1, 2, 4).  A block is
 // always 4 waves and holds 4/S (4/VS) items.
-template <class T, int S>
-__device__ __forceinline__ T combine_shares(T (*part)[kW2], int wv, int lane) {
-    const int w0 = wv - wv % S;
-    if (S == 1) return part[w0][lane];
-    if (S == 2) return part[w0][lane] + part[w0 + 1][lane];
-    return (part[w0][lane] + part[w0 + 1][lane]) + (part[w0 + 2][lane] + part[w0 + 3][lane]);
-}
-
 template <class Tin, class Tout, class T, int FOOT, int S>
 __global__ __launch_bounds__(256) void fbct_proj_batch_kernel(const Slice2<T>* __restrict__ sl, int nslices,
                                                               int nitems, const Tin* __restrict__ img,
@@ -307,13 +339,24 @@ template <class Tin, class Tout, class T>
 static hipError_t proj2(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* img, void* proj,
                         int kind, int exhaustive, hipStream_t st) {
     const long long nut = (g.nu + kW2 - 1) / kW2;
-    const unsigned blocks = (unsigned)(((long long)nviews * nut + 3) / 4);
-    if (blocks == 0) return hipSuccess;
+    const long long items = (long long)nviews * nut;
+    if (items == 0) return hipSuccess;
+    if (items < kFewItems2D) {
+        const unsigned blocks = (unsigned)items;
+        if (kind == F2_DD)
+            hipLaunchKernelGGL((fbct_proj_kernel<Tin, Tout, T, F2_DD, 4>), dim3(blocks), dim3(256), 0, st, g, views,
+                               vstride, nviews, (const Tin*)img, (Tout*)proj, exhaustive);
+        else
+            hipLaunchKernelGGL((fbct_proj_kernel<Tin, Tout, T, F2_TF, 4>), dim3(blocks), dim3(256), 0, st, g, views,
+                               vstride, nviews, (const Tin*)img, (Tout*)proj, exhaustive);
+        return hipGetLastError();
+    }
+    const unsigned blocks = (unsigned)((items + 3) / 4);
     if (kind == F2_DD)
-        hipLaunchKernelGGL((fbct_proj_kernel<Tin, Tout, T, F2_DD>), dim3(blocks), dim3(256), 0, st, g, views, vstride,
+        hipLaunchKernelGGL((fbct_proj_kernel<Tin, Tout, T, F2_DD, 1>), dim3(blocks), dim3(256), 0, st, g, views, vstride,
                            nviews, (const Tin*)img, (Tout*)proj, exhaustive);
     else
-        hipLaunchKernelGGL((fbct_proj_kernel<Tin, Tout, T, F2_TF>), dim3(blocks), dim3(256), 0, st, g, views, vstride,
+        hipLaunchKernelGGL((fbct_proj_kernel<Tin, Tout, T, F2_TF, 1>), dim3(blocks), dim3(256), 0, st, g, views, vstride,
                            nviews, (const Tin*)img, (Tout*)proj, exhaustive);
     return hipGetLastError();
 }

@@ -11,6 +11,9 @@ namespace ctsim {
 enum DType { DT_F32 = 0, DT_F16 = 1, DT_BF16 = 2, DT_F64 = 3 };
 
 inline size_t dtype_size(int dt) { return dt == DT_F64 ? 8 : dt == DT_F32 ? 4 : 2; }
+// below this many (view, 64-cell tile) items a 2D forward projection splits each item over
+// the four waves of a block (fewer items would leave the chip with under eight waves per SIMD)
+constexpr long long kFewItems2D = 8192;
 
 // 3D cone beam.  foot: 0 = DD, 1 = SF.  Volume in ZXY order.
 template <class T>

@@ -209,6 +209,7 @@ FAN_CASES = [
     dict(seed=11, n=(64, 64), nu=128, nviews=36, dettype=1),
     dict(seed=12, n=(45, 77), nu=100, nviews=25, dettype=1, offsets=True),
     dict(seed=13, n=(50, 50), nu=90, nviews=30, dettype=0),
+    dict(seed=14, n=(24, 20), nu=600, nviews=1000, dettype=1),  # 10000 (view, tile) items: one wave per item
 ]
 
 
@@ -378,6 +379,8 @@ def test_batched_uneven_2d_matches_single_slices():
     assert packed.size == ni.value
     d_img = DeviceArray.from_numpy(packed, "f32")
     d_sino = DeviceArray(ns.value, "f32")
+    # the batch's default schedule splits the forward items over 4 waves below 8192 items,
+    # as the single-slice launches do: the same sums in the same order
     for kind in ("dd", "tf"):
         _lib.call("ctsim_batch2_proj", h, f"proj,{kind}".encode(), d_img.vp, 0, d_sino.vp, 0, None)
         got = d_sino.to_numpy()
