@@ -3,7 +3,9 @@ passes), so that a profile names only the kernels it is about.
 
     python -m ctsim_amd.profiling copy [reps]          # 256 MiB FP32 copy kernel (ctsim_convert)
     python -m ctsim_amd.profiling pwls-subset [reps]   # proj,tf + back,tf of one 41-view subset,
-                                                       # 512x512x128 volume, 1024x64 detector, FP16
+                                                       # 512x512x128 volume, 1024x64 detector: FP32
+                                                       # projector input and output, BF16 sinogram
+                                                       # into the backprojector (as in PwlsOS mixed)
     python -m ctsim_amd.profiling pair-slab [reps]     # proj,tf + back,tf of 24 of 720 views,
                                                        # 512x512x256 volume, 1024x384 detector, FP32
     python -m ctsim_amd.profiling fdk-back [reps]      # back,hpd of 720 filtered views (FP16) into a
@@ -33,21 +35,26 @@ def copy(reps: int) -> None:
     synchronize()
 
 
-def _pair(g, vol, nsub, vstep, dtype, reps):
+def _pair(g, vol, nsub, vstep, dtype, reps, back_dtype=None):
+    """proj,tf of the volume stored in dtype into a dtype sinogram, then back,tf of a
+    back_dtype copy of it (default: the same buffer) into an FP32 volume"""
     P = Projector(g)
     x = P.upload_volume(vol, dtype=dtype)
     s = DeviceArray(P.n_sino(nsub), dtype)
+    sb = s if back_dtype is None else DeviceArray(P.n_sino(nsub), back_dtype)
     b = DeviceArray(P.n_img, "f32")
     for _ in range(reps):
         P.proj(x, s, op="proj,tf", view0=0, vstep=vstep, nsub=nsub)
-        P.back(s, b, op="back,tf", view0=0, vstep=vstep, nsub=nsub)
+        if sb is not s:
+            sb.copy_from(s)
+        P.back(sb, b, op="back,tf", view0=0, vstep=vstep, nsub=nsub)
     synchronize()
 
 
 def pwls_subset(reps: int) -> None:
     g = workloads.pwls_512x128(with_map=True)
     vol = workloads.body_phantom(g.reconSize, g.reconSpacing, 10, 7)
-    _pair(g, vol, 41, 24, "f16", reps)
+    _pair(g, vol, 41, 24, "f32", reps, back_dtype="bf16")
 
 
 def pair_slab(reps: int) -> None:
