@@ -554,7 +554,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             const ViewCol<T>& c = vw[cur];
             const int cells = uni(c.cells), rows = uni(c.rows);
             const int nw = cells >> 16, r0 = rows & 0xffff, r1 = rows >> 16;
-            const Tin* pr = proj + ((size_t)uni(c.vb) * g.nu + (cells & 0xffff)) * g.nv + min(r0 + lane, r1);
+            const Tin* pr = proj + ((size_t)(b0 + cur) * g.nu + (cells & 0xffff)) * g.nv + min(r0 + lane, r1);
             pre0 = pr[0];
             if (nw > 1) pre1 = pr[g.nv];
             if (nw > 2) pre2 = pr[2 * g.nv];
@@ -562,7 +562,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
         }
         while (cur >= 0) {
             const ViewCol<T>& c = vw[cur];
-            const int cells = uni(c.cells), rows = uni(c.rows), zr = uni(c.zr), vb = uni(c.vb);
+            const int cells = uni(c.cells), rows = uni(c.rows), zr = uni(c.zr), vb = b0 + cur;
             const int ilo = cells & 0xffff, nw = cells >> 16, r0 = rows & 0xffff, r1 = rows >> 16;
             const int iz0 = zr & 0xffff, iz1 = zr >> 16;
             const T w0 = uni(c.w0), w1 = uni(c.w1), w2 = uni(c.w2), w3 = uni(c.w3);
@@ -579,7 +579,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                     const int cellsn = uni(cn.cells), rowsn = uni(cn.rows);
                     const int nwn = cellsn >> 16, r0n = rowsn & 0xffff, r1n = rowsn >> 16;
                     const Tin* pr =
-                        proj + ((size_t)uni(cn.vb) * g.nu + (cellsn & 0xffff)) * g.nv + min(r0n + lane, r1n);
+                        proj + ((size_t)(b0 + nxt) * g.nu + (cellsn & 0xffff)) * g.nv + min(r0n + lane, r1n);
                     pre0 = pr[0];
                     if (nwn > 1) pre1 = pr[g.nv];
                     if (nwn > 2) pre2 = pr[2 * g.nv];
