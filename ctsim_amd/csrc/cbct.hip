@@ -10,6 +10,8 @@
 //
 // Device layouts: volume "ZXY" (z fastest: vol[(iy*nx + ix)*nz + iz]) so that a
 // wavefront walks one voxel column; sinogram [view][u][v] (v fastest) as in MATLAB.
+#include <type_traits>
+
 #include "internal.h"
 
 namespace ctsim {
@@ -35,27 +37,42 @@ enum Foot { FOOT_DD = 0, FOOT_TF = 1 };
 // (value times the cone factor ampv, in LDS), and fly while it is gathered: every row
 // sums the voxels whose v footprint covers it, with the exact clamps of the reference.
 // Columns of up to kFastSpan voxels (the common case) take one straight-line pass with
-// no window loop; longer ones are staged in windows of kZWin voxels.
+// no window loop, staging only the 64-voxel slots their range needs; longer ones are
+// staged in windows of kZWin voxels.  FP32 volumes of at most kFastSpan slices take the
+// ring form below (cbct_proj_ring_kernel) instead.
 // ---------------------------------------------------------------------------------
 constexpr int kZWin = 256;  // voxels staged per LDS window and wave
 constexpr int kPre = 3;     // voxel loads per lane issued one column ahead (3 x 64 voxels)
 static_assert(kPre == 3, "the pipeline below holds three prefetch registers");
 constexpr int kZPad = 4;    // zero words kept after the staged voxels (the gather reads K - 1 <= 3 past them)
-// columns of fewer voxels than this stage their three prefetched slots whole: the zeros the
-// range-checked loads return past zb are the pads
-constexpr int kFastSpan = kPre * kWave - kZPad;
-static_assert(kPre * kWave <= kZWin, "the prefetched slots fit the staging window");
+// columns of at most this many voxels stage the prefetched slots they need (lanes past zb
+// read zeros) and four zero words after the last slot
+constexpr int kFastSpan = kPre * kWave;
+static_assert(kPre * kWave + kZPad <= kZWin + kZPad, "the prefetched slots and pads fit the staging window");
 
-// Eleven registers per lane (no LDS: a block needs 20.5 KB for its staging windows and
-// 16-cell accumulators, and 80 VGPRs give six waves per SIMD).  1/zinc is recomputed from
-// zinc (the same v_rcp, the same value); the footprint corners, needed only for
-// footprints wider than four tile cells, are recomputed from the column on that path.
+// voxel range and gather class of a candidate column, packed in one word:
+// za | zb << 13 | class << 26 (nz <= 8192).  Class 0..2: the fixed-count gathers of
+// 2, 3, 4 voxels per row (zinc in [1, 14], [1/2, 1), [1/3, 1/2)); 3: the counted loop.
+constexpr int kZBits = 13;
+constexpr int kZMask = (1 << kZBits) - 1;
+constexpr int kMaxNz = 1 << kZBits;
+
+template <class T>
+__device__ __forceinline__ int gather_class(T zinc) {
+    if (zinc > T(14) || zinc < T(1) / T(3)) return 3;
+    return zinc >= T(1) ? 0 : (zinc >= T(0.5) ? 1 : 2);
+}
+
+// Ten registers per lane (no LDS).  1/zinc is recomputed from zinc (the same v_rcp, the
+// same value); the footprint corners, needed only for footprints wider than four tile
+// cells, are recomputed from the column on that path.  The tile weights carry the
+// column's amp / r, so the staged voxel value is sqrt(r^2 + zc^2) * img.
 template <class T>
 struct ColParams {
     int col;    // image column iy * nx + ix
     int cells;  // footprint cells ilo | ihi << 16 (detectors narrower than 32768 cells)
-    int zab;    // voxel range za | zb << 16 needed by the tile's rows (nz <= 512)
-    T ampr, r2;  // amp / r and r^2: the staged value of voxel z is sqrt(r^2 + zc^2) * ampr * img
+    int zab;    // voxel range and gather class (see kZBits)
+    T r2;       // r^2
     T zinc, vstart, w0, w1, w2, w3;
 };
 
@@ -122,13 +139,13 @@ __device__ __forceinline__ T gather_row(const T* zwin, int zw0, int zw1, int ja,
     static_assert(K - 1 <= kZPad, "pad covers the voxels past the window");
     ja = min(max(ja, zw0 - 1), zw1 + 1);
     const T* pz = zwin + (ja - zw0 + 1);
-    T x = T(ja) * zinc - tt;
-    T cl = clamp01(x);
+    // voxel edges x_k = x_0 + k zinc, each one FMA from x_0 with the clamp folded in
+    const T x0 = T(ja) * zinc - tt;
+    T cl = clamp01(x0);
     T s = T(0);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        x = x + zinc;
-        const T ch = clamp01(x);
+        const T ch = clamp01(fma_t(T(k + 1), zinc, x0));
         s += (ch - cl) * pz[k];
         cl = ch;
     }
@@ -179,6 +196,9 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
     if (lane == 0) zw[0] = T(0);  // the word before the first staged voxel stays zero
     const T wzdz = v.wz * g.dz;                       // zc = z dz + wz dz
     const T fiv_lo = T(iv_a + lane) - T(0.5);          // lower edge of this lane's row
+    // this lane's z offset in each prefetch slot: voxel za + i 64 + lane sits at
+    // zc = (za dz + wz dz) + lzd[i]
+    const T lzd0 = T(lane) * g.dz, lzd1 = T(kWave + lane) * g.dz, lzd2 = T(2 * kWave + lane) * g.dz;
 
     T* acc = &accs[wv][0][lane];
 #pragma unroll
@@ -222,9 +242,10 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
                     ok = col_corners<T, FOOT>(g, v, ix, iy, p0, p1, p2, p3, ilo, ihi, amp, r, d);
                     ok = ok && ihi >= iu_a && ilo <= iu_b;
                     cp.cells = ilo | (ihi << 16);
+                    T ampr = T(0);
                     if (ok) {
                         ok = v_window(g, d, v.wz, cp.zinc, cp.vstart, iz0, iz1) && iz0 < iz1 && cp.zinc > T(0);
-                        cp.ampr = amp * frcp(r);
+                        ampr = amp * frcp(r);
                         cp.r2 = r * r;
                     }
                     if (ok) {
@@ -233,31 +254,33 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
                         const T rz = frcp(cp.zinc);
                         const int za = max((int)ceil((T(iv_a) - T(0.5) - cp.vstart) * rz - T(1)), iz0);
                         const int zb = min((int)ceil((T(iv_last) + T(0.5) - cp.vstart) * rz) - 1, iz1 - 1);
-                        cp.zab = za | (zb << 16);
+                        cp.zab = za | (zb << kZBits) | (gather_class(cp.zinc) << (2 * kZBits));
                         ok = za <= zb;
                     }
                     if (ok) {
-                        // tile weights of the first four footprint cells inside the tile
+                        // tile weights of the first four footprint cells inside the tile,
+                        // times amp / r
                         const int c0 = max(ilo, iu_a);
-                        cp.w0 = u_weight<T, FOOT>(c0 + 0, ilo, ihi, p0, p1, p2, p3);
-                        cp.w1 = u_weight<T, FOOT>(c0 + 1, ilo, ihi, p0, p1, p2, p3);
-                        cp.w2 = u_weight<T, FOOT>(c0 + 2, ilo, ihi, p0, p1, p2, p3);
-                        cp.w3 = u_weight<T, FOOT>(c0 + 3, ilo, ihi, p0, p1, p2, p3);
+                        cp.w0 = u_weight<T, FOOT>(c0 + 0, ilo, ihi, p0, p1, p2, p3) * ampr;
+                        cp.w1 = u_weight<T, FOOT>(c0 + 1, ilo, ihi, p0, p1, p2, p3) * ampr;
+                        cp.w2 = u_weight<T, FOOT>(c0 + 2, ilo, ihi, p0, p1, p2, p3) * ampr;
+                        cp.w3 = u_weight<T, FOOT>(c0 + 3, ilo, ihi, p0, p1, p2, p3) * ampr;
                         cp.col = iy * g.nx + ix;
                     }
                 }
                 // each lane keeps its candidate's parameters in registers; the column loop
                 // reads those of the column it walks with readlane (scalar registers)
                 unsigned long long mask = __ballot(ok);
-                // software pipeline over the surviving columns: loads of column i+1 are in
-                // flight while column i is staged and gathered
+                // software pipeline over the surviving columns: the loads of column i+1 are
+                // in flight while column i is staged and gathered.  All three loads are
+                // issued for every column (a buffer resource over voxels [za, zb] of the
+                // column, wave-uniform base and range, so loads past zb read zero): loads
+                // under a condition would make the compiler wait for them at once.
                 int cur = mask ? __ffsll((long long)mask) - 1 : -1;
                 if (cur >= 0) mask &= mask - 1;
                 Tin pre0{}, pre1{}, pre2{};
-                // the three voxel loads of column c: a buffer resource over voxels [za, zb]
-                // of the column (wave-uniform base and range), so lanes past zb read zero
                 auto prefetch = [&](int c) {
-                    const int zabn = rdl_i(cp.zab, c), zan = zabn & 0xffff, zbn = zabn >> 16;
+                    const int zabn = rdl_i(cp.zab, c), zan = zabn & kZMask, zbn = (zabn >> kZBits) & kZMask;
                     const auto rs = buf_rsrc(vol + (size_t)rdl_i(cp.col, c) * g.nz + zan,
                                              (zbn - zan + 1) * (int)sizeof(Tin));
                     pre0 = buf_ld<Tin>(rs, loff);
@@ -268,9 +291,9 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
                 while (cur >= 0) {
                     const int me = cur;  // cur moves on to the next column inside the loop
                     const int zab = rdl_i(cp.zab, me), cells = rdl_i(cp.cells, me);
-                    const int za = zab & 0xffff, zb = zab >> 16, ilo = cells & 0xffff, ihi = cells >> 16;
-                    const T ampr = rdl(cp.ampr, me), r2 = rdl(cp.r2, me), zinc = rdl(cp.zinc, me),
-                            vstart = rdl(cp.vstart, me), rz = frcp(zinc);
+                    const int za = zab & kZMask, zb = (zab >> kZBits) & kZMask, cls = zab >> (2 * kZBits);
+                    const int ilo = cells & 0xffff, ihi = cells >> 16;
+                    const T r2 = rdl(cp.r2, me), zinc = rdl(cp.zinc, me), vstart = rdl(cp.vstart, me), rz = frcp(zinc);
                     const int iv = iv_a + lane;
                     // row iv against the column's voxels: voxel j covers detector rows
                     // [x_j, x_j + zinc] relative to the row's lower edge, with
@@ -281,7 +304,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
                     const T tt = fiv_lo - vstart;
                     const int ja = (int)ceil(tt * rz - T(1));
                     auto gather = [&](int zw0, int zw1) -> T {
-                        if (zinc > T(14) || zinc < T(1) / T(3)) {
+                        if (cls == 3) {
                             // footprints that may reach the 16-cell cap, and thin voxels
                             T s = T(0);
                             if (iv <= iv_last) {
@@ -294,8 +317,8 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
                             }
                             return s;
                         }
-                        if (zinc >= T(1)) return gather_row<T, 2>(zw, zw0, zw1, ja, tt, zinc);
-                        if (zinc >= T(0.5)) return gather_row<T, 3>(zw, zw0, zw1, ja, tt, zinc);
+                        if (cls == 0) return gather_row<T, 2>(zw, zw0, zw1, ja, tt, zinc);
+                        if (cls == 1) return gather_row<T, 3>(zw, zw0, zw1, ja, tt, zinc);
                         return gather_row<T, 4>(zw, zw0, zw1, ja, tt, zinc);
                     };
                     auto next = [&]() {
@@ -306,15 +329,24 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
                         }
                     };
                     T q = T(0);
+                    const int ns = (zb - za) >> 6;  // 64-voxel slots the range needs, minus one
+                    const T zbase = T(za) * g.dz + wzdz;
                     if (zb - za < kFastSpan) {
-                        // the range and the zero words the gather reads past it fit the three
-                        // prefetched slots: stage all of them (lanes past zb write zeros)
-                        const Tin pv[kPre] = {pre0, pre1, pre2};
-#pragma unroll
-                        for (int i = 0; i < kPre; ++i) {
-                            const T zc = T(za + i * kWave + lane) * g.dz + wzdz;
-                            zw[1 + i * kWave + lane] = fsqrt(r2 + zc * zc) * ampr * T(to_f(pv[i]));
+                        // stage the slots the range needs (lanes past zb hold the zeros the
+                        // range-checked loads returned) and four zero words after them
+                        {
+                            const T zc = zbase + lzd0;
+                            zw[1 + lane] = fsqrt(fma_t(zc, zc, r2)) * T(to_f(pre0));
                         }
+                        if (ns >= 1) {
+                            const T zc = zbase + lzd1;
+                            zw[1 + kWave + lane] = fsqrt(fma_t(zc, zc, r2)) * T(to_f(pre1));
+                        }
+                        if (ns >= 2) {
+                            const T zc = zbase + lzd2;
+                            zw[1 + 2 * kWave + lane] = fsqrt(fma_t(zc, zc, r2)) * T(to_f(pre2));
+                        }
+                        if (lane < kZPad) zw[1 + (ns + 1) * kWave + lane] = T(0);
                         __builtin_amdgcn_wave_barrier();
                         next();  // the next column's loads fly while this one is gathered
                         q = gather(za, zb);
@@ -323,8 +355,9 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
                         const Tin* colp = vol + (size_t)rdl_i(cp.col, me) * g.nz;
                         for (int zw0 = za; zw0 <= zb; zw0 += kZWin) {
                             const int zw1 = min(zb, zw0 + kZWin - 1);
-                            // stage ampv * img of voxels [zw0, zw1] in LDS: the first window
-                            // from the registers loaded one column earlier, the rest directly
+                            // stage sqrt(r^2 + zc^2) * img of voxels [zw0, zw1] in LDS: the
+                            // first window from the registers loaded one column earlier, the
+                            // rest directly
                             int zfirst = zw0 + lane;
                             if (zw0 == za) {
                                 const Tin pv[kPre] = {pre0, pre1, pre2};
@@ -332,13 +365,13 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
                                 for (int i = 0; i < kPre; ++i) {
                                     const int zz = za + i * kWave + lane;
                                     const T zc = T(zz) * g.dz + wzdz;
-                                    zw[zz - zw0 + 1] = fsqrt(r2 + zc * zc) * ampr * T(to_f(pv[i]));
+                                    zw[zz - zw0 + 1] = fsqrt(fma_t(zc, zc, r2)) * T(to_f(pv[i]));
                                 }
                                 zfirst = za + kPre * kWave + lane;
                             }
                             for (int zz = zfirst; zz <= zw1; zz += kWave) {
                                 const T zc = T(zz) * g.dz + wzdz;
-                                zw[zz - zw0 + 1] = fsqrt(r2 + zc * zc) * ampr * ld<T>(colp + zz);
+                                zw[zz - zw0 + 1] = fsqrt(fma_t(zc, zc, r2)) * ld<T>(colp + zz);
                             }
                             if (lane < kZPad) zw[zw1 - zw0 + 2 + lane] = T(0);  // zero pads
                             __builtin_amdgcn_wave_barrier();
@@ -348,32 +381,27 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
                         }
                     }
                     // first and last tile cell of the footprint; the same on every lane, so
-                    // the four weighted adds touch one LDS word per lane and bank
-                    const int c0 = uni(max(ilo, iu_a) - iu_a);
-                    const int c1 = uni(min(ihi, iu_b) - iu_a);
-                    const T aq = q;  // amp is folded into the staged voxel values
-                    // cells c0 .. c1 in chunks of four: the first chunk's weights were made
-                    // with the column's parameters, footprints wider than four tile cells
-                    // (rare) evaluate the further ones here
-                    T w0 = rdl(cp.w0, me), w1 = rdl(cp.w1, me), w2 = rdl(cp.w2, me), w3 = rdl(cp.w3, me);
-                    for (int cc = c0; cc <= c1; cc += 4) {
-                        if (cc > c0) {
-                            // wider than four tile cells (rare): the corners again, from the column
-                            const int col = rdl_i(cp.col, me);
-                            T p0, p1, p2, p3, amp, r, d;
-                            int jlo, jhi;
-                            col_corners<T, FOOT>(g, v, col % g.nx, col / g.nx, p0, p1, p2, p3, jlo, jhi, amp, r, d);
-                            w0 = u_weight<T, FOOT>(iu_a + cc, ilo, ihi, p0, p1, p2, p3);
-                            w1 = cc + 1 <= c1 ? u_weight<T, FOOT>(iu_a + cc + 1, ilo, ihi, p0, p1, p2, p3) : T(0);
-                            w2 = cc + 2 <= c1 ? u_weight<T, FOOT>(iu_a + cc + 2, ilo, ihi, p0, p1, p2, p3) : T(0);
-                            w3 = cc + 3 <= c1 ? u_weight<T, FOOT>(iu_a + cc + 3, ilo, ihi, p0, p1, p2, p3) : T(0);
-                        }
-                        // the cells past c1 (zero weight) are skipped by uniform branches
-                        T* pa = acc + cc * kWave;
-                        pa[0] += w0 * aq;
-                        if (cc + 1 <= c1) pa[kWave] += w1 * aq;
-                        if (cc + 2 <= c1) pa[2 * kWave] += w2 * aq;
-                        if (cc + 3 <= c1) pa[3 * kWave] += w3 * aq;
+                    // the weighted adds touch one LDS word per lane and bank.  The first four
+                    // cells use the weights made with the column's parameters (scalar
+                    // registers); footprints wider than four tile cells (rare) evaluate the
+                    // further ones here.
+                    const int c0 = max(ilo, iu_a) - iu_a;
+                    const int c1 = min(ihi, iu_b) - iu_a;
+                    {
+                        T* pa = acc + c0 * kWave;
+                        pa[0] += rdl(cp.w0, me) * q;
+                        if (c0 + 1 <= c1) pa[kWave] += rdl(cp.w1, me) * q;
+                        if (c0 + 2 <= c1) pa[2 * kWave] += rdl(cp.w2, me) * q;
+                        if (c0 + 3 <= c1) pa[3 * kWave] += rdl(cp.w3, me) * q;
+                    }
+                    if (c0 + 4 <= c1) {
+                        const int col = rdl_i(cp.col, me);
+                        T p0, p1, p2, p3, amp, r, d;
+                        int jlo, jhi;
+                        col_corners<T, FOOT>(g, v, col % g.nx, col / g.nx, p0, p1, p2, p3, jlo, jhi, amp, r, d);
+                        const T ampr = amp * frcp(r);
+                        for (int cc = c0 + 4; cc <= c1; ++cc)
+                            acc[cc * kWave] += u_weight<T, FOOT>(iu_a + cc, ilo, ihi, p0, p1, p2, p3) * ampr * q;
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
@@ -387,6 +415,361 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
     }
     Tout* out = proj + (size_t)b * g.nu * g.nv;
     const int iv = iv_a + lane;
+    if (iv <= iv_last) {
+#pragma unroll
+        for (int t = 0; t < TU; ++t) {
+            const int iu = iu_a + t;
+            T sum = acc[t * kWave];
+#pragma unroll
+            for (int w = 1; w < SPLIT; ++w) sum += accs[w][t][lane];
+            if (iu < g.nu) out[(size_t)iu * g.nv + iv] = cvt_store<Tout>(sum);
+        }
+    }
+}
+
+// LDS ring of the ring form: kRing column slots per wave.  A slot holds a zero word, the
+// 192 words a column's three loads write (voxels za .. za + 191; MUBUF loads with the LDS
+// bit put lane t's word at the slot address + 4 t, and lanes past the buffer range write
+// zero) and kZPad zero words.  Only the loads write words 1 .. 192, so word 0 and the pads
+// stay zero for the whole kernel.
+constexpr int kRing = 3;
+constexpr int kSlotW = 1 + kPre * kWave + kZPad;
+
+// A buffer resource as four scalar words, for the inline assembly below.  The loads are
+// issued from inline assembly because the compiler's wait insertion, which cannot tell the
+// ring's slots apart, would otherwise wait for every outstanding LDS load before any LDS
+// read of the ring; the kernel's explicit vmcnt waits order each slot's reads after its
+// own loads instead.
+typedef int rsrc_words __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ rsrc_words dma_rsrc(const void* base, int nbytes) {
+    const unsigned long long a = (unsigned long long)base;
+    rsrc_words r;
+    r.x = (int)(unsigned)(a & 0xffffffffull);
+    r.y = (int)(unsigned)((a >> 32) & 0xffffull);  // stride 0
+    r.z = nbytes;                                  // range: loads at or past it read zero
+    r.w = 0x00020000;
+    return r;
+}
+
+// LDS byte address of a shared-memory pointer, in a scalar register
+template <class P>
+__device__ __forceinline__ unsigned lds_addr(P* p) {
+    return (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(size_t)(__attribute__((address_space(3))) P*)p);
+}
+
+// wait until at most N of this wave's vector-memory operations are outstanding; the
+// memory clobber keeps the compiler from moving LDS accesses across the wait
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// ---------------------------------------------------------------------------------
+// Forward projection, ring form (FP32 volumes with nz <= kFastSpan: every column range
+// fits three 64-voxel slots).  The same work items, strips and footprints as
+// cbct_proj_kernel; what differs is how a wave walks the surviving columns of a batch:
+//
+//  * the survivors' parameters are moved to lanes 0 .. n-1 (ds_permute), so column k's
+//    are read with readlane(k) and the walk needs no bit-mask bookkeeping;
+//  * each candidate's lane precomputes what the walk would otherwise derive in scalar
+//    registers: the byte address and length of its voxel range (the buffer resource),
+//    the lower edge of its first voxel on the detector, zc of its first voxel, its tile
+//    cells and gather class;
+//  * a column's three 64-voxel loads go straight into an LDS ring slot (MUBUF loads with
+//    the LDS bit), two columns ahead of the one being gathered.  Every column issues
+//    exactly one three-load group (groups past the batch's last survivor have an empty
+//    range and write zeros), so "this column's group has landed" is always vmcnt(6).
+// ---------------------------------------------------------------------------------
+struct RingCol {
+    int alo, ahi;   // byte address of the column's first needed voxel (ahi: the 16 address bits of a resource)
+    int nbytes;     // bytes of the voxel range [za, zb]
+    int meta;       // gather class | slots - 1 << 2 | cells past the first << 4 | first tile cell << 9
+    float zbase;    // zc of voxel za: za dz + wz dz
+    float r2, zinc, vst;  // r^2, magnified slice pitch, lower edge of voxel za on the detector (rows)
+    float w0, w1, w2, w3;  // tile weights of the first four footprint cells, times amp / r
+    int col, cells;  // image column and footprint cells ilo | ihi << 16 (wide footprints only)
+};
+
+__device__ __forceinline__ int permute_i(int dst_lane_bytes, int x) {
+    return __builtin_amdgcn_ds_permute(dst_lane_bytes, x);
+}
+__device__ __forceinline__ float permute_f(int dst_lane_bytes, float x) {
+    return __int_as_float(__builtin_amdgcn_ds_permute(dst_lane_bytes, __float_as_int(x)));
+}
+
+// three 64-lane dword loads of one column into the ring slot at LDS byte address la (the
+// instruction offsets 0, 256, 512 apply to both the voxel and the LDS address)
+__device__ __forceinline__ void ring_load(rsrc_words rs, unsigned la, int voff) {
+    asm volatile(
+        "s_mov_b32 m0, %1\n\t"
+        "s_nop 0\n\t"
+        "buffer_load_dword %0, %2, 0 offen lds\n\t"
+        "buffer_load_dword %0, %2, 0 offen offset:256 lds\n\t"
+        "buffer_load_dword %0, %2, 0 offen offset:512 lds"
+        :
+        : "v"(voff), "s"(la), "s"(rs)
+        : "memory");
+}
+
+// sum over the K voxels from ja on of (overlap with the row) * staged value, voxel j
+// covering [x_j, x_j + zinc] relative to the row's lower edge (x_j = j zinc - tt); zs[1 + j]
+// holds voxel j of the range, zs[0] and the words past the range are zero
+template <int K>
+__device__ __forceinline__ float ring_gather(const float* zs, int span, int ja, float tt, float zinc) {
+    ja = min(max(ja, -1), span + 1);
+    const float* pz = zs + (ja + 1);
+    const float x0 = float(ja) * zinc - tt;
+    float cl = clamp01(x0);
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const float ch = clamp01(fmaf(float(k + 1), zinc, x0));
+        s += (ch - cl) * pz[k];
+        cl = ch;
+    }
+    return s;
+}
+
+template <class Tout, int FOOT, int TU, int SPLIT>
+__global__ __launch_bounds__(kBlock, 6) void cbct_proj_ring_kernel(Geom<float> g, const View<float>* __restrict__ views,
+                                                                   int vstride, int nviews, const float* __restrict__ vol,
+                                                                   const uint8_t* __restrict__ map,
+                                                                   Tout* __restrict__ proj, int exhaustive) {
+    using T = float;
+    static_assert(SPLIT == 1 || SPLIT == kWavesPerBlock, "SPLIT is 1 or the block's wave count");
+    __shared__ T ring[kWavesPerBlock][kRing * kSlotW];
+    // each lane's row of the tile, [cell][lane]: one word per bank, one writer per word
+    __shared__ T accs[kWavesPerBlock][TU][kWave];
+    const int nut = (g.nu + TU - 1) / TU;
+    const int nvc = (g.nv + kWave - 1) / kWave;
+    const int items = nviews * nut * nvc;
+    const int bid = blockIdx.x;
+    const int wv = (int)(threadIdx.x / kWave);
+    const int item = SPLIT == 1 ? bid * kWavesPerBlock + wv : bid;
+    const int share = SPLIT == 1 ? 0 : wv;
+    if (item >= items) return;  // SPLIT > 1: the whole block leaves together
+    const int vc = item % nvc;
+    const int b = (item / nvc) % nviews;
+    // longest work first (tiles in order of their distance from the central ray's tile)
+    const int ut = tile_by_rank(item / (nvc * nviews), min(max((int)floor(-g.wu / T(TU)), 0), nut - 1), nut);
+    const int lane = lane_id();
+    const int loff = lane * 4;
+    const int iv_a = vc * kWave;
+    const int iu_a = ut * TU;
+    const int iu_b = min(iu_a + TU, g.nu) - 1;
+    const View<T> v = views[(size_t)b * vstride];
+    const bool row_major = fabs_t(v.c) >= fabs_t(v.s);
+    const int m_end = row_major ? g.ny : g.nx;
+    const T kL = u_to_slope(g, T(iu_a) - T(0.5));
+    const T kR = u_to_slope(g, T(iu_b) + T(0.5));
+    const int iv = iv_a + lane;
+    const int iv_last = min(iv_a + kWave, g.nv) - 1;
+    T* zr = ring[wv];
+    // word 0 and the pads of every slot stay zero; the loads write words 1 .. 192
+    for (int i = lane; i < kRing * kSlotW; i += kWave) zr[i] = T(0);
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // zeros in place before any load lands
+    const unsigned ring_lds = lds_addr(zr);
+    const unsigned slot_bytes = kSlotW * 4;
+    const unsigned ring_end = ring_lds + kRing * slot_bytes;
+    const T wzdz = v.wz * g.dz;
+    const T fiv_lo = T(iv) - T(0.5);  // lower edge of this lane's row
+    const T lzd0 = T(lane) * g.dz, lzd1 = T(kWave + lane) * g.dz, lzd2 = T(2 * kWave + lane) * g.dz;
+    const rsrc_words empty_rs = dma_rsrc(vol, 0);  // a group of loads that only writes zeros
+
+    T* acc = &accs[wv][0][lane];
+#pragma unroll
+    for (int t = 0; t < TU; ++t) acc[t * kWave] = T(0);
+
+    // ring position of the next group to issue (byte address of its slot's word 1)
+    unsigned isl = ring_lds + 4;
+    auto advance = [&](unsigned a) { return a + slot_bytes >= ring_end ? a + slot_bytes - kRing * slot_bytes : a + slot_bytes; };
+
+    for (int m0 = share * kWave; m0 < m_end; m0 += SPLIT * kWave) {
+        const int m = m0 + lane;
+        int lo = 0, hi = -1;
+        if (m < m_end) {
+            if (!strip_range(g, v, row_major, m, kL, kR, exhaustive != 0, lo, hi)) hi = lo - 1;
+        }
+        const int cnt = hi - lo + 1 > 0 ? hi - lo + 1 : 0;
+        const int incl = wave_incl_scan(cnt);
+        const int excl = incl - cnt;
+        const int total = rdl_i(incl, kWave - 1);
+        for (int k0 = 0; k0 < total; k0 += kWave) {
+            const int k = k0 + lane;
+            int j = 0;
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1) {
+                const int e = __shfl(excl, j + step, 64);
+                if (e <= k) j += step;
+            }
+            const int lo_j = __shfl(lo, j, 64);
+            const int ex_j = __shfl(excl, j, 64);
+            const int minor = lo_j + (k - ex_j);
+            const int ix = row_major ? minor : m0 + j;
+            const int iy = row_major ? m0 + j : minor;
+
+            // exact footprint of the candidate column, its weights on this tile and the
+            // quantities the walk needs
+            bool ok = k < total;
+            RingCol cp{};
+            if (ok && map != nullptr) ok = map[iy * g.nx + ix] != 0;
+            if (ok) {
+                T p0, p1, p2, p3, amp, r, d;
+                int ilo = 0, ihi = -1, iz0 = 0, iz1 = 0;
+                ok = col_corners<T, FOOT>(g, v, ix, iy, p0, p1, p2, p3, ilo, ihi, amp, r, d);
+                ok = ok && ihi >= iu_a && ilo <= iu_b;
+                T zinc = T(1), vstart = T(0);
+                if (ok) ok = v_window(g, d, v.wz, zinc, vstart, iz0, iz1) && iz0 < iz1 && zinc > T(0);
+                int za = 0, zb = -1;
+                if (ok) {
+                    // voxel range needed by this wave's rows (exact ranges; a voxel left out at
+                    // the rounding edge has a weight of the order of the rounding)
+                    const T rz = frcp(zinc);
+                    za = max((int)ceil((T(iv_a) - T(0.5) - vstart) * rz - T(1)), iz0);
+                    zb = min((int)ceil((T(iv_last) + T(0.5) - vstart) * rz) - 1, iz1 - 1);
+                    ok = za <= zb;
+                }
+                if (ok) {
+                    const T ampr = amp * frcp(r);
+                    const int c0 = max(ilo, iu_a);
+                    const int c1 = min(ihi, iu_b);
+                    cp.w0 = u_weight<T, FOOT>(c0 + 0, ilo, ihi, p0, p1, p2, p3) * ampr;
+                    cp.w1 = u_weight<T, FOOT>(c0 + 1, ilo, ihi, p0, p1, p2, p3) * ampr;
+                    cp.w2 = u_weight<T, FOOT>(c0 + 2, ilo, ihi, p0, p1, p2, p3) * ampr;
+                    cp.w3 = u_weight<T, FOOT>(c0 + 3, ilo, ihi, p0, p1, p2, p3) * ampr;
+                    const int span = zb - za;
+                    const unsigned long long a =
+                        (unsigned long long)(vol + ((size_t)(iy * g.nx + ix) * g.nz + za));
+                    cp.alo = (int)(unsigned)(a & 0xffffffffull);
+                    cp.ahi = (int)(unsigned)((a >> 32) & 0xffffull);
+                    cp.nbytes = (span + 1) * 4;
+                    cp.meta = gather_class(zinc) | ((span >> 6) << 2) | ((c1 - c0) << 4) | ((c0 - iu_a) << 9);
+                    cp.zbase = T(za) * g.dz + wzdz;
+                    cp.r2 = r * r;
+                    cp.zinc = zinc;
+                    cp.vst = vstart + T(za) * zinc;
+                    cp.col = iy * g.nx + ix;
+                    cp.cells = ilo | (ihi << 16);
+                }
+            }
+            // survivors to lanes 0 .. n-1 in lane order, the others after them
+            const unsigned long long mask = __ballot(ok);
+            const int n = __popcll(mask);
+            if (n == 0) continue;
+            {
+                const unsigned long long below = mask & ((1ull << lane) - 1ull);
+                const int rank = __popcll(below);
+                const int dst = (ok ? rank : n + (lane - rank)) * 4;
+                cp.alo = permute_i(dst, cp.alo);
+                cp.ahi = permute_i(dst, cp.ahi);
+                cp.nbytes = permute_i(dst, cp.nbytes);
+                cp.meta = permute_i(dst, cp.meta);
+                cp.zbase = permute_f(dst, cp.zbase);
+                cp.r2 = permute_f(dst, cp.r2);
+                cp.zinc = permute_f(dst, cp.zinc);
+                cp.vst = permute_f(dst, cp.vst);
+                cp.w0 = permute_f(dst, cp.w0);
+                cp.w1 = permute_f(dst, cp.w1);
+                cp.w2 = permute_f(dst, cp.w2);
+                cp.w3 = permute_f(dst, cp.w3);
+                cp.col = permute_i(dst, cp.col);
+                cp.cells = permute_i(dst, cp.cells);
+            }
+            // group c's loads into the next ring slot; past the last survivor, a group that
+            // writes zeros, so that every column has exactly two groups issued after it
+            auto issue = [&](int c) {
+                rsrc_words rs = empty_rs;
+                if (c < n) {
+                    rs.x = rdl_i(cp.alo, c);
+                    rs.y = rdl_i(cp.ahi, c);
+                    rs.z = rdl_i(cp.nbytes, c);
+                }
+                ring_load(rs, isl, loff);
+                isl = advance(isl);
+            };
+            issue(0);
+            issue(1);
+            unsigned psl = isl;  // slot of column 0: two slots behind the issue position
+            psl = advance(psl);
+            for (int c = 0; c < n; ++c) {
+                issue(c + 2);
+                wait_vm<6>();  // column c's group has landed
+                T* zs = (T*)((char*)zr + (psl - 4 - ring_lds));  // word 0 of the slot
+                psl = advance(psl);
+                const int meta = rdl_i(cp.meta, c);
+                const int cls = meta & 3, ns = (meta >> 2) & 3, nc = (meta >> 4) & 31, c0 = meta >> 9;
+                const T r2 = rdl(cp.r2, c), zinc = rdl(cp.zinc, c), vst = rdl(cp.vst, c), zbase = rdl(cp.zbase, c);
+                const T rz = frcp(zinc);
+                // scale the loaded voxels in place by sqrt(r^2 + zc^2) (the cone factor times
+                // r); words past the range are zero
+                {
+                    const T zc = zbase + lzd0;
+                    zs[1 + lane] = fsqrt(fmaf(zc, zc, r2)) * zs[1 + lane];
+                }
+                if (ns >= 1) {
+                    const T zc = zbase + lzd1;
+                    zs[1 + kWave + lane] = fsqrt(fmaf(zc, zc, r2)) * zs[1 + kWave + lane];
+                }
+                if (ns >= 2) {
+                    const T zc = zbase + lzd2;
+                    zs[1 + 2 * kWave + lane] = fsqrt(fmaf(zc, zc, r2)) * zs[1 + 2 * kWave + lane];
+                }
+                __builtin_amdgcn_wave_barrier();
+                // row iv against the column's voxels (relative to voxel za): voxel j covers
+                // rows [vst + j zinc, vst + (j + 1) zinc]; at most ceil(1/zinc) + 1 of them
+                // overlap a row, so thick voxels take a fixed unrolled count
+                const T tt = fiv_lo - vst;
+                const int ja = (int)ceil(tt * rz - T(1));
+                const int span = (rdl_i(cp.nbytes, c) >> 2) - 1;
+                T q;
+                if (cls == 1) {
+                    q = ring_gather<3>(zs, span, ja, tt, zinc);
+                } else if (cls == 2) {
+                    q = ring_gather<4>(zs, span, ja, tt, zinc);
+                } else if (cls == 0) {
+                    q = ring_gather<2>(zs, span, ja, tt, zinc);
+                } else {
+                    // footprints that may reach the 16-cell cap, and thin voxels
+                    q = T(0);
+                    if (iv <= iv_last) {
+                        const int jb = min((int)ceil((T(iv) + T(0.5) - vst) * rz) - 1, span);
+                        for (int jj = max(ja, 0); jj <= jb; ++jj) {
+                            const T vmin = vst + T(jj) * zinc;
+                            q += (zinc > T(14) ? dd_overlap16(iv, vmin, vmin + zinc, g.nv)
+                                               : dd_overlap(iv, vmin, vmin + zinc)) * zs[jj + 1];
+                        }
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                // tile cells c0 .. c0 + nc; the same on every lane, so each weighted add
+                // touches one LDS word per lane and bank
+                T* pa = acc + c0 * kWave;
+                pa[0] += rdl(cp.w0, c) * q;
+                if (nc >= 1) pa[kWave] += rdl(cp.w1, c) * q;
+                if (nc >= 2) pa[2 * kWave] += rdl(cp.w2, c) * q;
+                if (nc >= 3) pa[3 * kWave] += rdl(cp.w3, c) * q;
+                if (nc >= 4) {
+                    // wider than four tile cells (rare): the corners again, from the column
+                    const int col = rdl_i(cp.col, c), cells = rdl_i(cp.cells, c);
+                    const int ilo = cells & 0xffff, ihi = cells >> 16;
+                    T p0, p1, p2, p3, amp, r, d;
+                    int jlo, jhi;
+                    col_corners<T, FOOT>(g, v, col % g.nx, col / g.nx, p0, p1, p2, p3, jlo, jhi, amp, r, d);
+                    const T ampr = amp * frcp(r);
+                    for (int cc = c0 + 4; cc <= c0 + nc; ++cc)
+                        acc[cc * kWave] += u_weight<T, FOOT>(iu_a + cc, ilo, ihi, p0, p1, p2, p3) * ampr * q;
+                }
+            }
+        }
+    }
+    wait_vm<0>();  // the trailing zero groups have landed before the ring is released
+    if (SPLIT > 1) {
+        // fixed-order combine of the shares by wave 0: ((s0 + s1) + s2) + s3
+        __syncthreads();
+        if (wv > 0) return;
+    }
+    Tout* out = proj + (size_t)b * g.nu * g.nv;
     if (iv <= iv_last) {
 #pragma unroll
         for (int t = 0; t < TU; ++t) {
@@ -881,6 +1264,14 @@ constexpr int kTU = 16;  // detector columns per forward-projection wave
 template <class Tin, class Tout, class T, int FOOT, int RPL, int SPLIT>
 static void launch_proj(unsigned blocks, hipStream_t st, const Geom<T>& g, const View<T>* views, int vstride,
                         int nviews, const void* vol, const uint8_t* map, void* proj, int exhaustive) {
+    // FP32 volumes whose columns fit the three 64-voxel slots take the ring form
+    if constexpr (std::is_same<Tin, float>::value && std::is_same<T, float>::value) {
+        if (g.nz <= kFastSpan) {
+            hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU, SPLIT>), dim3(blocks), dim3(kBlock), 0, st, g,
+                               views, vstride, nviews, (const float*)vol, map, (Tout*)proj, exhaustive);
+            return;
+        }
+    }
     hipLaunchKernelGGL((cbct_proj_kernel<Tin, Tout, T, FOOT, kTU, RPL, SPLIT>), dim3(blocks), dim3(kBlock), 0, st, g,
                        views, vstride, nviews, (const Tin*)vol, map, (Tout*)proj, exhaustive);
 }
@@ -903,6 +1294,7 @@ static hipError_t proj3(const Geom<T>& g, const View<T>* views, int vstride, int
                         const uint8_t* map, void* proj, int foot, int exhaustive, hipStream_t st) {
     if (nviews <= 0) return hipSuccess;
     if (g.nu >= 32768) return hipErrorNotSupported;  // footprint cells are packed in 16 bits
+    if (g.nz > kMaxNz) return hipErrorNotSupported;   // voxel ranges are packed in kZBits bits
     if (foot == FOOT_DD)
         proj3_foot<Tin, Tout, T, FOOT_DD>(g, views, vstride, nviews, vol, map, proj, exhaustive, st);
     else

@@ -191,6 +191,27 @@ def test_cbct_plan_vs_oracle(case):
         assert rel_l2(got, want) <= _tol(op, g, sino, mp), (op, rel_l2(got, want))
 
 
+@pytest.mark.parametrize("seed", [1, 4, 9])
+def test_cbct_register_pipeline_vs_oracle(seed):
+    """FP16 volumes take the register-pipeline forward kernel (FP32 volumes of at most 192
+    slices take the LDS ring): against the oracle on the FP16-rounded volume"""
+    from oracle import oracle as O
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    case = next(c for c in CONE_CASES if c["seed"] == seed)
+    g, vol, sino, mp = _cone_case(**case)
+    say("register pipeline case", case)
+    v16 = vol.astype(np.float16).astype(np.float32)
+    P = Projector(geometry_from_g3(g, mp))
+    d_vol = P.upload_volume(v16, dtype="f16")
+    out = DeviceArray(P.n_sino(), "f32")
+    for kind in ("dd", "tf"):
+        P.proj(d_vol, out, f"proj,{kind}")
+        got = P.download_sino(out)
+        want = O.cbct(f"proj,{kind}", g, v16, mp)
+        assert rel_l2(got, want) <= _tol(f"proj,{kind}", g, v16, mp), (kind, rel_l2(got, want))
+
+
 def test_cbct_unsplit_launch_vs_oracle():
     """>= 16384 work items: each wave walks its whole strip (no LDS combine)"""
     from oracle import oracle as O
