@@ -401,7 +401,9 @@ int ctsim_plan_back(const ctsim_plan* p, const char* op, int32_t view0, int32_t 
         return fail(CTSIM_EUNSUP, "Error: dtype combination (sino %d, img %d) not supported for compute %d.", sino_dt,
                     img_dt, p->compute);
     if (e == hipErrorNotSupported)
-        return fail(CTSIM_EUNSUP, "Error: 3D backprojection supports up to 512 slices (nz = %d).", p->g.nz);
+        return fail(CTSIM_EUNSUP,
+                    "Error: 3D backprojection supports up to 512 slices (nz = %d) and views of under 2^31 bytes.",
+                    p->g.nz);
     HIP_TRY(e, "backprojection launch");
     return CTSIM_OK;
 }

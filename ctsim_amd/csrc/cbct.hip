@@ -798,13 +798,14 @@ template <class T>
 struct __align__(16) ViewCol {  // one view's footprint on one image column (80 bytes in FP32)
     int cells;                  // first footprint cell ilo | cell count nw << 16
     int rows;                   // rows covered r0 | r1 << 16
-    int zr;                     // voxel range iz0 | iz1 << 16
+    int cls;                    // gather class: rows per voxel 2, 3, 4 (0, 1, 2) or the capped loop (3)
     int vb;                     // view index within the selection
     T ampr, r2;                 // amp / r and r^2: the voxel factor is sqrt(r^2 + zc^2) amp / r
     T zinc, vs5, wzdz;          // v of voxel z spans [vs5 + z zinc, vs5 + (z + 1) zinc] (+1/2 folded);
                                 // zc = z dz + wzdz
     T vstart;                   // the unshifted start, for footprints that may reach the cap
-    T w0, w1, w2, w3, p0, p1, p2, p3;
+    T w0, w1, w2, w3;           // weights of the first four footprint cells, times amp / r
+    T p0, p1, p2, p3;           // footprint corners (cells past the fourth)
 };
 
 // Staged rows of a window [rw, rwe] sit at sw[1 + row - rw]; sw[0] and the kRowPad
@@ -827,13 +828,15 @@ inline size_t back_lds_bytes(int rowwin) {
     return (size_t)kWavesPerBlock * (size_t)back_rowbuf(rowwin) * sizeof(T);
 }
 
+// swr points at the word of row 0 (swr[row] = sw[1 + row - rw]); flo, fhi = rw - 1, rwe + 1
+// as T.  The clamp of floor(a) to [rw - 1, rwe + 1] is taken in T (one med3 in FP32).
 template <class T, int K>
-__device__ __forceinline__ T gather_rows(const T* sw, int rw, int rwe, T a, T b) {
+__device__ __forceinline__ T gather_rows(const T* swr, T flo, T fhi, T a, T b) {
     static_assert(K - 1 <= kRowPad, "pad covers the rows past the window");
-    const int ra = min(max((int)floor(a), rw - 1), rwe + 1);
-    const T* pr = sw + (ra - rw + 1);
-    T f = T(ra) + T(1);
-    T s = ((b < f ? b : f) - a) * pr[0];
+    const T fa = min(max(floor(a), flo), fhi);
+    const T* pr = swr + (int)fa;
+    T f = fa + T(1);
+    T s = (fmin(b, f) - a) * pr[0];
 #pragma unroll
     for (int k = 1; k < K; ++k) {
         s += clamp01(b - f) * pr[k];
@@ -842,7 +845,9 @@ __device__ __forceinline__ T gather_rows(const T* sw, int rw, int rwe, T a, T b)
     return s;
 }
 
-template <class Tin, class Tout, class T, int FOOT, int NZL>
+// SHORT: detectors of at most 64 rows, where every view takes the one-pass row staging
+// (the windowed form is compiled out, which keeps the loop's registers in place)
+template <class Tin, class Tout, class T, int FOOT, int NZL, bool SHORT>
 __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View<T>* __restrict__ views, int vstride,
                                                            int nviews, const Tin* __restrict__ proj,
                                                            const uint8_t* __restrict__ map,
@@ -913,12 +918,12 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             if (ok) {
                 vc.cells = ilo | ((ihi - ilo + 1) << 16);
                 vc.rows = r0 | (r1 << 16);
-                vc.zr = iz0 | (iz1 << 16);
-                vc.w0 = u_weight<T, FOOT>(ilo + 0, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3);
-                vc.w1 = u_weight<T, FOOT>(ilo + 1, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3);
-                vc.w2 = u_weight<T, FOOT>(ilo + 2, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3);
-                vc.w3 = u_weight<T, FOOT>(ilo + 3, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3);
+                vc.cls = vc.zinc <= T(1) ? 0 : (vc.zinc <= T(2) ? 1 : (vc.zinc <= T(3) ? 2 : 3));
                 vc.ampr = cg.amp * frcp(cg.r);
+                vc.w0 = u_weight<T, FOOT>(ilo + 0, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3) * vc.ampr;
+                vc.w1 = u_weight<T, FOOT>(ilo + 1, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3) * vc.ampr;
+                vc.w2 = u_weight<T, FOOT>(ilo + 2, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3) * vc.ampr;
+                vc.w3 = u_weight<T, FOOT>(ilo + 3, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3) * vc.ampr;
                 vc.r2 = cg.r * cg.r;
                 vc.vs5 = vstart + T(0.5);
                 vc.vstart = vstart;
@@ -933,25 +938,33 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
         int cur = mask ? __ffsll((long long)mask) - 1 : -1;
         if (cur >= 0) mask &= mask - 1;
         Tin pre0{}, pre1{}, pre2{}, pre3{};
+        // the four footprint columns of view vb, rows r0 + lane (at most r1): loads through a
+        // resource over the view's sinogram, so that columns past the detector's last read
+        // zero; all four are loaded (loads under a condition would be waited for at once),
+        // the columns past the footprint are not used
+        const int view_bytes = g.nu * g.nv * (int)sizeof(Tin);
+        auto load_rows = [&](int vb, int cells, int rows) {
+            const int r0 = rows & 0xffff, r1 = rows >> 16;
+            const auto rs = buf_rsrc(proj + (size_t)vb * g.nu * g.nv, view_bytes);
+            const int off = ((cells & 0xffff) * g.nv + min(r0 + lane, r1)) * (int)sizeof(Tin);
+            const int step = g.nv * (int)sizeof(Tin);
+            pre0 = buf_ld<Tin>(rs, off);
+            pre1 = buf_ld<Tin>(rs, off + step);
+            pre2 = buf_ld<Tin>(rs, off + 2 * step);
+            pre3 = buf_ld<Tin>(rs, off + 3 * step);
+        };
         if (cur >= 0) {
             const ViewCol<T>& c = vw[cur];
-            const int cells = uni(c.cells), rows = uni(c.rows);
-            const int nw = cells >> 16, r0 = rows & 0xffff, r1 = rows >> 16;
-            const Tin* pr = proj + ((size_t)(b0 + cur) * g.nu + (cells & 0xffff)) * g.nv + min(r0 + lane, r1);
-            pre0 = pr[0];
-            if (nw > 1) pre1 = pr[g.nv];
-            if (nw > 2) pre2 = pr[2 * g.nv];
-            if (nw > 3) pre3 = pr[3 * g.nv];
+            load_rows(b0 + cur, uni(c.cells), uni(c.rows));
         }
         while (cur >= 0) {
             const ViewCol<T>& c = vw[cur];
-            const int cells = uni(c.cells), rows = uni(c.rows), zr = uni(c.zr), vb = b0 + cur;
+            const int cells = uni(c.cells), rows = uni(c.rows), cls = uni(c.cls), vb = b0 + cur;
             const int ilo = cells & 0xffff, nw = cells >> 16, r0 = rows & 0xffff, r1 = rows >> 16;
-            const int iz0 = zr & 0xffff, iz1 = zr >> 16;
-            const T w0 = uni(c.w0), w1 = uni(c.w1), w2 = uni(c.w2), w3 = uni(c.w3);
+            const T w0 = uni(c.w0), w1 = uni(c.w1), w2 = uni(c.w2), w3 = uni(c.w3);  // amp / r folded in
             const T zinc = uni(c.zinc), vs5 = uni(c.vs5);
-            const T ampr = uni(c.ampr), r2 = uni(c.r2);  // ampv * amp = sqrt(r^2 + zc^2) * amp / r
-            const T wzdz = uni(c.wzdz);                  // zc = z dz + wz dz
+            const T r2 = uni(c.r2);      // ampv * r = sqrt(r^2 + zc^2)
+            const T wzdz = uni(c.wzdz);  // zc = z dz + wz dz
             const Tin* pv = proj + ((size_t)vb * g.nu + ilo) * g.nv;
             // the next view's first row chunk starts loading once this one is staged
             auto next = [&]() {
@@ -959,34 +972,31 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 if (nxt >= 0) {
                     mask &= mask - 1;
                     const ViewCol<T>& cn = vw[nxt];
-                    const int cellsn = uni(cn.cells), rowsn = uni(cn.rows);
-                    const int nwn = cellsn >> 16, r0n = rowsn & 0xffff, r1n = rowsn >> 16;
-                    const Tin* pr =
-                        proj + ((size_t)(b0 + nxt) * g.nu + (cellsn & 0xffff)) * g.nv + min(r0n + lane, r1n);
-                    pre0 = pr[0];
-                    if (nwn > 1) pre1 = pr[g.nv];
-                    if (nwn > 2) pre2 = pr[2 * g.nv];
-                    if (nwn > 3) pre3 = pr[3 * g.nv];
+                    load_rows(b0 + nxt, uni(cn.cells), uni(cn.rows));
                 }
                 cur = nxt;
             };
-            // the column's voxels gather the staged rows [rw, rwe]
+            // the column's voxels gather the staged rows [rw, rwe].  Voxels outside the
+            // column's range [iz0, iz1) need no test: their edges lie more than half a row
+            // off the detector, so the clamped row index reads the zero words before or
+            // after the staged rows (or overlaps of zero), as do voxels past nz, whose sums
+            // are never stored.
             auto gather = [&](int rw, int rwe) {
-                const T zu = uni(zinc);
+                const T* swr = sw + 1 - rw;
+                const T flo = T(rw - 1), fhi = T(rwe + 1);
 #pragma unroll
                 for (int cz = 0; cz < NZL; ++cz) {
                     const int iz = cz * kWave + lane;
-                    if (iz < iz0 || iz >= iz1) continue;
                     // voxel edges in row units, +1/2 folded into vs5 (within an ulp of the
                     // reference's vstart + iz zinc + 1/2)
-                    const T a = fma_t(T(iz), zinc, vs5), bb = fma_t(T(iz + 1), zinc, vs5);
+                    const T a = fma_t(T(iz), zinc, vs5), bb = a + zinc;
                     T s;
-                    if (zu <= T(1)) {
-                        s = gather_rows<T, 2>(sw, rw, rwe, a, bb);
-                    } else if (zu <= T(2)) {
-                        s = gather_rows<T, 3>(sw, rw, rwe, a, bb);
-                    } else if (zu <= T(3)) {
-                        s = gather_rows<T, 4>(sw, rw, rwe, a, bb);
+                    if (cls == 0) {
+                        s = gather_rows<T, 2>(swr, flo, fhi, a, bb);
+                    } else if (cls == 1) {
+                        s = gather_rows<T, 3>(swr, flo, fhi, a, bb);
+                    } else if (cls == 2) {
+                        s = gather_rows<T, 4>(swr, flo, fhi, a, bb);
                     } else {
                         // footprints that may reach the 16-cell cap: the reference's edges
                         const T vmin = uni(c.vstart) + T(iz) * zinc;
@@ -1000,12 +1010,12 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                         }
                     }
                     const T zcm = T(iz) * g.dz + wzdz;
-                    img[cz] += ampr * fsqrt(r2 + zcm * zcm) * s;
+                    img[cz] += fsqrt(fma_t(zcm, zcm, r2)) * s;
                 }
             };
             // (short columns only: with more voxels per lane the second copy of the gather
             // costs the longer columns a wave per SIMD)
-            if (NZL <= 2 && (r1 - r0 + 1 + kRowPad <= kWave || g.nv <= kWave)) {
+            if (SHORT || (NZL <= 2 && r1 - r0 + 1 + kRowPad <= kWave)) {
                 // one row per lane: lanes past r1 write the zero words the gather reads past
                 // the rows (past the 64 slots they were zeroed at the start)
                 T s = w0 * T(to_f(pre0));
@@ -1013,7 +1023,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 if (nw > 2) s += w2 * T(to_f(pre2));
                 if (nw > 3) s += w3 * T(to_f(pre3));
                 for (int k = 4; k < nw; ++k)
-                    s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) *
+                    s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) * c.ampr *
                          ld<T>(pv + (size_t)k * g.nv + min(r0 + lane, r1));
                 sw[1 + lane] = r0 + lane <= r1 ? s : T(0);
                 __builtin_amdgcn_wave_barrier();
@@ -1033,7 +1043,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                         if (nw > 2) s += w2 * T(to_f(pre2));
                         if (nw > 3) s += w3 * T(to_f(pre3));
                         for (int k = 4; k < nw; ++k)
-                            s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) *
+                            s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) * c.ampr *
                                  ld<T>(pv + (size_t)k * g.nv + rfirst);
                         sw[rfirst - rw + 1] = s;
                     }
@@ -1047,7 +1057,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                     if (nw > 2) s += w2 * ld<T>(pr + 2 * g.nv);
                     if (nw > 3) s += w3 * ld<T>(pr + 3 * g.nv);
                     for (int k = 4; k < nw; ++k)
-                        s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) *
+                        s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) * c.ampr *
                              ld<T>(pr + (size_t)k * g.nv);
                     sw[row - rw + 1] = s;
                 }
@@ -1326,6 +1336,20 @@ PdLimits<float> pd_limits<float>(int nu, int nv) {
     return PdLimits<float>{float_at_least(d.lo), float_at_most(d.u_hi), float_at_most(d.v_hi)};
 }
 
+template <class Tin, class Tout, class T, int NZL, bool SHORT>
+static void launch_back_dd_tf(unsigned blocks, hipStream_t st, const Geom<T>& g, const View<T>* views, int vstride,
+                              int nviews, const void* proj, const uint8_t* map, void* vol, int foot, T scale,
+                              int accumulate, int rowwin) {
+    if (foot == FOOT_DD)
+        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_DD, NZL, SHORT>), dim3(blocks), dim3(kBlock),
+                           back_lds_bytes<T>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
+                           (Tout*)vol, scale, accumulate, rowwin);
+    else
+        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_TF, NZL, SHORT>), dim3(blocks), dim3(kBlock),
+                           back_lds_bytes<T>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
+                           (Tout*)vol, scale, accumulate, rowwin);
+}
+
 template <class Tin, class Tout, class T, int NZL>
 static void launch_back(unsigned blocks, hipStream_t st, const Geom<T>& g, const View<T>* views, int vstride,
                         int nviews, const void* proj, const uint8_t* map, void* vol, int foot, int pd_mode, T scale,
@@ -1339,14 +1363,12 @@ static void launch_back(unsigned blocks, hipStream_t st, const Geom<T>& g, const
         hipLaunchKernelGGL((cbct_back_pd_kernel<Tin, Tout, T, NZL, 0>), dim3(blocks), dim3(kBlock), 0, st, g, views,
                            vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate,
                            pd_limits<T>(g.nu, g.nv));
-    else if (foot == FOOT_DD)
-        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_DD, NZL>), dim3(blocks), dim3(kBlock),
-                           back_lds_bytes<T>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
-                           (Tout*)vol, scale, accumulate, rowwin);
+    else if (g.nv <= kWave)
+        launch_back_dd_tf<Tin, Tout, T, NZL, true>(blocks, st, g, views, vstride, nviews, proj, map, vol, foot, scale,
+                                                   accumulate, rowwin);
     else
-        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_TF, NZL>), dim3(blocks), dim3(kBlock),
-                           back_lds_bytes<T>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
-                           (Tout*)vol, scale, accumulate, rowwin);
+        launch_back_dd_tf<Tin, Tout, T, NZL, false>(blocks, st, g, views, vstride, nviews, proj, map, vol, foot, scale,
+                                                    accumulate, rowwin);
 }
 
 template <class Tin, class Tout, class T>
@@ -1359,6 +1381,8 @@ static hipError_t back3(const Geom<T>& g, const View<T>* views, int vstride, int
     if (cols == 0 || blocks == 0) return hipSuccess;
     if (foot == 2 && (g.nu < 2 || g.nv < 2)) return hipErrorInvalidValue;  // bilinear needs 2x2 cells
     if (g.nz > 8 * kWave) return hipErrorNotSupported;                      // voxel columns up to 512
+    if ((long long)g.nu * g.nv * (long long)sizeof(Tin) > 0x7fffffffLL)
+        return hipErrorNotSupported;  // a view's sinogram is addressed with 32-bit byte offsets
     const int nzl = (g.nz + kWave - 1) / kWave;
     if (nzl <= 1) launch_back<Tin, Tout, T, 1>(blocks, st, g, views, vstride, nviews, proj, map, vol, foot, pd_mode, scale, accumulate);
     else if (nzl <= 2) launch_back<Tin, Tout, T, 2>(blocks, st, g, views, vstride, nviews, proj, map, vol, foot, pd_mode, scale, accumulate);
