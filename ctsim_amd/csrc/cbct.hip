@@ -828,6 +828,20 @@ inline size_t back_lds_bytes(int rowwin) {
     return (size_t)kWavesPerBlock * (size_t)back_rowbuf(rowwin) * sizeof(T);
 }
 
+// Voxels of at most one row (zinc <= 1) meet rows ra = floor(a) and ra + 1:
+// s = (zinc - e) p[ra] + e p[ra + 1] with e = max(b - (ra + 1), 0) = max(frac(a) + zinc - 1, 0),
+// the same two DD overlaps as below.  a is clamped to [rw - 1, rwe + 1] first: a clamped
+// value is an integer, so e = 0 and the voxel reads only a zero word.
+template <class T>
+__device__ __forceinline__ T gather_rows2(const T* swr, T flo, T fhi, T a, T zinc, T zm1) {
+    const T ac = fmin(fmax(a, flo), fhi);
+    const T fa = floor(ac);
+    const T* pr = swr + (int)fa;
+    const T e = clamp01((ac - fa) + zm1);
+    const T p0 = pr[0], p1 = pr[1];
+    return fma_t(e, p1 - p0, zinc * p0);
+}
+
 // swr points at the word of row 0 (swr[row] = sw[1 + row - rw]); flo, fhi = rw - 1, rwe + 1
 // as T.  The clamp of floor(a) to [rw - 1, rwe + 1] is taken in T (one med3 in FP32).
 template <class T, int K>
@@ -942,11 +956,13 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
         // resource over the view's sinogram, so that columns past the detector's last read
         // zero; all four are loaded (loads under a condition would be waited for at once),
         // the columns past the footprint are not used
-        const int view_bytes = g.nu * g.nv * (int)sizeof(Tin);
         auto load_rows = [&](int vb, int cells, int rows) {
-            const int r0 = rows & 0xffff, r1 = rows >> 16;
-            const auto rs = buf_rsrc(proj + (size_t)vb * g.nu * g.nv, view_bytes);
-            const int off = ((cells & 0xffff) * g.nv + min(r0 + lane, r1)) * (int)sizeof(Tin);
+            // a resource over the view's sinogram up to the end of the footprint's last
+            // column, so that the loads of columns past the footprint (and past the
+            // detector) read zero
+            const int r0 = rows & 0xffff, r1 = rows >> 16, ilo = cells & 0xffff, nw = cells >> 16;
+            const auto rs = buf_rsrc(proj + (size_t)vb * g.nu * g.nv, (ilo + nw) * g.nv * (int)sizeof(Tin));
+            const int off = (ilo * g.nv + min(r0 + lane, r1)) * (int)sizeof(Tin);
             const int step = g.nv * (int)sizeof(Tin);
             pre0 = buf_ld<Tin>(rs, off);
             pre1 = buf_ld<Tin>(rs, off + step);
@@ -983,7 +999,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             // are never stored.
             auto gather = [&](int rw, int rwe) {
                 const T* swr = sw + 1 - rw;
-                const T flo = T(rw - 1), fhi = T(rwe + 1);
+                const T flo = T(rw - 1), fhi = T(rwe + 1), zm1 = zinc - T(1);
 #pragma unroll
                 for (int cz = 0; cz < NZL; ++cz) {
                     const int iz = cz * kWave + lane;
@@ -992,7 +1008,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                     const T a = fma_t(T(iz), zinc, vs5), bb = a + zinc;
                     T s;
                     if (cls == 0) {
-                        s = gather_rows<T, 2>(swr, flo, fhi, a, bb);
+                        s = gather_rows2<T>(swr, flo, fhi, a, zinc, zm1);
                     } else if (cls == 1) {
                         s = gather_rows<T, 3>(swr, flo, fhi, a, bb);
                     } else if (cls == 2) {
@@ -1018,10 +1034,8 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             if (SHORT || (NZL <= 2 && r1 - r0 + 1 + kRowPad <= kWave)) {
                 // one row per lane: lanes past r1 write the zero words the gather reads past
                 // the rows (past the 64 slots they were zeroed at the start)
-                T s = w0 * T(to_f(pre0));
-                if (nw > 1) s += w1 * T(to_f(pre1));
-                if (nw > 2) s += w2 * T(to_f(pre2));
-                if (nw > 3) s += w3 * T(to_f(pre3));
+                // columns past the footprint were loaded as zeros (and weigh zero)
+                T s = w0 * T(to_f(pre0)) + w1 * T(to_f(pre1)) + w2 * T(to_f(pre2)) + w3 * T(to_f(pre3));
                 for (int k = 4; k < nw; ++k)
                     s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) * c.ampr *
                          ld<T>(pv + (size_t)k * g.nv + min(r0 + lane, r1));
@@ -1038,10 +1052,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 int rfirst = rw + lane;
                 if (rw == r0) {
                     if (rfirst <= rwe) {
-                        T s = w0 * T(to_f(pre0));
-                        if (nw > 1) s += w1 * T(to_f(pre1));
-                        if (nw > 2) s += w2 * T(to_f(pre2));
-                        if (nw > 3) s += w3 * T(to_f(pre3));
+                        T s = w0 * T(to_f(pre0)) + w1 * T(to_f(pre1)) + w2 * T(to_f(pre2)) + w3 * T(to_f(pre3));
                         for (int k = 4; k < nw; ++k)
                             s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) * c.ampr *
                                  ld<T>(pv + (size_t)k * g.nv + rfirst);
