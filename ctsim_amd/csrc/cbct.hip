@@ -539,8 +539,10 @@ __global__ __launch_bounds__(kBlock, 6) void cbct_proj_ring_kernel(Geom<float> g
     using T = float;
     static_assert(SPLIT == 1 || SPLIT == kWavesPerBlock, "SPLIT is 1 or the block's wave count");
     __shared__ T ring[kWavesPerBlock][kRing * kSlotW];
-    // each lane's row of the tile, [cell][lane]: one word per bank, one writer per word
-    __shared__ T accs[kWavesPerBlock][TU][kWave];
+    // each lane's row of the tile, [cell][lane]: one word per bank, one writer per word.
+    // The three cells after the last wave's are never written: the four-cell update below
+    // reads cells c0 .. c0 + 3 whatever c0 is (values past the wave's 16 cells are not used)
+    __shared__ T accs[kWavesPerBlock * TU * kWave + 3 * kWave];
     const int nut = (g.nu + TU - 1) / TU;
     const int nvc = (g.nv + kWave - 1) / kWave;
     const int items = nviews * nut * nvc;
@@ -577,7 +579,7 @@ __global__ __launch_bounds__(kBlock, 6) void cbct_proj_ring_kernel(Geom<float> g
     const T lzd0 = T(lane) * g.dz, lzd1 = T(kWave + lane) * g.dz, lzd2 = T(2 * kWave + lane) * g.dz;
     const rsrc_words empty_rs = dma_rsrc(vol, 0);  // a group of loads that only writes zeros
 
-    T* acc = &accs[wv][0][lane];
+    T* acc = &accs[(wv * TU) * kWave + lane];
 #pragma unroll
     for (int t = 0; t < TU; ++t) acc[t * kWave] = T(0);
 
@@ -702,14 +704,13 @@ __global__ __launch_bounds__(kBlock, 6) void cbct_proj_ring_kernel(Geom<float> g
                 const T r2 = rdl(cp.r2, c), zinc = rdl(cp.zinc, c), vst = rdl(cp.vst, c), zbase = rdl(cp.zbase, c);
                 const T rz = frcp(zinc);
                 // scale the loaded voxels in place by sqrt(r^2 + zc^2) (the cone factor times
-                // r); words past the range are zero
+                // r); words past the range are zero.  The first two slots in one straight
+                // pass (one LDS round trip; a slot past the range holds zeros and stays zero)
                 {
-                    const T zc = zbase + lzd0;
-                    zs[1 + lane] = fsqrt(fmaf(zc, zc, r2)) * zs[1 + lane];
-                }
-                if (ns >= 1) {
-                    const T zc = zbase + lzd1;
-                    zs[1 + kWave + lane] = fsqrt(fmaf(zc, zc, r2)) * zs[1 + kWave + lane];
+                    const T v0 = zs[1 + lane], v1 = zs[1 + kWave + lane];
+                    const T zc0 = zbase + lzd0, zc1 = zbase + lzd1;
+                    zs[1 + lane] = fsqrt(fmaf(zc0, zc0, r2)) * v0;
+                    zs[1 + kWave + lane] = fsqrt(fmaf(zc1, zc1, r2)) * v1;
                 }
                 if (ns >= 2) {
                     const T zc = zbase + lzd2;
@@ -743,12 +744,16 @@ __global__ __launch_bounds__(kBlock, 6) void cbct_proj_ring_kernel(Geom<float> g
                 }
                 __builtin_amdgcn_wave_barrier();
                 // tile cells c0 .. c0 + nc; the same on every lane, so each weighted add
-                // touches one LDS word per lane and bank
+                // touches one LDS word per lane and bank.  The four words are read and
+                // updated in one pass (one LDS round trip); only the cells of the footprint
+                // inside the tile are written back.
                 T* pa = acc + c0 * kWave;
-                pa[0] += rdl(cp.w0, c) * q;
-                if (nc >= 1) pa[kWave] += rdl(cp.w1, c) * q;
-                if (nc >= 2) pa[2 * kWave] += rdl(cp.w2, c) * q;
-                if (nc >= 3) pa[3 * kWave] += rdl(cp.w3, c) * q;
+                const T a0 = pa[0] + rdl(cp.w0, c) * q, a1 = pa[kWave] + rdl(cp.w1, c) * q,
+                        a2 = pa[2 * kWave] + rdl(cp.w2, c) * q, a3 = pa[3 * kWave] + rdl(cp.w3, c) * q;
+                pa[0] = a0;
+                if (nc >= 1) pa[kWave] = a1;
+                if (nc >= 2) pa[2 * kWave] = a2;
+                if (nc >= 3) pa[3 * kWave] = a3;
                 if (nc >= 4) {
                     // wider than four tile cells (rare): the corners again, from the column
                     const int col = rdl_i(cp.col, c), cells = rdl_i(cp.cells, c);
@@ -776,7 +781,7 @@ __global__ __launch_bounds__(kBlock, 6) void cbct_proj_ring_kernel(Geom<float> g
             const int iu = iu_a + t;
             T sum = acc[t * kWave];
 #pragma unroll
-            for (int w = 1; w < SPLIT; ++w) sum += accs[w][t][lane];
+            for (int w = 1; w < SPLIT; ++w) sum += accs[(w * TU + t) * kWave + lane];
             if (iu < g.nu) out[(size_t)iu * g.nv + iv] = cvt_store<Tout>(sum);
         }
     }
