@@ -433,6 +433,8 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
 // zero) and kZPad zero words.  Only the loads write words 1 .. 192, so word 0 and the pads
 // stay zero for the whole kernel.
 constexpr int kRing = 3;
+constexpr int kRingWaves = 6;  // waves per SIMD the ring kernel is compiled for (the LDS of six
+                               // blocks per CU; seven would spill registers)
 constexpr int kSlotW = 1 + kPre * kWave + kZPad;
 
 // A buffer resource as four scalar words, for the inline assembly below.  The loads are
@@ -532,17 +534,18 @@ __device__ __forceinline__ float ring_gather(const float* zs, int span, int ja, 
 }
 
 template <class Tout, int FOOT, int TU, int SPLIT>
-__global__ __launch_bounds__(kBlock, 6) void cbct_proj_ring_kernel(Geom<float> g, const View<float>* __restrict__ views,
+__global__ __launch_bounds__(kBlock, kRingWaves) void cbct_proj_ring_kernel(Geom<float> g, const View<float>* __restrict__ views,
                                                                    int vstride, int nviews, const float* __restrict__ vol,
                                                                    const uint8_t* __restrict__ map,
                                                                    Tout* __restrict__ proj, int exhaustive) {
     using T = float;
     static_assert(SPLIT == 1 || SPLIT == kWavesPerBlock, "SPLIT is 1 or the block's wave count");
-    __shared__ T ring[kWavesPerBlock][kRing * kSlotW];
-    // each lane's row of the tile, [cell][lane]: one word per bank, one writer per word.
-    // The three cells after the last wave's are never written: the four-cell update below
-    // reads cells c0 .. c0 + 3 whatever c0 is (values past the wave's 16 cells are not used)
-    __shared__ T accs[kWavesPerBlock * TU * kWave + 3 * kWave];
+    // one block of LDS: the waves' tile accumulators, then their rings.  Accumulators: each
+    // lane's row of the tile, [cell][lane] (one word per bank, one writer per word).  The
+    // four-cell update below reads cells c0 .. c0 + 3 whatever c0 is; past a wave's 16 cells
+    // it reads the next wave's cells or, after the last wave's, its ring (values not used).
+    __shared__ T lds_block[kWavesPerBlock * TU * kWave + kWavesPerBlock * kRing * kSlotW];
+    T* const accs = lds_block;
     const int nut = (g.nu + TU - 1) / TU;
     const int nvc = (g.nv + kWave - 1) / kWave;
     const int items = nviews * nut * nvc;
@@ -567,7 +570,7 @@ __global__ __launch_bounds__(kBlock, 6) void cbct_proj_ring_kernel(Geom<float> g
     const T kR = u_to_slope(g, T(iu_b) + T(0.5));
     const int iv = iv_a + lane;
     const int iv_last = min(iv_a + kWave, g.nv) - 1;
-    T* zr = ring[wv];
+    T* zr = lds_block + kWavesPerBlock * TU * kWave + wv * kRing * kSlotW;
     // word 0 and the pads of every slot stay zero; the loads write words 1 .. 192
     for (int i = lane; i < kRing * kSlotW; i += kWave) zr[i] = T(0);
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // zeros in place before any load lands
@@ -690,13 +693,13 @@ __global__ __launch_bounds__(kBlock, 6) void cbct_proj_ring_kernel(Geom<float> g
                 ring_load(rs, isl, loff);
                 isl = advance(isl);
             };
-            issue(0);
-            issue(1);
-            unsigned psl = isl;  // slot of column 0: two slots behind the issue position
+#pragma unroll
+            for (int i = 0; i < kRing - 1; ++i) issue(i);
+            unsigned psl = isl;  // slot of column 0: kRing - 1 slots behind the issue position
             psl = advance(psl);
             for (int c = 0; c < n; ++c) {
-                issue(c + 2);
-                wait_vm<6>();  // column c's group has landed
+                issue(c + kRing - 1);
+                wait_vm<3 * (kRing - 1)>();  // column c's group has landed
                 T* zs = (T*)((char*)zr + (psl - 4 - ring_lds));  // word 0 of the slot
                 psl = advance(psl);
                 const int meta = rdl_i(cp.meta, c);
