@@ -212,6 +212,47 @@ def test_cbct_register_pipeline_vs_oracle(seed):
         assert rel_l2(got, want) <= _tol(f"proj,{kind}", g, v16, mp), (kind, rel_l2(got, want))
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_cbct_random_geometries_vs_oracle(seed):
+    """DD and SF, both directions, on random geometries: voxel and detector pitches, offsets,
+    couch positions, maps and both detector types; slice counts on either side of the
+    forward kernels' 192-slice switch and row counts on either side of the backprojector's
+    64-row form"""
+    from oracle import oracle as O
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    rng = np.random.default_rng(300 + seed)
+    n = (int(rng.integers(12, 28)), int(rng.integers(12, 28)), int(rng.choice([int(rng.integers(4, 190)), 250])))
+    nuv = (int(rng.integers(30, 90)), int(rng.choice([int(rng.integers(8, 64)), int(rng.integers(65, 140))])))
+    nviews = 5
+    dettype = int(seed % 2)
+    betas = np.sort(rng.uniform(0, 2 * np.pi, nviews))
+    d = tuple(float(v) for v in rng.uniform(0.4, 1.2, 3))
+    o = tuple(float(v) for v in rng.uniform(-1.5, 1.5, 3))
+    duv = tuple(float(v) for v in rng.uniform(0.3, 1.6, 2))
+    ouv = tuple(float(v) for v in rng.uniform(-2, 2, 2))
+    g = O.G3(n, d, o, nuv, duv, ouv, 1000.0, 0.0, 1500.0, betas,
+             couchz=rng.uniform(-3, 3, nviews) if seed % 3 == 0 else None, dettype=dettype)
+    vol = rng.uniform(0, 1, n).astype(np.float32)
+    sino = rng.uniform(0, 1, (nuv[1], nuv[0], nviews)).astype(np.float32)
+    mp = (rng.uniform(size=n[:2]) < 0.9) if seed % 2 == 0 else None
+    say("random cone case", seed, n, nuv, d, duv, dettype)
+    P = Projector(geometry_from_g3(g, mp))
+    d_vol = P.upload_volume(vol)
+    d_sino = P.upload_sino(sino)
+    out_s = DeviceArray(P.n_sino(), "f32")
+    out_v = DeviceArray(P.n_img, "f32")
+    for kind in ("dd", "tf"):
+        P.proj(d_vol, out_s, f"proj,{kind}")
+        got = P.download_sino(out_s)
+        want = O.cbct(f"proj,{kind}", g, vol, mp)
+        assert rel_l2(got, want) <= _tol(f"proj,{kind}", g, vol, mp), (kind, rel_l2(got, want))
+        P.back(d_sino, out_v, f"back,{kind}")
+        got = P.download_volume(out_v)
+        want = O.cbct(f"back,{kind}", g, sino, mp)
+        assert rel_l2(got, want) <= _tol(f"back,{kind}", g, sino, mp), (kind, rel_l2(got, want))
+
+
 def test_cbct_unsplit_launch_vs_oracle():
     """>= 16384 work items: each wave walks its whole strip (no LDS combine)"""
     from oracle import oracle as O
