@@ -186,8 +186,6 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
     const View<T> v = views[(size_t)b * vstride];
     const bool row_major = fabs_t(v.c) >= fabs_t(v.s);
     const int nmajor = row_major ? g.ny : g.nx;
-    // the SPLIT waves of a block take interleaved 64-row batches of the strip, so that the
-    // long middle of a chord is shared among them
     const int m_end = nmajor;
     const T kL = u_to_slope(g, T(iu_a) - T(0.5));
     const T kR = u_to_slope(g, T(iu_b) + T(0.5));
@@ -204,8 +202,11 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
 #pragma unroll
     for (int t = 0; t < TU; ++t) acc[t * kWave] = T(0);
 
-    for (int m0 = share * kWave; m0 < m_end; m0 += SPLIT * kWave) {
-        const int m = m0 + lane;
+    // the SPLIT waves of a block take every SPLIT-th image line of the strip (lane l of a
+    // batch takes line m0 + l SPLIT + share), so that their shares are alike wherever the
+    // strip is long: the block holds its LDS until its last wave ends
+    for (int m0 = 0; m0 < m_end; m0 += SPLIT * kWave) {
+        const int m = m0 + lane * SPLIT + share;
         int lo = 0, hi = -1;
         if (m < m_end) {
             if (!strip_range(g, v, row_major, m, kL, kR, exhaustive != 0, lo, hi)) hi = lo - 1;
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
                 }
                 const int lo_j = __shfl(lo, j, 64);
                 const int ex_j = __shfl(excl, j, 64);
-                const int major = m0 + j;
+                const int major = m0 + j * SPLIT + share;
                 const int minor = lo_j + (k - ex_j);
                 const int ix = row_major ? minor : major;
                 const int iy = row_major ? major : minor;
@@ -427,15 +428,17 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
     }
 }
 
-// LDS ring of the ring form: kRing column slots per wave.  A slot holds a zero word, the
-// 192 words a column's three loads write (voxels za .. za + 191; MUBUF loads with the LDS
-// bit put lane t's word at the slot address + 4 t, and lanes past the buffer range write
-// zero) and kZPad zero words.  Only the loads write words 1 .. 192, so word 0 and the pads
-// stay zero for the whole kernel.
-constexpr int kRing = 3;
+// LDS ring of the ring form: ring_len(NS) entries per wave, one per column.  An entry
+// holds a zero word, the NS * 64 words a column's NS loads write (voxels za ..
+// za + 64 NS - 1; MUBUF loads with the LDS bit put lane t's word at the entry address
+// + 4 t, and lanes past the buffer range write zero) and kZPad zero words.  Only the loads
+// write words 1 .. 64 NS, so word 0 and the pads stay zero for the whole kernel.  Columns
+// of at most 128 voxels (NS = 2) take four entries (three columns in flight), columns of
+// at most 192 (NS = 3) three: about the same LDS.
 constexpr int kRingWaves = 6;  // waves per SIMD the ring kernel is compiled for (the LDS of six
                                // blocks per CU; seven would spill registers)
-constexpr int kSlotW = 1 + kPre * kWave + kZPad;
+__host__ __device__ constexpr int ring_len(int ns) { return ns == 2 ? 4 : 3; }
+__host__ __device__ constexpr int ring_entry_words(int ns) { return 1 + ns * kWave + kZPad; }
 
 // A buffer resource as four scalar words, for the inline assembly below.  The loads are
 // issued from inline assembly because the compiler's wait insertion, which cannot tell the
@@ -478,10 +481,11 @@ __device__ __forceinline__ void wait_vm() {
 //    registers: the byte address and length of its voxel range (the buffer resource),
 //    the lower edge of its first voxel on the detector, zc of its first voxel, its tile
 //    cells and gather class;
-//  * a column's three 64-voxel loads go straight into an LDS ring slot (MUBUF loads with
-//    the LDS bit), two columns ahead of the one being gathered.  Every column issues
-//    exactly one three-load group (groups past the batch's last survivor have an empty
-//    range and write zeros), so "this column's group has landed" is always vmcnt(6).
+//  * a column's NS 64-voxel loads go straight into an LDS ring entry (MUBUF loads with
+//    the LDS bit), ring_len(NS) - 1 columns ahead of the one being gathered.  Every column
+//    issues exactly one group of NS loads (groups past the batch's last survivor have an
+//    empty range and write zeros), so "this column's group has landed" is always the same
+//    vmcnt: NS (ring_len(NS) - 1).
 // ---------------------------------------------------------------------------------
 struct RingCol {
     int alo, ahi;   // byte address of the column's first needed voxel (ahi: the 16 address bits of a resource)
@@ -500,18 +504,30 @@ __device__ __forceinline__ float permute_f(int dst_lane_bytes, float x) {
     return __int_as_float(__builtin_amdgcn_ds_permute(dst_lane_bytes, __float_as_int(x)));
 }
 
-// three 64-lane dword loads of one column into the ring slot at LDS byte address la (the
+// NS 64-lane dword loads of one column into the ring entry at LDS byte address la (the
 // instruction offsets 0, 256, 512 apply to both the voxel and the LDS address)
+template <int NS>
 __device__ __forceinline__ void ring_load(rsrc_words rs, unsigned la, int voff) {
-    asm volatile(
-        "s_mov_b32 m0, %1\n\t"
-        "s_nop 0\n\t"
-        "buffer_load_dword %0, %2, 0 offen lds\n\t"
-        "buffer_load_dword %0, %2, 0 offen offset:256 lds\n\t"
-        "buffer_load_dword %0, %2, 0 offen offset:512 lds"
-        :
-        : "v"(voff), "s"(la), "s"(rs)
-        : "memory");
+    static_assert(NS == 2 || NS == 3, "two or three slots per column");
+    if constexpr (NS == 2)
+        asm volatile(
+            "s_mov_b32 m0, %1\n\t"
+            "s_nop 0\n\t"
+            "buffer_load_dword %0, %2, 0 offen lds\n\t"
+            "buffer_load_dword %0, %2, 0 offen offset:256 lds"
+            :
+            : "v"(voff), "s"(la), "s"(rs)
+            : "memory");
+    else
+        asm volatile(
+            "s_mov_b32 m0, %1\n\t"
+            "s_nop 0\n\t"
+            "buffer_load_dword %0, %2, 0 offen lds\n\t"
+            "buffer_load_dword %0, %2, 0 offen offset:256 lds\n\t"
+            "buffer_load_dword %0, %2, 0 offen offset:512 lds"
+            :
+            : "v"(voff), "s"(la), "s"(rs)
+            : "memory");
 }
 
 // sum over the K voxels from ja on of (overlap with the row) * staged value, voxel j
@@ -533,7 +549,7 @@ __device__ __forceinline__ float ring_gather(const float* zs, int span, int ja, 
     return s;
 }
 
-template <class Tout, int FOOT, int TU, int SPLIT>
+template <class Tout, int FOOT, int TU, int SPLIT, int NS>
 __global__ __launch_bounds__(kBlock, kRingWaves) void cbct_proj_ring_kernel(Geom<float> g, const View<float>* __restrict__ views,
                                                                    int vstride, int nviews, const float* __restrict__ vol,
                                                                    const uint8_t* __restrict__ map,
@@ -544,6 +560,7 @@ __global__ __launch_bounds__(kBlock, kRingWaves) void cbct_proj_ring_kernel(Geom
     // lane's row of the tile, [cell][lane] (one word per bank, one writer per word).  The
     // four-cell update below reads cells c0 .. c0 + 3 whatever c0 is; past a wave's 16 cells
     // it reads the next wave's cells or, after the last wave's, its ring (values not used).
+    constexpr int kRing = ring_len(NS), kSlotW = ring_entry_words(NS);
     __shared__ T lds_block[kWavesPerBlock * TU * kWave + kWavesPerBlock * kRing * kSlotW];
     T* const accs = lds_block;
     const int nut = (g.nu + TU - 1) / TU;
@@ -580,7 +597,7 @@ __global__ __launch_bounds__(kBlock, kRingWaves) void cbct_proj_ring_kernel(Geom
     const T wzdz = v.wz * g.dz;
     const T fiv_lo = T(iv) - T(0.5);  // lower edge of this lane's row
     const T lzd0 = T(lane) * g.dz, lzd1 = T(kWave + lane) * g.dz, lzd2 = T(2 * kWave + lane) * g.dz;
-    const rsrc_words empty_rs = dma_rsrc(vol, 0);  // a group of loads that only writes zeros
+    const rsrc_words empty_rs = dma_rsrc(vol, 0);  // a group of loads that only writes zeros  // a group of loads that only writes zeros
 
     T* acc = &accs[(wv * TU) * kWave + lane];
 #pragma unroll
@@ -590,8 +607,11 @@ __global__ __launch_bounds__(kBlock, kRingWaves) void cbct_proj_ring_kernel(Geom
     unsigned isl = ring_lds + 4;
     auto advance = [&](unsigned a) { return a + slot_bytes >= ring_end ? a + slot_bytes - kRing * slot_bytes : a + slot_bytes; };
 
-    for (int m0 = share * kWave; m0 < m_end; m0 += SPLIT * kWave) {
-        const int m = m0 + lane;
+    // the SPLIT waves of a block take every SPLIT-th image line of the strip (lane l of a
+    // batch takes line m0 + l SPLIT + share), so that their shares are alike wherever the
+    // strip is long: the block holds its LDS until its last wave ends
+    for (int m0 = 0; m0 < m_end; m0 += SPLIT * kWave) {
+        const int m = m0 + lane * SPLIT + share;
         int lo = 0, hi = -1;
         if (m < m_end) {
             if (!strip_range(g, v, row_major, m, kL, kR, exhaustive != 0, lo, hi)) hi = lo - 1;
@@ -611,8 +631,8 @@ __global__ __launch_bounds__(kBlock, kRingWaves) void cbct_proj_ring_kernel(Geom
             const int lo_j = __shfl(lo, j, 64);
             const int ex_j = __shfl(excl, j, 64);
             const int minor = lo_j + (k - ex_j);
-            const int ix = row_major ? minor : m0 + j;
-            const int iy = row_major ? m0 + j : minor;
+            const int ix = row_major ? minor : m0 + j * SPLIT + share;
+            const int iy = row_major ? m0 + j * SPLIT + share : minor;
 
             // exact footprint of the candidate column, its weights on this tile and the
             // quantities the walk needs
@@ -690,7 +710,7 @@ __global__ __launch_bounds__(kBlock, kRingWaves) void cbct_proj_ring_kernel(Geom
                     rs.y = rdl_i(cp.ahi, c);
                     rs.z = rdl_i(cp.nbytes, c);
                 }
-                ring_load(rs, isl, loff);
+                ring_load<NS>(rs, isl, loff);
                 isl = advance(isl);
             };
 #pragma unroll
@@ -699,7 +719,7 @@ __global__ __launch_bounds__(kBlock, kRingWaves) void cbct_proj_ring_kernel(Geom
             psl = advance(psl);
             for (int c = 0; c < n; ++c) {
                 issue(c + kRing - 1);
-                wait_vm<3 * (kRing - 1)>();  // column c's group has landed
+                wait_vm<NS * (kRing - 1)>();  // column c's group has landed
                 T* zs = (T*)((char*)zr + (psl - 4 - ring_lds));  // word 0 of the slot
                 psl = advance(psl);
                 const int meta = rdl_i(cp.meta, c);
@@ -715,7 +735,7 @@ __global__ __launch_bounds__(kBlock, kRingWaves) void cbct_proj_ring_kernel(Geom
                     zs[1 + lane] = fsqrt(fmaf(zc0, zc0, r2)) * v0;
                     zs[1 + kWave + lane] = fsqrt(fmaf(zc1, zc1, r2)) * v1;
                 }
-                if (ns >= 2) {
+                if (NS == 3 && ns >= 2) {
                     const T zc = zbase + lzd2;
                     zs[1 + 2 * kWave + lane] = fsqrt(fmaf(zc, zc, r2)) * zs[1 + 2 * kWave + lane];
                 }
@@ -1295,8 +1315,13 @@ static void launch_proj(unsigned blocks, hipStream_t st, const Geom<T>& g, const
                         int nviews, const void* vol, const uint8_t* map, void* proj, int exhaustive) {
     // FP32 volumes whose columns fit the three 64-voxel slots take the ring form
     if constexpr (std::is_same<Tin, float>::value && std::is_same<T, float>::value) {
+        if (g.nz <= 2 * kWave) {
+            hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU, SPLIT, 2>), dim3(blocks), dim3(kBlock), 0, st, g,
+                               views, vstride, nviews, (const float*)vol, map, (Tout*)proj, exhaustive);
+            return;
+        }
         if (g.nz <= kFastSpan) {
-            hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU, SPLIT>), dim3(blocks), dim3(kBlock), 0, st, g,
+            hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU, SPLIT, 3>), dim3(blocks), dim3(kBlock), 0, st, g,
                                views, vstride, nviews, (const float*)vol, map, (Tout*)proj, exhaustive);
             return;
         }
