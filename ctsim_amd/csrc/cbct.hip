@@ -549,26 +549,28 @@ __device__ __forceinline__ float ring_gather(const float* zs, int span, int ja, 
     return s;
 }
 
-template <class Tout, int FOOT, int TU, int SPLIT, int NS>
-__global__ __launch_bounds__(kBlock, kRingWaves) void cbct_proj_ring_kernel(Geom<float> g, const View<float>* __restrict__ views,
+// WPB waves per block; SPLIT = WPB shares each item's strip among them (eight waves: short
+// view subsets make few items, and smaller shares even out the blocks' run times)
+template <class Tout, int FOOT, int TU, int SPLIT, int NS, int WPB>
+__global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel(Geom<float> g, const View<float>* __restrict__ views,
                                                                    int vstride, int nviews, const float* __restrict__ vol,
                                                                    const uint8_t* __restrict__ map,
                                                                    Tout* __restrict__ proj, int exhaustive) {
     using T = float;
-    static_assert(SPLIT == 1 || SPLIT == kWavesPerBlock, "SPLIT is 1 or the block's wave count");
+    static_assert(SPLIT == 1 || SPLIT == WPB, "SPLIT is 1 or the block's wave count");
     // one block of LDS: the waves' tile accumulators, then their rings.  Accumulators: each
     // lane's row of the tile, [cell][lane] (one word per bank, one writer per word).  The
     // four-cell update below reads cells c0 .. c0 + 3 whatever c0 is; past a wave's 16 cells
     // it reads the next wave's cells or, after the last wave's, its ring (values not used).
     constexpr int kRing = ring_len(NS), kSlotW = ring_entry_words(NS);
-    __shared__ T lds_block[kWavesPerBlock * TU * kWave + kWavesPerBlock * kRing * kSlotW];
+    __shared__ T lds_block[WPB * TU * kWave + WPB * kRing * kSlotW];
     T* const accs = lds_block;
     const int nut = (g.nu + TU - 1) / TU;
     const int nvc = (g.nv + kWave - 1) / kWave;
     const int items = nviews * nut * nvc;
     const int bid = blockIdx.x;
     const int wv = (int)(threadIdx.x / kWave);
-    const int item = SPLIT == 1 ? bid * kWavesPerBlock + wv : bid;
+    const int item = SPLIT == 1 ? bid * WPB + wv : bid;
     const int share = SPLIT == 1 ? 0 : wv;
     if (item >= items) return;  // SPLIT > 1: the whole block leaves together
     const int vc = item % nvc;
@@ -587,7 +589,7 @@ __global__ __launch_bounds__(kBlock, kRingWaves) void cbct_proj_ring_kernel(Geom
     const T kR = u_to_slope(g, T(iu_b) + T(0.5));
     const int iv = iv_a + lane;
     const int iv_last = min(iv_a + kWave, g.nv) - 1;
-    T* zr = lds_block + kWavesPerBlock * TU * kWave + wv * kRing * kSlotW;
+    T* zr = lds_block + WPB * TU * kWave + wv * kRing * kSlotW;
     // word 0 and the pads of every slot stay zero; the loads write words 1 .. 192
     for (int i = lane; i < kRing * kSlotW; i += kWave) zr[i] = T(0);
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // zeros in place before any load lands
@@ -793,7 +795,7 @@ __global__ __launch_bounds__(kBlock, kRingWaves) void cbct_proj_ring_kernel(Geom
     }
     wait_vm<0>();  // the trailing zero groups have landed before the ring is released
     if (SPLIT > 1) {
-        // fixed-order combine of the shares by wave 0: ((s0 + s1) + s2) + s3
+        // fixed-order combine of the shares by wave 0: ((s0 + s1) + s2) + ...
         __syncthreads();
         if (wv > 0) return;
     }
@@ -1315,14 +1317,16 @@ static void launch_proj(unsigned blocks, hipStream_t st, const Geom<T>& g, const
                         int nviews, const void* vol, const uint8_t* map, void* proj, int exhaustive) {
     // FP32 volumes whose columns fit the three 64-voxel slots take the ring form
     if constexpr (std::is_same<Tin, float>::value && std::is_same<T, float>::value) {
+        // split items (SPLIT > 1: one item per block) go to blocks of eight waves
+        constexpr int kRW = SPLIT > 1 ? 8 : kWavesPerBlock, kRS = SPLIT > 1 ? 8 : 1;
         if (g.nz <= 2 * kWave) {
-            hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU, SPLIT, 2>), dim3(blocks), dim3(kBlock), 0, st, g,
-                               views, vstride, nviews, (const float*)vol, map, (Tout*)proj, exhaustive);
+            hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU, kRS, 2, kRW>), dim3(blocks), dim3(kRW * kWave), 0,
+                               st, g, views, vstride, nviews, (const float*)vol, map, (Tout*)proj, exhaustive);
             return;
         }
         if (g.nz <= kFastSpan) {
-            hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU, SPLIT, 3>), dim3(blocks), dim3(kBlock), 0, st, g,
-                               views, vstride, nviews, (const float*)vol, map, (Tout*)proj, exhaustive);
+            hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU, kRS, 3, kRW>), dim3(blocks), dim3(kRW * kWave), 0,
+                               st, g, views, vstride, nviews, (const float*)vol, map, (Tout*)proj, exhaustive);
             return;
         }
     }
