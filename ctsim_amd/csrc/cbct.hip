@@ -1007,10 +1007,13 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             const ViewCol<T>& c = vw[cur];
             const int cells = uni(c.cells), rows = uni(c.rows), cls = uni(c.cls), vb = b0 + cur;
             const int ilo = cells & 0xffff, nw = cells >> 16, r0 = rows & 0xffff, r1 = rows >> 16;
-            const T w0 = uni(c.w0), w1 = uni(c.w1), w2 = uni(c.w2), w3 = uni(c.w3);  // amp / r folded in
-            const T zinc = uni(c.zinc), vs5 = uni(c.vs5);
-            const T r2 = uni(c.r2);      // ampv * r = sqrt(r^2 + zc^2)
-            const T wzdz = uni(c.wzdz);  // zc = z dz + wz dz
+            // the record's reals stay in vector registers (every lane reads the same LDS words):
+            // they are only operands of vector arithmetic, and moving them to scalar
+            // registers would cost one instruction each per view
+            const T w0 = c.w0, w1 = c.w1, w2 = c.w2, w3 = c.w3;  // amp / r folded in
+            const T zinc = c.zinc, vs5 = c.vs5;
+            const T r2 = c.r2;      // ampv * r = sqrt(r^2 + zc^2)
+            const T wzdz = c.wzdz;  // zc = z dz + wz dz
             const Tin* pv = proj + ((size_t)vb * g.nu + ilo) * g.nv;
             // the next view's first row chunk starts loading once this one is staged
             auto next = [&]() {
