@@ -450,7 +450,7 @@ def main():
     achieved = f_launch / (kt["proj"] * 1e-3) / 1e12
     roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                 frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=measured_traffic(),
-                kernel="cbct_proj_ring_kernel<float,SF,16,4> (one 41-view subset, FP32 input z - x0)",
+                kernel="cbct_proj_ring_kernel<float,SF,16,8,2,8> (one 41-view subset, FP32 input z - x0)",
                 note="FP32 vector ALU bound (no MFMA); achieved = (17U+3C+60K)/24 per launch / event time")
     res = {
         "metric": "PWLS-OS iter/s at 512²×128 and DD proj+backproj GUPS, mixed vs plain FP32",
