@@ -831,7 +831,8 @@ struct __align__(16) ViewCol {  // one view's footprint on one image column (80 
     int cls;                    // gather class: rows per voxel 2, 3, 4 (0, 1, 2) or the capped loop (3)
     int vb;                     // view index within the selection
     T ampr, r2;                 // amp / r and r^2: the voxel factor is sqrt(r^2 + zc^2) amp / r
-    T zinc, vs5, wzdz;          // v of voxel z spans [vs5 + z zinc, vs5 + (z + 1) zinc] (+1/2 folded);
+    T zinc, vs15, wzdz;         // v of voxel z spans [vs15 + z zinc, vs15 + (z + 1) zinc] - 1 (the +1/2 edge
+                                // shift and a +1 that keeps the clamped row index non-negative folded in);
                                 // zc = z dz + wzdz
     T vstart;                   // the unshifted start, for footprints that may reach the cap
     T w0, w1, w2, w3;           // weights of the first four footprint cells, times amp / r
@@ -858,10 +859,15 @@ inline size_t back_lds_bytes(int rowwin) {
     return (size_t)kWavesPerBlock * (size_t)back_rowbuf(rowwin) * sizeof(T);
 }
 
+// The gathers take a voxel's edges a, b in row units plus one (so that the clamped row
+// index is never negative), swr with swr[row + 1] = sw[1 + row - rw], and flo, fhi =
+// rw, rwe + 2 as T: a shift of a, b, flo, fhi by one and of swr by minus one changes no
+// weight and no word read.
+//
 // Voxels of at most one row (zinc <= 1) meet rows ra = floor(a) and ra + 1:
 // s = (zinc - e) p[ra] + e p[ra + 1] with e = max(b - (ra + 1), 0) = max(frac(a) + zinc - 1, 0),
-// the same two DD overlaps as below.  a is clamped to [rw - 1, rwe + 1] first: a clamped
-// value is an integer, so e = 0 and the voxel reads only a zero word.
+// the same two DD overlaps as below.  a is clamped to [flo, fhi] first: a clamped value is
+// an integer, so e = 0 and the voxel reads only a zero word.
 template <class T>
 __device__ __forceinline__ T gather_rows2(const T* swr, T flo, T fhi, T a, T zinc, T zm1) {
     const T ac = fmin(fmax(a, flo), fhi);
@@ -871,9 +877,19 @@ __device__ __forceinline__ T gather_rows2(const T* swr, T flo, T fhi, T a, T zin
     const T p0 = pr[0], p1 = pr[1];
     return fma_t(e, p1 - p0, zinc * p0);
 }
+// FP32: the clamp as one med3, the row as one truncating convert (flo >= 0, so ac >= 0) and
+// frac(ac) as v_fract (exact: ac - floor(ac))
+template <>
+__device__ __forceinline__ float gather_rows2<float>(const float* swr, float flo, float fhi, float a, float zinc,
+                                                      float zm1) {
+    const float ac = __builtin_amdgcn_fmed3f(a, flo, fhi);
+    const float* pr = swr + (int)ac;
+    const float e = clamp01(__builtin_amdgcn_fractf(ac) + zm1);
+    const float p0 = pr[0], p1 = pr[1];
+    return fmaf(e, p1 - p0, zinc * p0);
+}
 
-// swr points at the word of row 0 (swr[row] = sw[1 + row - rw]); flo, fhi = rw - 1, rwe + 1
-// as T.  The clamp of floor(a) to [rw - 1, rwe + 1] is taken in T (one med3 in FP32).
+// Thicker voxels: the clamp of floor(a) to [flo, fhi] is taken in T.
 template <class T, int K>
 __device__ __forceinline__ T gather_rows(const T* swr, T flo, T fhi, T a, T b) {
     static_assert(K - 1 <= kRowPad, "pad covers the rows past the window");
@@ -969,7 +985,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 vc.w2 = u_weight<T, FOOT>(ilo + 2, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3) * vc.ampr;
                 vc.w3 = u_weight<T, FOOT>(ilo + 3, ilo, ihi, vc.p0, vc.p1, vc.p2, vc.p3) * vc.ampr;
                 vc.r2 = cg.r * cg.r;
-                vc.vs5 = vstart + T(0.5);
+                vc.vs15 = vstart + T(1.5);
                 vc.vstart = vstart;
                 vc.wzdz = v.wz * g.dz;
                 vc.vb = b0 + lane;
@@ -1011,7 +1027,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             // they are only operands of vector arithmetic, and moving them to scalar
             // registers would cost one instruction each per view
             const T w0 = c.w0, w1 = c.w1, w2 = c.w2, w3 = c.w3;  // amp / r folded in
-            const T zinc = c.zinc, vs5 = c.vs5;
+            const T zinc = c.zinc, vs15 = c.vs15;
             const T r2 = c.r2;      // ampv * r = sqrt(r^2 + zc^2)
             const T wzdz = c.wzdz;  // zc = z dz + wz dz
             const Tin* pv = proj + ((size_t)vb * g.nu + ilo) * g.nv;
@@ -1031,14 +1047,16 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             // after the staged rows (or overlaps of zero), as do voxels past nz, whose sums
             // are never stored.
             auto gather = [&](int rw, int rwe) {
-                const T* swr = sw + 1 - rw;
-                const T flo = T(rw - 1), fhi = T(rwe + 1), zm1 = zinc - T(1);
+                // edges and rows shifted by one: swr[row + 1] = sw[1 + row - rw], and the clamp
+                // range [rw, rwe + 2] of floor(a + 1) is non-negative
+                const T* swr = sw - rw;
+                const T flo = T(rw), fhi = T(rwe + 2), zm1 = zinc - T(1);
 #pragma unroll
                 for (int cz = 0; cz < NZL; ++cz) {
                     const int iz = cz * kWave + lane;
-                    // voxel edges in row units, +1/2 folded into vs5 (within an ulp of the
+                    // voxel edges in row units plus one, +3/2 folded into vs15 (within an ulp of the
                     // reference's vstart + iz zinc + 1/2)
-                    const T a = fma_t(T(iz), zinc, vs5), bb = a + zinc;
+                    const T a = fma_t(T(iz), zinc, vs15), bb = a + zinc;
                     T s;
                     if (cls == 0) {
                         s = gather_rows2<T>(swr, flo, fhi, a, zinc, zm1);
