@@ -402,7 +402,7 @@ int ctsim_plan_back(const ctsim_plan* p, const char* op, int32_t view0, int32_t 
                     img_dt, p->compute);
     if (e == hipErrorNotSupported)
         return fail(CTSIM_EUNSUP,
-                    "Error: 3D backprojection supports up to 512 slices (nz = %d) and views of under 2^31 bytes.",
+                    "Error: 3D backprojection supports up to 512 slices (nz = %d) and 64 views under 4 GiB.",
                     p->g.nz);
     HIP_TRY(e, "backprojection launch");
     return CTSIM_OK;

@@ -829,7 +829,7 @@ struct __align__(16) ViewCol {  // one view's footprint on one image column (80 
     int cells;                  // first footprint cell ilo | cell count nw << 16
     int rows;                   // rows covered r0 | r1 << 16
     int cls;                    // gather class: rows per voxel 2, 3, 4 (0, 1, 2) or the capped loop (3)
-    int vb;                     // view index within the selection
+    unsigned voff;              // bytes from the batch's first view to row r0 of column ilo of this view
     T ampr, r2;                 // amp / r and r^2: the voxel factor is sqrt(r^2 + zc^2) amp / r
     T zinc, vs15, wzdz;         // v of voxel z spans [vs15 + z zinc, vs15 + (z + 1) zinc] - 1 (the +1/2 edge
                                 // shift and a +1 that keeps the clamped row index non-negative folded in);
@@ -988,7 +988,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 vc.vs15 = vstart + T(1.5);
                 vc.vstart = vstart;
                 vc.wzdz = v.wz * g.dz;
-                vc.vb = b0 + lane;
+                vc.voff = ((unsigned)(lane * g.nu + ilo) * (unsigned)g.nv + (unsigned)r0) * (unsigned)sizeof(Tin);
             }
         }
         vw[lane] = vc;
@@ -1002,13 +1002,14 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
         // resource over the view's sinogram, so that columns past the detector's last read
         // zero; all four are loaded (loads under a condition would be waited for at once),
         // the columns past the footprint are not used
-        auto load_rows = [&](int vb, int cells, int rows) {
-            // a resource over the view's sinogram up to the end of the footprint's last
-            // column, so that the loads of columns past the footprint (and past the
-            // detector) read zero
-            const int r0 = rows & 0xffff, r1 = rows >> 16, ilo = cells & 0xffff, nw = cells >> 16;
-            const auto rs = buf_rsrc(proj + (size_t)vb * g.nu * g.nv, (ilo + nw) * g.nv * (int)sizeof(Tin));
-            const int off = (ilo * g.nv + min(r0 + lane, r1)) * (int)sizeof(Tin);
+        // the batch's first view; a record's voff reaches any row of its view from here
+        const char* bbase = reinterpret_cast<const char*>(proj + (size_t)b0 * g.nu * g.nv);
+        auto load_rows = [&](unsigned voff, int cells, int rows) {
+            // a resource from row r0 of the footprint's first column to the end of its last,
+            // so that the loads of columns past the footprint (and past the detector) read zero
+            const int r0 = rows & 0xffff, r1 = rows >> 16, nw = cells >> 16;
+            const auto rs = buf_rsrc(bbase + voff, (nw * g.nv - r0) * (int)sizeof(Tin));
+            const int off = min(lane, r1 - r0) * (int)sizeof(Tin);
             const int step = g.nv * (int)sizeof(Tin);
             pre0 = buf_ld<Tin>(rs, off);
             pre1 = buf_ld<Tin>(rs, off + step);
@@ -1017,7 +1018,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
         };
         if (cur >= 0) {
             const ViewCol<T>& c = vw[cur];
-            load_rows(b0 + cur, uni(c.cells), uni(c.rows));
+            load_rows((unsigned)uni((int)c.voff), uni(c.cells), uni(c.rows));
         }
         while (cur >= 0) {
             const ViewCol<T>& c = vw[cur];
@@ -1037,7 +1038,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 if (nxt >= 0) {
                     mask &= mask - 1;
                     const ViewCol<T>& cn = vw[nxt];
-                    load_rows(b0 + nxt, uni(cn.cells), uni(cn.rows));
+                    load_rows((unsigned)uni((int)cn.voff), uni(cn.cells), uni(cn.rows));
                 }
                 cur = nxt;
             };
@@ -1450,8 +1451,8 @@ static hipError_t back3(const Geom<T>& g, const View<T>* views, int vstride, int
     if (cols == 0 || blocks == 0) return hipSuccess;
     if (foot == 2 && (g.nu < 2 || g.nv < 2)) return hipErrorInvalidValue;  // bilinear needs 2x2 cells
     if (g.nz > 8 * kWave) return hipErrorNotSupported;                      // voxel columns up to 512
-    if ((long long)g.nu * g.nv * (long long)sizeof(Tin) > 0x7fffffffLL)
-        return hipErrorNotSupported;  // a view's sinogram is addressed with 32-bit byte offsets
+    if (foot != 2 && (long long)kWave * g.nu * g.nv * (long long)sizeof(Tin) > 0xffffffffLL)
+        return hipErrorNotSupported;  // 64 views' sinograms are addressed with 32-bit byte offsets
     const int nzl = (g.nz + kWave - 1) / kWave;
     if (nzl <= 1) launch_back<Tin, Tout, T, 1>(blocks, st, g, views, vstride, nviews, proj, map, vol, foot, pd_mode, scale, accumulate);
     else if (nzl <= 2) launch_back<Tin, Tout, T, 2>(blocks, st, g, views, vstride, nviews, proj, map, vol, foot, pd_mode, scale, accumulate);
