@@ -1052,34 +1052,43 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 // range [rw, rwe + 2] of floor(a + 1) is non-negative
                 const T* swr = sw - rw;
                 const T flo = T(rw), fhi = T(rwe + 2), zm1 = zinc - T(1);
+                // the class is the same for all of the column's voxels: one branch per view,
+                // each class with its own unrolled voxel loop
+                auto voxels = [&](auto cls_k) {
+                    constexpr int CLS = decltype(cls_k)::value;
 #pragma unroll
-                for (int cz = 0; cz < NZL; ++cz) {
-                    const int iz = cz * kWave + lane;
-                    // voxel edges in row units plus one, +3/2 folded into vs15 (within an ulp of the
-                    // reference's vstart + iz zinc + 1/2)
-                    const T a = fma_t(T(iz), zinc, vs15), bb = a + zinc;
-                    T s;
-                    if (cls == 0) {
-                        s = gather_rows2<T>(swr, flo, fhi, a, zinc, zm1);
-                    } else if (cls == 1) {
-                        s = gather_rows<T, 3>(swr, flo, fhi, a, bb);
-                    } else if (cls == 2) {
-                        s = gather_rows<T, 4>(swr, flo, fhi, a, bb);
-                    } else {
-                        // footprints that may reach the 16-cell cap: the reference's edges
-                        const T vmin = uni(c.vstart) + T(iz) * zinc;
-                        T clo, chi;
-                        int vlo, vhi;
-                        s = T(0);
-                        if (clamp_cells(vmin, vmin + zinc, g.nv, 16, clo, chi, vlo, vhi)) {
-                            const int ra = max(vlo, rw), rb = min(vhi, rwe);
-                            for (int iv = ra; iv <= rb; ++iv)
-                                s += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * sw[iv - rw + 1];
+                    for (int cz = 0; cz < NZL; ++cz) {
+                        const int iz = cz * kWave + lane;
+                        // voxel edges in row units plus one, +3/2 folded into vs15 (within an
+                        // ulp of the reference's vstart + iz zinc + 1/2)
+                        const T a = fma_t(T(iz), zinc, vs15);
+                        T s;
+                        if constexpr (CLS == 0) {
+                            s = gather_rows2<T>(swr, flo, fhi, a, zinc, zm1);
+                        } else if constexpr (CLS == 1) {
+                            s = gather_rows<T, 3>(swr, flo, fhi, a, a + zinc);
+                        } else if constexpr (CLS == 2) {
+                            s = gather_rows<T, 4>(swr, flo, fhi, a, a + zinc);
+                        } else {
+                            // footprints that may reach the 16-cell cap: the reference's edges
+                            const T vmin = uni(c.vstart) + T(iz) * zinc;
+                            T clo, chi;
+                            int vlo, vhi;
+                            s = T(0);
+                            if (clamp_cells(vmin, vmin + zinc, g.nv, 16, clo, chi, vlo, vhi)) {
+                                const int ra = max(vlo, rw), rb = min(vhi, rwe);
+                                for (int iv = ra; iv <= rb; ++iv)
+                                    s += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * sw[iv - rw + 1];
+                            }
                         }
+                        const T zcm = T(iz) * g.dz + wzdz;
+                        img[cz] += fsqrt(fma_t(zcm, zcm, r2)) * s;
                     }
-                    const T zcm = T(iz) * g.dz + wzdz;
-                    img[cz] += fsqrt(fma_t(zcm, zcm, r2)) * s;
-                }
+                };
+                if (cls == 0) voxels(std::integral_constant<int, 0>{});
+                else if (cls == 1) voxels(std::integral_constant<int, 1>{});
+                else if (cls == 2) voxels(std::integral_constant<int, 2>{});
+                else voxels(std::integral_constant<int, 3>{});
             };
             // (short columns only: with more voxels per lane the second copy of the gather
             // costs the longer columns a wave per SIMD)
