@@ -558,13 +558,14 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
                                                                    Tout* __restrict__ proj, int exhaustive) {
     using T = float;
     static_assert(SPLIT == 1 || SPLIT == WPB, "SPLIT is 1 or the block's wave count");
-    // one block of LDS: the waves' tile accumulators, then their rings.  Accumulators: each
-    // lane's row of the tile, [cell][lane] (one word per bank, one writer per word).  The
-    // four-cell update below reads cells c0 .. c0 + 3 whatever c0 is; past a wave's 16 cells
-    // it reads the next wave's cells or, after the last wave's, its ring (values not used).
+    // one block of LDS: the waves' rings, then their tile accumulators, then three pad cells.
+    // Accumulators: each lane's row of the tile, [cell][lane] (one word per bank, one writer
+    // per word).  The four-cell update below reads cells c0 .. c0 + 3 whatever c0 is; past a
+    // wave's 16 cells it reads the next wave's cells or the pad (values not used).  The rings
+    // come first so that their addresses fit the LDS instructions' offset fields.
     constexpr int kRing = ring_len(NS), kSlotW = ring_entry_words(NS);
-    __shared__ T lds_block[WPB * TU * kWave + WPB * kRing * kSlotW];
-    T* const accs = lds_block;
+    __shared__ T lds_block[WPB * kRing * kSlotW + WPB * TU * kWave + 3 * kWave];
+    T* const accs = lds_block + WPB * kRing * kSlotW;
     const int nut = (g.nu + TU - 1) / TU;
     const int nvc = (g.nv + kWave - 1) / kWave;
     const int items = nviews * nut * nvc;
@@ -589,7 +590,7 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
     const T kR = u_to_slope(g, T(iu_b) + T(0.5));
     const int iv = iv_a + lane;
     const int iv_last = min(iv_a + kWave, g.nv) - 1;
-    T* zr = lds_block + WPB * TU * kWave + wv * kRing * kSlotW;
+    T* zr = lds_block + wv * kRing * kSlotW;
     // word 0 and the pads of every slot stay zero; the loads write words 1 .. 192
     for (int i = lane; i < kRing * kSlotW; i += kWave) zr[i] = T(0);
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // zeros in place before any load lands
