@@ -1025,13 +1025,15 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             const ViewCol<T>& c = vw[cur];
             const int cells = uni(c.cells), rows = uni(c.rows), cls = uni(c.cls), vb = b0 + cur;
             const int ilo = cells & 0xffff, nw = cells >> 16, r0 = rows & 0xffff, r1 = rows >> 16;
-            // the record's reals stay in vector registers (every lane reads the same LDS words):
-            // they are only operands of vector arithmetic, and moving them to scalar
-            // registers would cost one instruction each per view
-            const T w0 = c.w0, w1 = c.w1, w2 = c.w2, w3 = c.w3;  // amp / r folded in
-            const T zinc = c.zinc, vs15 = c.vs15;
-            const T r2 = c.r2;      // ampv * r = sqrt(r^2 + zc^2)
-            const T wzdz = c.wzdz;  // zc = z dz + wz dz
+            // SHORT: the record's reals stay in vector registers (every lane reads the same LDS
+            // words): they are only operands of vector arithmetic, and moving them to scalar
+            // registers would cost one instruction each per view.  The windowed form, with
+            // more voxels per lane, moves them to scalar registers to keep its occupancy.
+            auto rd = [&](const T& x) -> T { return SHORT ? x : uni(x); };
+            const T w0 = rd(c.w0), w1 = rd(c.w1), w2 = rd(c.w2), w3 = rd(c.w3);  // amp / r folded in
+            const T zinc = rd(c.zinc), vs15 = rd(c.vs15);
+            const T r2 = rd(c.r2);      // ampv * r = sqrt(r^2 + zc^2)
+            const T wzdz = rd(c.wzdz);  // zc = z dz + wz dz
             const Tin* pv = proj + ((size_t)vb * g.nu + ilo) * g.nv;
             // the next view's first row chunk starts loading once this one is staged
             auto next = [&]() {
@@ -1091,6 +1093,38 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 else if (cls == 2) voxels(std::integral_constant<int, 2>{});
                 else voxels(std::integral_constant<int, 3>{});
             };
+            // the windowed form (tall detectors) instantiates the gather twice; there one copy of
+            // the voxel loop with the class chosen per voxel keeps the code small enough for the
+            // instruction cache
+            auto gather_small = [&](int rw, int rwe) {
+                const T* swr = sw - rw;
+                const T flo = T(rw), fhi = T(rwe + 2), zm1 = zinc - T(1);
+#pragma unroll
+                for (int cz = 0; cz < NZL; ++cz) {
+                    const int iz = cz * kWave + lane;
+                    const T a = fma_t(T(iz), zinc, vs15);
+                    T s;
+                    if (cls == 0) {
+                        s = gather_rows2<T>(swr, flo, fhi, a, zinc, zm1);
+                    } else if (cls == 1) {
+                        s = gather_rows<T, 3>(swr, flo, fhi, a, a + zinc);
+                    } else if (cls == 2) {
+                        s = gather_rows<T, 4>(swr, flo, fhi, a, a + zinc);
+                    } else {
+                        const T vmin = uni(c.vstart) + T(iz) * zinc;
+                        T clo, chi;
+                        int vlo, vhi;
+                        s = T(0);
+                        if (clamp_cells(vmin, vmin + zinc, g.nv, 16, clo, chi, vlo, vhi)) {
+                            const int ra = max(vlo, rw), rb = min(vhi, rwe);
+                            for (int iv = ra; iv <= rb; ++iv)
+                                s += dd_overlap16(iv, vmin, vmin + zinc, g.nv) * sw[iv - rw + 1];
+                        }
+                    }
+                    const T zcm = T(iz) * g.dz + wzdz;
+                    img[cz] += fsqrt(fma_t(zcm, zcm, r2)) * s;
+                }
+            };
             // (short columns only: with more voxels per lane the second copy of the gather
             // costs the longer columns a wave per SIMD)
             if (SHORT || (NZL <= 2 && r1 - r0 + 1 + kRowPad <= kWave)) {
@@ -1104,7 +1138,8 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 sw[1 + lane] = r0 + lane <= r1 ? s : T(0);
                 __builtin_amdgcn_wave_barrier();
                 next();
-                gather(r0, r1);
+                if constexpr (SHORT) gather(r0, r1);
+                else gather_small(r0, r1);
                 __builtin_amdgcn_wave_barrier();
                 continue;
             }
@@ -1136,7 +1171,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 }
                 __builtin_amdgcn_wave_barrier();
                 if (rw == r0) next();
-                gather(rw, rwe);
+                gather_small(rw, rwe);
                 __builtin_amdgcn_wave_barrier();
             }
         }
