@@ -128,12 +128,22 @@ def kernel_times(solver, reps=1):
                      solver.x0.vp if solver.x0 is not None else None, nx, ny, nz, solver.beta, solver.delta,
                      solver.pfun, 1.0, 1.0, solver._sv())
             t["update"] += tm.ms
+            if solver.fused:
+                with Timer(st) as tm:
+                    rw = solver._window(solver.r, 0, ns)
+                    call("ctsim_plan_back_nesterov", C.c_void_p(solver.P.handle), f"back,{solver.kind}".encode(), v0,
+                         vs, ns, rw.vp, rw.dtype, float(solver.M) * solver.wscale, solver.x[a].vp, solver.z[a].vp,
+                         solver.s.vp, solver.precom.vp, solver.x[b].vp, solver.z[b].vp,
+                         solver.zc.vp if solver.zc is not None else None,
+                         solver.x0.vp if solver.x0 is not None else None, solver.beta, solver.delta, solver.pfun,
+                         1.0, 1.0, None, None, 0, solver._sv())
+                t["back+update fused"] = t.get("back+update fused", 0.0) + tm.ms
     n = reps * len(solver.subsets)
     return {k: v / n for k, v in t.items()}
 
 
 def run_pwls(form, geom, y, w, x0, steps, warmup, sino_dtype="bf16", numos=24, beta=1e4, delta=1e-2,
-             total_iters=0):
+             total_iters=0, fused=False):
     """warmup + steps iterations (steps timed) of the form 'mixed', 'plain' or 'f64', then
     untimed iterations up to total_iters"""
     from ctsim_amd.device import synchronize
@@ -141,7 +151,7 @@ def run_pwls(form, geom, y, w, x0, steps, warmup, sino_dtype="bf16", numos=24, b
     precision = {"mixed": "mixed", "plain": "f32", "f64": "f64"}[form]
     n_all = max(warmup + steps, total_iters)
     solver = PwlsOS(geom, y, w, numos, beta, delta, precision=precision, sino_dtype=sino_dtype,
-                    graph=form == "mixed")
+                    graph=form == "mixed", fused=fused)
     solver.setup(x0)
     solver.prepare(n_all)
     for _ in range(warmup):
@@ -405,6 +415,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fdk", action="store_true")
     ap.add_argument("--no-2d", action="store_true")
+    ap.add_argument("--fuse", action="store_true",
+                    help="run the backprojection and the Nesterov-SQS update as one kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -429,7 +441,8 @@ def main():
     f_iter = 2 * flops(c_it)  # forward + back over all views per iteration
 
     n_acc = args.accuracy_iters
-    solver, dt, outs = run_pwls("mixed", geom, y, w, x0, args.steps, args.warmup, args.sino_dtype)
+    solver, dt, outs = run_pwls("mixed", geom, y, w, x0, args.steps, args.warmup, args.sino_dtype,
+                                   fused=args.fuse)
     ms_mixed = dt / args.steps * 1e3
     log(f"mixed: {ms_mixed:.1f} ms/iter, phi {outs[-1]['phi']:.6e}, rmsd {outs[-1]['rmsd']:.3e}")
     kt = kernel_times(solver)

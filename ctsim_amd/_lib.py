@@ -72,6 +72,8 @@ SIGNATURES = {
     "ctsim_plan_proj": (C.c_int, [vp, cp, i32, i32, i32, vp, i32, vp, i32, vp]),
     "ctsim_plan_proj_exhaustive": (C.c_int, [vp, cp, i32, i32, i32, vp, i32, vp, i32, vp]),
     "ctsim_plan_back": (C.c_int, [vp, cp, i32, i32, i32, vp, i32, vp, i32, f64, i32, vp]),
+    "ctsim_plan_back_nesterov": (C.c_int, [vp, cp, i32, i32, i32, vp, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp, f64,
+                                           f64, i32, f64, f64, vp, vp, i32, vp]),
     "ctsim_xyz_to_zxy": (C.c_int, [vp, i32, vp, i32, i32, i32, i32, vp]),
     "ctsim_zxy_to_xyz": (C.c_int, [vp, i32, vp, i32, i32, i32, i32, vp]),
     "ctsim_batch2_create": (C.c_int, [C.POINTER(vp), i32, C.POINTER(CtsimGeom), P_i32, C.POINTER(P_f64), i32]),

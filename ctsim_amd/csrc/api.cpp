@@ -408,6 +408,34 @@ int ctsim_plan_back(const ctsim_plan* p, const char* op, int32_t view0, int32_t 
     return CTSIM_OK;
 }
 
+int ctsim_plan_back_nesterov(const ctsim_plan* p, const char* op, int32_t view0, int32_t vstep, int32_t nsub,
+                             const void* d_sino, int32_t sino_dt, double scale, const float* x, const float* z,
+                             float* s, const float* precom, float* x_out, float* z_out, float* zcopy,
+                             const float* zref, double beta, double delta, int32_t pfun, double t, double t_next,
+                             const double* d_tseq, const int32_t* d_base, int32_t k, void* stream) {
+    if (!p) return fail(CTSIM_EINVAL, "Error: null plan.");
+    const OpKind kind = parse_op(op);
+    if (kind != OP_BACK_DD && kind != OP_BACK_TF) return fail(CTSIM_EINVAL, "Error: unknown geom type.");
+    if (p->ndim != 3 || p->compute != CTSIM_F32 || pfun != 0)
+        return fail(CTSIM_EUNSUP, "Error: the fused update is 3D, FP32, Huber penalty.");
+    if (int rc = check_views(p, view0, vstep, nsub)) return rc;
+    if (!d_sino || !x || !z || !s || !precom || !x_out || !z_out) return fail(CTSIM_EINVAL, "Error: null argument.");
+    if (x_out == x || z_out == z || x_out == z || z_out == x)
+        return fail(CTSIM_EINVAL, "Error: outputs must not alias the stencil inputs.");
+    if (!valid_dtype(sino_dt)) return fail(CTSIM_EINVAL, "Error: bad dtype.");
+    NestArgs na;
+    na.x = x; na.z = z; na.s = s; na.precom = precom; na.x_out = x_out; na.z_out = z_out;
+    na.zcopy = zcopy; na.zref = zref; na.beta = beta; na.delta = (float)delta;
+    na.t = t; na.t_next = t_next; na.tseq = d_tseq; na.base = (const int*)d_base; na.k = k;
+    hipError_t e = cbct_back_nesterov_launch(p->gf, p->d_vf + view0, vstep, nsub, d_sino, sino_dt, p->d_map,
+                                             kind == OP_BACK_DD ? 0 : 1, (float)scale, na, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: sinogram dtype %d not supported.", sino_dt);
+    if (e == hipErrorNotSupported)
+        return fail(CTSIM_EUNSUP, "Error: the fused update needs a detector of at most 64 rows and nz <= 512.");
+    HIP_TRY(e, "fused backprojection and update");
+    return CTSIM_OK;
+}
+
 int ctsim_xyz_to_zxy(const void* src, int32_t sdt, void* dst, int32_t ddt, int32_t nx, int32_t ny, int32_t nz,
                      void* stream) {
     if (!src || !dst || nx <= 0 || ny <= 0 || nz <= 0) return fail(CTSIM_EINVAL, "Error: bad layout arguments.");

@@ -13,6 +13,7 @@
 #include <type_traits>
 
 #include "internal.h"
+#include "penalty.h"
 
 namespace ctsim {
 
@@ -908,12 +909,16 @@ __device__ __forceinline__ T gather_rows(const T* swr, T flo, T fhi, T a, T b) {
 
 // SHORT: detectors of at most 64 rows, where every view takes the one-pass row staging
 // (the windowed form is compiled out, which keeps the loop's registers in place)
-template <class Tin, class Tout, class T, int FOOT, int NZL, bool SHORT>
+// FUSE: the column's finished gradient goes straight into the Nesterov-SQS update of its
+// voxels (the update kernel's arithmetic with the Huber penalty, na's buffers) instead of
+// to vol; columns outside the map are updated with a zero gradient.
+template <class Tin, class Tout, class T, int FOOT, int NZL, bool SHORT, bool FUSE>
 __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View<T>* __restrict__ views, int vstride,
                                                            int nviews, const Tin* __restrict__ proj,
                                                            const uint8_t* __restrict__ map,
                                                            Tout* __restrict__ vol, T scale, int accumulate,
-                                                           int rowwin) {
+                                                           int rowwin, NestArgs na) {
+    static_assert(!FUSE || (std::is_same<T, float>::value && SHORT), "the fused update is FP32, short detectors");
     // row buffers (rowwin + 1 + kRowPad words per wave) in dynamic LDS, sized at launch
     // for the detector, so short detectors leave room for more resident waves
     extern __shared__ __align__(16) unsigned char back_dyn_lds[];
@@ -926,7 +931,8 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
     const int ix = colid % g.nx, iy = colid / g.nx;
     const int lane = lane_id();
     Tout* ocol = vol + (size_t)colid * g.nz;
-    if (map != nullptr && map[iy * g.nx + ix] == 0) {
+    const bool masked = map != nullptr && map[iy * g.nx + ix] == 0;
+    if (masked && !FUSE) {
         if (!accumulate)
             for (int iz = lane; iz < g.nz; iz += kWave) ocol[iz] = cvt_store<Tout>(T(0));
         return;
@@ -940,7 +946,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
     T img[NZL];
 #pragma unroll
     for (int c = 0; c < NZL; ++c) img[c] = T(0);
-    for (int b0 = 0; b0 < nviews; b0 += kWave) {
+    for (int b0 = 0; b0 < (masked ? 0 : nviews); b0 += kWave) {
         // footprint of view b0 + lane on this column
         bool ok = b0 + lane < nviews;
         ViewCol<T> vc{};
@@ -1176,12 +1182,44 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
             }
         }
     }
+    if constexpr (FUSE) {
+        // reconPwlsSeNesterovSqs.m:67-87 for the column's voxels, as the update kernel does it
+        double t = na.t, t_next = na.t_next;
+        if (na.tseq) {
+            const int i0 = *na.base + na.k;
+            t = na.tseq[i0];
+            t_next = na.tseq[i0 + 1];
+        }
+        const double a = 1.0 - 1.0 / t_next, b = 1.0 / t_next;
 #pragma unroll
-    for (int c = 0; c < NZL; ++c) {
-        const int iz = c * kWave + lane;
-        if (iz < g.nz) {
-            const T val = scale * img[c];
-            ocol[iz] = cvt_store<Tout>(accumulate ? ld<T>(ocol + iz) + val : val);
+        for (int c = 0; c < NZL; ++c) {
+            const int iz = c * kWave + lane;
+            if (iz < g.nz) {
+                const size_t i = (size_t)colid * g.nz + iz;
+                const float sz = pen_point<1, POT_HUBER>(na.z, g.nx, g.ny, g.nz, ix, iy, iz, na.delta);
+                const float tx = pen_point<2, POT_HUBER>(na.x, g.nx, g.ny, g.nz, ix, iy, iz, na.delta);
+                // the gradient rounded as a stored one is (no fused multiply-add across it)
+                const float grad = __fmul_rn(scale, img[c]) + (float)(na.beta * (double)sz);
+                const double den = (double)na.precom[i] + na.beta * (double)tx;
+                const float du = (float)((double)grad / den);
+                const float xn = na.z[i] - du;
+                const float sn = na.s[i] - (float)(t * (double)du);
+                const float zn = (float)(a * (double)xn + b * (double)sn);
+                na.x_out[i] = scrub(xn);
+                na.s[i] = scrub(sn);
+                const float zs = scrub(zn);
+                na.z_out[i] = zs;
+                if (na.zcopy) na.zcopy[i] = na.zref ? zs - na.zref[i] : zs;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < NZL; ++c) {
+            const int iz = c * kWave + lane;
+            if (iz < g.nz) {
+                const T val = scale * img[c];
+                ocol[iz] = cvt_store<Tout>(accumulate ? ld<T>(ocol + iz) + val : val);
+            }
         }
     }
 }
@@ -1456,13 +1494,13 @@ static void launch_back_dd_tf(unsigned blocks, hipStream_t st, const Geom<T>& g,
                               int nviews, const void* proj, const uint8_t* map, void* vol, int foot, T scale,
                               int accumulate, int rowwin) {
     if (foot == FOOT_DD)
-        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_DD, NZL, SHORT>), dim3(blocks), dim3(kBlock),
+        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_DD, NZL, SHORT, false>), dim3(blocks), dim3(kBlock),
                            back_lds_bytes<T>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
-                           (Tout*)vol, scale, accumulate, rowwin);
+                           (Tout*)vol, scale, accumulate, rowwin, NestArgs{});
     else
-        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_TF, NZL, SHORT>), dim3(blocks), dim3(kBlock),
+        hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_TF, NZL, SHORT, false>), dim3(blocks), dim3(kBlock),
                            back_lds_bytes<T>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
-                           (Tout*)vol, scale, accumulate, rowwin);
+                           (Tout*)vol, scale, accumulate, rowwin, NestArgs{});
 }
 
 template <class Tin, class Tout, class T, int NZL>
@@ -1534,6 +1572,54 @@ hipError_t cbct_proj_launch<double>(const Geom<double>& g, const View<double>* v
                                     int exhaustive, hipStream_t st) {
     if (vol_dt != DT_F64 || proj_dt != DT_F64) return hipErrorInvalidValue;
     return proj3<double, double, double>(g, views, vstride, nviews, vol, map, proj, foot, exhaustive, st);
+}
+
+// backprojection of a view selection into the gradient, fused with the Nesterov-SQS update
+// of every voxel (Huber penalty); FP32, detectors of at most 64 rows
+template <class Tin, int NZL>
+static void launch_back_nesterov(unsigned blocks, hipStream_t st, const Geom<float>& g, const View<float>* views,
+                                 int vstride, int nviews, const void* proj, const uint8_t* map, int foot, float scale,
+                                 const NestArgs& na) {
+    const int rowwin = back_rowwin(g.nv);
+    if (foot == FOOT_DD)
+        hipLaunchKernelGGL((cbct_back_kernel<Tin, float, float, FOOT_DD, NZL, true, true>), dim3(blocks), dim3(kBlock),
+                           back_lds_bytes<float>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
+                           (float*)nullptr, scale, 0, rowwin, na);
+    else
+        hipLaunchKernelGGL((cbct_back_kernel<Tin, float, float, FOOT_TF, NZL, true, true>), dim3(blocks), dim3(kBlock),
+                           back_lds_bytes<float>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
+                           (float*)nullptr, scale, 0, rowwin, na);
+}
+
+template <class Tin>
+static hipError_t back_nesterov3(const Geom<float>& g, const View<float>* views, int vstride, int nviews,
+                                 const void* proj, const uint8_t* map, int foot, float scale, const NestArgs& na,
+                                 hipStream_t st) {
+    if (g.nv > kWave) return hipErrorNotSupported;
+    if (g.nz > 8 * kWave) return hipErrorNotSupported;
+    if ((long long)kWave * g.nu * g.nv * (long long)sizeof(Tin) > 0xffffffffLL) return hipErrorNotSupported;
+    const long long cols = (long long)g.nx * g.ny;
+    const unsigned blocks = (unsigned)((cols + kWavesPerBlock - 1) / kWavesPerBlock);
+    if (cols == 0) return hipSuccess;
+    const int nzl = (g.nz + kWave - 1) / kWave;
+    if (nzl <= 1) launch_back_nesterov<Tin, 1>(blocks, st, g, views, vstride, nviews, proj, map, foot, scale, na);
+    else if (nzl <= 2) launch_back_nesterov<Tin, 2>(blocks, st, g, views, vstride, nviews, proj, map, foot, scale, na);
+    else if (nzl <= 4) launch_back_nesterov<Tin, 4>(blocks, st, g, views, vstride, nviews, proj, map, foot, scale, na);
+    else launch_back_nesterov<Tin, 8>(blocks, st, g, views, vstride, nviews, proj, map, foot, scale, na);
+    return hipGetLastError();
+}
+
+hipError_t cbct_back_nesterov_launch(const Geom<float>& g, const View<float>* views, int vstride, int nviews,
+                                     const void* proj, int proj_dt, const uint8_t* map, int foot, float scale,
+                                     const NestArgs& na, hipStream_t st) {
+    if (foot != FOOT_DD && foot != FOOT_TF) return hipErrorInvalidValue;
+    if (nviews <= 0) return hipErrorInvalidValue;
+    switch (proj_dt) {
+        case DT_F32: return back_nesterov3<float>(g, views, vstride, nviews, proj, map, foot, scale, na, st);
+        case DT_F16: return back_nesterov3<__half>(g, views, vstride, nviews, proj, map, foot, scale, na, st);
+        case DT_BF16: return back_nesterov3<__hip_bfloat16>(g, views, vstride, nviews, proj, map, foot, scale, na, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 template <>

@@ -20,6 +20,27 @@ template <class T>
 hipError_t cbct_proj_launch(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* vol,
                             int vol_dt, const uint8_t* map, void* proj, int proj_dt, int foot, int exhaustive,
                             hipStream_t st);
+// buffers and scalars of a Nesterov-SQS update fused into the backprojection (FP32; the
+// update kernel's arguments, the gradient being the backprojection itself)
+struct NestArgs {
+    const float* x = nullptr;
+    const float* z = nullptr;
+    float* s = nullptr;
+    const float* precom = nullptr;
+    float* x_out = nullptr;
+    float* z_out = nullptr;
+    float* zcopy = nullptr;       // z' (or z' - zref), FP32, or null
+    const float* zref = nullptr;
+    double beta = 0.0;
+    float delta = 0.f;
+    double t = 1.0, t_next = 1.0;
+    const double* tseq = nullptr;  // when set, t and t_next are tseq[*base + k] and tseq[*base + k + 1]
+    const int* base = nullptr;
+    int k = 0;
+};
+hipError_t cbct_back_nesterov_launch(const Geom<float>& g, const View<float>* views, int vstride, int nviews,
+                                     const void* proj, int proj_dt, const uint8_t* map, int foot, float scale,
+                                     const NestArgs& na, hipStream_t st);
 template <class T>
 hipError_t cbct_back_launch(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* proj,
                             int proj_dt, const uint8_t* map, void* vol, int vol_dt, int foot, T scale, int accumulate,

@@ -183,6 +183,21 @@ int ctsim_plan_back(const ctsim_plan* plan, const char* op, int32_t view0, int32
                     const void* d_sino, int32_t sino_dtype, void* d_img, int32_t img_dtype, double scale,
                     int32_t accumulate, void* stream);
 
+/* Backprojection of a view selection fused with the Nesterov-SQS subset update of
+ * statsrecon/pwls/reconPwlsSeNesterovSqs.m:67-87 (Huber penalty, penalty/huberPenalty.m):
+ * the gradient scale * A^T sino of each voxel column goes straight into the update of its
+ * voxels, as ctsim_pwls_nesterov_update (or _seq, when d_tseq is not null) computes it from
+ * a gradient volume, without storing that volume.  3D FP32 plans, op "back,dd" or
+ * "back,tf", pfun 0 (Huber), detectors of at most 64 rows; CTSIM_EUNSUP otherwise (the
+ * caller then runs ctsim_plan_back and the update).  zcopy (FP32) receives z' - zref (z'
+ * when zref is null) unless null.  No counterpart in the reference, where the
+ * backprojection and the update are separate MATLAB statements. */
+int ctsim_plan_back_nesterov(const ctsim_plan* plan, const char* op, int32_t view0, int32_t vstep, int32_t nsub,
+                             const void* d_sino, int32_t sino_dtype, double scale, const float* x, const float* z,
+                             float* s, const float* precom, float* x_out, float* z_out, float* zcopy,
+                             const float* zref, double beta, double delta, int32_t pfun, double t, double t_next,
+                             const double* d_tseq, const int32_t* d_base, int32_t k, void* stream);
+
 /* Exhaustive-candidate variant of ctsim_plan_proj used by tests: every image column
  * is considered for every detector tile (no strip pruning); results must be
  * identical to ctsim_plan_proj. */

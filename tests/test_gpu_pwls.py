@@ -282,3 +282,23 @@ def test_nesterov_update_step_both_kernel_forms(nz, pfun):
     np.testing.assert_allclose(back(ds), sn, rtol=1e-5, atol=1e-8)
     np.testing.assert_allclose(back(zo), zn, rtol=1e-5, atol=1e-8)
     np.testing.assert_allclose(back(zc), zn, rtol=1e-3, atol=1e-6)
+
+
+@pytest.mark.parametrize("mixed,graph,with_map", [(True, True, False), (False, False, False), (True, True, True)])
+def test_fused_backprojection_update_equals_separate_kernels(mixed, graph, with_map):
+    """ctsim_plan_back_nesterov (the backprojection with the update in its epilogue) agrees with
+    ctsim_plan_back followed by the update kernel, masked columns included. Not bit for bit: the
+    separate update kernel's four-voxel form sums the penalty's neighbour terms in another
+    order than the fused epilogue's one-voxel form (a last-ulp difference per step)"""
+    from helpers import geometry_from_g3
+    from ctsim_amd.pwls import PwlsOS
+    g, y, w, x0 = _problem(seed=7)
+    mp = (np.random.default_rng(8).uniform(size=g.n[:2]) < 0.85) if with_map else None
+    beta, delta, M, its = 50.0, 1e-3, 4, 3
+    a = PwlsOS(geometry_from_g3(g, mp), y, w, M, beta, delta, mixed=mixed, graph=graph, fused=True)
+    b = PwlsOS(geometry_from_g3(g, mp), y, w, M, beta, delta, mixed=mixed, graph=graph, fused=False)
+    assert a.fused and not b.fused
+    xa, pa, _ = a.run(x0.astype(np.float32), its)
+    xb, pb, _ = b.run(x0.astype(np.float32), its)
+    assert rel_l2(xa, xb) < 1e-6, rel_l2(xa, xb)
+    np.testing.assert_allclose(pa, pb, rtol=1e-6)
