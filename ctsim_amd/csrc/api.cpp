@@ -69,6 +69,8 @@ Geom<T> make_geom(const ctsim_geom& g) {
     k.wu = (T)(-(g.nu - 1.0) / 2.0 + (double)g.ou);
     k.wv = (T)(-(g.nv - 1.0) / 2.0 + (double)g.ov);
     k.ou = (T)g.ou;
+    k.zstride = g.nz;
+    k.zoff = T(0);
     return k;
 }
 
@@ -341,6 +343,9 @@ static int plan_proj_impl(const ctsim_plan* p, const char* op, int view0, int vs
     if (e == hipErrorInvalidValue)
         return fail(CTSIM_EUNSUP, "Error: dtype combination (img %d, sino %d) not supported for compute %d.", img_dt,
                     sino_dt, p->compute);
+    if (e == hipErrorNotSupported)
+        return fail(CTSIM_EUNSUP, "Error: 3D forward projection supports nu <= 32767 and nz <= 8192 (nu = %d, nz = %d).",
+                    p->g.nu, p->g.nz);
     HIP_TRY(e, "forward projection launch");
     return CTSIM_OK;
 }
@@ -402,8 +407,8 @@ int ctsim_plan_back(const ctsim_plan* p, const char* op, int32_t view0, int32_t 
                     img_dt, p->compute);
     if (e == hipErrorNotSupported)
         return fail(CTSIM_EUNSUP,
-                    "Error: 3D backprojection supports up to 512 slices (nz = %d) and 64 views under 4 GiB.",
-                    p->g.nz);
+                    "Error: 3D backprojection needs 64 views of the detector under 4 GiB (nu = %d, nv = %d).",
+                    p->g.nu, p->g.nv);
     HIP_TRY(e, "backprojection launch");
     return CTSIM_OK;
 }

@@ -57,6 +57,7 @@ static_assert(kPre * kWave + kZPad <= kZWin + kZPad, "the prefetched slots and p
 constexpr int kZBits = 13;
 constexpr int kZMask = (1 << kZBits) - 1;
 constexpr int kMaxNz = 1 << kZBits;
+constexpr int kBackSlab = 8 * kWave;  // slices per backprojector launch (8 voxels per lane)
 
 template <class T>
 __device__ __forceinline__ int gather_class(T zinc) {
@@ -930,7 +931,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
     if (colid >= g.nx * g.ny) return;
     const int ix = colid % g.nx, iy = colid / g.nx;
     const int lane = lane_id();
-    Tout* ocol = vol + (size_t)colid * g.nz;
+    Tout* ocol = vol + (size_t)colid * g.zstride;
     const bool masked = map != nullptr && map[iy * g.nx + ix] == 0;
     if (masked && !FUSE) {
         if (!accumulate)
@@ -970,8 +971,9 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 ok = clamp_cells(tau[0], tau[3], g.nu, 16, clo, chi, ilo, ihi);
                 vc.p0 = tau[0]; vc.p1 = tau[1]; vc.p2 = tau[2]; vc.p3 = tau[3];
             }
+            const T wz = v.wz + g.zoff;
             if (ok)
-                ok = v_window(g, cg.d, v.wz, vc.zinc, vstart, iz0, iz1) && iz0 < iz1 && vc.zinc > T(0);
+                ok = v_window(g, cg.d, wz, vc.zinc, vstart, iz0, iz1) && iz0 < iz1 && vc.zinc > T(0);
             if (ok) {
                 // rows touched by voxels iz0 .. iz1-1
                 T a0 = vstart + T(iz0) * vc.zinc + T(0.5);
@@ -994,7 +996,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 vc.r2 = cg.r * cg.r;
                 vc.vs15 = vstart + T(1.5);
                 vc.vstart = vstart;
-                vc.wzdz = v.wz * g.dz;
+                vc.wzdz = wz * g.dz;
                 vc.voff = ((unsigned)(lane * g.nu + ilo) * (unsigned)g.nv + (unsigned)r0) * (unsigned)sizeof(Tin);
             }
         }
@@ -1266,7 +1268,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_pd_kernel(Geom<T> g, const V
     if (ix >= g.nx || iy >= g.ny) return;
     const int colid = iy * g.nx + ix;
     const int lane = lane_id();
-    Tout* ocol = vol + (size_t)colid * g.nz;
+    Tout* ocol = vol + (size_t)colid * g.zstride;
     if (map != nullptr && map[iy * g.nx + ix] == 0) {
         if (!accumulate)
             for (int iz = lane; iz < g.nz; iz += kWave) ocol[iz] = cvt_store<Tout>(T(0));
@@ -1300,11 +1302,11 @@ __global__ __launch_bounds__(kBlock) void cbct_back_pd_kernel(Geom<T> g, const V
         // v of voxel iz: (iz + wz) dz w2 / dv - wv, evaluated as iz A + B with the view's
         // A and B (one FMA per voxel; within an ulp or two of the reference's order)
         const T A = g.dz * w2 / g.dv;
-        const T B = v.wz * A - g.wv;
+        const T B = (v.wz + g.zoff) * A - g.wv;
         // 'back,pd' stops at the first voxel above the detector; v is monotone in iz, so
-        // that is a per-voxel test when v grows and "nothing if voxel 0 is already above"
-        // when it shrinks
-        if (off_boundary && !(A > T(0)) && B > lim.v_hi) continue;
+        // that is a per-voxel test when v grows and "nothing if voxel 0 of the volume is
+        // already above" when it shrinks
+        if (off_boundary && !(A > T(0)) && v.wz * A - g.wv > lim.v_hi) continue;
         const Tin* p = proj + ((size_t)b * g.nu + iu) * g.nv;
 #pragma unroll
         for (int c = 0; c < NZL; ++c) {
@@ -1533,9 +1535,22 @@ static hipError_t back3(const Geom<T>& g, const View<T>* views, int vstride, int
                                       : (unsigned)((cols + kWavesPerBlock - 1) / kWavesPerBlock);
     if (cols == 0 || blocks == 0) return hipSuccess;
     if (foot == 2 && (g.nu < 2 || g.nv < 2)) return hipErrorInvalidValue;  // bilinear needs 2x2 cells
-    if (g.nz > 8 * kWave) return hipErrorNotSupported;                      // voxel columns up to 512
     if (foot != 2 && (long long)kWave * g.nu * g.nv * (long long)sizeof(Tin) > 0xffffffffLL)
         return hipErrorNotSupported;  // 64 views' sinograms are addressed with 32-bit byte offsets
+    if (g.nz > kBackSlab) {
+        // a lane holds at most 8 voxels of a column: taller volumes are backprojected in
+        // z-slabs of kBackSlab slices, each a launch over its slices of every column
+        for (int z0 = 0; z0 < g.nz; z0 += kBackSlab) {
+            Geom<T> gs = g;
+            gs.nz = std::min(kBackSlab, g.nz - z0);
+            gs.zstride = g.zstride;
+            gs.zoff = g.zoff + T(z0);
+            hipError_t e = back3<Tin, Tout, T>(gs, views, vstride, nviews, proj, map,
+                                               static_cast<Tout*>(vol) + z0, foot, pd_mode, scale, accumulate, st);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     const int nzl = (g.nz + kWave - 1) / kWave;
     if (nzl <= 1) launch_back<Tin, Tout, T, 1>(blocks, st, g, views, vstride, nviews, proj, map, vol, foot, pd_mode, scale, accumulate);
     else if (nzl <= 2) launch_back<Tin, Tout, T, 2>(blocks, st, g, views, vstride, nviews, proj, map, vol, foot, pd_mode, scale, accumulate);

@@ -97,6 +97,11 @@ struct Geom {
     T sad, sdd, sdd_du, zk;
     T wx, wy, wu, wv;
     T ou;  // arc pd offset term uses offset_u directly (geom/cbct_os/cbct_back.c:204)
+    // backprojection of a z-slab of a taller volume: the column stride of the output
+    // buffer (nz of the whole volume) and the slab's first slice, added to each view's wz;
+    // zstride = nz and zoff = 0 for a whole volume
+    int zstride;
+    T zoff;
 };
 
 // In-plane column quantities (cbct_proj.c:491-533).

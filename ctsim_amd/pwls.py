@@ -1,7 +1,8 @@
 """Device-resident PWLS ordered-subset reconstruction.
 
-NesterovSqs restates statsrecon/pwls/reconPwlsSeNesterovSqs.m:37-137 (and Sqs the plain
-SQS loop of statsrecon/pwls/reconPwlsSeSqs.m:36-110) with the separable-footprint
+PwlsOS restates statsrecon/pwls/reconPwlsSeNesterovSqs.m:37-137 (PwlsSQS the plain SQS
+loop of statsrecon/pwls/reconPwlsSeSqs.m:36-110, PwlsLALM reconPwlsLALMOs.m, PwlsNUSQS
+reconPwlsSeNusqs.m; each can replay its iterations as captured hipGraphs) with the separable-footprint
 operators of geom/shell/loadPojectors.m and the Huber penalty of penalty/huberPenalty.m:
 
     precom = At(w .* A(ones))
@@ -141,7 +142,10 @@ class PwlsOS:
         # apart (the penalty's neighbour reads stall the backprojector's waves), so it is off
         can_fuse = (not self.f64 and self.pfun == 0 and op in ("dd", "tf") and nv <= 64 and nz <= 512
                     and (self.zc is None or self.zdt == F32))
-        self.fused = bool(fused) and can_fuse
+        if fused and not can_fuse:
+            raise ValueError("fused=True needs FP32 volumes, the Huber penalty, the dd or tf operator, a detector "
+                             "of at most 64 rows, at most 512 slices and an FP32 projector input copy")
+        self.fused = bool(fused)
         self.x0 = DeviceArray(nvox, F32) if self.offset else None  # reference image of the offset form
         self.scal = DeviceArray(4, "f64")  # phi, R, rmsd numerator, spare
         self.cur = 0  # which of the x / z double buffers is current
@@ -269,6 +273,21 @@ class PwlsOS:
         if device_t:
             call("ctsim_add_i32", self._base.vp, self.M, self._sv())
 
+    def _cost_tail(self) -> None:
+        """R(x) and the RMSD numerator of the iteration into the FP64 scalars (on the stream)"""
+        nx, ny, nz = self.nxyz
+        xc = self.x[self.cur]
+        sfx = "_f64" if self.f64 else ""
+        call("ctsim_penalty" + sfx, xc.vp, nx, ny, nz, self.pfun, self.delta, 0, None, C.c_void_p(self.scal.ptr + 8),
+             self._sv())
+        call("ctsim_sqdiff_sum" + sfx, xc.vp, self.xold.vp, xc.count, C.c_void_p(self.scal.ptr + 16), self._sv())
+
+    def _scalars(self) -> dict:
+        """phi, beta R(x) and rmsd of the finished iteration (after a stream synchronise)"""
+        sc = self.scal.to_numpy()
+        return dict(phi=float(sc[0] + self.beta * sc[1]), penalty=float(self.beta * sc[1]),
+                    rmsd=float(math.sqrt(sc[2] / self.P.n_img)))
+
     def prepare(self, itnlim: int) -> None:
         """for the graph form: momentum sequence on the device, one iteration captured"""
         if not self.graph_mode:
@@ -333,7 +352,8 @@ class PwlsLALM:
     volume buffers."""
 
     def __init__(self, geom: Geometry, y: np.ndarray, w: np.ndarray, numos: int, beta: float, delta: float,
-                 rho: float = 0.1, pfun: str = "huber", op: str = "tf", inner: int = 10, stream: Stream | None = None):
+                 rho: float = 0.1, pfun: str = "huber", op: str = "tf", inner: int = 10, stream: Stream | None = None,
+                 graph: bool = False):
         self.base = PwlsOS(geom, y, w, numos, beta, delta, precision="f32", graph=False, op=op, stream=stream,
                            pfun=pfun)
         self.rho = float(rho)
@@ -341,6 +361,7 @@ class PwlsLALM:
         nvox = self.base.P.n_img
         self.gacc = DeviceArray(nvox, F32)
         self.sdir = DeviceArray(nvox, F32)
+        self.graph = _IterationGraph(self.base, self._body) if graph else None
 
     def setup(self, x0: np.ndarray) -> None:
         self.base.setup(x0)
@@ -348,6 +369,17 @@ class PwlsLALM:
         self.itn = 0
 
     def iterate(self) -> dict:
+        b = self.base
+        if self.graph is not None:
+            self.graph.run()
+        else:
+            self._body()
+        b.stream.synchronize()
+        self.itn += 1
+        return b._scalars()
+
+    def _body(self) -> None:
+        """one iteration on the stream (every scalar is fixed, so it can be captured once)"""
         b = self.base
         st = b.stream
         nx, ny, nz = b.nxyz
@@ -366,14 +398,7 @@ class PwlsLALM:
                 call("ctsim_pwls_lalm_inner", src.vp, b.xold.vp, self.sdir.vp, b.precom.vp, dst.vp, nx, ny, nz,
                      b.beta, b.delta, b.pfun, self.rho, b._sv())
                 b.cur = 1 - b.cur
-        xc = b.x[b.cur]
-        call("ctsim_penalty", xc.vp, nx, ny, nz, b.pfun, b.delta, 0, None, C.c_void_p(b.scal.ptr + 8), b._sv())
-        call("ctsim_sqdiff_sum", xc.vp, b.xold.vp, xc.count, C.c_void_p(b.scal.ptr + 16), b._sv())
-        st.synchronize()
-        self.itn += 1
-        sc = b.scal.to_numpy()
-        return dict(phi=float(sc[0] + b.beta * sc[1]), penalty=float(b.beta * sc[1]),
-                    rmsd=float(math.sqrt(sc[2] / b.P.n_img)))
+        b._cost_tail()
 
     def image(self) -> np.ndarray:
         return self.base.image()
@@ -409,9 +434,12 @@ class PwlsNUSQS:
     NLOOP, NFIX = 3, 10
 
     def __init__(self, geom: Geometry, y: np.ndarray, w: np.ndarray, numos: int, beta: float, delta: float,
-                 pfun: str = "huber", op: str = "tf", stream: Stream | None = None):
+                 pfun: str = "huber", op: str = "tf", stream: Stream | None = None, graph: bool = False):
         self.base = PwlsOS(geom, y, w, numos, beta, delta, precision="f32", graph=False, op=op, stream=stream,
                            pfun=pfun)
+        if graph and self.base.M % 2:
+            raise ValueError("graph replay needs an even number of subsets (x double buffers)")
+        self.graph = _IterationGraph(self.base, self._body) if graph else None
         nvox = self.base.P.n_img
         self.v = DeviceArray(nvox, F32)
         self.dl = DeviceArray(nvox, F32)    # dL of the seed image
@@ -455,6 +483,20 @@ class PwlsNUSQS:
             call("ctsim_absdiff_floor", b.x[b.cur].vp, self.xref.vp, self.tmp.vp, self.tmp.count, self.EPS, b._sv())
             call("ctsim_dynamic_range", self.tmp.vp, self.tmp.count, 40.0, 0.05, self.v.vp, b._sv())
             self._data_curvature(self.dlt)
+        if self.graph is not None:
+            self.graph.run()
+        else:
+            self._body()
+        st.synchronize()
+        self.itn = itn
+        return b._scalars()
+
+    def _body(self) -> None:
+        """the subset loop of one iteration with the current dLtilde and H (capturable: the
+        schedule above changes buffer contents only)"""
+        b = self.base
+        st = b.stream
+        nx, ny, nz = b.nxyz
         b.xold.copy_from(b.x[b.cur], stream=st)
         call("ctsim_fill", C.c_void_p(b.scal.ptr), 3, 1, 0.0, b._sv())
         for k, (v0, vs, ns, off) in enumerate(b.subsets):
@@ -467,20 +509,103 @@ class PwlsNUSQS:
             call("ctsim_pwls_nusqs_update", x.vp, b.x[1 - b.cur].vp, b.g.vp, self.dlt.vp, self.h.vp, nx, ny, nz,
                  b.beta, b.delta, b.pfun, b._sv())
             b.cur = 1 - b.cur
-        xc = b.x[b.cur]
-        call("ctsim_penalty", xc.vp, nx, ny, nz, b.pfun, b.delta, 0, None, C.c_void_p(b.scal.ptr + 8), b._sv())
-        call("ctsim_sqdiff_sum", xc.vp, b.xold.vp, xc.count, C.c_void_p(b.scal.ptr + 16), b._sv())
-        st.synchronize()
-        self.itn = itn
-        sc = b.scal.to_numpy()
-        return dict(phi=float(sc[0] + b.beta * sc[1]), penalty=float(b.beta * sc[1]),
-                    rmsd=float(math.sqrt(sc[2] / b.P.n_img)))
+        b._cost_tail()
 
     def image(self) -> np.ndarray:
         return self.base.image()
 
     def run(self, x0: np.ndarray, itnlim: int, stop_crit: float = 1e-5):
         """reconPwlsSeNusqs main loop (stops after iteration 5 once rmsd < stop_crit)"""
+        self.setup(x0)
+        phis, rmsds = [], []
+        for itn in range(1, itnlim + 1):
+            out = self.iterate()
+            phis.append(out["phi"])
+            rmsds.append(out["rmsd"])
+            if itn > 5 and out["rmsd"] < stop_crit:
+                break
+        return self.image(), np.array(phis), np.array(rmsds)
+
+
+class _IterationGraph:
+    """One captured hipGraph of an iteration body that ping-pongs the x buffers of a PwlsOS
+    base.  The body is captured on the first run; its buffer choice is the one at capture
+    time, so the body must swap the buffers an even number of times (then every replay
+    starts and ends on the same buffer, as the eager loop does)."""
+
+    def __init__(self, base: PwlsOS, body):
+        self.base, self.body = base, body
+        self._graph = None
+
+    def run(self) -> None:
+        b = self.base
+        if self._graph is None:
+            cur0 = b.cur
+            with Graph(b.stream) as gr:
+                self.body()
+            if b.cur != cur0:
+                raise ValueError("graph replay needs an even number of x buffer swaps per iteration "
+                                 "(an even number of subsets)")
+            self._graph = gr
+        self._graph.launch()
+
+
+class PwlsSQS:
+    """Plain SQS with ordered subsets on the device (statsrecon/pwls/reconPwlsSeSqs.m:36-110),
+    FP32 storage, FP64 denominators and reductions:
+
+        precom = At(w .* A(ones))
+        per subset:  d = Aos(x) - Os(y);  x = max(x - (Atos(Os(w) .* d) + beta S(x)) ./
+                                                      (precom + beta T(x)), 0)
+        per iteration: phi = sum_sub sum(w d^2)/2 + beta R(x);  rmsd = rms(x - xold)
+
+    graph=True replays each iteration as one captured hipGraph (an even number of subsets)."""
+
+    def __init__(self, geom: Geometry, y: np.ndarray, w: np.ndarray, numos: int, beta: float, delta: float,
+                 pfun: str = "huber", op: str = "tf", stream: Stream | None = None, graph: bool = False):
+        self.base = PwlsOS(geom, y, w, numos, beta, delta, precision="f32", graph=False, op=op, stream=stream,
+                           pfun=pfun)
+        self.graph = _IterationGraph(self.base, self._body) if graph else None
+        if graph and self.base.M % 2:
+            raise ValueError("graph replay needs an even number of subsets (x double buffers)")
+
+    def setup(self, x0: np.ndarray) -> None:
+        self.base.setup(x0)
+        self.itn = 0
+
+    def _body(self) -> None:
+        b = self.base
+        st = b.stream
+        nx, ny, nz = b.nxyz
+        b.xold.copy_from(b.x[b.cur], stream=st)
+        call("ctsim_fill", C.c_void_p(b.scal.ptr), 3, 1, 0.0, b._sv())
+        for k, (v0, vs, ns, off) in enumerate(b.subsets):
+            x = b.x[b.cur]
+            pw = b._window(b.p, 0, ns)
+            b.P.proj(x, pw, f"proj,{b.kind}", v0, vs, ns, stream=st)
+            call("ctsim_pwls_residual", pw.vp, b.p.dtype, b._sub_ptr(b.y, off), b.y.dtype, b._sub_ptr(b.w, off),
+                 b.w.dtype, b.r.vp, b.r.dtype, ns * b.cell, b.wscale, C.c_void_p(b.scal.ptr), b._sv())
+            b.P.back(b._window(b.r, 0, ns), b.g, f"back,{b.kind}", v0, vs, ns, scale=float(b.M) * b.wscale, stream=st)
+            call("ctsim_pwls_sqs_update", x.vp, b.x[1 - b.cur].vp, b.g.vp, b.precom.vp, nx, ny, nz, b.beta, b.delta,
+                 b.pfun, b._sv())
+            b.cur = 1 - b.cur
+        b._cost_tail()
+
+    def iterate(self) -> dict:
+        b = self.base
+        if self.graph is not None:
+            self.graph.run()
+        else:
+            self._body()
+        b.stream.synchronize()
+        self.itn += 1
+        return b._scalars()
+
+    def image(self) -> np.ndarray:
+        return self.base.image()
+
+    def run(self, x0: np.ndarray, itnlim: int, stop_crit: float = 1e-5):
+        """reconPwlsSeSqs main loop (stops after iteration 5 once rmsd < stop_crit, :106)"""
         self.setup(x0)
         phis, rmsds = [], []
         for itn in range(1, itnlim + 1):

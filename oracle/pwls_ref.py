@@ -14,6 +14,7 @@ TEST INFRASTRUCTURE ONLY (imported by tests/): the checker of ctsim_amd.pwls.
   dynamic_range(x, t, eta)       statsrecon/pwls/dynamicRangeAdjustment.m
   nusqs(...)                     statsrecon/pwls/reconPwlsSeNusqs.m:32-121
   nesterov_sqs(...)              statsrecon/pwls/reconPwlsSeNesterovSqs.m:37-137
+  sqs(...)                       statsrecon/pwls/reconPwlsSeSqs.m:36-110
   lalm_os(...)                   statsrecon/pwls/reconPwlsLALMOs.m:37-110
 
 The projectors are the oracle's (oracle.cbct, FP64), called per ordered subset with the
@@ -274,6 +275,32 @@ def nesterov_sqs(y, w, g: O.G3, beta, delta, itnlim, numos, x0, map_=None, kind=
             z = (1 - 1 / t) * x + (1 / t) * s
             for v in (x, s, z):
                 v[(v < 0) | ~np.isfinite(v)] = 0
+            phi += np.sum(w[:, :, idx] * d ** 2) / 2
+        phi += beta * penalty(pfun, x, 0, delta)
+        phis.append(phi)
+        rmsds.append(math.sqrt(np.mean((x - xold) ** 2)))
+    return x, np.array(phis), np.array(rmsds)
+
+
+def sqs(y, w, g: O.G3, beta, delta, itnlim, numos, x0, pfun="huber", map_=None, kind="tf"):
+    """reconPwlsSeSqs.m:39-76: per subset x = max(x - (Atos(Os(w) d) + beta S(x)) ./
+    (precom + beta T(x)), 0) with d = Aos(x) - Os(y); returns x, phis, rmsds (FP64)"""
+    A = lambda v: O.cbct(f"proj,{kind}", g, v, map_, prec=64)  # noqa: E731
+    At = lambda s: O.cbct(f"back,{kind}", g, s, map_, prec=64)  # noqa: E731
+    subs = [subset_geom(g, numos, isub) for isub in range(1, numos + 1)]
+    x = np.asarray(x0, np.float64).copy()
+    precom = At(w * A(np.ones_like(x)))
+    phis, rmsds = [], []
+    for itn in range(1, itnlim + 1):
+        xold = x.copy()
+        phi = 0.0
+        for isub in range(1, numos + 1):
+            gs, idx = subs[isub - 1]
+            d = O.cbct(f"proj,{kind}", gs, x, map_, prec=64) - y[:, :, idx]
+            num = numos * O.cbct(f"back,{kind}", gs, w[:, :, idx] * d, map_, prec=64) + beta * penalty(
+                pfun, x, 1, delta)
+            x = x - num / (precom + beta * penalty(pfun, x, 2, delta))
+            x[(x < 0) | ~np.isfinite(x)] = 0
             phi += np.sum(w[:, :, idx] * d ** 2) / 2
         phi += beta * penalty(pfun, x, 0, delta)
         phis.append(phi)

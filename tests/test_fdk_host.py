@@ -115,3 +115,30 @@ def test_sliver_weights_of_conjugate_rays_sum_to_one():
         w1 = sliver_weight(np.array([g]), beta, tau)[0]
         w2 = sliver_weight(np.array([-g]), beta + math.pi - 2 * g, tau)[0]
         assert w1 + w2 == pytest.approx(1.0, abs=1e-12), (g, beta, w1, w2)
+
+
+@pytest.mark.parametrize("flat", [1, 0])
+def test_fdk_cosine_weight_special_cases(flat):
+    """With an all-pass filter (H = 1) fdk_filter returns the cosine weight of reconFDK.m:39
+    (flat) / :44 (arc).  On the mid row (v = 0) the flat weight reduces to SAD/SDD for every u
+    (the row-wise form, not the textbook SAD/sqrt(u^2+v^2+SDD^2)), and on the mid column
+    (u = 0) it is SAD/sqrt(v^2 + SDD^2); the arc weight on the mid row is SAD/SDD cos(u/SDD)
+    and on the mid column SAD/SDD."""
+    nu, nv = 41, 33
+    g = Geometry(reconSize=(8, 8, 8), reconSpacing=(1, 1, 1), reconOffset=(0, 0, 0), detSize=(nu, nv),
+                 detSpacing=(3.0, 4.0), detOffset=(0, 0), SAD=400.0, ADD=250.0, betas=circular_betas(4),
+                 flatPanel=flat)
+    one = np.ones((nv, nu, 1))
+    wgt = fdk_ref.fdk_filter(one, g, np.ones(64))[:, :, 0]
+    u = (np.arange(nu) - (nu - 1) / 2) * 3.0
+    v = (np.arange(nv) - (nv - 1) / 2) * 4.0
+    mid_v, mid_u = (nv - 1) // 2, (nu - 1) // 2
+    if flat:
+        np.testing.assert_allclose(wgt[mid_v, :], 400.0 / 650.0, rtol=1e-13)
+        np.testing.assert_allclose(wgt[:, mid_u], 400.0 / np.sqrt(v ** 2 + 650.0 ** 2), rtol=1e-13)
+        assert wgt[0, mid_u] < wgt[mid_v, mid_u]
+    else:
+        np.testing.assert_allclose(wgt[mid_v, :], 400.0 / 650.0 * np.cos(u / 650.0), rtol=1e-13)
+        # cos(u / sqrt(SDD^2 + v^2)) is one on the mid column whatever v is
+        np.testing.assert_allclose(wgt[:, mid_u], 400.0 / 650.0, rtol=1e-13)
+        assert wgt[0, 0] > wgt[mid_v, 0]

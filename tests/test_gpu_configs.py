@@ -1,0 +1,393 @@
+"""GPU parity on the five target workloads at their full sizes, against the oracle.
+
+Every projector result is held to the strict gate: relative L2 <= 1e-5 against the FP32
+oracle (which equals the reference's compiled CPU kernels bit for bit), with the relative
+L2 against the FP64 oracle printed beside it, and matched-pair adjoint mismatch
+|<Ax,y> - <x,A^T y>| / (|Ax| |y|) < 1e-6.  None of these geometries reaches the 16-cell
+footprint cap (the widest u footprint is about 3 cells, the tallest v footprint about 2
+rows), so no tolerance is widened.
+
+The oracle runs with OpenMP over views on the host's cores; where a workload has too many
+views for that to finish in seconds the test takes view subsets of the full volume and
+detector (every view of a circular orbit costs the same, and each subset exercises the
+same kernels and launch forms as the full scan).
+
+  1  fan_256         256^2, 360 views, 512 bins: full DD pair
+  2  sf_pair_512     512^2 x 256, 1024 x 384: 8-view SF/DD pairs and a 48-view forward
+                     (>= 16384 work items at nz = 256: the one-wave-per-item forward kernel
+                     the full 720-view operator runs), FP32 and FP16 volumes
+  3  fdk_512         1024 x 768 detector, 512^3: FP16 filtered views of 8 views, back,hpd;
+                     the device row filter against the FP64 numpy filter
+  4  pwls_512x128    512^2 x 128, 1024 x 64, 984 views in 24 subsets: one 41-view subset
+                     forward and back (x M) with the FOV map, the residual, and one Nesterov-SQS
+                     subset step against an FP64 restatement on the oracle's projections
+  5  batch2_uneven   64 uneven fan-beam slices in one launch: FP32 per slice against the
+                     oracle, FP16 storage within its precision
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+ADJ = 1e-6
+NT = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu_present():
+    from ctsim_amd import device
+    if device.device_count() < 1:
+        pytest.fail("no GPU visible to the HIP runtime")
+
+
+def g3_of(geom, idx=None):
+    from oracle import oracle as O
+    b = geom.betas if idx is None else geom.betas[idx]
+    cz = geom.couchZ if idx is None else geom.couchZ[idx]
+    return O.G3(geom.reconSize, geom.reconSpacing, geom.reconOffset, geom.detSize, geom.detSpacing, geom.detOffset,
+                geom.SAD, geom.ADD, geom.SDD, b, couchz=cz, dettype=geom.flatPanel)
+
+
+def g2_of(geom):
+    from oracle import oracle as O
+    return O.G2(geom.reconSize[0], geom.reconSize[1], geom.reconSpacing[0], geom.reconSpacing[1],
+                geom.reconOffset[0], geom.reconOffset[1], geom.detSize[0], geom.detSpacing[0], geom.detOffset[0],
+                geom.SAD, geom.ADD, geom.SDD, geom.betas, geom.flatPanel)
+
+
+def adjoint_mismatch(ax, y, x, aty):
+    ax, y, x, aty = (np.asarray(a, np.float64).ravel() for a in (ax, y, x, aty))
+    return abs(ax.dot(y) - x.dot(aty)) / (np.linalg.norm(ax) * np.linalg.norm(y))
+
+
+def check(name, got, want32, want64=None):
+    e32 = rel_l2(got, want32)
+    e64 = rel_l2(got, want64) if want64 is not None else float("nan")
+    say(f"  {name}: rel-L2 vs FP32 oracle {e32:.2e}, vs FP64 oracle {e64:.2e}")
+    assert e32 <= TOL, (name, e32)
+
+
+# ---------------------------------------------------------------------------------------
+# 1: 2D DD pair, 256^2, 360 views, 512 bins
+# ---------------------------------------------------------------------------------------
+def test_config1_fan_256_dd_pair():
+    from oracle import oracle as O
+    from ctsim_amd import workloads as W
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    geom = W.fan_256()
+    g = g2_of(geom)
+    img = W.ellipse_image(256, 0.8, 10, seed=1)
+    say("config 1: fan_256", geom.reconSize, geom.detSize, geom.noViews)
+    P = Projector(geom)
+    d_sino = DeviceArray(P.n_sino(), "f32")
+    P.proj(P.upload_volume(img), d_sino, "proj,dd")
+    sino = P.download_sino(d_sino)
+    check("proj,dd", sino, O.fbct("proj,dd", g, img, nthreads=NT), O.fbct("proj,dd", g, img, prec=64, nthreads=NT))
+    d_img = DeviceArray(P.n_img, "f32")
+    P.back(P.upload_sino(sino), d_img, "back,dd")
+    check("back,dd", P.download_volume(d_img), O.fbct("back,dd", g, sino, nthreads=NT),
+          O.fbct("back,dd", g, sino, prec=64, nthreads=NT))
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(geom.reconSize).astype(np.float32)
+    y = rng.standard_normal(geom.sino_shape()).astype(np.float32)
+    P.proj(P.upload_volume(x), d_sino, "proj,dd")
+    P.back(P.upload_sino(y), d_img, "back,dd")
+    m = adjoint_mismatch(P.download_sino(d_sino), y, x, P.download_volume(d_img))
+    say(f"  adjoint mismatch {m:.2e}")
+    assert m < ADJ
+
+
+# ---------------------------------------------------------------------------------------
+# 2: 3D SF (and DD) pair, 512^2 x 256, 1024 x 384
+# ---------------------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def config2():
+    from ctsim_amd import workloads as W
+    geom = W.sf_pair_512()
+    vol = W.ellipsoid_phantom(geom.reconSize, geom.reconSpacing, 20, seed=2)
+    return geom, vol
+
+
+@pytest.mark.timeout(300)
+def test_config2_sf_pair_512_eight_views(config2):
+    from oracle import oracle as O
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    geom, vol = config2
+    v0, vs, ns = 3, 90, 8
+    idx = np.arange(v0, geom.noViews, vs)[:ns]
+    g = g3_of(geom, idx)
+    say("config 2: sf_pair_512, views", idx.tolist())
+    P = Projector(geom)
+    d_vol = P.upload_volume(vol)
+    d_sino = DeviceArray(P.n_sino(ns), "f32")
+    d_img = DeviceArray(P.n_img, "f32")
+    for kind in ("tf", "dd"):
+        P.proj(d_vol, d_sino, f"proj,{kind}", v0, vs, ns)
+        sino = P.download_sino(d_sino, ns)
+        check(f"proj,{kind}", sino, O.cbct(f"proj,{kind}", g, vol, nthreads=NT),
+              O.cbct(f"proj,{kind}", g, vol, prec=64, nthreads=NT) if kind == "tf" else None)
+        P.back(P.upload_sino(sino), d_img, f"back,{kind}", v0, vs, ns)
+        check(f"back,{kind}", P.download_volume(d_img), O.cbct(f"back,{kind}", g, sino, nthreads=NT),
+              O.cbct(f"back,{kind}", g, sino, prec=64, nthreads=NT) if kind == "tf" else None)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(geom.reconSize).astype(np.float32)
+    y = rng.standard_normal(geom.sino_shape(ns)).astype(np.float32)
+    for kind in ("tf", "dd"):
+        P.proj(P.upload_volume(x), d_sino, f"proj,{kind}", v0, vs, ns)
+        P.back(P.upload_sino(y), d_img, f"back,{kind}", v0, vs, ns)
+        m = adjoint_mismatch(P.download_sino(d_sino, ns), y, x, P.download_volume(d_img))
+        say(f"  adjoint mismatch {kind} {m:.2e}")
+        assert m < ADJ
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("vdtype", ["f32", "f16"])
+def test_config2_forward_one_wave_per_item(config2, vdtype):
+    """48 views x 64 tiles x 6 row chunks = 18432 work items at 256 slices: the forward kernel
+    that the full 720-view operator runs (each wave walks its whole strip, no split combine)"""
+    from oracle import oracle as O
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    geom, vol = config2
+    v0, vs, ns = 0, 15, 48
+    idx = np.arange(v0, geom.noViews, vs)[:ns]
+    assert ns * (geom.detSize[0] // 16) * math.ceil(geom.detSize[1] / 64) >= 16384
+    g = g3_of(geom, idx)
+    v = vol.astype(np.float16).astype(np.float32) if vdtype == "f16" else vol
+    say("config 2: 48-view forward,", vdtype, "volume")
+    P = Projector(geom)
+    d_vol = P.upload_volume(v, dtype=vdtype)
+    d_sino = DeviceArray(P.n_sino(ns), "f32")
+    for kind in ("tf", "dd"):
+        P.proj(d_vol, d_sino, f"proj,{kind}", v0, vs, ns)
+        check(f"proj,{kind} ({vdtype} volume)", P.download_sino(d_sino, ns), O.cbct(f"proj,{kind}", g, v, nthreads=NT))
+
+
+# ---------------------------------------------------------------------------------------
+# 3: FDK, 1024 x 768 detector, 512^3
+# ---------------------------------------------------------------------------------------
+@pytest.mark.timeout(300)
+def test_config3_fdk_512_filter_and_back_hpd():
+    from oracle import fdk_ref
+    from oracle import oracle as O
+    from ctsim_amd import fdk
+    from ctsim_amd import workloads as W
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    geom = W.fdk_512()
+    v0, vs, ns = 5, 90, 8
+    idx = np.arange(v0, geom.noViews, vs)[:ns]
+    nu, nv = geom.detSize
+    rng = np.random.default_rng(4)
+    # line integrals of a smooth body-sized object plus Poisson noise at I0 = 1e5
+    u = (np.arange(nu) - (nu - 1) / 2) * geom.detSpacing[0]
+    vv = (np.arange(nv) - (nv - 1) / 2) * geom.detSpacing[1]
+    U, V = np.meshgrid(u, vv)
+    p = 6.0 * np.sqrt(np.clip(1 - (U / 170.0) ** 2 - (V / 140.0) ** 2, 0, None))
+    counts = rng.poisson(1e5 * np.exp(-p), size=(ns, nv, nu)).astype(np.float64)
+    sino = np.asfortranarray((-np.log(np.maximum(counts, 10) / 1e5)).transpose(1, 2, 0)).astype(np.float32)
+    say("config 3: fdk_512, views", idx.tolist())
+    H = fdk.fdk_filter_response(geom, "hann")
+    vw = fdk.fdk_view_weights(geom)
+    want_f = fdk_ref.fdk_filter(sino, geom, H, None if vw is None else vw[idx])
+    filt = fdk.FdkFilter(geom, "hann")
+    d_in = DeviceArray.from_numpy(sino.ravel(order="F"), "f32")
+    # the plan numbers views 0..nsub-1 from view0; a circular full scan has no per-view weights
+    assert vw is None
+    f32 = filt(d_in, DeviceArray(d_in.count, "f32"), view0=0, nsub=ns).to_numpy().reshape(sino.shape, order="F")
+    e = rel_l2(f32, want_f)
+    say(f"  row filter FP32 vs FP64 numpy filter {e:.2e}")
+    assert e <= TOL
+    d16 = filt(d_in, DeviceArray(d_in.count, "f16"), view0=0, nsub=ns)
+    f16 = d16.to_numpy().reshape(sino.shape, order="F").astype(np.float32)
+    assert rel_l2(f16, want_f) < 1e-3
+    g = g3_of(geom, idx)
+    P = Projector(geom)
+    d_img = DeviceArray(P.n_img, "f32")
+    P.back(d16, d_img, "back,hpd", v0, vs, ns)
+    check("back,hpd (FP16 filtered views)", P.download_volume(d_img), O.cbct("back,hpd", g, f16, nthreads=NT),
+          O.cbct("back,hpd", g, f16, prec=64, nthreads=NT))
+
+
+# ---------------------------------------------------------------------------------------
+# 4: PWLS-OS, 512^2 x 128, 1024 x 64, 984 views, 24 subsets
+# ---------------------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def config4():
+    from ctsim_amd import workloads as W
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    geom = W.pwls_512x128()
+    xt = W.body_phantom(geom.reconSize, geom.reconSpacing, 10, seed=4)
+    P = Projector(geom)
+    d_p = DeviceArray(P.n_sino(), "f32")
+    P.proj(P.upload_volume(xt), d_p, "proj,tf")
+    p = P.download_sino(d_p).astype(np.float64)
+    rng = np.random.default_rng(5)
+    counts = rng.poisson(2e4 * np.exp(-p)).astype(np.float64)
+    y = -np.log(np.maximum(counts, 10) / 2e4)
+    return geom, xt, y, counts
+
+
+@pytest.mark.timeout(300)
+def test_config4_pwls_subset_pair(config4):
+    from oracle import oracle as O
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    geom, xt, y, w = config4
+    M, isub = 24, 1
+    v0, vs, ns = Projector.subset(geom.noViews, M, isub)
+    assert ns == 41
+    idx = np.arange(v0, geom.noViews, vs)
+    g = g3_of(geom, idx)
+    say("config 4: pwls_512x128 subset", isub, "of", M, "views", idx[:4].tolist(), "...")
+    P = Projector(geom)
+    d_s = DeviceArray(P.n_sino(ns), "f32")
+    P.proj(P.upload_volume(xt), d_s, "proj,tf", v0, vs, ns)
+    check("proj,tf subset", P.download_sino(d_s, ns), O.cbct("proj,tf", g, xt, geom.map, nthreads=NT),
+          O.cbct("proj,tf", g, xt, geom.map, prec=64, nthreads=NT))
+    r = np.asfortranarray(w[:, :, idx] * (P.download_sino(d_s, ns) - y[:, :, idx])).astype(np.float32)
+    d_img = DeviceArray(P.n_img, "f32")
+    P.back(P.upload_sino(r), d_img, "back,tf", v0, vs, ns, scale=M)
+    want = np.float32(M) * O.cbct("back,tf", g, r, geom.map, nthreads=NT)
+    check("back,tf subset x M", P.download_volume(d_img), want,
+          M * O.cbct("back,tf", g, r, geom.map, prec=64, nthreads=NT))
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(geom.reconSize).astype(np.float32)
+    yy = rng.standard_normal(geom.sino_shape(ns)).astype(np.float32)
+    P.proj(P.upload_volume(x), d_s, "proj,tf", v0, vs, ns)
+    P.back(P.upload_sino(yy), d_img, "back,tf", v0, vs, ns)
+    m = adjoint_mismatch(P.download_sino(d_s, ns), yy, x * geom.map[:, :, None], P.download_volume(d_img))
+    say(f"  adjoint mismatch {m:.2e}")
+    assert m < ADJ
+
+
+@pytest.mark.timeout(300)
+def test_config4_pwls_subset_step(config4):
+    """one Nesterov-SQS subset step of the plain FP32 form at full size (Huber, beta 1e4,
+    delta 1e-2 /cm) against reconPwlsSeNesterovSqs.m:65-87 in FP64 on the oracle's
+    projections; precom is the device's (it sums all 984 views)"""
+    from oracle import oracle as O
+    from oracle import pwls_ref as R
+    from ctsim_amd.pwls import PwlsOS
+    geom, xt, y, w = config4
+    M, beta, delta = 24, 1e4, 1e-2
+    x0 = (0.9 * xt).astype(np.float32)
+    sol = PwlsOS(geom, y, w, M, beta, delta, precision="f32")
+    sol.setup(x0)
+    precom = sol.P.download_volume(sol.precom).astype(np.float64)
+    t = 1.0
+    t_next = (1.0 + math.sqrt(5.0)) / 2.0
+    sol._subset_step(0, t, t_next)
+    sol.stream.synchronize()
+    x1 = sol.P.download_volume(sol.x[sol.cur])
+    z1 = sol.P.download_volume(sol.z[sol.cur])
+    phi_data = float(sol.scal.to_numpy()[0])
+    # FP64: isub = 1 selects views 2, 26, ... (1-based) -> view0 = 1
+    v0, vs, ns, _ = sol.subsets[0]
+    idx = np.arange(v0, geom.noViews, vs)
+    g = g3_of(geom, idx)
+    z = x0.astype(np.float64)
+    d = O.cbct("proj,tf", g, z, geom.map, prec=64, nthreads=NT) - y[:, :, idx]
+    grad = M * O.cbct("back,tf", g, w[:, :, idx] * d, geom.map, prec=64, nthreads=NT) + beta * R.huber(z, 1, delta)
+    du = grad / (precom + beta * R.huber(z, 2, delta))
+    xw = z - du
+    sw = z - t * du
+    zw = (1 - 1 / t_next) * xw + sw / t_next
+    xw[xw < 0] = 0
+    zw[zw < 0] = 0
+    e_du = rel_l2(x0 - x1.astype(np.float64), x0 - xw)
+    say(f"  step: x rel-L2 {rel_l2(x1, xw):.2e}, z rel-L2 {rel_l2(z1, zw):.2e}, step size rel-L2 {e_du:.2e}")
+    assert rel_l2(x1, xw) <= TOL and rel_l2(z1, zw) <= TOL
+    assert e_du <= 1e-4
+    phi_want = float(np.sum(w[:, :, idx] * d * d) / 2)
+    say(f"  data term {phi_data:.6e} vs {phi_want:.6e}")
+    assert phi_data == pytest.approx(phi_want, rel=1e-5)
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("sdt,tol", [("f32", 1e-6), ("bf16", 4e-3)])
+def test_config4_residual_subset(config4, sdt, tol):
+    """r = w (p - y) over one 41-view subset with FP64 cost, in the storage dtypes of the
+    plain (FP32) and mixed (BF16) forms"""
+    from ctsim_amd._lib import call
+    from ctsim_amd.device import DeviceArray, bf16_from_f32, f32_from_bf16
+    geom, xt, y, w = config4
+    idx = np.arange(1, geom.noViews, 24)
+    rng = np.random.default_rng(7)
+    ys = np.asfortranarray(y[:, :, idx]).ravel(order="F").astype(np.float32)
+    ws = (np.asfortranarray(w[:, :, idx]).ravel(order="F") / w.max()).astype(np.float32)
+    p = (ys + rng.normal(0, 0.05, ys.size)).astype(np.float32)
+    if sdt == "bf16":
+        ys, ws = f32_from_bf16(bf16_from_f32(ys)), f32_from_bf16(bf16_from_f32(ws))
+    dp = DeviceArray.from_numpy(p, "f32")
+    dy = DeviceArray.from_numpy(ys, sdt)
+    dw = DeviceArray.from_numpy(ws, sdt)
+    dr = DeviceArray(p.size, sdt)
+    phi = DeviceArray(1, "f64").fill(0.0)
+    call("ctsim_pwls_residual", dp.vp, dp.dtype, dy.vp, dy.dtype, dw.vp, dw.dtype, dr.vp, dr.dtype, p.size,
+         float(w.max()), phi.vp, None)
+    d = p.astype(np.float64) - ys
+    r = dr.to_numpy().astype(np.float64)
+    e = rel_l2(r, ws * d)
+    say(f"  residual {sdt}: rel-L2 {e:.2e}")
+    assert e < tol
+    assert phi.to_numpy()[0] == pytest.approx(w.max() * np.sum(ws * d * d) / 2, rel=1e-9)
+
+
+# ---------------------------------------------------------------------------------------
+# 5: 64 uneven fan-beam slices in one launch
+# ---------------------------------------------------------------------------------------
+@pytest.mark.timeout(600)
+def test_config5_batch2_uneven_64_slices():
+    from oracle import oracle as O
+    from ctsim_amd import workloads as W
+    from ctsim_amd.batch2 import Batch2
+    from ctsim_amd.device import DeviceArray
+    geoms, imgs = W.batch2_uneven()
+    say("config 5: batch2_uneven,", len(geoms), "slices")
+    B = Batch2(geoms)
+    gs = [g2_of(g) for g in geoms]
+    d_img = DeviceArray.from_numpy(B.pack_images(imgs), "f32")
+    d_sino = DeviceArray(B.sino_elems, "f32")
+    d_back = DeviceArray(B.img_elems, "f32")
+    B.proj(d_img, d_sino, "proj,dd")
+    sinos = B.unpack_sinos(d_sino.to_numpy())
+    B.back(d_sino, d_back, "back,dd")
+    backs = B.unpack_images(d_back.to_numpy())
+    worst_p = worst_b = 0.0
+    for g, im, s, bk in zip(gs, imgs, sinos, backs):
+        worst_p = max(worst_p, rel_l2(s, O.fbct("proj,dd", g, im, nthreads=NT)))
+        worst_b = max(worst_b, rel_l2(bk, O.fbct("back,dd", g, s, nthreads=NT)))
+    say(f"  FP32 worst slice: proj,dd {worst_p:.2e}, back,dd {worst_b:.2e}")
+    assert worst_p <= TOL and worst_b <= TOL
+    # FP16 images and sinograms, FP32 accumulation: against the oracle on the FP16-rounded
+    # inputs; the FP16 output rounding (11-bit significand) bounds the error
+    imgs16 = [im.astype(np.float16).astype(np.float32) for im in imgs]
+    d_img16 = DeviceArray.from_numpy(B.pack_images(imgs16), "f16")
+    d_sino16 = DeviceArray(B.sino_elems, "f16")
+    B.proj(d_img16, d_sino16, "proj,dd")
+    s16 = [s.astype(np.float32) for s in B.unpack_sinos(d_sino16.to_numpy())]
+    d_back16 = DeviceArray(B.img_elems, "f32")
+    B.back(d_sino16, d_back16, "back,dd")
+    b16 = B.unpack_images(d_back16.to_numpy())
+    worst_p = worst_b = 0.0
+    for g, im, s, bk in zip(gs, imgs16, s16, b16):
+        worst_p = max(worst_p, rel_l2(s, O.fbct("proj,dd", g, im, nthreads=NT)))
+        worst_b = max(worst_b, rel_l2(bk, O.fbct("back,dd", g, s, nthreads=NT)))
+    say(f"  FP16 storage worst slice: proj,dd {worst_p:.2e}, back,dd (FP32 out) {worst_b:.2e}")
+    assert worst_p < 1e-3 and worst_b <= TOL

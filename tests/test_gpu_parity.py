@@ -578,3 +578,24 @@ def test_proj_rd_random_geometries_vs_oracle(seed):
     want2 = O.fbct("proj,rd", g2, img)
     got2 = _per_op("ctsim_fbct_proj_rd", g, betas, None, None, img, (nu, nviews))
     assert rel_l2(got2, want2) <= RD_TOL
+
+
+def test_cbct_back_taller_than_one_launch_vs_oracle():
+    """more than 512 slices: the backprojectors run in z-slabs of 512 (a lane holds at most
+    eight voxels of a column); every slab against the oracle's whole-column loop"""
+    from oracle import oracle as O
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    g, vol, sino, mp = _cone_case(12, n=(10, 12, 1100), nuv=(40, 180), nviews=6, dz=0.2, dv=1.4)
+    say("slab case", g.n, g.nuv)
+    P = Projector(geometry_from_g3(g, mp))
+    d_sino = P.upload_sino(sino)
+    out = DeviceArray(P.n_img, "f32")
+    for op in ("back,dd", "back,tf", "back,pd", "back,hpd"):
+        P.back(d_sino, out, op)
+        got = P.download_volume(out)
+        want = O.cbct(op, g, sino, mp)
+        assert rel_l2(got, want) <= _tol(op, g, sino, mp), (op, rel_l2(got, want))
+    # accumulate adds every slab once
+    P.back(d_sino, out, "back,tf", accumulate=True)
+    assert rel_l2(P.download_volume(out), 2 * O.cbct("back,tf", g, sino, mp)) <= TOL

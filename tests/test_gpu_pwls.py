@@ -302,3 +302,50 @@ def test_fused_backprojection_update_equals_separate_kernels(mixed, graph, with_
     xb, pb, _ = b.run(x0.astype(np.float32), its)
     assert rel_l2(xa, xb) < 1e-6, rel_l2(xa, xb)
     np.testing.assert_allclose(pa, pb, rtol=1e-6)
+
+
+def test_sqs_matches_fp64_oracle():
+    """the plain SQS loop (reconPwlsSeSqs.m:51-76) on the device against its FP64 restatement"""
+    from oracle import pwls_ref as R
+    from ctsim_amd.pwls import PwlsSQS
+    g, y, w, x0 = _problem(seed=15)
+    beta, delta, M, its = 50.0, 1e-3, 4, 3
+    want, phis, rmsds = R.sqs(y, w, g, beta, delta, its, M, x0)
+    sol = PwlsSQS(_geometry(g), y, w, M, beta, delta)
+    x, ph, rm = sol.run(x0.astype(np.float32), its)
+    print("sqs vs fp64 oracle:", rel_l2(x, want), ph, phis)
+    assert rel_l2(x, want) < 1e-4
+    np.testing.assert_allclose(ph, phis, rtol=1e-4)
+    np.testing.assert_allclose(rm, rmsds, rtol=1e-3)
+
+
+@pytest.mark.parametrize("loop", ["sqs", "lalm", "nusqs"])
+def test_graph_replay_equals_stream_execution(loop):
+    """each iteration captured once as a hipGraph and replayed: the same kernels with the same
+    arguments, so every replay gives the bits of the launch-by-launch loop (four NU-SQS
+    iterations cover the schedule steps that run between replays)"""
+    from ctsim_amd.pwls import PwlsLALM, PwlsNUSQS, PwlsSQS
+    cls = {"sqs": PwlsSQS, "lalm": PwlsLALM, "nusqs": PwlsNUSQS}[loop]
+    g, y, w, x0 = _problem(seed=16)
+    beta, delta, M, its = 50.0, 1e-3, 4, 4 if loop == "nusqs" else 3
+    a = cls(_geometry(g), y, w, M, beta, delta, graph=True)
+    xa, pa, ra = a.run(x0.astype(np.float32), its)
+    b = cls(_geometry(g), y, w, M, beta, delta, graph=False)
+    xb, pb, rb = b.run(x0.astype(np.float32), its)
+    assert np.array_equal(xa, xb)
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(ra, rb)
+
+
+def test_graph_replay_rejects_odd_subset_counts():
+    from ctsim_amd.pwls import PwlsSQS
+    g, y, w, x0 = _problem(seed=16)
+    with pytest.raises(ValueError, match="even number of subsets"):
+        PwlsSQS(_geometry(g), y, w, 3, 50.0, 1e-3, graph=True)
+
+
+def test_fused_request_that_cannot_be_honoured_raises():
+    from ctsim_amd.pwls import PwlsOS
+    g, y, w, x0 = _problem(seed=16)
+    with pytest.raises(ValueError, match="fused=True"):
+        PwlsOS(_geometry(g), y, w, 4, 50.0, 1e-3, precision="f32", pfun="hyper", fused=True)
