@@ -1,9 +1,9 @@
 /*
  * cbct_sf_cuda - mexFunction answering to CTSim's cbct_sf_cuda name, replacing
- * geom/sf_cuda/cbct_sf_cuda.cu (16 arguments):proj,tf | back,tf
- * with libctsim_hip.so.  ops: .
+ * geom/sf_cuda/cbct_sf_cuda.cu with libctsim_hip.so.  16 arguments.
+ * ops: proj,tf | back,tf.
  *   out = cbct_sf_cuda(op, nxyz, dxyz, cxyz, nuv, duv, cuv, SAD, ADD, SDD, noviews, betas,
- *               couchz, detector_type, data, map)
+ *                      couchz, detector_type, data, map)
  */
 #include "ctsim_mex_common.h"
 
