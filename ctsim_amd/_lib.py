@@ -54,6 +54,8 @@ SIGNATURES = {
     "ctsim_cbct_dd_cuda": (C.c_int, _GATEWAY3),
     "ctsim_cbct_sf_cuda": (C.c_int, _GATEWAY3),
     "ctsim_cbct_back_pd_cuda": (C.c_int, _GATEWAY3),
+    "ctsim_cbct_geom_pmat_mex": (C.c_int, [C.c_char_p, P_i32, P_f32, P_f32, P_i32, f32, f32, i32, P_f32, P_f32,
+                                           P_f32, P_f32, P_f32, P_u8, P_f32, C.c_char_p, sz]),
     "ctsim_fbct_proj_rd": (C.c_int, _PER_OP),
     "ctsim_cbct_proj_rd": (C.c_int, _PER_OP),
     "ctsim_fbct_proj_dd": (C.c_int, _PER_OP),

@@ -18,7 +18,7 @@ OUT = os.path.join(HERE, "libctsim_hip.so")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("CTSIM_ARCH", "gfx950")
 
-SOURCES = ["cbct.hip", "fbct.hip", "pwls.hip", "fdk.hip", "sim.hip", "rd.hip", "api.cpp"]
+SOURCES = ["cbct.hip", "fbct.hip", "pwls.hip", "fdk.hip", "sim.hip", "rd.hip", "pmat.hip", "api.cpp"]
 FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

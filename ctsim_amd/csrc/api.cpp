@@ -633,6 +633,124 @@ int ctsim_cbct_back_pd_cuda(const char* op, const int32_t nxyz[3], const float d
                                    betas, couchz, dettype, in, map, out, errbuf, errlen);
 }
 
+}  // extern "C"
+
+namespace {
+
+// device buffers of one PMat gateway call, freed on every return path
+struct DevBufs {
+    void* p[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    ~DevBufs() {
+        for (void* q : p)
+            if (q) (void)hipFree(q);
+    }
+};
+
+float f32_at_least(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, INFINITY);
+    return f;
+}
+float f32_at_most(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+
+int pmat_gateway(const char* op, const int32_t* nxyz, const float* dxyz, const float* cxyz, const int32_t* nuv,
+                 float du, float dv, int32_t noviews, const float* pmat, const float* cameras, const float* sdds,
+                 const float* cuvs, const float* in, const uint8_t* map, float* out) {
+    if (!op) return fail(CTSIM_EINVAL, "The 1st argument is invalid!");
+    int kind;
+    const bool proj = !std::strncmp(op, "proj", 4);
+    if (!std::strcmp(op, "proj,rd"))
+        return fail(CTSIM_EINVAL, "Error: ray driven method is not supported. Please consider using distance driven "
+                                  "method.");
+    else if (!std::strcmp(op, "proj,dd") || !std::strcmp(op, "proj,tf")) kind = PMAT_PROJ_DD;
+    else if (!std::strcmp(op, "back,dd") || !std::strcmp(op, "back,tf")) kind = PMAT_BACK_DD;
+    else if (!std::strcmp(op, "back,pd")) kind = PMAT_BACK_PD;
+    else if (!std::strcmp(op, "back,hpd")) kind = PMAT_BACK_HPD;
+    else return fail(CTSIM_EINVAL, "Error: unknown geom type.");
+    if (!nxyz || !dxyz || !cxyz || !nuv) return fail(CTSIM_EINVAL, "Error: incorrect number of arguments.");
+    const int nx = nxyz[0], ny = nxyz[1], nz = nxyz[2], nu = nuv[0], nv = nuv[1];
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nu <= 0 || nv <= 0 || noviews <= 0)
+        return fail(CTSIM_EINVAL, "Error: sizes must be positive.");
+    if (!in)
+        return fail(CTSIM_EINVAL, proj ? "Error: mx_image must have [nx ny nz] float array."
+                                       : "Error: mx_proj must have [nv nu noviews] float array.");
+    if (!pmat) return fail(CTSIM_EINVAL, "Error: mx_pmat must have [3 4 noViews ] float array.");
+    if (!cameras) return fail(CTSIM_EINVAL, "Error: mx_cameras must have noviews X 2 float array.");
+    if (!cuvs) return fail(CTSIM_EINVAL, "Error: mx_cuvs must have noviews X 2 float array.");
+    if (!sdds) return fail(CTSIM_EINVAL, "Error: mx_sdds must have noviews X 1 float array.");
+    if (!map) return fail(CTSIM_EINVAL, "Error: mx_map must have [nx ny] bool array.");
+    if (!out) return fail(CTSIM_EINVAL, "Error: null data pointer.");
+    if (nz > 512) return fail(CTSIM_EUNSUP, "Error: the PMat operators take at most 512 slices (nz = %d).", nz);
+    if ((kind == PMAT_BACK_PD || kind == PMAT_BACK_HPD) && (nu < 2 || nv < 2))
+        return fail(CTSIM_EUNSUP, "Error: back,pd needs at least 2 x 2 detector cells.");
+    PmatConst c;
+    c.nx = nx; c.ny = ny; c.nz = nz; c.nu = nu; c.nv = nv; c.nviews = noviews;
+    c.dx = dxyz[0]; c.dy = dxyz[1]; c.dz = dxyz[2]; c.du = du; c.dv = dv;
+    // float wx = - ( nx - 1.0 ) / 2.0 + offset_x; wu, wv without offsets (cbct_proj_pmat.c:34-38)
+    c.wx = (float)(-(nx - 1.0) / 2.0 + (double)cxyz[0]);
+    c.wy = (float)(-(ny - 1.0) / 2.0 + (double)cxyz[1]);
+    c.wz = (float)(-(nz - 1.0) / 2.0 + (double)cxyz[2]);
+    c.wu = (float)(-(nu - 1.0) / 2.0);
+    c.wv = (float)(-(nv - 1.0) / 2.0);
+    c.lo = f32_at_least(1e-6);
+    c.u_hi = f32_at_most(nu - 1 - 1e-6);
+    c.v_hi = f32_at_most(nv - 1 - 1e-6);
+    std::vector<PmatView> views((size_t)noviews);
+    for (int b = 0; b < noviews; ++b) {
+        PmatView& v = views[(size_t)b];
+        std::memcpy(v.pm, pmat + 12 * (size_t)b, sizeof v.pm);
+        v.cx = cameras[3 * (size_t)b];
+        v.cy = cameras[3 * (size_t)b + 1];
+        v.sdd = sdds[b];
+        v.ou = cuvs[2 * (size_t)b];
+    }
+    std::lock_guard<std::mutex> lock(g_gateway_mutex);
+    const size_t nimg = (size_t)nx * ny * nz, nsino = (size_t)noviews * nu * nv;
+    DevBufs d;
+    // views, map, input, output (MATLAB layout), ZXY volume, amplitude-scaled sinogram (back,dd)
+    HIP_TRY(hipMalloc(&d.p[0], views.size() * sizeof(PmatView)), "device buffer");
+    HIP_TRY(hipMalloc(&d.p[1], (size_t)nx * ny), "device buffer");
+    HIP_TRY(hipMalloc(&d.p[2], (proj ? nimg : nsino) * 4), "device buffer");
+    HIP_TRY(hipMalloc(&d.p[3], (proj ? nsino : nimg) * 4), "device buffer");
+    HIP_TRY(hipMalloc(&d.p[4], nimg * 4), "device buffer");
+    if (kind == PMAT_BACK_DD) HIP_TRY(hipMalloc(&d.p[5], nsino * 4), "device buffer");
+    HIP_TRY(hipMemcpy(d.p[0], views.data(), views.size() * sizeof(PmatView), hipMemcpyHostToDevice), "upload");
+    HIP_TRY(hipMemcpy(d.p[1], map, (size_t)nx * ny, hipMemcpyHostToDevice), "upload");
+    HIP_TRY(hipMemcpy(d.p[2], in, (proj ? nimg : nsino) * 4, hipMemcpyHostToDevice), "upload");
+    hipStream_t st = nullptr;
+    const PmatView* dview = (const PmatView*)d.p[0];
+    const uint8_t* dmap = (const uint8_t*)d.p[1];
+    if (proj) {
+        HIP_TRY(xyz_zxy_launch(d.p[2], DT_F32, d.p[4], DT_F32, nx * ny, nz, true, st), "layout");
+        HIP_TRY(pmat_launch(kind, c, dview, dmap, (const float*)d.p[4], (float*)d.p[3], nullptr, st),
+                "PMat projection");
+        HIP_TRY(hipMemcpy(out, d.p[3], nsino * 4, hipMemcpyDeviceToHost), "download");
+    } else {
+        HIP_TRY(pmat_launch(kind, c, dview, dmap, (const float*)d.p[2], (float*)d.p[4], (float*)d.p[5], st),
+                "PMat backprojection");
+        HIP_TRY(xyz_zxy_launch(d.p[4], DT_F32, d.p[3], DT_F32, nx * ny, nz, false, st), "layout");
+        HIP_TRY(hipMemcpy(out, d.p[3], nimg * 4, hipMemcpyDeviceToHost), "download");
+    }
+    return CTSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ctsim_cbct_geom_pmat_mex(const char* op, const int32_t nxyz[3], const float dxyz[3], const float cxyz[3],
+                             const int32_t nuv[2], float du, float dv, int32_t noviews, const float* pmat,
+                             const float* cameras, const float* sdds, const float* cuvs, const float* in,
+                             const uint8_t* map, float* out, char* errbuf, size_t errlen) {
+    const int rc = pmat_gateway(op, nxyz, dxyz, cxyz, nuv, du, dv, noviews, pmat, cameras, sdds, cuvs, in, map, out);
+    if (rc) copy_err(errbuf, errlen);
+    return rc;
+}
+
 // one entry point per (dimension, operator): the gateway with the op string fixed
 #define CTSIM_PER_OP(NAME, NDIM, OPSTR)                                                                            \
     int NAME(const ctsim_geom* g, int32_t nviews, const double* betas, const float* couchz, const uint8_t* map,   \

@@ -1251,6 +1251,26 @@ static_assert(kWavesPerBlock == 4, "a block's waves form a 2 x 2 column patch");
 __host__ __device__ inline int pd_blocks(int nx, int ny) {
     return ((nx + kPdTile - 1) / kPdTile) * ((ny + kPdTile - 1) / kPdTile) * (kPdTile * kPdTile / kWavesPerBlock);
 }
+// One view's u, w2 and w2s at the column (xc, yc) in the reference's float operations,
+// one rounding each and no contraction (cbct_back.c:195-212): u, its cell and weight, and
+// w2s are then the reference's own.
+__device__ __forceinline__ void pd_view_col_f32(const Geom<float>& g, const View<float>& v, float xc, float yc,
+                                                float& u, float& w2, float& w2s) {
+#pragma clang fp contract(off)
+    const float xr = xc * v.c - yc * v.s;
+    const float yr = xc * v.s + yc * v.c;
+    if (g.dettype == 1) {
+        w2 = g.sdd / (g.sad + yr);
+        w2s = w2 * w2;
+        u = xr * w2 / g.du - g.wu;
+    } else {
+        const float roff = g.ou * g.sad / g.sdd;
+        u = atanf((xr - roff) / (g.sad + yr)) * g.sdd / g.du - g.wu;
+        w2s = g.sdd * g.sdd / ((g.sad + yr) * (g.sad + yr) + (xr - roff) * (xr - roff));
+        w2 = g.sdd / (g.sad + yr);
+    }
+}
+
 template <class Tin, class Tout, class T, int NZL, int OFF_BOUNDARY>
 __global__ __launch_bounds__(kBlock) void cbct_back_pd_kernel(Geom<T> g, const View<T>* __restrict__ views,
                                                               int vstride, int nviews, const Tin* __restrict__ proj,
@@ -1283,18 +1303,22 @@ __global__ __launch_bounds__(kBlock) void cbct_back_pd_kernel(Geom<T> g, const V
     for (int c = 0; c < NZL; ++c) img[c] = T(0);
     for (int b = 0; b < nviews; ++b) {
         const View<T> v = views[(size_t)b * vstride];
-        const T xr = xc * v.c - yc * v.s;
-        const T yr = xc * v.s + yc * v.c;
         T u, w2, w2s;
-        if (g.dettype == 1) {
-            w2 = g.sdd / (g.sad + yr);
-            w2s = w2 * w2;
-            u = xr * w2 / g.du - g.wu;
+        if constexpr (std::is_same<T, float>::value) {
+            pd_view_col_f32(g, v, xc, yc, u, w2, w2s);
         } else {
-            const T roff = g.ou * g.sad / g.sdd;
-            u = atan((xr - roff) / (g.sad + yr)) * g.sdd / g.du - g.wu;
-            w2s = g.sdd * g.sdd / ((g.sad + yr) * (g.sad + yr) + (xr - roff) * (xr - roff));
-            w2 = g.sdd / (g.sad + yr);
+            const T xr = xc * v.c - yc * v.s;
+            const T yr = xc * v.s + yc * v.c;
+            if (g.dettype == 1) {
+                w2 = g.sdd / (g.sad + yr);
+                w2s = w2 * w2;
+                u = xr * w2 / g.du - g.wu;
+            } else {
+                const T roff = g.ou * g.sad / g.sdd;
+                u = atan((xr - roff) / (g.sad + yr)) * g.sdd / g.du - g.wu;
+                w2s = g.sdd * g.sdd / ((g.sad + yr) * (g.sad + yr) + (xr - roff) * (xr - roff));
+                w2 = g.sdd / (g.sad + yr);
+            }
         }
         if (!(u >= lim.lo && u <= lim.u_hi)) continue;  // also skips NaN
         const int iu = (int)floor(u);

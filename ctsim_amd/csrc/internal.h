@@ -160,4 +160,23 @@ hipError_t dynamic_range_launch(const float* x, size_t n, double t, double eta, 
 hipError_t nusqs_launch(const float* x, float* x_out, const float* g, const float* dl, const float* h, int nx, int ny,
                         int nz, double beta, double delta, int pfun, hipStream_t st);
 
+
+// ---------------------------------------------------------------------------------
+// projection-matrix (PMat / C-arm) cone beam, pmat.hip
+// ---------------------------------------------------------------------------------
+struct PmatView {  // one view: the 3x4 matrix as stored by the gateway (column-major), camera x, y, SDD, u offset
+    float pm[12];
+    float cx, cy, sdd, ou;
+};
+struct PmatConst {
+    int nx, ny, nz, nu, nv, nviews;
+    float dx, dy, dz, du, dv;
+    float wx, wy, wz, wu, wv;
+    // the reference's double tests u, v >= 1e-6, u <= nu - 1 - 1e-6, v <= nv - 1 - 1e-6 as
+    // float thresholds
+    float lo, u_hi, v_hi;
+};
+enum { PMAT_PROJ_DD = 0, PMAT_BACK_DD = 1, PMAT_BACK_PD = 2, PMAT_BACK_HPD = 3 };
+hipError_t pmat_launch(int kind, const PmatConst& c, const PmatView* views, const uint8_t* map, const float* in,
+                       float* out, float* tmp, hipStream_t st);
 }  // namespace ctsim

@@ -66,27 +66,25 @@ static const double* ctsim_mx_betas(const mxArray* a, int32_t noviews) {
     return (const double*)mxGetData(a);
 }
 
-/* couchz: single, noviews elements */
+/* couchz: single, noviews elements (the reference's message says double:
+ * geom/cbct_os/cbct_geom_mex.c:143-144) */
 static const float* ctsim_mx_couchz(const mxArray* a, int32_t noviews) {
     if (!mxIsSingle(a) || (int64_t)mxGetNumberOfElements(a) != noviews)
-        mexErrMsgTxt("Error: mx_couchz must have noviews X 1 single array.");
+        mexErrMsgTxt("Error: mx_couchz must have noviews X 1 double array.");
     return (const float*)mxGetData(a);
 }
 
 /* map: logical (or uint8) [nx ny] */
 static const uint8_t* ctsim_mx_map(const mxArray* a, int32_t nx, int32_t ny) {
     if (!(mxIsLogical(a) || mxGetClassID(a) == mxUINT8_CLASS) || (int64_t)mxGetNumberOfElements(a) != (int64_t)nx * ny)
-        mexErrMsgTxt("Error: map must be a logical nx X ny array.");
+        mexErrMsgTxt("Error: mx_map must have [nx ny] bool array.");
     return (const uint8_t*)mxGetData(a);
 }
 
-/* data: single with exactly n elements (trailing singleton dimensions are dropped by MATLAB) */
-static const float* ctsim_mx_data(const mxArray* a, int64_t n, const char* what) {
-    if (!mxIsSingle(a) || mxIsComplex(a) || (int64_t)mxGetNumberOfElements(a) != n) {
-        char msg[160];
-        snprintf(msg, sizeof msg, "Error: %s must be a real single array of %lld elements.", what, (long long)n);
-        mexErrMsgTxt(msg);
-    }
+/* data: single with exactly n elements (trailing singleton dimensions are dropped by
+ * MATLAB); msg is the reference gateway's message for it */
+static const float* ctsim_mx_data(const mxArray* a, int64_t n, const char* msg) {
+    if (!mxIsSingle(a) || mxIsComplex(a) || (int64_t)mxGetNumberOfElements(a) != n) mexErrMsgTxt(msg);
     return (const float*)mxGetData(a);
 }
 
@@ -135,7 +133,9 @@ static void ctsim_mex_cbct(ctsim_cbct_fn fn, int require_map, int nlhs, mxArray*
     const int proj = ctsim_mx_is_proj(op);
     const int64_t nimg = (int64_t)nxyz[0] * nxyz[1] * nxyz[2];
     const int64_t nsino = (int64_t)nuv[0] * nuv[1] * noviews;
-    const float* in = ctsim_mx_data(prhs[14], proj ? nimg : nsino, proj ? "image" : "sinogram");
+    const float* in = ctsim_mx_data(prhs[14], proj ? nimg : nsino,
+                                    proj ? "Error: mx_image must have [nx ny nz] float array."
+                                         : "Error: mx_proj must have [nv nu noviews] float array.");
     const uint8_t* map = nrhs == 16 ? ctsim_mx_map(prhs[15], nxyz[0], nxyz[1]) : NULL;
     float* out = proj ? ctsim_mx_out(plhs, nuv[1], nuv[0], noviews) : ctsim_mx_out(plhs, nxyz[0], nxyz[1], nxyz[2]);
     if (fn(op, nxyz, dxyz, cxyz, nuv, duv, cuv, sad, add, sdd, noviews, betas, couchz, dettype, in, map, out, err,

@@ -30,7 +30,9 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     const int proj = ctsim_mx_is_proj(op);
     const int64_t nimg = (int64_t)nxy[0] * nxy[1];
     const int64_t nsino = (int64_t)nu * noviews;
-    const float* in = ctsim_mx_data(prhs[13], proj ? nimg : nsino, proj ? "image" : "sinogram");
+    const float* in = ctsim_mx_data(prhs[13], proj ? nimg : nsino,
+                                    proj ? "Error: mx_image must have [nx ny] float array."
+                                         : "Error: mx_proj must have [nu noviews] float array.");
     float* out = proj ? ctsim_mx_out(plhs, nu, noviews, 1) : ctsim_mx_out(plhs, nxy[0], nxy[1], 1);
     if (ctsim_fbct_geom_mex(op, nxy, dxy, oxy, nu, du, ou, sad, add, sdd, noviews, betas, dettype, in, out, err,
                             sizeof err))

@@ -1,0 +1,331 @@
+// Projection-matrix (PMat / C-arm) cone-beam operators: the reference's geom/pcbct
+// gateway (cbct_geom_pmat_mex.c), whose views are given by a 3x4 projection matrix, a
+// camera position, a source-to-detector distance and a detector offset each.
+//
+//   proj,dd  (proj,tf runs it too)   geom/pcbct/cbct_proj_pmat.c:8-215
+//   back,dd  (back,tf runs it too)   geom/pcbct/cbct_back_pmat.c:137-321
+//   back,pd / back,hpd               geom/pcbct/cbct_back_pmat.c:9-134
+//
+// Volumes are device ZXY (z fastest), sinograms [view][u][v].  FP32 throughout; the
+// footprint arithmetic follows the reference's float / double mix with no contraction,
+// so footprints, weights and amplitudes are the reference's own.
+//
+//  * back,pd / back,hpd and back,dd gather: one wavefront per image column, lanes are the
+//    column's voxels, every view in order; one writer per voxel, no atomics.
+//  * proj,dd: one wavefront per (view, image column), lanes are voxels, each voxel adding
+//    its footprint cells with FP32 global atomics (the reference's voxel-driven form; its
+//    per-cell amplitude is applied afterwards).  Results agree with the reference to FP32
+//    rounding; the order of the atomic additions is not fixed, so the last bits can vary
+//    between runs.  The footprint of a general projection matrix has no cheap inverse
+//    (the circular kernels' strip walk assumes the orbit), and this gateway is not on a
+//    timed path (no MATLAB caller in the reference tree).
+#include <algorithm>
+
+#include "internal.h"
+
+namespace ctsim {
+
+constexpr int kWave = 64;      // wavefront width
+constexpr int kPmFootU = 16;  // u cells per footprint (iumin + 15, cbct_proj_pmat.c:118)
+constexpr int kPmFootV = 10;  // v rows per voxel (FOOTPRINT_SIZE_V, ct_def.h:82)
+
+// u footprint of column (ix, iy) in view pv (cbct_proj_pmat.c:69-147): cells
+// [iumin, iumax], weights w, ampu, and v0 / f0 and dv/dz of the column; false when the
+// column misses the detector.
+__device__ __forceinline__ bool pmat_ufoot(const PmatConst& c, const PmatView& pv, int ix, int iy, float& ampu,
+                                           int& iumin, int& iumax, float* w, float& v0f, float& dvdz) {
+#pragma clang fp contract(off)
+    const float* pm = pv.pm;
+    const float xc = ((float)ix + c.wx) * c.dx;
+    const float yc = ((float)iy + c.wy) * c.dy;
+    const float u0 = xc * pm[0] + yc * pm[3] + c.wz * c.dz * pm[6] + pm[9];
+    const float v0 = xc * pm[1] + yc * pm[4] + c.wz * c.dz * pm[7] + pm[10];
+    const float f0 = xc * pm[2] + yc * pm[5] + pm[11];
+    const float dsp_x = (float)fabs((double)(pv.cx - xc));
+    const float dsp_y = (float)fabs((double)(pv.cy - yc));
+    float umin, umax;
+    const double dist = sqrt((double)(dsp_x * dsp_x + dsp_y * dsp_y));
+    if (fabs((double)dsp_x) > fabs((double)dsp_y)) {
+        umin = (u0 - c.dy * pm[3] / 2.0f) / (f0 - c.dy * pm[5] / 2.0f);
+        umax = (u0 + c.dy * pm[3] / 2.0f) / (f0 + c.dy * pm[5] / 2.0f);
+        ampu = (float)((double)c.dx * dist / (double)dsp_x / 10.0);
+    } else {
+        umin = (u0 - c.dx * pm[0] / 2.0f) / (f0 - c.dx * pm[2] / 2.0f);
+        umax = (u0 + c.dx * pm[0] / 2.0f) / (f0 + c.dx * pm[2] / 2.0f);
+        ampu = (float)((double)c.dx * dist / (double)dsp_y / 10.0);
+    }
+    if (umin >= umax) {
+        const float t = umax;
+        umax = umin;
+        umin = t;
+    }
+    const double a = (double)umin + 0.5, b = (double)umax + 0.5;
+    umin = (float)(a > 0.0 ? a : 0.0);
+    umax = (float)(b < (double)c.nu ? b : (double)c.nu);
+    if (umin > umax) return false;
+    iumin = (int)floorf(umin);
+    iumax = min((int)floorf(umax), c.nu - 1);
+    iumax = min(iumax, iumin + kPmFootU - 1);
+    if (iumin == iumax) {
+        w[0] = umax - umin;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kPmFootU; ++k) {
+            const int iu = iumin + k;
+            w[k] = iu == iumin ? (float)((double)iumin + 1.0 - (double)umin)
+                               : (iu == iumax ? umax - (float)iumax : (iu < iumax ? 1.0f : 0.0f));
+        }
+    }
+    v0f = v0 / f0;
+    dvdz = pm[7] / f0;
+    return true;
+}
+
+// v footprint of voxel iz (cbct_proj_pmat.c:155-190): rows [ivmin, ivmax] and weights
+__device__ __forceinline__ bool pmat_vfoot(const PmatConst& c, int iz, float v0f, float dvdz, int& ivmin,
+                                           int& ivmax, float* w) {
+#pragma clang fp contract(off)
+    const double fz = (double)(float)iz;
+    double lo, hi;
+    if (dvdz > 0.0f) {
+        lo = (double)v0f + (fz - 0.5) * (double)c.dz * (double)dvdz;
+        hi = (double)v0f + (fz + 0.5) * (double)c.dz * (double)dvdz;
+    } else {
+        lo = (double)v0f + (fz + 0.5) * (double)c.dz * (double)dvdz;
+        hi = (double)v0f + (fz - 0.5) * (double)c.dz * (double)dvdz;
+    }
+    float vmin = (float)lo, vmax = (float)hi;
+    const double a = (double)vmin + 0.5, b = (double)vmax + 0.5;
+    vmin = (float)(a > 0.0 ? a : 0.0);
+    vmax = (float)(b < (double)c.nv ? b : (double)c.nv);
+    if (vmin > vmax) return false;
+    ivmin = (int)floorf(vmin);
+    ivmax = min((int)floorf(vmax), c.nv - 1);
+    ivmax = min(ivmax, ivmin + kPmFootV - 1);
+    if (ivmin == ivmax) {
+        w[0] = vmax - vmin;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kPmFootV; ++k) {
+            const int iv = ivmin + k;
+            w[k] = iv == ivmin ? (float)((double)(float)ivmin + 1.0 - (double)vmin)
+                               : (iv == ivmax ? vmax - (float)ivmax : (iv < ivmax ? 1.0f : 0.0f));
+        }
+    }
+    return true;
+}
+
+// amplitude of detector cell (iu, iv) of a view (cbct_proj_pmat.c:197-210); the v
+// coordinate takes the u offset, as the reference writes it
+__device__ __forceinline__ float pmat_cell_amp(const PmatConst& c, const PmatView& pv, int iu, int iv) {
+#pragma clang fp contract(off)
+    const float u = ((float)iu + c.wu + pv.ou) * c.du;
+    const float uu = pv.sdd * pv.sdd + u * u;
+    const float v = ((float)iv + c.wv + pv.ou) * c.dv;
+    return (float)sqrt((double)((uu + v * v) / uu));
+}
+
+__global__ __launch_bounds__(256) void pmat_amp_kernel(PmatConst c, const PmatView* __restrict__ views,
+                                                       const float* __restrict__ in, float* __restrict__ out,
+                                                       size_t n) {
+    const size_t cells = (size_t)c.nu * c.nv;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int b = (int)(i / cells);
+        const int r = (int)(i - (size_t)b * cells);
+        const int iu = r / c.nv, iv = r - iu * c.nv;
+        out[i] = in[i] * pmat_cell_amp(c, views[b], iu, iv);
+    }
+}
+
+// forward: one wave per (view, image column), lanes are voxels
+__global__ __launch_bounds__(256) void pmat_proj_kernel(PmatConst c, const PmatView* __restrict__ views,
+                                                        const float* __restrict__ vol, const uint8_t* __restrict__ map,
+                                                        float* __restrict__ acc) {
+#pragma clang fp contract(off)
+    const int ncol = c.nx * c.ny;
+    const long long wid = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (wid >= (long long)ncol * c.nviews) return;
+    const int b = (int)(wid / ncol), col = (int)(wid - (long long)b * ncol);
+    if (!map[col]) return;
+    const int ix = col % c.nx, iy = col / c.nx;
+    const PmatView pv = views[b];
+    float wu[kPmFootU], wv[kPmFootV];
+    float ampu, v0f, dvdz;
+    int iumin, iumax;
+    if (!pmat_ufoot(c, pv, ix, iy, ampu, iumin, iumax, wu, v0f, dvdz)) return;
+    float* pview = acc + (size_t)b * c.nu * c.nv;
+    const float* vc = vol + (size_t)col * c.nz;
+    for (int iz = lane_id(); iz < c.nz; iz += kWave) {
+        const float val = vc[iz];
+        if (val == 0.0f) continue;
+        const float voxel = ampu * val;
+        int ivmin, ivmax;
+        if (!pmat_vfoot(c, iz, v0f, dvdz, ivmin, ivmax, wv)) continue;
+        for (int iu = iumin; iu <= iumax; ++iu)
+            for (int iv = ivmin; iv <= ivmax; ++iv)
+                atomicAdd(&pview[(size_t)iu * c.nv + iv], wu[iu - iumin] * wv[iv - ivmin] * voxel);
+    }
+}
+
+// back,dd: one wave per image column, lanes are voxels; sino is already scaled by the
+// cell amplitudes
+template <int NZL>
+__global__ __launch_bounds__(256) void pmat_back_dd_kernel(PmatConst c, const PmatView* __restrict__ views,
+                                                           const float* __restrict__ sino,
+                                                           const uint8_t* __restrict__ map, float* __restrict__ vol) {
+#pragma clang fp contract(off)
+    const int col = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (col >= c.nx * c.ny) return;
+    const int lane = lane_id();
+    float* ocol = vol + (size_t)col * c.nz;
+    float img[NZL];
+#pragma unroll
+    for (int k = 0; k < NZL; ++k) img[k] = 0.0f;
+    if (map[col]) {
+        const int ix = col % c.nx, iy = col / c.nx;
+        float wu[kPmFootU], wv[kPmFootV];
+        for (int b = 0; b < c.nviews; ++b) {
+            const PmatView pv = views[b];
+            float ampu, v0f, dvdz;
+            int iumin, iumax;
+            if (!pmat_ufoot(c, pv, ix, iy, ampu, iumin, iumax, wu, v0f, dvdz)) continue;
+            const float* pview = sino + (size_t)b * c.nu * c.nv;
+#pragma unroll
+            for (int k = 0; k < NZL; ++k) {
+                const int iz = k * kWave + lane;
+                if (iz >= c.nz) continue;
+                int ivmin, ivmax;
+                if (!pmat_vfoot(c, iz, v0f, dvdz, ivmin, ivmax, wv)) continue;
+                float voxel = 0.0f;
+                for (int iu = iumin; iu <= iumax; ++iu)
+                    for (int iv = ivmin; iv <= ivmax; ++iv)
+                        voxel += pview[(size_t)iu * c.nv + iv] * wu[iu - iumin] * wv[iv - ivmin];
+                img[k] += ampu * voxel;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NZL; ++k) {
+        const int iz = k * kWave + lane;
+        if (iz < c.nz) ocol[iz] = img[k];
+    }
+}
+
+// back,pd (OFF = 1: skip below the detector, stop the column above it) and back,hpd
+// (OFF = 0: clamp v to [0.5, nv - 1.5]); one wave per image column, lanes are voxels
+template <int NZL, int OFF>
+__global__ __launch_bounds__(256) void pmat_back_pd_kernel(PmatConst c, const PmatView* __restrict__ views,
+                                                           const float* __restrict__ sino,
+                                                           const uint8_t* __restrict__ map, float* __restrict__ vol) {
+#pragma clang fp contract(off)
+    const int col = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (col >= c.nx * c.ny) return;
+    const int lane = lane_id();
+    float* ocol = vol + (size_t)col * c.nz;
+    float img[NZL];
+#pragma unroll
+    for (int k = 0; k < NZL; ++k) img[k] = 0.0f;
+    if (map[col]) {
+        const int ix = col % c.nx, iy = col / c.nx;
+        const float xc = ((float)ix + c.wx) * c.dx;
+        const float yc = ((float)iy + c.wy) * c.dy;
+        for (int b = 0; b < c.nviews; ++b) {
+            const PmatView pv = views[b];
+            const float* pm = pv.pm;
+            const float u0 = xc * pm[0] + yc * pm[3] + c.wz * c.dz * pm[6] + pm[9];
+            const float v0 = xc * pm[1] + yc * pm[4] + c.wz * c.dz * pm[7] + pm[10];
+            const float f0 = xc * pm[2] + yc * pm[5] + c.wz * c.dz * pm[8] + pm[11];
+            const float w2s = pv.sdd * pv.sdd / ((pv.cx - xc) * (pv.cx - xc) + (pv.cy - yc) * (pv.cy - yc));
+            float uu[NZL], vv[NZL];
+            bool uok[NZL];
+            // back,pd breaks out of the column at the first voxel (in z order) whose u is on
+            // the detector and whose v is past its top: the lowest such iz over the lanes
+            int first_break = c.nz;
+#pragma unroll
+            for (int k = 0; k < NZL; ++k) {
+                const int iz = k * kWave + lane;
+                const float fz = (float)iz * c.dz;
+                const float den = f0 + fz * pm[8];
+                uu[k] = (u0 + fz * pm[6]) / den;
+                vv[k] = (v0 + fz * pm[7]) / den;
+                uok[k] = iz < c.nz && uu[k] >= c.lo && uu[k] <= c.u_hi;
+                if (OFF) {
+                    const unsigned long long brk = __ballot(uok[k] && vv[k] > c.v_hi);
+                    if (brk && first_break == c.nz) first_break = k * kWave + __ffsll((long long)brk) - 1;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NZL; ++k) {
+                const int iz = k * kWave + lane;
+                if (!uok[k] || iz >= first_break) continue;
+                float v = vv[k];
+                if (OFF) {
+                    if (!(v >= c.lo)) continue;
+                } else {
+                    v = (double)v > 0.5 ? v : 0.5f;
+                    v = (double)v < (double)c.nv - 1.5 ? v : (float)((double)c.nv - 1.5);
+                }
+                const float u = uu[k];
+                const int iu = (int)floorf(u);
+                const float wul = u - (float)iu;
+                const int iv = max(0, min((int)floorf(v), c.nv - 2));
+                const float wvl = v - (float)iv;
+                const float* p = sino + ((size_t)b * c.nu + iu) * c.nv + iv;
+                img[k] = img[k] + w2s * ((1.0f - wul) * ((1.0f - wvl) * p[0] + wvl * p[1]) +
+                                         wul * ((1.0f - wvl) * p[c.nv] + wvl * p[c.nv + 1]));
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NZL; ++k) {
+        const int iz = k * kWave + lane;
+        if (iz < c.nz) ocol[iz] = img[k];
+    }
+}
+
+template <int NZL>
+static hipError_t pmat_back_nzl(int kind, const PmatConst& c, const PmatView* views, const float* sino,
+                                const uint8_t* map, float* vol, hipStream_t st) {
+    const unsigned blocks = (unsigned)((c.nx * c.ny + 3) / 4);
+    if (kind == PMAT_BACK_DD)
+        hipLaunchKernelGGL(pmat_back_dd_kernel<NZL>, dim3(blocks), dim3(256), 0, st, c, views, sino, map, vol);
+    else if (kind == PMAT_BACK_PD)
+        hipLaunchKernelGGL((pmat_back_pd_kernel<NZL, 1>), dim3(blocks), dim3(256), 0, st, c, views, sino, map, vol);
+    else
+        hipLaunchKernelGGL((pmat_back_pd_kernel<NZL, 0>), dim3(blocks), dim3(256), 0, st, c, views, sino, map, vol);
+    return hipGetLastError();
+}
+
+hipError_t pmat_launch(int kind, const PmatConst& c, const PmatView* views, const uint8_t* map, const float* in,
+                       float* out, float* tmp, hipStream_t st) {
+    if (c.nz > 8 * kWave) return hipErrorNotSupported;
+    const size_t nsino = (size_t)c.nviews * c.nu * c.nv;
+    if (kind == PMAT_PROJ_DD) {
+        hipError_t e = fill_launch(out, DT_F32, nsino, 0.0, st);
+        if (e != hipSuccess) return e;
+        const long long waves = (long long)c.nx * c.ny * c.nviews;
+        hipLaunchKernelGGL(pmat_proj_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, c, views, in, map,
+                           out);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(pmat_amp_kernel, dim3((unsigned)std::min<size_t>((nsino + 255) / 256, 65536)), dim3(256), 0,
+                           st, c, views, out, out, nsino);
+        return hipGetLastError();
+    }
+    const float* src = in;
+    if (kind == PMAT_BACK_DD) {
+        // the reference scales its input in place (cbct_back_pmat.c:190-202); here a copy
+        hipLaunchKernelGGL(pmat_amp_kernel, dim3((unsigned)std::min<size_t>((nsino + 255) / 256, 65536)), dim3(256), 0,
+                           st, c, views, in, tmp, nsino);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        src = tmp;
+    }
+    const int nzl = (c.nz + kWave - 1) / kWave;
+    if (nzl <= 1) return pmat_back_nzl<1>(kind, c, views, src, map, out, st);
+    if (nzl <= 2) return pmat_back_nzl<2>(kind, c, views, src, map, out, st);
+    if (nzl <= 4) return pmat_back_nzl<4>(kind, c, views, src, map, out, st);
+    return pmat_back_nzl<8>(kind, c, views, src, map, out, st);
+}
+
+}  // namespace ctsim

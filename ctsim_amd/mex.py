@@ -142,3 +142,51 @@ def cbct_back_pd_cuda(op, nxyz, dxyz, cxyz, nuv, duv, cuv, sad, add, sdd, noview
         raise ValueError("Error: unknown geom type.")  # the CUDA FDK gateway rejects 'proj' (:155-157)
     return _cbct("ctsim_cbct_back_pd_cuda", ("back,pd", "back,hpd"), op, nxyz, dxyz, cxyz, nuv, duv, cuv, sad, add,
                  sdd, noviews, betas, couchz, detector_type, data, map, require_map=True)
+
+
+_OPS_PMAT = ("proj,rd", "proj,dd", "proj,tf", "back,pd", "back,hpd", "back,dd", "back,tf")
+
+
+def cbct_geom_pmat_mex(op, nxyz, dxyz, cxyz, nuv, duv, noviews, pmat, cameras, sdds, cuvs, data, map):
+    """Projection-matrix cone-beam gateway (geom/pcbct/cbct_geom_pmat_mex.c, 13 arguments).
+
+    pmat [3 4 noviews], cameras [3 noviews], sdds [noviews], cuvs [2 noviews], all single;
+    map logical [nx ny] (required).  duv of three elements is read as the reference reads it
+    (duv[1], duv[2]); two elements as (du, dv)."""
+    _check_op(op, _OPS_PMAT)
+    nxyz = _arr(nxyz, np.int32, 3)
+    dxyz = _arr(dxyz, np.float32, 3)
+    cxyz = _arr(cxyz, np.float32, 3)
+    nuv = _arr(nuv, np.int32, 2)
+    duv = _arr(duv, np.float32, 2)
+    du, dv = (float(duv[1]), float(duv[2])) if duv.size >= 3 else (float(duv[0]), float(duv[1]))
+    noviews = int(np.atleast_1d(noviews)[0])
+
+    def single(x, n, msg):
+        a = np.asarray(x)
+        if a.dtype != np.float32 or a.size != n:
+            raise ValueError(msg)
+        return np.asfortranarray(a)
+
+    pm = single(pmat, 12 * noviews, "Error: mx_pmat must have [3 4 noViews ] float array.")
+    cams = single(cameras, 3 * noviews, "Error: mx_cameras must have noviews X 2 float array.")
+    sd = single(sdds, noviews, "Error: mx_sdds must have noviews X 1 float array.")
+    cu = single(cuvs, 2 * noviews, "Error: mx_cuvs must have noviews X 2 float array.")
+    nx, ny, nz = (int(v) for v in nxyz)
+    nu, nv = int(nuv[0]), int(nuv[1])
+    if op.startswith("proj"):
+        src = _f32_data(data, (nx, ny, nz), "Error: mx_image must have [nx ny nz] float array.")
+        out = np.empty((nv, nu, noviews), np.float32, order="F")
+    else:
+        src = _f32_data(data, (nv, nu, noviews), "Error: mx_proj must have [nv nu noviews] float array.")
+        out = np.empty((nx, ny, nz), np.float32, order="F")
+    m = np.asarray(map)
+    if m.dtype != np.bool_ or m.size != nx * ny:
+        raise ValueError("Error: mx_map must have [nx ny] bool array.")
+    mp = np.ascontiguousarray(np.asfortranarray(m.reshape((nx, ny), order="F")).view(np.uint8).ravel(order="F"))
+    err = C.create_string_buffer(512)
+    call("ctsim_cbct_geom_pmat_mex", op.encode(), _ptr(nxyz, P_i32), _ptr(dxyz, P_f32), _ptr(cxyz, P_f32),
+         _ptr(nuv, P_i32), du, dv, noviews, pm.ctypes.data_as(P_f32), cams.ctypes.data_as(P_f32),
+         sd.ctypes.data_as(P_f32), cu.ctypes.data_as(P_f32), src.ctypes.data_as(P_f32), mp.ctypes.data_as(P_u8),
+         out.ctypes.data_as(P_f32), err, 512)
+    return out

@@ -96,6 +96,23 @@ int ctsim_cbct_back_pd_cuda(const char* op, const int32_t nxyz[3], const float d
                             int32_t detector_type, const float* in, const uint8_t* map, float* out, char* errbuf,
                             size_t errlen);
 
+/* Projection-matrix (PMat / C-arm) cone beam: geom/pcbct/cbct_geom_pmat_mex.c:260-309
+ * (13-argument gateway: op, nxyz, dxyz, cxyz, nuv, duv, noviews, pmat, cameras, sdds,
+ * cuvs, data, map).  pmat [3 4 noviews] single (column-major, u/v/w rows of each view),
+ * cameras [3 noviews] (x, y used), sdds [noviews], cuvs [2 noviews] (the u offset is the
+ * one used, for both detector coordinates, as in cbct_proj_pmat.c:197-205), map [nx ny]
+ * required.  du, dv are the detector pitches: the reference reads duv[1] and duv[2]
+ * (cbct_geom_pmat_mex.c:49-50); the MEX shim passes those for a three-element duv and
+ * duv[0], duv[1] for a two-element one.  ops proj,dd | proj,tf (runs dd, as the
+ * reference does after its warning) | back,dd | back,tf (dd) | back,pd | back,hpd;
+ * proj,rd is refused with the reference's message.  FP32, at most 512 slices.  proj,dd
+ * adds voxel footprints with FP32 atomics, so its last bits can vary from run to run;
+ * the backprojectors gather (one writer per voxel). */
+int ctsim_cbct_geom_pmat_mex(const char* op, const int32_t nxyz[3], const float dxyz[3], const float cxyz[3],
+                             const int32_t nuv[2], float du, float dv, int32_t noviews, const float* pmat,
+                             const float* cameras, const float* sdds, const float* cuvs, const float* in,
+                             const uint8_t* map, float* out, char* errbuf, size_t errlen);
+
 /* Geometry of one operator; mirrors the fields of CTSim's geom struct that the
  * gateways receive (reconSize, reconSpacing, reconOffset, detSize, detSpacing,
  * detOffset, SAD, ADD, SDD, flatPanel; parameters/loadProjectionGeometryCT.m).

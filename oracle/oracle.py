@@ -54,6 +54,12 @@ def lib():
                       C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.c_float,
                       C.c_float, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_float), C.c_int32, C.c_void_p,
                       C.POINTER(C.c_uint8), C.c_void_p, C.c_int]
+        f = L.oracle_pmat
+        f.restype = C.c_int
+        f.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                      C.POINTER(C.c_int32), C.c_float, C.c_float, C.c_int32, C.POINTER(C.c_float),
+                      C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint8),
+                      C.c_void_p, C.c_void_p, C.c_int]
         f = L.oracle_cbct_count
         f.restype = C.c_int
         f.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float),
@@ -203,4 +209,66 @@ def ref_cbct(op: str, g: G3, data: np.ndarray, map_=None) -> np.ndarray:
                         _p(m, C.c_uint8), _p(out, C.c_float))
     if rc:
         raise ValueError(f"ref_cbct({op!r}) failed")
+    return out
+
+
+class GP:
+    """Projection-matrix cone-beam geometry in the argument form of the PMat gateway
+    (geom/pcbct/cbct_geom_pmat_mex.c:19-128): pmat [3 4 nviews], cameras [3 nviews],
+    sdds [nviews], offset_uv [2 nviews] (float32, Fortran order), map [nx ny]."""
+
+    def __init__(self, n, d, o, nuv, du, dv, pmat, cameras, sdds, ouv, map_=None):
+        self.n = np.array(n, np.int32)
+        self.d = np.array(d, np.float32)
+        self.o = np.array(o, np.float32)
+        self.nuv = np.array(nuv, np.int32)
+        self.du, self.dv = float(du), float(dv)
+        self.pmat = np.asfortranarray(pmat, np.float32)
+        self.nviews = int(self.pmat.shape[2])
+        self.cameras = np.asfortranarray(cameras, np.float32)
+        self.sdds = np.ascontiguousarray(sdds, np.float32)
+        self.ouv = np.asfortranarray(ouv, np.float32)
+        self.map = (np.ones((int(n[0]), int(n[1])), np.uint8) if map_ is None
+                    else np.asfortranarray(map_, np.uint8))
+
+    def img_shape(self):
+        return tuple(int(x) for x in self.n)
+
+    def sino_shape(self):
+        return (int(self.nuv[1]), int(self.nuv[0]), self.nviews)
+
+
+def circular_pmat(betas, sad, sdd, nuv, duv, ouv=(0.0, 0.0)):
+    """Projection matrices, camera positions and distances of a circular flat-detector orbit
+    in the conventions of geom/cbct_os (source at (-SAD sin b, -SAD cos b), detector
+    coordinate t = x cos b - y sin b, depth D = SAD + x sin b + y cos b): u, v are detector
+    cell indices with cell centres at integers, u = (SDD / du) t / D - wu,
+    v = (SDD / dv) z / D - wv, w = -(n - 1) / 2 + offset.  Float64 arrays."""
+    betas = np.asarray(betas, np.float64)
+    nu, nv = nuv
+    wu = -(nu - 1) / 2.0 + ouv[0]
+    wv = -(nv - 1) / 2.0 + ouv[1]
+    P = np.zeros((3, 4, len(betas)))
+    cams = np.zeros((3, len(betas)))
+    for k, b in enumerate(betas):
+        s, c = np.sin(b), np.cos(b)
+        depth = np.array([s, c, 0.0, sad])
+        P[0, :, k] = (sdd / duv[0]) * np.array([c, -s, 0.0, 0.0]) - wu * depth
+        P[1, :, k] = (sdd / duv[1]) * np.array([0.0, 0.0, 1.0, 0.0]) - wv * depth
+        P[2, :, k] = depth
+        cams[:, k] = (-sad * s, -sad * c, 0.0)
+    ouvs = np.tile(np.asarray(ouv, np.float64)[:, None], (1, len(betas)))
+    return P, cams, np.full(len(betas), float(sdd)), ouvs
+
+
+def pmat(op: str, g: GP, data: np.ndarray, prec: int = 32, nthreads: int = 1) -> np.ndarray:
+    dt = np.float32 if prec == 32 else np.float64
+    src = np.asfortranarray(data, dt)
+    out = np.zeros(g.sino_shape() if op.startswith("proj") else g.img_shape(), dt, order="F")
+    rc = lib().oracle_pmat(op.encode(), prec, _p(g.n, C.c_int32), _p(g.d, C.c_float), _p(g.o, C.c_float),
+                           _p(g.nuv, C.c_int32), g.du, g.dv, g.nviews, _p(g.pmat, C.c_float),
+                           _p(g.cameras, C.c_float), _p(g.sdds, C.c_float), _p(g.ouv, C.c_float),
+                           _p(g.map, C.c_uint8), src.ctypes.data, out.ctypes.data, nthreads)
+    if rc:
+        raise ValueError(f"oracle_pmat({op!r}) failed with {rc}")
     return out

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 MEX_SRC = os.path.join(ROOT, "ctsim_amd", "csrc", "mex")
 LIB_DIR = os.path.join(HERE, "_lib")
-SHIMS = ["fbct_geom_mex", "cbct_geom_mex", "cbct_dd_cuda", "cbct_sf_cuda", "cbct_back_pd_cuda"]
+SHIMS = ["fbct_geom_mex", "cbct_geom_mex", "cbct_dd_cuda", "cbct_sf_cuda", "cbct_back_pd_cuda", "cbct_geom_pmat_mex"]
 CFLAGS = ["-std=c11", "-O1", "-fPIC", "-Wall", "-Wextra", "-Werror", "-Wno-unused-parameter",
           "-Wno-unused-function"]
 

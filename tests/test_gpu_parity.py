@@ -596,6 +596,7 @@ def test_cbct_back_taller_than_one_launch_vs_oracle():
         got = P.download_volume(out)
         want = O.cbct(op, g, sino, mp)
         assert rel_l2(got, want) <= _tol(op, g, sino, mp), (op, rel_l2(got, want))
-    # accumulate adds every slab once
+    # accumulate adds every slab once (onto a back,tf result)
+    P.back(d_sino, out, "back,tf")
     P.back(d_sino, out, "back,tf", accumulate=True)
     assert rel_l2(P.download_volume(out), 2 * O.cbct("back,tf", g, sino, mp)) <= TOL

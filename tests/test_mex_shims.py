@@ -50,7 +50,8 @@ def cone_args(op, data, map_=True, nviews=10, dettype=1, classes="matlab"):
 
 
 def test_shims_compile_against_the_mex_signatures(shims):
-    assert set(shims.fn) == {"fbct_geom_mex", "cbct_geom_mex", "cbct_dd_cuda", "cbct_sf_cuda", "cbct_back_pd_cuda"}
+    assert set(shims.fn) == {"fbct_geom_mex", "cbct_geom_mex", "cbct_dd_cuda", "cbct_sf_cuda", "cbct_back_pd_cuda",
+                             "cbct_geom_pmat_mex"}
 
 
 def test_argument_count_and_op_checks(shims):
@@ -70,6 +71,33 @@ def test_argument_count_and_op_checks(shims):
     assert shims.call("cbct_sf_cuda", *cone_args("proj,dd", vol))[1] == "Error: unknown geom type."
 
 
+def test_pmat_argument_checks(shims):
+    n, nuv, nviews = (8, 8, 4), (16, 8), 3
+    a = ["proj,dd", np.array(n, np.int32), np.ones(3, np.float32), np.zeros(3, np.float32), np.array(nuv, np.int32),
+         np.ones(2, np.float32), np.int32(nviews), np.zeros((3, 4, nviews), np.float32),
+         np.zeros((3, nviews), np.float32), np.ones(nviews, np.float32), np.zeros((2, nviews), np.float32),
+         np.zeros(n, np.float32), np.ones(n[:2], bool)]
+    assert shims.call("cbct_geom_pmat_mex", *a[:12])[1] == "Error: incorrect number of arguments."
+    b = list(a)
+    b[7] = b[7].astype(np.float64)
+    assert shims.call("cbct_geom_pmat_mex", *b)[1] == "Error: mx_pmat must have [3 4 noViews ] float array."
+    b = list(a)
+    b[8] = b[8][:2]
+    assert shims.call("cbct_geom_pmat_mex", *b)[1] == "Error: mx_cameras must have noviews X 2 float array."
+    b = list(a)
+    b[9] = b[9][:2]
+    assert shims.call("cbct_geom_pmat_mex", *b)[1] == "Error: mx_sdds must have noviews X 1 float array."
+    b = list(a)
+    b[10] = b[10].astype(np.float64)
+    assert shims.call("cbct_geom_pmat_mex", *b)[1] == "Error: mx_cuvs must have noviews X 2 float array."
+    b = list(a)
+    b[12] = b[12].astype(np.uint8)
+    assert shims.call("cbct_geom_pmat_mex", *b)[1] == "Error: mx_map must have [nx ny] bool array."
+    b = list(a)
+    b[0] = "back,xx"
+    assert shims.call("cbct_geom_pmat_mex", *b)[1] == "Error: unknown geom type."
+
+
 def test_argument_class_checks(shims):
     vol = np.zeros((20, 18, 12), np.float32)
     a = cone_args("proj,tf", vol)
@@ -80,15 +108,15 @@ def test_argument_class_checks(shims):
     assert shims.call("cbct_geom_mex", *a)[1] == "Error: mx_betas must have noviews X 1 double array."
     a = cone_args("proj,tf", vol)
     a[12] = a[12].astype(np.float64)
-    assert shims.call("cbct_geom_mex", *a)[1] == "Error: mx_couchz must have noviews X 1 single array."
+    assert shims.call("cbct_geom_mex", *a)[1] == "Error: mx_couchz must have noviews X 1 double array."
     a = cone_args("proj,tf", vol)
     a[14] = vol.astype(np.float64)
-    assert "image must be a real single array" in shims.call("cbct_geom_mex", *a)[1]
+    assert shims.call("cbct_geom_mex", *a)[1] == "Error: mx_image must have [nx ny nz] float array."
     a = cone_args("back,tf", vol)  # a volume where the sinogram belongs
-    assert "sinogram must be a real single array of 9600 elements" in shims.call("cbct_geom_mex", *a)[1]
+    assert shims.call("cbct_geom_mex", *a)[1] == "Error: mx_proj must have [nv nu noviews] float array."
     a = cone_args("proj,tf", vol)
     a[15] = a[15][:, :17]
-    assert shims.call("cbct_geom_mex", *a)[1] == "Error: map must be a logical nx X ny array."
+    assert shims.call("cbct_geom_mex", *a)[1] == "Error: mx_map must have [nx ny] bool array."
     a = cone_args("proj,tf", vol)
     a[1] = np.array([20, 18], np.int32)
     assert "nxyz must be a real numeric array of 3" in shims.call("cbct_geom_mex", *a)[1]
@@ -139,3 +167,31 @@ def test_shim_calls_match_the_oracle(shims, oracle_mod):
         print(f"  {name} {op} ({'map' if with_map else 'no map'}, {classes} classes): rel-L2 {e:.2e}", flush=True)
         assert e <= 1e-5, (name, op, e)
     assert np.array_equal(vol, vol0) and np.array_equal(sino, sino0)  # inputs untouched
+
+
+@pytest.mark.gpu
+def test_pmat_shim_matches_the_oracle(shims, oracle_mod):
+    """cbct_geom_pmat_mex through its shim: a three-element duv read as the reference reads it
+    (duv[1], duv[2]), a two-element one as (du, dv), against the PMat oracle."""
+    O = oracle_mod
+    n, nuv, nviews = (20, 18, 12), (40, 24), 9
+    P, cams, sdds, ouv = O.circular_pmat(np.linspace(0, 2 * np.pi, nviews, endpoint=False), 1000.0, 1500.0, nuv,
+                                         (0.5, 0.6))
+    m = np.ones(n[:2], bool)
+    m[:3, :] = False
+    g = O.GP(n, (0.6, 0.6, 0.8), (0.25, -0.5, 0.5), nuv, 0.5, 0.6, P, cams, sdds, ouv, map_=m)
+    rng = np.random.default_rng(5)
+    vol = np.asfortranarray(rng.uniform(0, 1, n).astype(np.float32))
+    sino = np.asfortranarray(rng.uniform(0, 1, g.sino_shape()).astype(np.float32))
+    for duv in (np.array([0.5, 0.6], np.float32), np.array([9.0, 0.5, 0.6], np.float32)):
+        for op in ("proj,dd", "back,dd", "back,hpd"):
+            data = vol if op.startswith("proj") else sino
+            out, err = shims.call("cbct_geom_pmat_mex", op, np.array(n, np.int32), g.d, g.o, np.array(nuv, np.int32),
+                                  duv, np.int32(nviews), g.pmat, g.cameras, g.sdds, g.ouv, data, m)
+            assert not err, (op, err)
+            e = rel_l2(out, O.pmat(op, g, data))
+            print(f"  cbct_geom_pmat_mex {op} (duv of {duv.size}): rel-L2 {e:.2e}", flush=True)
+            assert e <= 1e-5, (op, e)
+    out, err = shims.call("cbct_geom_pmat_mex", "proj,rd", np.array(n, np.int32), g.d, g.o, np.array(nuv, np.int32),
+                          np.array([0.5, 0.6], np.float32), np.int32(nviews), g.pmat, g.cameras, g.sdds, g.ouv, vol, m)
+    assert out is None and "ray driven method is not supported" in err
