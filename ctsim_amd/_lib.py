@@ -107,6 +107,7 @@ SIGNATURES = {
     "ctsim_graph_destroy": (C.c_int, [vp]),
     "ctsim_device_count": (C.c_int, [P_i32]),
     "ctsim_pwls_residual": (C.c_int, [vp, i32, vp, i32, vp, i32, vp, i32, sz, f64, vp, vp]),
+    "ctsim_pwls_residual_noalloc": (C.c_int, [vp, i32, vp, i32, vp, i32, vp, i32, sz, f64, vp, vp]),
     "ctsim_huber": (C.c_int, [vp, i32, i32, i32, f64, i32, vp, vp, vp]),
     "ctsim_pwls_nesterov_update": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, i32, i32, i32, f64, f64, i32, f64, f64, vp]),
     "ctsim_pwls_nesterov_update_seq": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, i32, i32, i32, f64, f64, i32, vp, vp, i32, vp]),

@@ -1122,6 +1122,15 @@ int ctsim_pwls_residual(const void* p, int32_t pdt, const void* y, int32_t ydt, 
     return CTSIM_OK;
 }
 
+int ctsim_pwls_residual_noalloc(const void* p, int32_t pdt, const void* y, int32_t ydt, const void* w, int32_t wdt,
+                                void* r, int32_t rdt, size_t n, double wscale, double* phi, void* stream) {
+    if (!p || !y || !w || !r || !phi) return fail(CTSIM_EINVAL, "Error: null argument.");
+    hipError_t e = residual_launch_noalloc(p, pdt, y, ydt, w, wdt, r, rdt, n, wscale, phi, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype combination not supported.");
+    HIP_TRY(e, "pwls residual (allocation-free)");
+    return CTSIM_OK;
+}
+
 }  // extern "C"
 
 template <class V>

@@ -110,6 +110,8 @@ namespace ctsim {
 hipError_t reduction_scratch_init();
 hipError_t residual_launch(const void* p, int pdt, const void* y, int ydt, const void* w, int wdt, void* r, int rdt,
                            size_t n, double wscale, double* phi, hipStream_t st);
+hipError_t residual_launch_noalloc(const void* p, int pdt, const void* y, int ydt, const void* w, int wdt, void* r, int rdt,
+                           size_t n, double wscale, double* phi, hipStream_t st);
 hipError_t add_i32_launch(int* p, int inc, hipStream_t st);
 hipError_t sqdiff_launch(const float* a, const float* b, size_t n, double* out, hipStream_t st);
 hipError_t sqdiff_launch(const double* a, const double* b, size_t n, double* out, hipStream_t st);

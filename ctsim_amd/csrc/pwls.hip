@@ -6,6 +6,8 @@
 // (data-fit cost, penalty cost, RMSD) accumulate in FP64 with a fixed-order two-level
 // reduction: every block writes a partial, the last block to finish adds the partials
 // in index order.  Results are therefore bit-reproducible run to run.
+#include <mutex>
+
 #include "internal.h"
 #include "penalty.h"
 
@@ -642,7 +644,7 @@ static RedScratch g_scratch[16];
 
 hipError_t reduction_scratch_init();
 
-static hipError_t scratch(RedScratch& rs) {
+static hipError_t lib_scratch(RedScratch& rs) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
@@ -664,7 +666,7 @@ static hipError_t scratch(RedScratch& rs) {
 
 hipError_t reduction_scratch_init() {
     RedScratch rs;
-    return scratch(rs);
+    return lib_scratch(rs);
 }
 
 static unsigned red_grid(size_t n) {
@@ -707,11 +709,8 @@ static hipError_t residual_y(const void* p, const void* y, int ydt, const void* 
     }
 }
 
-hipError_t residual_launch(const void* p, int pdt, const void* y, int ydt, const void* w, int wdt, void* r, int rdt,
-                           size_t n, double ws, double* phi, hipStream_t st) {
-    RedScratch rs;
-    hipError_t e = scratch(rs);
-    if (e != hipSuccess) return e;
+static hipError_t residual_launch_rs(const void* p, int pdt, const void* y, int ydt, const void* w, int wdt, void* r,
+                                     int rdt, size_t n, double ws, double* phi, RedScratch rs, hipStream_t st) {
     if (pdt == DT_F64 || ydt == DT_F64 || wdt == DT_F64 || rdt == DT_F64) {
         if (pdt != DT_F64 || ydt != DT_F64 || wdt != DT_F64 || rdt != DT_F64) return hipErrorInvalidValue;
         hipLaunchKernelGGL((residual_kernel<double, double, double, double>), dim3(red_grid(n)), dim3(kRB), 0, st,
@@ -724,6 +723,66 @@ hipError_t residual_launch(const void* p, int pdt, const void* y, int ydt, const
         case DT_BF16: return residual_y<__hip_bfloat16>(p, y, ydt, w, wdt, r, rdt, n, ws, rs, phi, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t residual_launch(const void* p, int pdt, const void* y, int ydt, const void* w, int wdt, void* r, int rdt,
+                           size_t n, double ws, double* phi, hipStream_t st) {
+    RedScratch rs;
+    hipError_t e = lib_scratch(rs);
+    if (e != hipSuccess) return e;
+    return residual_launch_rs(p, pdt, y, ydt, w, wdt, r, rdt, n, ws, phi, rs, st);
+}
+
+// ---------------------------------------------------------------------------------
+// Allocation-free reduction scratch: the same layout as the library-allocated one
+// (kRedBlocks FP64 partial sums, then the counter) in a file-scope device array, one
+// copy per GPU, held until the process ends.  Host code never passes the array's name
+// to a kernel: the first call on a device looks its address up with
+// hipGetSymbolAddress and keeps it, and every launch takes that address.
+// ---------------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = kRedBlocks * sizeof(double) + 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static void* g_noalloc_ptr[16];
+static std::mutex g_noalloc_mutex;
+
+__global__ void zero_words_kernel(unsigned int* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+static hipError_t noalloc_scratch(RedScratch& rs, hipStream_t st) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 16) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (!g_noalloc_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!ptr) return hipErrorInvalidSymbol;
+        // the counter must start at zero (every reduction returns it to zero); cleared by a
+        // kernel on the caller's stream, ahead of this call's own reduction
+        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, st,
+                           (unsigned int*)((char*)ptr + kRedBlocks * sizeof(double)), 64);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        g_noalloc_ptr[dev] = ptr;
+    }
+    rs.partial = (double*)g_noalloc_ptr[dev];
+    rs.count = (unsigned int*)((char*)g_noalloc_ptr[dev] + kRedBlocks * sizeof(double));
+    return hipSuccess;
+}
+
+hipError_t residual_launch_noalloc(const void* p, int pdt, const void* y, int ydt, const void* w, int wdt, void* r,
+                                   int rdt, size_t n, double ws, double* phi, hipStream_t st) {
+    // the reduction grid is at most kRedBlocks blocks whatever n is, so the array always holds
+    // its partial sums
+    if (red_grid(n) > (unsigned)kRedBlocks) return hipErrorInvalidValue;
+    RedScratch rs;
+    hipError_t e = noalloc_scratch(rs, st);
+    if (e != hipSuccess) return e;
+    return residual_launch_rs(p, pdt, y, ydt, w, wdt, r, rdt, n, ws, phi, rs, st);
 }
 
 template <int POT, class V>
@@ -746,7 +805,7 @@ static hipError_t pen_launch_p(const V* x, int nx, int ny, int nz, double delta,
         }
     } else if (mode == 0) {
         RedScratch rs;
-        hipError_t e = scratch(rs);
+        hipError_t e = lib_scratch(rs);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((pen_cost_kernel<POT, V>), dim3(red_grid(n)), dim3(kRB), 0, st, x, nx, ny, nz, delta, rs,
                            cost);
@@ -890,7 +949,7 @@ hipError_t dynamic_range_launch(const float* x, size_t n, double t, double eta, 
         g_dra[dev] = (DraStats*)p;
     }
     RedScratch rs;
-    e = scratch(rs);
+    e = lib_scratch(rs);
     if (e != hipSuccess) return e;
     DraStats* s = g_dra[dev];
     hipLaunchKernelGGL(dra_init_kernel, dim3(1), dim3(1), 0, st, s);
@@ -950,7 +1009,7 @@ hipError_t lalm_inner_launch(const float* x, const float* xold, const float* s, 
 template <class V>
 static hipError_t sqdiff_launch_t(const V* a, const V* b, size_t n, double* out, hipStream_t st) {
     RedScratch rs;
-    hipError_t e = scratch(rs);
+    hipError_t e = lib_scratch(rs);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sqdiff_kernel<V>, dim3(red_grid(n)), dim3(kRB), 0, st, a, b, n, rs, out);
     return hipGetLastError();

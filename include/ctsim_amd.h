@@ -301,6 +301,20 @@ int ctsim_pwls_residual(const void* d_p, int32_t p_dtype, const void* d_y, int32
                         int32_t w_dtype, void* d_r, int32_t r_dtype, size_t n, double wscale, double* d_phi,
                         void* stream);
 
+/* ctsim_pwls_residual without any device allocation.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and errors as ctsim_pwls_residual.  The reduction's scratch (1024 FP64 partial
+ * sums and a counter, 8448 bytes) is a fixed device array, one per GPU, held until the
+ * process ends; the first call on a device finds its address and clears the counter on
+ * the caller's stream.  The reduction grid never exceeds the array's 1024 partial sums,
+ * so every n is accepted.  No hipMalloc, hipFree or allocating library call is made.
+ * Calls that overlap on one GPU (other streams, other threads) share the array and
+ * race: serialise them. */
+int ctsim_pwls_residual_noalloc(const void* d_p, int32_t p_dtype, const void* d_y, int32_t y_dtype,
+                                const void* d_w, int32_t w_dtype, void* d_r, int32_t r_dtype, size_t n,
+                                double wscale, double* d_phi, void* stream);
+
 /* Roughness penalties of a ZXY volume (penalty/loadPenaltyOperator.m:22-53): pfun
  * CTSIM_PEN_HUBER (huberPenalty.m:101-161), CTSIM_PEN_QUAD (quadPenalty.m; T is the
  * constant 4 + 4/sqrt2 (+1 in 3D) of the reference), CTSIM_PEN_HYPER (hyperbolaPenalty.m,
