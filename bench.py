@@ -104,7 +104,7 @@ def kernel_times(solver, reps=1):
     st = solver.stream
     nx, ny, nz = solver.nxyz
     t = {"proj": 0.0, "back": 0.0, "residual": 0.0, "update": 0.0}
-    src = solver.zc if solver.mixed else solver.z[solver.cur]
+    src = solver.zc if solver.zc is not None else solver.z[solver.cur]
     for _ in range(reps):
         for k, (v0, vs, ns, off) in enumerate(solver.subsets):
             pw = solver._window(solver.p, 0, ns)
@@ -112,6 +112,9 @@ def kernel_times(solver, reps=1):
                 solver.P.proj(src, pw, f"proj,{solver.kind}", v0, vs, ns, stream=st)
             t["proj"] += tm.ms
             with Timer(st) as tm:
+                if solver.ax0 is not None:
+                    call("ctsim_sub", pw.vp, solver._sub_ptr(solver.ax0, off), pw.vp, solver.p.dtype, ns * solver.cell,
+                         solver._sv())
                 call("ctsim_pwls_residual", pw.vp, solver.p.dtype, solver._sub_ptr(solver.y, off), solver.y.dtype,
                      solver._sub_ptr(solver.w, off), solver.w.dtype, solver.r.vp, solver.r.dtype, ns * solver.cell,
                      solver.wscale, C.c_void_p(solver.scal.ptr), solver._sv())
@@ -153,6 +156,9 @@ def run_pwls(form, geom, y, w, x0, steps, warmup, sino_dtype="bf16", numos=24, b
     solver = PwlsOS(geom, y, w, numos, beta, delta, precision=precision, sino_dtype=sino_dtype,
                     graph=form == "mixed", fused=fused)
     solver.setup(x0)
+    # the batched (mixed) form takes the autotuner's schedules; the plain form keeps the
+    # plan's defaults
+    solver.tuning = solver.autotune(reps=3) if form == "mixed" else None
     solver.prepare(n_all)
     for _ in range(warmup):
         solver.iterate()
@@ -187,6 +193,9 @@ def pair_gups(kinds=("dd", "tf"), reps=2):
     counts = load_counts()
     out = {}
     for kind in kinds:
+        # the autotuner's schedules for this geometry (times of every candidate reported)
+        tp_sched = P.autotune(vol, sino, f"proj,{kind}", reps=1, stream=st)
+        tb_sched = P.autotune(sino, img, f"back,{kind}", reps=1, stream=st)
         P.proj(vol, sino, f"proj,{kind}", stream=st)
         P.back(sino, img, f"back,{kind}", stream=st)
         st.synchronize()
@@ -205,7 +214,10 @@ def pair_gups(kinds=("dd", "tf"), reps=2):
         out[kind] = dict(proj_ms=round(tp, 3), back_ms=round(tb, 3),
                          gups_pair=round(2 * c["U"] / ((tp + tb) * 1e-3) / 1e9, 2),
                          proj_tflops=round(pt, 2), back_tflops=round(bt, 2),
-                         proj_frac=round(pt / PEAK_FP32_TFLOPS, 4), back_frac=round(bt / PEAK_FP32_TFLOPS, 4))
+                         proj_frac=round(pt / PEAK_FP32_TFLOPS, 4), back_frac=round(bt / PEAK_FP32_TFLOPS, 4),
+                         schedule={"proj": tp_sched, "back": tb_sched,
+                                   "codes": "proj form|split<<2 (form 0 auto, 1 ring, 2 register; split 0 auto, "
+                                            "1 whole, 2 shared); back 0 linear, 1 super-tile"})
         log(f"pair {kind}: proj {tp:.1f} ms  back {tb:.1f} ms  {out[kind]['gups_pair']} GUPS")
     return out
 
@@ -437,6 +449,35 @@ def accuracy_lockstep(geom, y, w, x0, n_iter, numos=24, beta=1e4, delta=1e-2):
                     "with 1e-3 over the iterations where f64_ulp stays within 1e-4"}
 
 
+def accuracy_m1(geom, y, w, x0, n_iter, beta=1e4, delta=1e-2):
+    """One subset (M = 1): the same loop without ordered subsets, where the iterates do not
+    oscillate.  Relative RMSE of each form against the FP64 iterate after every iteration
+    (the mixed forms without graph replay: one subset is an odd count for the x/z double
+    buffers; the storage precisions are those of the timed form)."""
+    from ctsim_amd.pwls import PwlsOS
+    forms = {"f64": dict(precision="f64", graph=False), "plain": dict(precision="f32", graph=False),
+             "mixed_bf16": dict(precision="mixed", sino_dtype="bf16", graph=False),
+             "mixed_f16": dict(precision="mixed", sino_dtype="f16", graph=False)}
+    solvers = {}
+    for name, kw in forms.items():
+        sv = PwlsOS(geom, y, w, 1, beta, delta, **kw)
+        sv.setup(x0)
+        sv.prepare(n_iter)
+        solvers[name] = sv
+    curves = {k: [] for k in forms if k != "f64"}
+    for it in range(n_iter):
+        imgs = {}
+        for name, sv in solvers.items():
+            sv.iterate()
+            imgs[name] = sv.image()
+        for k in curves:
+            curves[k].append(float(f"{rel_rmse(imgs[k], imgs['f64']):.3e}"))
+    log("accuracy M=1 final:", {k: v[-1] for k, v in curves.items()})
+    del solvers
+    return {"subsets": 1, "iterations": n_iter, "rel_rmse_vs_fp64_per_iter": curves,
+            "rel_rmse_vs_fp64_final": {k: v[-1] for k, v in curves.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -479,6 +520,9 @@ def main():
                                    fused=args.fuse)
     ms_mixed = dt / args.steps * 1e3
     log(f"mixed: {ms_mixed:.1f} ms/iter, phi {outs[-1]['phi']:.6e}, rmsd {outs[-1]['rmsd']:.3e}")
+    log("autotuned schedules:", solver.tuning)
+    tuning = solver.tuning
+    sched = solver.P.schedule()
     kt = kernel_times(solver)
     log("kernel ms (avg per subset):", {k: round(v, 3) for k, v in kt.items()})
     del solver
@@ -491,13 +535,19 @@ def main():
         kt_plain = kernel_times(sp)
         del sp
     accuracy = accuracy_lockstep(geom, y, w, x0, n_acc) if n_acc else None
+    if accuracy is not None:
+        accuracy["m1"] = accuracy_m1(geom, y, w, x0, n_acc)
     M = 24
     # dominant kernel: the forward projector (per-subset launch)
     f_launch = flops(c_it) / M
     achieved = f_launch / (kt["proj"] * 1e-3) / 1e12
+    ring = (sched[0] & 3) != 2
+    split = (sched[0] >> 2) & 3
+    kname = ("cbct_proj_ring_kernel<float,SF,16,8,2,8>" if ring and split != 1 else
+             "cbct_proj_ring_kernel<float,SF,16,1,2,4>" if ring else "cbct_proj_kernel<float,float,float,SF,16,1,*>")
     roof = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                 frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=measured_traffic(),
-                kernel="cbct_proj_ring_kernel<float,SF,16,8,2,8> (one 41-view subset, FP32 input z - x0)",
+                kernel=f"{kname} (one 41-view subset, FP32 input z - x0, autotuned schedule {sched[0]})",
                 note="FP32 vector ALU bound, no MFMA (not a dense contraction); peak = MI355X's published FP32 "
                      "figure; achieved = (17U+3C+60K)/24 per launch / HIP-event time on the launch stream")
     res = {
@@ -523,6 +573,10 @@ def main():
         "gups_iteration": round(2 * c_it["U"] / (ms_mixed * 1e-3) / 1e9, 2),
         "iteration_tflops": round(f_iter / (ms_mixed * 1e-3) / 1e12, 3),
         "kernel_ms_per_subset": {k: round(v, 4) for k, v in kt.items()},
+        "autotune": {"schedule": {"proj": sched[0], "back": sched[1]}, "proj": tuning["proj"],
+                     "back": tuning["back"],
+                     "codes": "proj form|split<<2 (form 0 auto, 1 ring, 2 register; split 0 auto, 1 whole, "
+                              "2 shared); back 0 linear, 1 super-tile; times in ms per 41-view launch"},
         "roofline": roof,
         "accuracy": accuracy,
     }

@@ -71,6 +71,10 @@ SIGNATURES = {
     "ctsim_cbct_back_hpd": (C.c_int, _PER_OP),
     "ctsim_plan_create": (C.c_int, [C.POINTER(vp), i32, C.POINTER(CtsimGeom), i32, P_f64, P_f32, P_u8, i32]),
     "ctsim_plan_destroy": (C.c_int, [vp]),
+    "ctsim_plan_set_schedule": (C.c_int, [vp, i32, i32]),
+    "ctsim_plan_get_schedule": (C.c_int, [vp, P_i32, P_i32]),
+    "ctsim_plan_autotune": (C.c_int, [vp, C.c_char_p, i32, i32, i32, vp, i32, vp, i32, i32, P_i32, P_f32, P_i32, i32,
+                                      vp]),
     "ctsim_plan_proj": (C.c_int, [vp, cp, i32, i32, i32, vp, i32, vp, i32, vp]),
     "ctsim_plan_proj_exhaustive": (C.c_int, [vp, cp, i32, i32, i32, vp, i32, vp, i32, vp]),
     "ctsim_plan_back": (C.c_int, [vp, cp, i32, i32, i32, vp, i32, vp, i32, f64, i32, vp]),

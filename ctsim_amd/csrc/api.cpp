@@ -71,6 +71,8 @@ Geom<T> make_geom(const ctsim_geom& g) {
     k.ou = (T)g.ou;
     k.zstride = g.nz;
     k.zoff = T(0);
+    k.sched_proj = 0;
+    k.sched_back = 1;  // super-tile column order: measured faster at 41 and 720 views
     return k;
 }
 
@@ -411,6 +413,86 @@ int ctsim_plan_back(const ctsim_plan* p, const char* op, int32_t view0, int32_t 
                     p->g.nu, p->g.nv);
     HIP_TRY(e, "backprojection launch");
     return CTSIM_OK;
+}
+
+int ctsim_plan_set_schedule(ctsim_plan* p, int32_t proj_sched, int32_t back_sched) {
+    if (!p) return fail(CTSIM_EINVAL, "Error: null plan.");
+    if ((proj_sched & 3) == 3 || ((proj_sched >> 2) & 3) == 3 || (proj_sched >> 4) != 0 || back_sched < 0 ||
+        back_sched > 1)
+        return fail(CTSIM_EINVAL, "Error: unknown schedule (proj %d, back %d).", proj_sched, back_sched);
+    p->gf.sched_proj = p->gd.sched_proj = proj_sched;
+    p->gf.sched_back = p->gd.sched_back = back_sched;
+    return CTSIM_OK;
+}
+
+int ctsim_plan_get_schedule(const ctsim_plan* p, int32_t* proj_sched, int32_t* back_sched) {
+    if (!p || !proj_sched || !back_sched) return fail(CTSIM_EINVAL, "Error: null argument.");
+    *proj_sched = p->compute == CTSIM_F32 ? p->gf.sched_proj : p->gd.sched_proj;
+    *back_sched = p->compute == CTSIM_F32 ? p->gf.sched_back : p->gd.sched_back;
+    return CTSIM_OK;
+}
+
+int ctsim_plan_autotune(ctsim_plan* p, const char* op, int32_t view0, int32_t vstep, int32_t nsub, const void* d_in,
+                        int32_t in_dt, void* d_out, int32_t out_dt, int32_t reps, int32_t* best, float* ms_out,
+                        int32_t* sched_out, int32_t max_out, void* stream) {
+    if (!p || !op || !d_in || !d_out || !best) return fail(CTSIM_EINVAL, "Error: null argument.");
+    const OpKind k = parse_op(op);
+    const bool proj = k == OP_PROJ_DD || k == OP_PROJ_TF;
+    const bool back = k == OP_BACK_DD || k == OP_BACK_TF;
+    if (p->ndim != 3 || (!proj && !back))
+        return fail(CTSIM_EINVAL, "Error: the 3D autotuner takes proj,dd proj,tf back,dd back,tf.");
+    if (d_in == d_out) return fail(CTSIM_EINVAL, "Error: the tuning output must not alias the input.");
+    if (reps < 1) reps = 1;
+    // the forward schedules (form | split << 2): automatic, LDS ring with whole and with shared
+    // items, register pipeline with whole and with shared items; the backward column orders
+    static const int kProj[5] = {0, 1 | (1 << 2), 1 | (2 << 2), 2 | (1 << 2), 2 | (2 << 2)};
+    static const int kBack[2] = {0, 1};
+    const int n = proj ? 5 : 2;
+    const int keep_p = p->gf.sched_proj, keep_b = p->gf.sched_back;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0), "event");
+    hipError_t he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(he, "event");
+    }
+    int rc = CTSIM_OK, pick = -1;
+    float best_ms = 3.4e38f;
+    for (int i = 0; i < n && rc == CTSIM_OK; ++i) {
+        const int sp = proj ? kProj[i] : keep_p, sb = proj ? keep_b : kBack[i];
+        rc = ctsim_plan_set_schedule(p, sp, sb);
+        auto run = [&]() {
+            return proj ? ctsim_plan_proj(p, op, view0, vstep, nsub, d_in, in_dt, d_out, out_dt, stream)
+                        : ctsim_plan_back(p, op, view0, vstep, nsub, d_in, in_dt, d_out, out_dt, 1.0, 0, stream);
+        };
+        if (rc == CTSIM_OK) rc = run();  // warm-up
+        if (rc) break;
+        if ((he = hipEventRecord(e0, st)) != hipSuccess) { rc = hip_fail(he, "event"); break; }
+        for (int r = 0; r < reps && rc == CTSIM_OK; ++r) rc = run();
+        if (rc) break;
+        float ms = 0.f;
+        if ((he = hipEventRecord(e1, st)) != hipSuccess || (he = hipEventSynchronize(e1)) != hipSuccess ||
+            (he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) {
+            rc = hip_fail(he, "event timing");
+            break;
+        }
+        ms /= reps;
+        if (ms_out && i < max_out) ms_out[i] = ms;
+        if (sched_out && i < max_out) sched_out[i] = proj ? sp : sb;
+        if (ms < best_ms) {
+            best_ms = ms;
+            pick = i;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != CTSIM_OK || pick < 0) {
+        (void)ctsim_plan_set_schedule(p, keep_p, keep_b);
+        return rc ? rc : fail(CTSIM_EINVAL, "Error: no schedule ran.");
+    }
+    *best = proj ? kProj[pick] : kBack[pick];
+    return ctsim_plan_set_schedule(p, proj ? kProj[pick] : keep_p, proj ? keep_b : kBack[pick]);
 }
 
 int ctsim_plan_back_nesterov(const ctsim_plan* p, const char* op, int32_t view0, int32_t vstep, int32_t nsub,

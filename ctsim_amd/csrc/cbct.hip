@@ -58,6 +58,7 @@ constexpr int kZBits = 13;
 constexpr int kZMask = (1 << kZBits) - 1;
 constexpr int kMaxNz = 1 << kZBits;
 constexpr int kBackSlab = 8 * kWave;  // slices per backprojector launch (8 voxels per lane)
+constexpr int kSuperTile = 32;        // super-tile edge in columns of the backprojectors' patch order
 
 template <class T>
 __device__ __forceinline__ int gather_class(T zinc) {
@@ -510,13 +511,32 @@ __device__ __forceinline__ float permute_f(int dst_lane_bytes, float x) {
 // instruction offsets 0, 256, 512 apply to both the voxel and the LDS address)
 template <int NS>
 __device__ __forceinline__ void ring_load(rsrc_words rs, unsigned la, int voff) {
-    static_assert(NS == 2 || NS == 3, "two or three slots per column");
-    if constexpr (NS == 2)
+    static_assert(NS >= 1 && NS <= 4, "one to four slots per column");
+    if constexpr (NS == 1)
+        asm volatile(
+            "s_mov_b32 m0, %1\n\t"
+            "s_nop 0\n\t"
+            "buffer_load_dword %0, %2, 0 offen lds"
+            :
+            : "v"(voff), "s"(la), "s"(rs)
+            : "memory");
+    else if constexpr (NS == 2)
         asm volatile(
             "s_mov_b32 m0, %1\n\t"
             "s_nop 0\n\t"
             "buffer_load_dword %0, %2, 0 offen lds\n\t"
             "buffer_load_dword %0, %2, 0 offen offset:256 lds"
+            :
+            : "v"(voff), "s"(la), "s"(rs)
+            : "memory");
+    else if constexpr (NS == 4)
+        asm volatile(
+            "s_mov_b32 m0, %1\n\t"
+            "s_nop 0\n\t"
+            "buffer_load_dword %0, %2, 0 offen lds\n\t"
+            "buffer_load_dword %0, %2, 0 offen offset:256 lds\n\t"
+            "buffer_load_dword %0, %2, 0 offen offset:512 lds\n\t"
+            "buffer_load_dword %0, %2, 0 offen offset:768 lds"
             :
             : "v"(voff), "s"(la), "s"(rs)
             : "memory");
@@ -552,22 +572,39 @@ __device__ __forceinline__ float ring_gather(const float* zs, int span, int ja, 
 }
 
 // WPB waves per block; SPLIT = WPB shares each item's strip among them (eight waves: short
-// view subsets make few items, and smaller shares even out the blocks' run times)
-template <class Tout, int FOOT, int TU, int SPLIT, int NS, int WPB>
+// view subsets make few items, and smaller shares even out the blocks' run times).
+//
+// HALF: an FP16 volume (the mixed PWLS form's projector input z - x0).  A slot of the ring
+// is then 64 dwords holding 128 voxels (NS slots: up to 128 NS voxels per column), so the
+// ring can be RING entries deep in the LDS of the FP32 form's four; the voxel range of a
+// column is widened to whole dwords (an even first voxel: columns of an even nz start on a
+// dword), and the scaling pass converts each lane's two voxels and writes them, scaled, as
+// FP32 to a staging buffer that the row gather reads as the FP32 form reads its slot.  The
+// one or two voxels the widening adds lie outside every row of the wave and weigh zero.
+template <class Tout, int FOOT, int TU, int SPLIT, int NS, int WPB, bool HALF = false, int RING = ring_len(NS)>
 __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel(Geom<float> g, const View<float>* __restrict__ views,
-                                                                   int vstride, int nviews, const float* __restrict__ vol,
+                                                                   int vstride, int nviews,
+                                                                   const typename std::conditional<HALF, __half, float>::type* __restrict__ vol,
                                                                    const uint8_t* __restrict__ map,
                                                                    Tout* __restrict__ proj, int exhaustive) {
     using T = float;
+    using Tin = typename std::conditional<HALF, __half, float>::type;
     static_assert(SPLIT == 1 || SPLIT == WPB, "SPLIT is 1 or the block's wave count");
-    // one block of LDS: the waves' rings, then their tile accumulators, then three pad cells.
-    // Accumulators: each lane's row of the tile, [cell][lane] (one word per bank, one writer
-    // per word).  The four-cell update below reads cells c0 .. c0 + 3 whatever c0 is; past a
-    // wave's 16 cells it reads the next wave's cells or the pad (values not used).  The rings
-    // come first so that their addresses fit the LDS instructions' offset fields.
-    constexpr int kRing = ring_len(NS), kSlotW = ring_entry_words(NS);
-    __shared__ T lds_block[WPB * kRing * kSlotW + WPB * TU * kWave + 3 * kWave];
-    T* const accs = lds_block + WPB * kRing * kSlotW;
+    static_assert(HALF ? (NS == 1 || NS == 2) : (NS >= 2 && NS <= 4), "slots per column");
+    // one block of LDS: the waves' rings, (HALF) their staging buffers, then their tile
+    // accumulators, then three pad cells.  Accumulators: each lane's row of the tile,
+    // [cell][lane] (one word per bank, one writer per word).  The four-cell update below
+    // reads cells c0 .. c0 + 3 whatever c0 is; past a wave's 16 cells it reads the next
+    // wave's cells or the pad (values not used).  The rings come first so that their
+    // addresses fit the LDS instructions' offset fields.
+    constexpr int kRing = RING;
+    constexpr int kSlotW = HALF ? NS * kWave : ring_entry_words(NS);
+    // staging (HALF): an unused word, the zero word, 128 NS voxels, kZPad zero words; even
+    // length, so that voxel pairs sit on 8-byte boundaries
+    constexpr int kStageW = HALF ? 2 + 2 * NS * kWave + kZPad : 0;
+    static_assert(!HALF || (kStageW % 2 == 0 && (kRing * kSlotW) % 2 == 0), "staging pairs on 8-byte boundaries");
+    __shared__ __align__(16) T lds_block[WPB * kRing * kSlotW + WPB * kStageW + WPB * TU * kWave + 3 * kWave];
+    T* const accs = lds_block + WPB * (kRing * kSlotW + kStageW);
     const int nut = (g.nu + TU - 1) / TU;
     const int nvc = (g.nv + kWave - 1) / kWave;
     const int items = nviews * nut * nvc;
@@ -593,23 +630,32 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
     const int iv = iv_a + lane;
     const int iv_last = min(iv_a + kWave, g.nv) - 1;
     T* zr = lds_block + wv * kRing * kSlotW;
-    // word 0 and the pads of every slot stay zero; the loads write words 1 .. 192
+    // FP32: word 0 and the pads of every slot stay zero; the loads write words 1 .. 192.
+    // HALF: the staging buffer's zero word and pads stay zero; the scaling pass writes the rest
+    T* const zstage = lds_block + WPB * kRing * kSlotW + wv * kStageW + 1;  // its zero word
     for (int i = lane; i < kRing * kSlotW; i += kWave) zr[i] = T(0);
+    if constexpr (HALF)
+        for (int i = lane; i < kStageW - 1; i += kWave) zstage[i] = T(0);
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // zeros in place before any load lands
     const unsigned ring_lds = lds_addr(zr);
     const unsigned slot_bytes = kSlotW * 4;
     const unsigned ring_end = ring_lds + kRing * slot_bytes;
     const T wzdz = v.wz * g.dz;
     const T fiv_lo = T(iv) - T(0.5);  // lower edge of this lane's row
-    const T lzd0 = T(lane) * g.dz, lzd1 = T(kWave + lane) * g.dz, lzd2 = T(2 * kWave + lane) * g.dz;
-    const rsrc_words empty_rs = dma_rsrc(vol, 0);  // a group of loads that only writes zeros  // a group of loads that only writes zeros
+    // zc of the voxels a lane scales, relative to the range's first: FP32 lane + 64 k,
+    // HALF 2 lane (+ 1) + 128 k
+    const T lzd0 = T(HALF ? 2 * lane : lane) * g.dz, lzd1 = T(kWave + lane) * g.dz,
+            lzd2 = T(2 * kWave + lane) * g.dz, lzd3 = T(3 * kWave + lane) * g.dz;
+    const T lzh1 = T(2 * kWave + 2 * lane) * g.dz;
+    const rsrc_words empty_rs = dma_rsrc(vol, 0);  // a group of loads that only writes zeros
 
     T* acc = &accs[(wv * TU) * kWave + lane];
 #pragma unroll
     for (int t = 0; t < TU; ++t) acc[t * kWave] = T(0);
 
-    // ring position of the next group to issue (byte address of its slot's word 1)
-    unsigned isl = ring_lds + 4;
+    // ring position of the next group to issue (byte address of its slot's word 1; HALF: of
+    // the slot's first dword)
+    unsigned isl = ring_lds + (HALF ? 0 : 4);
     auto advance = [&](unsigned a) { return a + slot_bytes >= ring_end ? a + slot_bytes - kRing * slot_bytes : a + slot_bytes; };
 
     // the SPLIT waves of a block take every SPLIT-th image line of the strip (lane l of a
@@ -661,6 +707,11 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
                     ok = za <= zb;
                 }
                 if (ok) {
+                    if constexpr (HALF) {
+                        // whole dwords: an even first voxel and an odd last one (nz is even)
+                        za &= ~1;
+                        zb |= 1;
+                    }
                     const T ampr = amp * frcp(r);
                     const int c0 = max(ilo, iu_a);
                     const int c1 = min(ihi, iu_b);
@@ -673,8 +724,9 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
                         (unsigned long long)(vol + ((size_t)(iy * g.nx + ix) * g.nz + za));
                     cp.alo = (int)(unsigned)(a & 0xffffffffull);
                     cp.ahi = (int)(unsigned)((a >> 32) & 0xffffull);
-                    cp.nbytes = (span + 1) * 4;
-                    cp.meta = gather_class(zinc) | ((span >> 6) << 2) | ((c1 - c0) << 4) | ((c0 - iu_a) << 9);
+                    cp.nbytes = (span + 1) * (int)sizeof(Tin);
+                    cp.meta = gather_class(zinc) | ((span >> (HALF ? 7 : 6)) << 2) | ((c1 - c0) << 4) |
+                              ((c0 - iu_a) << 9);
                     cp.zbase = T(za) * g.dz + wzdz;
                     cp.r2 = r * r;
                     cp.zinc = zinc;
@@ -725,24 +777,49 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
             for (int c = 0; c < n; ++c) {
                 issue(c + kRing - 1);
                 wait_vm<NS * (kRing - 1)>();  // column c's group has landed
-                T* zs = (T*)((char*)zr + (psl - 4 - ring_lds));  // word 0 of the slot
+                const unsigned slot_off = psl - (HALF ? 0 : 4) - ring_lds;  // the slot's first word
+                T* zs = HALF ? zstage : (T*)((char*)zr + slot_off);         // word 0: zero
                 psl = advance(psl);
                 const int meta = rdl_i(cp.meta, c);
                 const int cls = meta & 3, ns = (meta >> 2) & 3, nc = (meta >> 4) & 31, c0 = meta >> 9;
                 const T r2 = rdl(cp.r2, c), zinc = rdl(cp.zinc, c), vst = rdl(cp.vst, c), zbase = rdl(cp.zbase, c);
                 const T rz = frcp(zinc);
-                // scale the loaded voxels in place by sqrt(r^2 + zc^2) (the cone factor times
-                // r); words past the range are zero.  The first two slots in one straight
-                // pass (one LDS round trip; a slot past the range holds zeros and stays zero)
-                {
-                    const T v0 = zs[1 + lane], v1 = zs[1 + kWave + lane];
-                    const T zc0 = zbase + lzd0, zc1 = zbase + lzd1;
-                    zs[1 + lane] = fsqrt(fmaf(zc0, zc0, r2)) * v0;
-                    zs[1 + kWave + lane] = fsqrt(fmaf(zc1, zc1, r2)) * v1;
-                }
-                if (NS == 3 && ns >= 2) {
-                    const T zc = zbase + lzd2;
-                    zs[1 + 2 * kWave + lane] = fsqrt(fmaf(zc, zc, r2)) * zs[1 + 2 * kWave + lane];
+                if constexpr (HALF) {
+                    // each lane's dword is voxels 2 lane, 2 lane + 1 of a slot (zero past the
+                    // range): converted, scaled by sqrt(r^2 + zc^2) and stored as an FP32 pair
+                    const unsigned* slot = (const unsigned*)((const char*)zr + slot_off);
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) {
+                        // every slot, in range or not: the staging buffer must hold this
+                        // column's zeros past its range, not the previous column's voxels
+                        const unsigned h2 = slot[k * kWave + lane];
+                        const T v0 = __half2float(__ushort_as_half((unsigned short)(h2 & 0xffffu)));
+                        const T v1 = __half2float(__ushort_as_half((unsigned short)(h2 >> 16)));
+                        const T zc0 = zbase + (k == 0 ? lzd0 : lzh1);
+                        const T zc1 = zc0 + g.dz;
+                        float2 o;
+                        o.x = fsqrt(fmaf(zc0, zc0, r2)) * v0;
+                        o.y = fsqrt(fmaf(zc1, zc1, r2)) * v1;
+                        *(float2*)(zs + 1 + 2 * (k * kWave + lane)) = o;
+                    }
+                } else {
+                    // scale the loaded voxels in place by sqrt(r^2 + zc^2) (the cone factor times
+                    // r); words past the range are zero.  The first two slots in one straight
+                    // pass (one LDS round trip; a slot past the range holds zeros and stays zero)
+                    {
+                        const T v0 = zs[1 + lane], v1 = zs[1 + kWave + lane];
+                        const T zc0 = zbase + lzd0, zc1 = zbase + lzd1;
+                        zs[1 + lane] = fsqrt(fmaf(zc0, zc0, r2)) * v0;
+                        zs[1 + kWave + lane] = fsqrt(fmaf(zc1, zc1, r2)) * v1;
+                    }
+                    if (NS >= 3 && ns >= 2) {
+                        const T zc = zbase + lzd2;
+                        zs[1 + 2 * kWave + lane] = fsqrt(fmaf(zc, zc, r2)) * zs[1 + 2 * kWave + lane];
+                    }
+                    if (NS >= 4 && ns >= 3) {
+                        const T zc = zbase + lzd3;
+                        zs[1 + 3 * kWave + lane] = fsqrt(fmaf(zc, zc, r2)) * zs[1 + 3 * kWave + lane];
+                    }
                 }
                 __builtin_amdgcn_wave_barrier();
                 // row iv against the column's voxels (relative to voxel za): voxel j covers
@@ -750,7 +827,7 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
                 // overlap a row, so thick voxels take a fixed unrolled count
                 const T tt = fiv_lo - vst;
                 const int ja = (int)ceil(tt * rz - T(1));
-                const int span = (rdl_i(cp.nbytes, c) >> 2) - 1;
+                const int span = (rdl_i(cp.nbytes, c) >> (HALF ? 1 : 2)) - 1;
                 T q;
                 if (cls == 1) {
                     q = ring_gather<3>(zs, span, ja, tt, zinc);
@@ -918,7 +995,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                                                            int nviews, const Tin* __restrict__ proj,
                                                            const uint8_t* __restrict__ map,
                                                            Tout* __restrict__ vol, T scale, int accumulate,
-                                                           int rowwin, NestArgs na) {
+                                                           int rowwin, NestArgs na, int order) {
     static_assert(!FUSE || (std::is_same<T, float>::value && SHORT), "the fused update is FP32, short detectors");
     // row buffers (rowwin + 1 + kRowPad words per wave) in dynamic LDS, sized at launch
     // for the detector, so short detectors leave room for more resident waves
@@ -927,9 +1004,24 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
     __shared__ ViewCol<T> vbuf[kWavesPerBlock][kWave];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int wv = (int)(threadIdx.x / kWave);
-    const int colid = bid * kWavesPerBlock + wv;
-    if (colid >= g.nx * g.ny) return;
-    const int ix = colid % g.nx, iy = colid / g.nx;
+    // column order: 0 = four consecutive columns per block (a contiguous run of columns per
+    // XCD); 1 = 2 x 2 column patches in super-tiles of kSuperTile x kSuperTile columns, a
+    // contiguous range of super-tiles per XCD (the FDK backprojector's order)
+    int ix, iy;
+    if (order == 0) {
+        const int colid = bid * kWavesPerBlock + wv;
+        if (colid >= g.nx * g.ny) return;
+        ix = colid % g.nx;
+        iy = colid / g.nx;
+    } else {
+        constexpr int kHalfT = kSuperTile / 2;
+        const int stl = bid / (kHalfT * kHalfT), r = bid % (kHalfT * kHalfT);
+        const int stx = (g.nx + kSuperTile - 1) / kSuperTile;
+        ix = (stl % stx) * kSuperTile + (r % kHalfT) * 2 + (wv & 1);
+        iy = (stl / stx) * kSuperTile + (r / kHalfT) * 2 + (wv >> 1);
+        if (ix >= g.nx || iy >= g.ny) return;
+    }
+    const int colid = iy * g.nx + ix;
     const int lane = lane_id();
     Tout* ocol = vol + (size_t)colid * g.zstride;
     const bool masked = map != nullptr && map[iy * g.nx + ix] == 0;
@@ -1237,7 +1329,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
 // takes a contiguous range of super-tiles: the resident columns form a compact patch
 // whose per-view footprint is about a tenth of the detector, not two full image rows.
 // ---------------------------------------------------------------------------------
-constexpr int kPdTile = 32;  // super-tile edge in columns
+constexpr int kPdTile = kSuperTile;  // super-tile edge in columns
 
 // The reference tests u and v in double against 1e-6 and n - 1 - 1e-6.  In T these are
 // the same tests against lo = the smallest T whose double value is >= 1e-6 and hi = the
@@ -1442,12 +1534,35 @@ __global__ __launch_bounds__(256) void zxy_to_xyz_f64(const double* __restrict__
 namespace ctsim {
 
 constexpr int kTU = 16;  // detector columns per forward-projection wave
+// ring depth of the FP16 ring form: one 64-dword slot per column (at most 128 slices), two
+// (at most 256); six entries keep five columns in flight in the LDS the FP32 form's four take
+constexpr int kHalfRing1 = 6, kHalfRing2 = 4;
 
 template <class Tin, class Tout, class T, int FOOT, int RPL, int SPLIT>
 static void launch_proj(unsigned blocks, hipStream_t st, const Geom<T>& g, const View<T>* views, int vstride,
                         int nviews, const void* vol, const uint8_t* map, void* proj, int exhaustive) {
-    // FP32 volumes whose columns fit the three 64-voxel slots take the ring form
-    if constexpr (std::is_same<Tin, float>::value && std::is_same<T, float>::value) {
+    const int form = g.sched_proj & 3;  // 0 auto, 1 ring, 2 register pipeline
+    // FP16 volumes of an even number of at most 256 slices: the ring form with half slots
+    // (auto: the register pipeline, which measured faster at 128 slices)
+    if constexpr (std::is_same<Tin, __half>::value && std::is_same<T, float>::value) if (form == 1) {
+        constexpr int kRW = SPLIT > 1 ? 8 : kWavesPerBlock, kRS = SPLIT > 1 ? 8 : 1;
+        if (g.nz % 2 == 0 && g.nz <= 2 * kWave) {
+            hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU, kRS, 1, kRW, true, kHalfRing1>), dim3(blocks),
+                               dim3(kRW * kWave), 0, st, g, views, vstride, nviews, (const __half*)vol, map,
+                               (Tout*)proj, exhaustive);
+            return;
+        }
+        if (g.nz % 2 == 0 && g.nz <= 4 * kWave) {
+            hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU, kRS, 2, kRW, true, kHalfRing2>), dim3(blocks),
+                               dim3(kRW * kWave), 0, st, g, views, vstride, nviews, (const __half*)vol, map,
+                               (Tout*)proj, exhaustive);
+            return;
+        }
+    }
+    // FP32 volumes whose columns fit the three 64-voxel slots take the ring form (auto);
+    // four slots (at most 256 slices) for whole work items (auto: 5% faster than the
+    // register pipeline on the 720-view 512^2 x 256 operator) or on request
+    if constexpr (std::is_same<Tin, float>::value && std::is_same<T, float>::value) if (form != 2) {
         // split items (SPLIT > 1: one item per block) go to blocks of eight waves
         constexpr int kRW = SPLIT > 1 ? 8 : kWavesPerBlock, kRS = SPLIT > 1 ? 8 : 1;
         if (g.nz <= 2 * kWave) {
@@ -1457,6 +1572,11 @@ static void launch_proj(unsigned blocks, hipStream_t st, const Geom<T>& g, const
         }
         if (g.nz <= kFastSpan) {
             hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU, kRS, 3, kRW>), dim3(blocks), dim3(kRW * kWave), 0,
+                               st, g, views, vstride, nviews, (const float*)vol, map, (Tout*)proj, exhaustive);
+            return;
+        }
+        if ((form == 1 || (form == 0 && SPLIT == 1)) && g.nz <= 4 * kWave) {
+            hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU, kRS, 4, kRW>), dim3(blocks), dim3(kRW * kWave), 0,
                                st, g, views, vstride, nviews, (const float*)vol, map, (Tout*)proj, exhaustive);
             return;
         }
@@ -1471,8 +1591,10 @@ static void proj3_foot(const Geom<T>& g, const View<T>* views, int vstride, int 
     const long long nut = (g.nu + kTU - 1) / kTU, nvc = (g.nv + kWave - 1) / kWave;
     const long long items = (long long)nviews * nut * nvc;
     // few work items (short view subsets, thin detectors): split each item's strip over
-    // the four waves of a block so that the chip still holds several waves per SIMD
-    const bool split = items < 16384;
+    // the waves of a block so that the chip still holds several waves per SIMD; the plan's
+    // schedule can force either form
+    const int ssplit = (g.sched_proj >> 2) & 3;
+    const bool split = ssplit == 1 ? false : (ssplit == 2 ? true : items < 16384);
     const unsigned blocks = (unsigned)(split ? items : (items + kWavesPerBlock - 1) / kWavesPerBlock);
     if (split) launch_proj<Tin, Tout, T, FOOT, 1, kWavesPerBlock>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
     else launch_proj<Tin, Tout, T, FOOT, 1, 1>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
@@ -1522,11 +1644,11 @@ static void launch_back_dd_tf(unsigned blocks, hipStream_t st, const Geom<T>& g,
     if (foot == FOOT_DD)
         hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_DD, NZL, SHORT, false>), dim3(blocks), dim3(kBlock),
                            back_lds_bytes<T>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
-                           (Tout*)vol, scale, accumulate, rowwin, NestArgs{});
+                           (Tout*)vol, scale, accumulate, rowwin, NestArgs{}, g.sched_back);
     else
         hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_TF, NZL, SHORT, false>), dim3(blocks), dim3(kBlock),
                            back_lds_bytes<T>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
-                           (Tout*)vol, scale, accumulate, rowwin, NestArgs{});
+                           (Tout*)vol, scale, accumulate, rowwin, NestArgs{}, g.sched_back);
 }
 
 template <class Tin, class Tout, class T, int NZL>
@@ -1555,8 +1677,8 @@ static hipError_t back3(const Geom<T>& g, const View<T>* views, int vstride, int
                         const uint8_t* map, void* vol, int foot, int pd_mode, T scale, int accumulate,
                         hipStream_t st) {
     const long long cols = (long long)g.nx * g.ny;
-    const unsigned blocks = foot == 2 ? (unsigned)pd_blocks(g.nx, g.ny)
-                                      : (unsigned)((cols + kWavesPerBlock - 1) / kWavesPerBlock);
+    const unsigned blocks = (foot == 2 || g.sched_back == 1) ? (unsigned)pd_blocks(g.nx, g.ny)
+                                                             : (unsigned)((cols + kWavesPerBlock - 1) / kWavesPerBlock);
     if (cols == 0 || blocks == 0) return hipSuccess;
     if (foot == 2 && (g.nu < 2 || g.nv < 2)) return hipErrorInvalidValue;  // bilinear needs 2x2 cells
     if (foot != 2 && (long long)kWave * g.nu * g.nv * (long long)sizeof(Tin) > 0xffffffffLL)
@@ -1623,11 +1745,11 @@ static void launch_back_nesterov(unsigned blocks, hipStream_t st, const Geom<flo
     if (foot == FOOT_DD)
         hipLaunchKernelGGL((cbct_back_kernel<Tin, float, float, FOOT_DD, NZL, true, true>), dim3(blocks), dim3(kBlock),
                            back_lds_bytes<float>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
-                           (float*)nullptr, scale, 0, rowwin, na);
+                           (float*)nullptr, scale, 0, rowwin, na, 0);
     else
         hipLaunchKernelGGL((cbct_back_kernel<Tin, float, float, FOOT_TF, NZL, true, true>), dim3(blocks), dim3(kBlock),
                            back_lds_bytes<float>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
-                           (float*)nullptr, scale, 0, rowwin, na);
+                           (float*)nullptr, scale, 0, rowwin, na, 0);
 }
 
 template <class Tin>

@@ -102,6 +102,12 @@ struct Geom {
     // zstride = nz and zoff = 0 for a whole volume
     int zstride;
     T zoff;
+    // launch schedule (host side only; set by ctsim_plan_set_schedule / the autotuner):
+    // sched_proj = form | split << 2 with form 0 auto, 1 LDS ring, 2 register pipeline and
+    // split 0 auto, 1 one wave per work item, 2 a block's waves share an item;
+    // sched_back = 0 consecutive columns per block, 1 2 x 2 column patches in 32 x 32 super-tiles
+    int sched_proj;
+    int sched_back;
 };
 
 // In-plane column quantities (cbct_proj.c:491-533).

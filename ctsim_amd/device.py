@@ -89,7 +89,23 @@ class DeviceArray:
     def vp(self) -> C.c_void_p:
         return C.c_void_p(self.ptr)
 
+    @classmethod
+    def window(cls, base: "DeviceArray", offset: int, count: int) -> "DeviceArray":
+        """A view of count elements of base from element offset on; it does not own the
+        memory (freeing it is a no-op) and must not outlive base."""
+        if offset < 0 or count < 0 or offset + count > base.count:
+            raise ValueError("window outside the array")
+        v = cls.__new__(cls)
+        v.dtype, v.count, v.shape = base.dtype, int(count), (int(count),)
+        v.nbytes = v.count * _SIZE[v.dtype]
+        v.ptr = base.ptr + offset * _SIZE[base.dtype]
+        v._view = True
+        return v
+
     def free(self) -> None:
+        if getattr(self, "_view", False):
+            self.ptr = 0
+            return
         if getattr(self, "ptr", 0):
             try:
                 lib().ctsim_free(C.c_void_p(self.ptr))

@@ -74,6 +74,36 @@ class Projector:
         call("ctsim_plan_back", C.c_void_p(self.handle), op.encode(), view0, vstep, nsub, sino.vp, sino.dtype,
              img.vp, img.dtype, float(scale), int(accumulate), C.c_void_p(stream.handle if stream else 0))
 
+    # launch schedules and the autotuner (3D) ------------------------------------------
+    def schedule(self) -> tuple:
+        """(proj_sched, back_sched) of the plan; see ctsim_plan_set_schedule"""
+        a, b = C.c_int32(), C.c_int32()
+        call("ctsim_plan_get_schedule", C.c_void_p(self.handle), C.byref(a), C.byref(b))
+        return int(a.value), int(b.value)
+
+    def set_schedule(self, proj_sched: int | None = None, back_sched: int | None = None) -> None:
+        a, b = self.schedule()
+        call("ctsim_plan_set_schedule", C.c_void_p(self.handle), a if proj_sched is None else int(proj_sched),
+             b if back_sched is None else int(back_sched))
+
+    def autotune(self, d_in: DeviceArray, d_out: DeviceArray, op: str, view0=0, vstep=1, nsub=None, reps=3,
+                 stream=None) -> dict:
+        """Time every schedule of op (proj,dd / proj,tf: forward forms and splits; back,dd /
+        back,tf: column orders) on these buffers and keep the fastest in the plan.  Returns
+        {"best": code, "candidates": [(code, ms), ...]}."""
+        nsub = self.geom.noViews if nsub is None else nsub
+        if op.startswith("proj"):
+            self._check(d_in, d_out, view0, vstep, nsub)
+        else:
+            self._check(d_out, d_in, view0, vstep, nsub)
+        best = C.c_int32()
+        ms = (C.c_float * 8)()
+        codes = (C.c_int32 * 8)()
+        call("ctsim_plan_autotune", C.c_void_p(self.handle), op.encode(), view0, vstep, nsub, d_in.vp, d_in.dtype,
+             d_out.vp, d_out.dtype, int(reps), C.byref(best), ms, codes, 8, C.c_void_p(stream.handle if stream else 0))
+        n = 5 if op.startswith("proj") else 2
+        return {"best": int(best.value), "candidates": [(int(codes[i]), round(float(ms[i]), 4)) for i in range(n)]}
+
     # host helpers (MATLAB-order numpy in, MATLAB-order numpy out) --------------------
     def upload_volume(self, vol: np.ndarray, dtype="f32", stream=None) -> DeviceArray:
         """MATLAB [nx ny nz] (or [nx ny]) -> device (ZXY for 3D)."""

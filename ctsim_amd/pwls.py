@@ -19,11 +19,13 @@ Three forms:
   * mixed:  y, w and the residual in FP16 or BF16, FP32 footprint sums and volumes,
             FP64 step denominators and reductions, and each iteration replayed as one
             captured hipGraph.  By default the stored data are y - A x0 (x0 the start
-            image, projected once in FP32) and the forward projector reads z - x0,
-            written by the update kernel: A is linear, so A z - y = A(z - x0) - (y - A x0)
-            exactly, and the half-precision rounding of the data is taken on the much
-            smaller y - A x0 (offset=False stores y itself and projects an FP16 copy
-            of z);
+            image, projected once in FP32), so that the half-precision rounding of the
+            data is taken on the much smaller y - A x0: A is linear, so
+            A z - y = (A z - A x0) - (y - A x0) exactly.  offset_mode "residual" (the
+            default) projects z and subtracts the stored FP32 A x0 of the subset (10 MB)
+            before the residual; "input" has the update kernel write z - x0 for the
+            projector (two extra volume passes per subset).  offset=False stores y itself
+            and projects an FP16 copy of z;
   * fp64:   every array, footprint and sum in double (precision="f64"), the accuracy
             reference the other two are measured against.
 """
@@ -69,7 +71,7 @@ class PwlsOS:
                  mixed: bool = True, sino_dtype: str = "f16", zcopy_dtype: str | None = None,
                  graph: bool | None = None, op: str = "tf", stream: Stream | None = None,
                  precision: str | None = None, pfun: str = "huber", offset: bool | None = None,
-                 fused: bool = False):
+                 fused: bool = False, offset_mode: str = "residual"):
         if geom.ndim != 3:
             raise ValueError("PwlsOS drives the 3D cone-beam operators")
         self.f64 = precision == "f64"
@@ -86,6 +88,10 @@ class PwlsOS:
         self.pfun = penalty_code(pfun)
         self.mixed = bool(mixed)
         self.offset = self.mixed and (True if offset is None else bool(offset))
+        if offset_mode not in ("residual", "input"):
+            raise ValueError("offset_mode is 'residual' or 'input'")
+        # residual: A x0 stored and subtracted from A z; input: the projector reads z - x0
+        self.offset_input = self.offset and offset_mode == "input"
         self.graph_mode = self.mixed if graph is None else bool(graph)
         if self.f64 and self.graph_mode:
             raise ValueError("the FP64 form runs without graph capture")
@@ -135,7 +141,10 @@ class PwlsOS:
         self.g = DeviceArray(nvox, vdt)
         self.precom = DeviceArray(nvox, vdt)
         self.xold = DeviceArray(nvox, vdt)
-        self.zc = DeviceArray(nvox, self.zdt) if self.mixed else None
+        # projector input copy: z - x0 (offset_mode "input") or a half-precision z (offset=False)
+        self.zc = DeviceArray(nvox, self.zdt) if self.mixed and (self.offset_input or not self.offset) else None
+        # A x0 of every view, subset-major, FP32 (offset_mode "residual")
+        self.ax0 = DeviceArray(nviews * self.cell, F32) if self.offset and not self.offset_input else None
         # backprojection and update in one kernel (ctsim_plan_back_nesterov), on request only:
         # FP32, Huber, detectors of at most 64 rows, an FP32 projector input copy. At 512x512x128
         # with 41 views per subset it takes 1.53 ms against 1.02 + 0.30 ms for the two kernels
@@ -146,7 +155,7 @@ class PwlsOS:
             raise ValueError("fused=True needs FP32 volumes, the Huber penalty, the dd or tf operator, a detector "
                              "of at most 64 rows, at most 512 slices and an FP32 projector input copy")
         self.fused = bool(fused)
-        self.x0 = DeviceArray(nvox, F32) if self.offset else None  # reference image of the offset form
+        self.x0 = DeviceArray(nvox, F32) if self.offset_input else None  # reference image of the input form
         self.scal = DeviceArray(4, "f64")  # phi, R, rmsd numerator, spare
         self.cur = 0  # which of the x / z double buffers is current
         self.t = 1.0
@@ -187,15 +196,19 @@ class PwlsOS:
         a = DeviceArray(max(s[2] for s in self.subsets) * self.cell, vdt)
         wa = DeviceArray(a.count, vdt)
         if self.offset:
-            # stored data y - A x0 (FP32 projection, one rounding to the storage dtype);
-            # the projector input z - x0 starts at zero
-            self.x0.copy_from(x0d, stream=st)
-            self.zc.fill(0.0, stream=st)
+            # stored data y - A x0 (FP32 projection, one rounding to the storage dtype); the
+            # input form's projector input z - x0 starts at zero, the residual form keeps A x0
+            if self.offset_input:
+                self.x0.copy_from(x0d, stream=st)
+                self.zc.fill(0.0, stream=st)
             yf = DeviceArray.from_numpy(self._y_host, F32, stream=st)
             for v0, vs, ns, off in self.subsets:
                 self.P.proj(x0d, a, f"proj,{self.kind}", v0, vs, ns, stream=st)
                 call("ctsim_sub", self._sub_ptr(yf, off), a.vp, self._sub_ptr(self.y, off), self.y.dtype,
                      ns * self.cell, self._sv())
+                if self.ax0 is not None:
+                    DeviceArray.window(self.ax0, off * self.cell, ns * self.cell).copy_from(
+                        DeviceArray.window(a, 0, ns * self.cell), stream=st)
             st.synchronize()
             del yf
         elif self.zc is not None:
@@ -207,6 +220,7 @@ class PwlsOS:
                  self._sv())
             self.P.back(self._View(wa.ptr, ns * self.cell, vdt), self.precom, f"back,{self.kind}", v0, vs, ns,
                         scale=self.wscale, accumulate=k > 0, stream=st)
+        self.scal.fill(0.0, stream=st)  # phi, R, rmsd accumulate from zero
         st.synchronize()
         self.cur = 0
         self.t = 1.0
@@ -217,9 +231,12 @@ class PwlsOS:
         """one ordered subset (index k into self.subsets) on the stream"""
         nx, ny, nz = self.nxyz
         v0, vs, ns, off = self.subsets[k]
-        src = self.zc if self.mixed else self.z[self.cur]
+        src = self.zc if self.zc is not None else self.z[self.cur]
         pw = self._window(self.p, 0, ns)
         self.P.proj(src, pw, f"proj,{self.kind}", v0, vs, ns, stream=self.stream)
+        if self.ax0 is not None:
+            # A z - A x0, in place (the stored data are y - A x0)
+            call("ctsim_sub", pw.vp, self._sub_ptr(self.ax0, off), pw.vp, F32, ns * self.cell, self._sv())
         call("ctsim_pwls_residual", pw.vp, self.p.dtype, self._sub_ptr(self.y, off), self.y.dtype,
              self._sub_ptr(self.w, off), self.w.dtype, self.r.vp, self.r.dtype, ns * self.cell, self.wscale,
              C.c_void_p(self.scal.ptr), self._sv())
@@ -287,6 +304,21 @@ class PwlsOS:
         sc = self.scal.to_numpy()
         return dict(phi=float(sc[0] + self.beta * sc[1]), penalty=float(self.beta * sc[1]),
                     rmsd=float(math.sqrt(sc[2] / self.P.n_img)))
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Pick the forward and backward launch schedules with the autotuner on the first
+        subset's launches (cached per geometry, view selection and dtypes; see
+        ctsim_amd.autotune).  Call after setup() and before prepare(): a captured iteration
+        keeps the schedules it was captured with."""
+        from .autotune import tuned
+        v0, vs, ns, _ = self.subsets[0]
+        src = self.zc if self.zc is not None else self.z[self.cur]
+        pw = DeviceArray.window(self.p, 0, ns * self.cell)
+        rw = DeviceArray.window(self.r, 0, ns * self.cell)
+        f = tuned(self.P, src, pw, f"proj,{self.kind}", v0, vs, ns, reps=reps, stream=self.stream)
+        b = tuned(self.P, rw, self.g, f"back,{self.kind}", v0, vs, ns, reps=reps, stream=self.stream)
+        self.stream.synchronize()
+        return {"proj": f, "back": b}
 
     def prepare(self, itnlim: int) -> None:
         """for the graph form: momentum sequence on the device, one iteration captured"""

@@ -183,6 +183,30 @@ typedef struct ctsim_plan ctsim_plan;
 int ctsim_plan_create(ctsim_plan** plan, int32_t ndim, const ctsim_geom* g, int32_t nviews, const double* betas,
                       const float* couchz, const uint8_t* map, int32_t compute);
 int ctsim_plan_destroy(ctsim_plan* plan);
+/* Launch schedules of a 3D plan (no counterpart in the reference, whose CUDA path uses
+ * fixed 16 x 16 blocks, geom/dd_cuda/cbct_dd_proj.c:441-442).  proj_sched = form | split << 2:
+ * form 0 automatic, 1 LDS-ring forward kernel (FP32 volumes of at most 256 slices, FP16
+ * volumes of an even number of at most 256), 2 register-pipeline forward kernel; split 0
+ * automatic (a block's waves share a work item below 16384 items), 1 one wave per item,
+ * 2 shared items.  back_sched: 0 four consecutive image columns per block, 1 2 x 2 column
+ * patches in 32 x 32 super-tiles (both with a contiguous range per XCD).  A new plan has
+ * (0, 1).  Every schedule
+ * is deterministic; the forward forms and splits add a ray's terms in different orders, so
+ * their results agree to FP32 rounding, not bit for bit. */
+int ctsim_plan_set_schedule(ctsim_plan* plan, int32_t proj_sched, int32_t back_sched);
+int ctsim_plan_get_schedule(const ctsim_plan* plan, int32_t* proj_sched, int32_t* back_sched);
+
+/* The project's autotuner for 3D plans: times every forward schedule (op proj,dd or
+ * proj,tf: automatic, ring whole, ring shared, register whole, register shared) or every
+ * backprojector column order (back,dd or back,tf) on the caller's view selection and
+ * buffers with HIP events on the stream (one warm-up, then reps timed launches each),
+ * keeps the fastest in the plan and returns it in *best.  d_out is overwritten; it must not
+ * alias d_in.  ms_out / sched_out (may be NULL) receive each candidate's mean time and
+ * schedule code, at most max_out of them. */
+int ctsim_plan_autotune(ctsim_plan* plan, const char* op, int32_t view0, int32_t vstep, int32_t nsub,
+                        const void* d_in, int32_t in_dtype, void* d_out, int32_t out_dtype, int32_t reps,
+                        int32_t* best, float* ms_out, int32_t* sched_out, int32_t max_out, void* stream);
+
 
 /* Forward projection of the views view0, view0 + vstep, ... (nsub of them), the
  * ordered-subset selection of geom/shell/forwardProjectMex.m:35-39.

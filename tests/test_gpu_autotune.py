@@ -1,0 +1,80 @@
+"""The 3D launch schedules and the autotuner (ctsim_plan_set_schedule / ctsim_plan_autotune).
+
+Every forward schedule (automatic, LDS ring with whole or shared work items, register
+pipeline with whole or shared items) and both backprojector column orders against the
+oracle, on geometries on either side of the ring forms' slice limits (128, 192 and 256
+slices, FP32 and FP16 volumes), with the same tolerance as the other parity tests; the
+autotuner on a subset launch returns one of its candidates, keeps it in the plan, and the
+schedule cache stores and reuses it."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from test_gpu_parity import _cone_case, _tol, geometry_from_g3
+
+pytestmark = pytest.mark.gpu
+
+PROJ_SCHEDS = [0, 1 | 1 << 2, 1 | 2 << 2, 2 | 1 << 2, 2 | 2 << 2]
+CASES = [
+    dict(seed=21, n=(40, 36, 100), nuv=(80, 40), nviews=12),               # ring, two slots
+    dict(seed=22, n=(36, 40, 180), nuv=(70, 70), nviews=9, dz=0.3, dv=0.9),  # ring, three slots
+    dict(seed=23, n=(30, 34, 240), nuv=(60, 90), nviews=8, dz=0.3, dv=0.9),  # ring, four slots (on request)
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[str(c["seed"]) for c in CASES])
+@pytest.mark.parametrize("vdt", ["f32", "f16"])
+def test_every_schedule_matches_the_oracle(case, vdt):
+    from oracle import oracle as O
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    g, vol, sino, mp = _cone_case(**case)
+    v = vol.astype(np.float16).astype(np.float32) if vdt == "f16" else vol
+    P = Projector(geometry_from_g3(g, mp))
+    d_vol = P.upload_volume(v, dtype=vdt)
+    out = DeviceArray(P.n_sino(), "f32")
+    print(f"schedules case {case['seed']} n={g.n.tolist()} {vdt}", flush=True)
+    for kind in ("dd", "tf"):
+        want = O.cbct(f"proj,{kind}", g, v, mp)
+        tol = _tol(f"proj,{kind}", g, v, mp)
+        for sc in PROJ_SCHEDS:
+            P.set_schedule(proj_sched=sc)
+            P.proj(d_vol, out, f"proj,{kind}")
+            e = rel_l2(P.download_sino(out), want)
+            assert e <= tol, (kind, sc, e)
+    if vdt == "f32":
+        d_sino = P.upload_sino(sino)
+        img = DeviceArray(P.n_img, "f32")
+        for kind in ("dd", "tf"):
+            want = O.cbct(f"back,{kind}", g, sino, mp)
+            for sb in (0, 1):
+                P.set_schedule(back_sched=sb)
+                P.back(d_sino, img, f"back,{kind}")
+                e = rel_l2(P.download_volume(img), want)
+                assert e <= _tol(f"back,{kind}", g, sino, mp), (kind, sb, e)
+
+
+def test_autotuner_picks_and_caches(tmp_path, monkeypatch):
+    from ctsim_amd import autotune
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    monkeypatch.setenv("CTSIM_SCHEDULE_CACHE", str(tmp_path / "schedules.json"))
+    g, vol, sino, mp = _cone_case(**CASES[0])
+    P = Projector(geometry_from_g3(g, mp))
+    d_vol = P.upload_volume(vol)
+    d_sino = DeviceArray(P.n_sino(6), "f32")
+    img = DeviceArray(P.n_img, "f32")
+    e = autotune.tuned(P, d_vol, d_sino, "proj,tf", 1, 2, 6, reps=2)
+    print("forward candidates", e["candidates"], "best", e["best"], flush=True)
+    assert e["best"] in PROJ_SCHEDS and P.schedule()[0] == e["best"]
+    assert [c for c, _ in e["candidates"]] == PROJ_SCHEDS and all(ms > 0 for _, ms in e["candidates"])
+    b = autotune.tuned(P, d_sino, img, "back,tf", 1, 2, 6, reps=2)
+    assert b["best"] in (0, 1) and P.schedule() == (e["best"], b["best"])
+    assert os.path.exists(tmp_path / "schedules.json")
+    # a second plan of the same geometry takes the cached schedules without timing
+    P2 = Projector(geometry_from_g3(g, mp))
+    P2.set_schedule(proj_sched=0, back_sched=0)
+    e2 = autotune.tuned(P2, d_vol, d_sino, "proj,tf", 1, 2, 6)
+    assert e2 == e and P2.schedule()[0] == e["best"]

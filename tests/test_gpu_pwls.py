@@ -100,8 +100,16 @@ def test_pwls_mixed_graph_equals_stream_and_is_close_to_plain():
     # taken on the small difference, so the iterate stays close to the plain FP32 one
     d = PwlsOS(_geometry(g), y, w, M, beta, delta, mixed=True, graph=True, offset=False)
     xd, _, _ = d.run(x0.astype(np.float32), its)
-    print("mixed vs plain:", rel_l2(xa, xc), "without the offset:", rel_l2(xd, xc))
+    # the offset taken at the projector input (the update kernel writes z - x0) instead of
+    # at the residual (A x0 stored and subtracted): the same data, other FP32 roundings
+    e = PwlsOS(_geometry(g), y, w, M, beta, delta, mixed=True, graph=True, offset_mode="input")
+    assert e.zc is not None and a.zc is None and a.ax0 is not None
+    xe, _, _ = e.run(x0.astype(np.float32), its)
+    print("mixed vs plain:", rel_l2(xa, xc), "without the offset:", rel_l2(xd, xc),
+          "offset at the projector input:", rel_l2(xe, xc), "residual vs input offset:", rel_l2(xa, xe))
     assert rel_l2(xa, xc) < 5e-4
+    assert rel_l2(xe, xc) < 5e-4
+    assert rel_l2(xa, xe) < 5e-4
     assert rel_l2(xd, xc) < 2e-3
     np.testing.assert_allclose(pa, pc, rtol=1e-3)
 
