@@ -128,6 +128,7 @@ SIGNATURES = {
     "ctsim_poisson_counts": (C.c_int, [vp, i32, sz, f64, C.c_uint64, vp, vp]),
     "ctsim_log_attenuation": (C.c_int, [vp, sz, f64, f64, vp, i32, vp]),
     "ctsim_log_attenuation_ex": (C.c_int, [vp, sz, i32, f64, f64, i32, vp, vp, vp, i32, vp]),
+    "ctsim_water_correction": (C.c_int, [vp, i32, vp, i32, i32, i32, i32, vp, i32, i32, vp]),
     "ctsim_pwls_weights": (C.c_int, [vp, sz, f64, f64, vp, i32, vp]),
     "ctsim_penalty": (C.c_int, [vp, i32, i32, i32, i32, f64, i32, vp, vp, vp]),
     "ctsim_penalty_f64": (C.c_int, [vp, i32, i32, i32, i32, f64, i32, vp, vp, vp]),

@@ -1516,6 +1516,20 @@ int ctsim_log_attenuation_ex(const float* d_counts, size_t n, int32_t ncell, dou
     return CTSIM_OK;
 }
 
+int ctsim_water_correction(const void* d_in, int32_t in_dtype, void* d_out, int32_t out_dtype, int32_t nv, int32_t nu,
+                           int32_t nviews, const float* d_coeffs, int32_t order, int32_t per_channel, void* stream) {
+    if (!d_in || !d_out || !d_coeffs) return fail(CTSIM_EINVAL, "Error: null argument.");
+    if (nv <= 0 || nu <= 0 || nviews < 0) return fail(CTSIM_EINVAL, "Error: bad sinogram size.");
+    if (order < 1 || order > 16) return fail(CTSIM_EINVAL, "Error: polynomial order must be 1 .. 16.");
+    const size_t n = (size_t)nv * nu * nviews;
+    if (n == 0) return CTSIM_OK;
+    hipError_t e = water_launch(d_in, in_dtype, d_out, out_dtype, n, nv, nu, d_coeffs, order, per_channel ? 1 : 0,
+                                (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(CTSIM_EUNSUP, "Error: dtype pair not supported.");
+    HIP_TRY(e, "water correction");
+    return CTSIM_OK;
+}
+
 int ctsim_pwls_weights(const float* d_counts, size_t n, double r, double v, void* d_w, int32_t w_dtype, void* stream) {
     if ((!d_counts || !d_w) && n) return fail(CTSIM_EINVAL, "Error: null argument.");
     if (n == 0) return CTSIM_OK;

@@ -135,6 +135,8 @@ hipError_t poisson_launch(const void* p, int pdt, size_t n, double i0, unsigned 
 hipError_t neglog_launch(const float* counts, size_t n, double i0, double low, void* y, int ydt, hipStream_t st);
 hipError_t neglog_ex_launch(const float* counts, size_t n, int ncell, double i0, double low, int compound,
                             const float* ff, const float* profile, void* y, int ydt, hipStream_t st);
+hipError_t water_launch(const void* in, int idt, void* out, int odt, size_t n, int nv, int nu, const float* coeffs,
+                        int order, int per_channel, hipStream_t st);
 hipError_t weights_launch(const float* s, size_t n, double r, double v, void* w, int wdt, hipStream_t st);
 
 // penalties (pfun 0 Huber, 1 quadratic, 2 hyperbola) and the PWLS updates (pwls.hip)

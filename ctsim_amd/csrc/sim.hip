@@ -120,6 +120,30 @@ __global__ void weights_kernel(const float* __restrict__ s, size_t n, double r, 
     }
 }
 
+// Water beam-hardening correction (artifact/beamhardening/beamHardeningWarterCorrection.m:
+// 40-69): out = sum_k c_k(u) p^k, k = 1 .. order, evaluated as the reference writes it
+// (p c_1, then + p^k c_k in turn, single precision), with the coefficients of detector
+// channel u (the sinogram's column, [view][u][v] order; one set for all channels when
+// per_channel is 0).  coeffs[u * order + k - 1] = c_k(u).  In place is allowed.
+template <class Tin, class Tout>
+__global__ void water_poly_kernel(const Tin* in, Tout* out, size_t n, int nv, int nu,
+                                  const float* __restrict__ coeffs, int order, int per_channel) {
+#pragma clang fp contract(off)
+    const size_t cells = (size_t)nu * nv;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int u = per_channel ? (int)((i % cells) / (size_t)nv) : 0;
+        const float* c = coeffs + (size_t)u * order;
+        const float p = ld_f(in + i);
+        float acc = p * c[0];
+        float pk = p;
+        for (int k = 1; k < order; ++k) {
+            pk = pk * p;
+            acc = acc + pk * c[k];
+        }
+        out[i] = cvt_store<Tout>(acc);
+    }
+}
+
 static unsigned grid_sim(size_t n) {
     size_t b = (n + 255) / 256;
     return (unsigned)(b < 65536 ? (b ? b : 1) : 65536);
@@ -158,6 +182,29 @@ hipError_t neglog_ex_launch(const float* counts, size_t n, int ncell, double i0,
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <class Tin>
+static hipError_t water_launch_in(const void* in, void* out, int odt, size_t n, int nv, int nu, const float* c,
+                                  int order, int per_channel, hipStream_t st) {
+    const dim3 g(grid_sim(n)), b(256);
+    switch (odt) {
+        case DT_F32: hipLaunchKernelGGL((water_poly_kernel<Tin, float>), g, b, 0, st, (const Tin*)in, (float*)out, n, nv, nu, c, order, per_channel); break;
+        case DT_F16: hipLaunchKernelGGL((water_poly_kernel<Tin, __half>), g, b, 0, st, (const Tin*)in, (__half*)out, n, nv, nu, c, order, per_channel); break;
+        case DT_BF16: hipLaunchKernelGGL((water_poly_kernel<Tin, __hip_bfloat16>), g, b, 0, st, (const Tin*)in, (__hip_bfloat16*)out, n, nv, nu, c, order, per_channel); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t water_launch(const void* in, int idt, void* out, int odt, size_t n, int nv, int nu, const float* coeffs,
+                        int order, int per_channel, hipStream_t st) {
+    switch (idt) {
+        case DT_F32: return water_launch_in<float>(in, out, odt, n, nv, nu, coeffs, order, per_channel, st);
+        case DT_F16: return water_launch_in<__half>(in, out, odt, n, nv, nu, coeffs, order, per_channel, st);
+        case DT_BF16: return water_launch_in<__hip_bfloat16>(in, out, odt, n, nv, nu, coeffs, order, per_channel, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t weights_launch(const float* s, size_t n, double r, double v, void* w, int wdt, hipStream_t st) {

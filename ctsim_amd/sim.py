@@ -3,8 +3,9 @@
 source_intensity() is the expected unattenuated count of the spectrum, as the two log
 functions compute it on the host; log_attenuation() runs the device log with the
 tube-current profile, the bowtie flat-field ratio and the clamp of the chosen noise
-model.  The water beam-hardening correction (beamHardeningWarterCorrection.m) and the
-material tables behind the detector response are not part of this path: the compound
+model.  process_ct_raw_data() chains it with the water beam-hardening correction
+(beamhardening.water_correction, coefficients per detector channel from
+beamhardening.water_coefficient_table), as processCTRawData.m:37-40 does.  The compound
 model takes the scintillator's attenuation at the bin energies from the caller.
 """
 from __future__ import annotations
@@ -65,4 +66,18 @@ def log_attenuation(counts: DeviceArray, ncell: int, source_intensity: float, co
          float(low_count), int(bool(compound)), flat_field.vp if flat_field is not None else None,
          tube_profile.vp if tube_profile is not None else None, y.vp, y.dtype,
          stream.vp if stream is not None else None)
+    return y
+
+
+def process_ct_raw_data(counts: DeviceArray, nv: int, nu: int, source_intensity: float, compound: bool = False,
+                        flat_field: DeviceArray | None = None, tube_profile: DeviceArray | None = None,
+                        water_coeffs: np.ndarray | None = None, out_dtype: str = "f32", stream=None) -> DeviceArray:
+    """processCTRawData.m:1-43 on a device sinogram [nviews][nu][nv] (nv = 1 for fan beam):
+    the tube-current division and -log (log_attenuation), then, when water_coeffs is given
+    ([nu, order] per channel, or [1, order]), the water polynomial in place."""
+    from .beamhardening import water_correction
+    y = log_attenuation(counts, nv * nu, source_intensity, compound, None, flat_field, tube_profile, out_dtype,
+                        stream)
+    if water_coeffs is not None:
+        water_correction(y, nv, nu, counts.count // (nv * nu), water_coeffs, out=y, stream=stream)
     return y

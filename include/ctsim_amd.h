@@ -503,6 +503,17 @@ int ctsim_log_attenuation(const float* d_counts, size_t n, double i0, double low
 int ctsim_log_attenuation_ex(const float* d_counts, size_t n, int32_t ncell, double source_intensity,
                              double low_count, int32_t compound, const float* d_flat_field,
                              const float* d_tube_profile, void* d_y, int32_t y_dtype, void* stream);
+/* Water beam-hardening correction of a sinogram of line integrals
+ * (artifact/beamhardening/beamHardeningWarterCorrection.m:40-69, the last step of
+ * sinogram/log/processCTRawData.m:40-43): out = sum_{k=1..order} c_k(u) in^k in single
+ * precision, per detector channel u (the column of a [nv nu nviews] sinogram; nv = 1 for
+ * a 2D [nu nviews] one) when per_channel is nonzero, else one coefficient set.
+ * d_coeffs: device floats, [channel][order] with the order fastest (order values when
+ * per_channel is 0); the coefficients come from the host fit of
+ * beamHardeningCorrectionPolynomialCoefficients.m (ctsim_amd.beamhardening).  F32, F16 or
+ * BF16 in and out; in place allowed. */
+int ctsim_water_correction(const void* d_in, int32_t in_dtype, void* d_out, int32_t out_dtype, int32_t nv, int32_t nu,
+                           int32_t nviews, const float* d_coeffs, int32_t order, int32_t per_channel, void* stream);
 /* w = (s - r)^2 / (s + v) in single precision, 0 where s <= r (statsrecon/pwls/computeWeightsPwls.m:8-9). */
 int ctsim_pwls_weights(const float* d_counts, size_t n, double r, double v, void* d_w, int32_t w_dtype, void* stream);
 

@@ -1,5 +1,6 @@
-"""GPU data side: Poisson counts (distribution checks), the log attenuation and the PWLS
-weights (exact against numpy restatements of the reference formulas)."""
+"""GPU data side: Poisson counts (distribution checks), the log attenuation, the PWLS
+weights and the water beam-hardening polynomial (exact against numpy restatements of the
+reference formulas)."""
 import math
 
 import numpy as np
@@ -92,3 +93,98 @@ def test_log_attenuation_ex_matches_the_raw_data_restatement(compound, with_ff, 
     np.testing.assert_allclose(y, want, rtol=1e-14, atol=1e-14)
     y32 = sim.log_attenuation(d_raw, nc, i0, compound, None, d_ff, d_prof, "f32").to_numpy()
     np.testing.assert_allclose(y32, want.astype(np.float32), rtol=2e-7, atol=1e-7)
+
+
+def _water_f32(p, coeffs, nv, nu):
+    """The correction in float32, one rounding per operation, in the reference's order
+    (beamHardeningWarterCorrection.m:56-60: p c_1, then + p^k c_k for k = 2 .. order)."""
+    p = np.asarray(p, np.float32).reshape(-1, nu, nv)
+    c = np.asarray(coeffs, np.float32)
+    c = np.broadcast_to(c[:, None, :], (nu, nv, c.shape[1])) if c.shape[0] > 1 else np.broadcast_to(
+        c[0], (nu, nv, c.shape[1]))
+    acc = (p * c[..., 0]).astype(np.float32)
+    pk = p.copy()
+    for k in range(1, c.shape[-1]):
+        pk = (pk * p).astype(np.float32)
+        acc = (acc + (pk * c[..., k]).astype(np.float32)).astype(np.float32)
+    return acc.reshape(-1)
+
+
+@pytest.mark.parametrize("nv,nu,nviews,order,per_channel", [(1, 367, 29, 3, True), (16, 89, 7, 3, True),
+                                                            (16, 89, 7, 4, False), (1, 5, 3, 1, True)])
+@pytest.mark.parametrize("in_dt", ["f32", "f16", "bf16"])
+def test_water_correction_matches_float32_evaluation(nv, nu, nviews, order, per_channel, in_dt):
+    from ctsim_amd.beamhardening import water_correction
+    from ctsim_amd.device import DeviceArray, bf16_from_f32, f32_from_bf16
+    rng = np.random.default_rng(nv * 1000 + nu + order)
+    p = rng.uniform(0.0, 9.0, nv * nu * nviews).astype(np.float32)
+    p[:7] = [0.0, 1e-30, 0.5, 1.0, 8.999, 3.25, 2.0][:min(7, p.size)]
+    if in_dt == "f16":
+        p = p.astype(np.float16).astype(np.float32)
+    elif in_dt == "bf16":
+        p = f32_from_bf16(bf16_from_f32(p))
+    coeffs = np.stack([np.r_[1.0 + 0.01 * u, rng.uniform(-0.02, 0.05, order - 1)] for u in range(nu)]).astype(
+        np.float32)
+    if not per_channel:
+        coeffs = coeffs[:1]
+    want = _water_f32(p, coeffs, nv, nu)
+    d = DeviceArray.from_numpy(p, in_dt)
+    got = water_correction(d, nv, nu, nviews, coeffs, out=DeviceArray(p.size, "f32")).to_numpy()
+    np.testing.assert_array_equal(got, want)
+    # narrower output: the float32 result rounded once
+    got16 = water_correction(d, nv, nu, nviews, coeffs, out=DeviceArray(p.size, "f16")).to_numpy()
+    np.testing.assert_array_equal(got16, want.astype(np.float16))
+    gotb = water_correction(d, nv, nu, nviews, coeffs, out=DeviceArray(p.size, "bf16")).to_numpy()
+    np.testing.assert_array_equal(gotb, f32_from_bf16(bf16_from_f32(want)))
+    # in place over the input
+    water_correction(d, nv, nu, nviews, coeffs, out=d)
+    stored = {"f32": want, "f16": want.astype(np.float16), "bf16": f32_from_bf16(bf16_from_f32(want))}[in_dt]
+    np.testing.assert_array_equal(d.to_numpy(), stored)
+
+
+def test_water_correction_with_fitted_coefficients_linearises_water():
+    """The fitted polynomial of a filtered 120 kVp spectrum, applied on the device to the
+    polychromatic line integrals of water slabs, returns L mu_eff to the fit's accuracy."""
+    import os
+    from conftest import GOLDEN
+    from ctsim_amd.beamhardening import (interp_geometric, spectral_attenuation, water_coefficient_table,
+                                         water_correction)
+    from ctsim_amd.device import DeviceArray
+    with np.load(os.path.join(GOLDEN, "material_tables.npz"), allow_pickle=False) as z:
+        t = {k: z[k] for k in z.files}
+    e = np.arange(20.0, 121.0)
+    ph = (120.0 - e) / e * (1 - np.exp(-((e - 15.0) / 12.0) ** 2)) * 1e6
+    eeff = float(np.sum(e * ph) / np.sum(ph))
+    nu, nviews = 24, 5
+    thick = np.linspace(0.0, 30.0, nu)
+    tab = water_coefficient_table(e, ph, t["water_E"], t["water_mu"], eeff,
+                                  bowtie=(t["aluminum_E"], t["aluminum_mu"], thick))
+    m = spectral_attenuation(e, t["water_E"], t["water_mu"])
+    meff = float(interp_geometric(e, m, [eeff])[0])
+    L = np.linspace(1.0, 40.0, nviews)
+    sino = np.empty((nviews, nu), np.float32)
+    for iv in range(nviews):
+        for u in range(nu):
+            pc = ph * e * np.exp(-spectral_attenuation(e, t["aluminum_E"], t["aluminum_mu"]) * thick[u] / 10.0)
+            sino[iv, u] = -np.log(np.sum(pc * np.exp(-L[iv] * m)) / np.sum(pc))
+    got = water_correction(DeviceArray.from_numpy(sino, "f32"), 1, nu, nviews, tab).to_numpy().reshape(nviews, nu)
+    np.testing.assert_allclose(got, np.broadcast_to((L * meff)[:, None], got.shape), atol=5e-3 * 50.0 * meff)
+    # the device result is the float32 polynomial of those coefficients
+    np.testing.assert_array_equal(got.reshape(-1), _water_f32(sino.reshape(-1), tab.astype(np.float32), 1, nu))
+
+
+def test_process_ct_raw_data_chains_log_and_water():
+    from ctsim_amd import sim
+    from ctsim_amd.device import DeviceArray
+    rng = np.random.default_rng(9)
+    nviews, nu, nv, i0 = 6, 41, 4, 2e4
+    raw = rng.poisson(i0 * 0.1, (nviews, nu, nv)).astype(np.float32)
+    prof = rng.uniform(0.5, 2.0, nviews).astype(np.float32)
+    coeffs = np.stack([[1.0 + 0.002 * u, 0.03, -0.001] for u in range(nu)]).astype(np.float32)
+    d_raw, d_prof = DeviceArray.from_numpy(raw, "f32"), DeviceArray.from_numpy(prof, "f32")
+    y = sim.log_attenuation(d_raw, nu * nv, i0, tube_profile=d_prof).to_numpy()
+    got = sim.process_ct_raw_data(d_raw, nv, nu, i0, tube_profile=d_prof, water_coeffs=coeffs).to_numpy()
+    assert got.shape == raw.shape
+    np.testing.assert_array_equal(got.reshape(-1), _water_f32(y, coeffs, nv, nu))
+    plain = sim.process_ct_raw_data(d_raw, nv, nu, i0, tube_profile=d_prof).to_numpy()
+    np.testing.assert_array_equal(plain, y)
