@@ -1456,6 +1456,220 @@ __global__ __launch_bounds__(kBlock) void cbct_back_pd_kernel(Geom<T> g, const V
 }
 
 // ---------------------------------------------------------------------------------
+// FDK backprojection, staged form (FP32 arithmetic, detectors of at most kPdRows rows).
+// The same columns, views, tests and per-voxel v as cbct_back_pd_kernel; what differs:
+//
+//  * the per-view quantities of the column (u, its cell and weight, w2s, the v line
+//    A iz + B and the detector rows the column's voxels reach) are evaluated one view per
+//    lane, 64 views at a time, and read with readlane in the view loop;
+//  * per view the wave interpolates the two detector columns iu, iu + 1 in u once, row by
+//    row, c(iv) = (1 - wul) p(iu, iv) + wul p(iu + 1, iv), over the rows its voxels reach
+//    (coalesced 8-byte loads), into its LDS row buffer; each voxel then interpolates c in v:
+//    w2s ((1 - wvl) c(iv) + wvl c(iv + 1)).  This is the reference's bilinear sum with the
+//    u and v interpolations taken in the other order (the same value up to FP32 rounding).
+//
+// The rows a column reaches: iv(v) = clamp(floor(m(v)), 0, nv - 2), m being the
+// boundary rule of the mode (hpd: 0.5 below, nv - 1.5 above; pd: voxels outside are
+// skipped, so the clamp to [lo, v_hi] bounds the ones kept), is non-decreasing in v, and
+// v is monotone in iz, so the rows lie between those of the first and last voxel.
+// ---------------------------------------------------------------------------------
+constexpr int kPdRows = 1024;             // detector rows a wave stages (taller: the direct kernel)
+constexpr int kPdPad = 4;                 // words before the first staged row
+constexpr int kPdStageW = kPdRows + 16;   // words per wave: pad, rows, the last load's overshoot
+
+// the E = 8 / sizeof(Tin) elements from p on as one 8-byte word; when only avail < E of
+// them lie inside the column, those, and zero bits for the rest
+template <class Tin>
+__device__ __forceinline__ uint2 pd_load8(const Tin* p, int avail) {
+    uint2 x = make_uint2(0u, 0u);
+    if (avail >= 8 / (int)sizeof(Tin)) {
+        __builtin_memcpy(&x, p, 8);
+    } else if constexpr (sizeof(Tin) == 4) {
+        if (avail > 0) x.x = *(const unsigned*)p;
+    } else {
+        const unsigned short* q = (const unsigned short*)p;
+        if (avail > 0) x.x = q[0];
+        if (avail > 1) x.x |= (unsigned)q[1] << 16;
+        if (avail > 2) x.y = q[2];
+    }
+    return x;
+}
+
+template <class Tin>
+__device__ __forceinline__ void pd_unpack8(uint2 x, float* out);
+template <>
+__device__ __forceinline__ void pd_unpack8<float>(uint2 x, float* out) {
+    out[0] = __uint_as_float(x.x);
+    out[1] = __uint_as_float(x.y);
+}
+template <>
+__device__ __forceinline__ void pd_unpack8<__half>(uint2 x, float* out) {
+    out[0] = __half2float(__ushort_as_half((unsigned short)(x.x & 0xffffu)));
+    out[1] = __half2float(__ushort_as_half((unsigned short)(x.x >> 16)));
+    out[2] = __half2float(__ushort_as_half((unsigned short)(x.y & 0xffffu)));
+    out[3] = __half2float(__ushort_as_half((unsigned short)(x.y >> 16)));
+}
+template <>
+__device__ __forceinline__ void pd_unpack8<__hip_bfloat16>(uint2 x, float* out) {
+    out[0] = __uint_as_float(x.x << 16);
+    out[1] = __uint_as_float(x.x & 0xffff0000u);
+    out[2] = __uint_as_float(x.y << 16);
+    out[3] = __uint_as_float(x.y & 0xffff0000u);
+}
+
+// iv of a voxel at v (the row pair it reads), for the row range of a column
+template <int OFF_BOUNDARY>
+__device__ __forceinline__ int pd_row_of(float v, const PdLimits<float>& lim, int nv) {
+    float m;
+    if (OFF_BOUNDARY) {
+        m = fminf(fmaxf(v, lim.lo), lim.v_hi);
+    } else {
+        m = v;
+        if (m < lim.lo) m = 0.5f;
+        if (m > lim.v_hi) m = float(nv) - 1.5f;
+    }
+    return max(0, min((int)floorf(m), nv - 2));
+}
+
+template <class Tin, class Tout, int NZL, int OFF_BOUNDARY>
+__global__ __launch_bounds__(kBlock) void cbct_back_pd_stage_kernel(Geom<float> g, const View<float>* __restrict__ views,
+                                                                    int vstride, int nviews,
+                                                                    const Tin* __restrict__ proj,
+                                                                    const uint8_t* __restrict__ map,
+                                                                    Tout* __restrict__ vol, float scale,
+                                                                    int accumulate, PdLimits<float> lim) {
+    using T = float;
+    constexpr bool off_boundary = OFF_BOUNDARY != 0;
+    constexpr int E = 8 / (int)sizeof(Tin);  // rows per 8-byte load
+    __shared__ __align__(16) float stage[kWavesPerBlock * kPdStageW];
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int wv = (int)(threadIdx.x / kWave);
+    constexpr int kHalf = kPdTile / 2;
+    const int st = bid / (kHalf * kHalf), r = bid % (kHalf * kHalf);
+    const int stx = (g.nx + kPdTile - 1) / kPdTile;
+    const int ix = (st % stx) * kPdTile + (r % kHalf) * 2 + (wv & 1);
+    const int iy = (st / stx) * kPdTile + (r / kHalf) * 2 + (wv >> 1);
+    if (ix >= g.nx || iy >= g.ny) return;
+    const int colid = iy * g.nx + ix;
+    const int lane = lane_id();
+    Tout* ocol = vol + (size_t)colid * g.zstride;
+    if (map != nullptr && map[iy * g.nx + ix] == 0) {
+        if (!accumulate)
+            for (int iz = lane; iz < g.nz; iz += kWave) ocol[iz] = cvt_store<Tout>(T(0));
+        return;
+    }
+    float* cs = stage + wv * kPdStageW;
+    const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
+    T izf[NZL];
+#pragma unroll
+    for (int c = 0; c < NZL; ++c) izf[c] = T(c * kWave + lane);
+    T img[NZL];
+#pragma unroll
+    for (int c = 0; c < NZL; ++c) img[c] = T(0);
+    for (int b0 = 0; b0 < nviews; b0 += kWave) {
+        // view b0 + lane: cell iu (-1: the column is skipped in this view), weight wul, w2s,
+        // the v line (A, B) and the rows r0 .. r1 its voxels read
+        int h_iu = -1, h_rows = 0;
+        T h_wul = T(0), h_w2s = T(0), h_A = T(0), h_B = T(0);
+        if (b0 + lane < nviews) {
+            const View<T> v = views[(size_t)(b0 + lane) * vstride];
+            T u, w2, w2s;
+            pd_view_col_f32(g, v, xc, yc, u, w2, w2s);
+            if (u >= lim.lo && u <= lim.u_hi) {
+                const T A = g.dz * w2 / g.dv;
+                const T B = (v.wz + g.zoff) * A - g.wv;
+                // (A and B are finite wherever u passes its test; a view where they are not
+                // is skipped, so that every voxel's v below is a number)
+                if (!(off_boundary && !(A > T(0)) && v.wz * A - g.wv > lim.v_hi) && __builtin_isfinite(A) &&
+                    __builtin_isfinite(B)) {
+                    h_iu = (int)floor(u);
+                    h_wul = u - T(h_iu);
+                    h_w2s = w2s;
+                    h_A = A;
+                    h_B = B;
+                    const T va = B, vb = fma_t(T(g.nz - 1), A, B);
+                    int ra = 0, rb = g.nv - 2;
+                    if (va == va && vb == vb) {
+                        ra = pd_row_of<OFF_BOUNDARY>(va, lim, g.nv);
+                        rb = pd_row_of<OFF_BOUNDARY>(vb, lim, g.nv);
+                    }
+                    h_rows = min(ra, rb) | ((max(ra, rb) + 1) << 16);
+                }
+            }
+        }
+        const int nb = min(kWave, nviews - b0);
+        for (int k = 0; k < nb; ++k) {
+            const int iu = rdl_i(h_iu, k);
+            if (iu < 0) continue;
+            const T wul = rdl(h_wul, k), w2s = rdl(h_w2s, k), A = rdl(h_A, k), B = rdl(h_B, k);
+            const int rows = rdl_i(h_rows, k);
+            const int r0 = rows & 0xffff, r1 = rows >> 16;
+            const int rbase = r0 & ~(E - 1);
+            // the previous view's reads of the row buffer come before these writes
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const Tin* p0 = proj + ((size_t)(b0 + k) * g.nu + iu) * g.nv;
+            const Tin* p1 = p0 + g.nv;
+            for (int ra = rbase + lane * E; ra <= r1; ra += kWave * E) {
+                const int avail = g.nv - ra;
+                const uint2 x0 = pd_load8<Tin>(p0 + (unsigned)ra, avail), x1 = pd_load8<Tin>(p1 + (unsigned)ra, avail);
+                T a[E], bb[E], c[E];
+                pd_unpack8<Tin>(x0, a);
+                pd_unpack8<Tin>(x1, bb);
+#pragma unroll
+                for (int e = 0; e < E; ++e) c[e] = fmaf(wul, bb[e], fmaf(a[e], -wul, a[e]));
+                float* dst = cs + (ra - rbase + kPdPad);
+                if constexpr (E == 4)
+                    *(float4*)dst = make_float4(c[0], c[1], c[2], c[3]);
+                else
+                    *(float2*)dst = make_float2(c[0], c[1]);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const float* crow = cs + (kPdPad - rbase);
+            // B in a vector register once per view (the voxel FMA takes A from a scalar one)
+            T Bv;
+            asm volatile("v_mov_b32 %0, %1" : "=v"(Bv) : "s"(B));
+            auto voxel = [&](int c) {
+                T vv = fma_t(izf[c], A, Bv);
+                if (off_boundary) {
+                    if (!(vv >= lim.lo && vv <= lim.v_hi)) return;
+                } else {
+                    if (vv < lim.lo) vv = T(0.5);
+                    if (vv > lim.v_hi) vv = T(g.nv) - T(1.5);
+                }
+                // v >= 0 here, so truncation is floor; floor(v) lies in [r0, r1 - 1], the rows
+                // staged: the voxel FMA is monotone in iz, so v lies between the first and the
+                // last voxel's, and the row of v is non-decreasing in v
+                const int iv = (int)vv;
+                const T wvl = __builtin_amdgcn_fractf(vv);
+                const T c0 = crow[iv], c1 = crow[iv + 1];
+                img[c] = fmaf(w2s, fmaf(wvl, c1 - c0, c0), img[c]);
+            };
+            if (g.nz >= NZL * kWave) {
+                // every lane's NZL voxels exist: one straight pass (the LDS reads overlap)
+#pragma unroll
+                for (int c = 0; c < NZL; ++c) voxel(c);
+            } else {
+#pragma unroll
+                for (int c = 0; c < NZL; ++c)
+                    if (c * kWave + lane < g.nz) voxel(c);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NZL; ++c) {
+        const int iz = c * kWave + lane;
+        if (iz < g.nz) {
+            const T val = scale * img[c];
+            ocol[iz] = cvt_store<Tout>(accumulate ? ld<T>(ocol + iz) + val : val);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // Layout changes between MATLAB order (x fastest) and the device ZXY order.
 // 32x32 tiles through LDS so that both sides are coalesced.
 // ---------------------------------------------------------------------------------
@@ -1644,11 +1858,11 @@ static void launch_back_dd_tf(unsigned blocks, hipStream_t st, const Geom<T>& g,
     if (foot == FOOT_DD)
         hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_DD, NZL, SHORT, false>), dim3(blocks), dim3(kBlock),
                            back_lds_bytes<T>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
-                           (Tout*)vol, scale, accumulate, rowwin, NestArgs{}, g.sched_back);
+                           (Tout*)vol, scale, accumulate, rowwin, NestArgs{}, g.sched_back & 1);
     else
         hipLaunchKernelGGL((cbct_back_kernel<Tin, Tout, T, FOOT_TF, NZL, SHORT, false>), dim3(blocks), dim3(kBlock),
                            back_lds_bytes<T>(rowwin), st, g, views, vstride, nviews, (const Tin*)proj, map,
-                           (Tout*)vol, scale, accumulate, rowwin, NestArgs{}, g.sched_back);
+                           (Tout*)vol, scale, accumulate, rowwin, NestArgs{}, g.sched_back & 1);
 }
 
 template <class Tin, class Tout, class T, int NZL>
@@ -1656,6 +1870,20 @@ static void launch_back(unsigned blocks, hipStream_t st, const Geom<T>& g, const
                         int nviews, const void* proj, const uint8_t* map, void* vol, int foot, int pd_mode, T scale,
                         int accumulate) {
     const int rowwin = back_rowwin(g.nv);
+    if constexpr (std::is_same<T, float>::value) {
+        // the staged FDK form unless the schedule asks for the direct one (sched_back bit 1)
+        if (foot == 2 && (g.sched_back & 2) == 0 && g.nv <= kPdRows) {
+            if (pd_mode)
+                hipLaunchKernelGGL((cbct_back_pd_stage_kernel<Tin, Tout, NZL, 1>), dim3(blocks), dim3(kBlock), 0, st,
+                                   g, views, vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate,
+                                   pd_limits<T>(g.nu, g.nv));
+            else
+                hipLaunchKernelGGL((cbct_back_pd_stage_kernel<Tin, Tout, NZL, 0>), dim3(blocks), dim3(kBlock), 0, st,
+                                   g, views, vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate,
+                                   pd_limits<T>(g.nu, g.nv));
+            return;
+        }
+    }
     if (foot == 2 && pd_mode)
         hipLaunchKernelGGL((cbct_back_pd_kernel<Tin, Tout, T, NZL, 1>), dim3(blocks), dim3(kBlock), 0, st, g, views,
                            vstride, nviews, (const Tin*)proj, map, (Tout*)vol, scale, accumulate,
@@ -1677,7 +1905,7 @@ static hipError_t back3(const Geom<T>& g, const View<T>* views, int vstride, int
                         const uint8_t* map, void* vol, int foot, int pd_mode, T scale, int accumulate,
                         hipStream_t st) {
     const long long cols = (long long)g.nx * g.ny;
-    const unsigned blocks = (foot == 2 || g.sched_back == 1) ? (unsigned)pd_blocks(g.nx, g.ny)
+    const unsigned blocks = (foot == 2 || (g.sched_back & 1)) ? (unsigned)pd_blocks(g.nx, g.ny)
                                                              : (unsigned)((cols + kWavesPerBlock - 1) / kWavesPerBlock);
     if (cols == 0 || blocks == 0) return hipSuccess;
     if (foot == 2 && (g.nu < 2 || g.nv < 2)) return hipErrorInvalidValue;  // bilinear needs 2x2 cells

@@ -105,7 +105,9 @@ struct Geom {
     // launch schedule (host side only; set by ctsim_plan_set_schedule / the autotuner):
     // sched_proj = form | split << 2 with form 0 auto, 1 LDS ring, 2 register pipeline and
     // split 0 auto, 1 one wave per work item, 2 a block's waves share an item;
-    // sched_back = 0 consecutive columns per block, 1 2 x 2 column patches in 32 x 32 super-tiles
+    // sched_back bit 0: 0 consecutive columns per block, 1 2 x 2 column patches in 32 x 32
+    // super-tiles (DD / SF); bit 1: the FDK (pd) backprojection's direct form instead of the
+    // LDS-staged one
     int sched_proj;
     int sched_back;
 };

@@ -89,7 +89,8 @@ class Projector:
     def autotune(self, d_in: DeviceArray, d_out: DeviceArray, op: str, view0=0, vstep=1, nsub=None, reps=3,
                  stream=None) -> dict:
         """Time every schedule of op (proj,dd / proj,tf: forward forms and splits; back,dd /
-        back,tf: column orders) on these buffers and keep the fastest in the plan.  Returns
+        back,tf: column orders; back,pd / back,hpd: the staged and direct FDK forms) on these
+        buffers and keep the fastest in the plan.  Returns
         {"best": code, "candidates": [(code, ms), ...]}."""
         nsub = self.geom.noViews if nsub is None else nsub
         if op.startswith("proj"):

@@ -188,17 +188,20 @@ int ctsim_plan_destroy(ctsim_plan* plan);
  * form 0 automatic, 1 LDS-ring forward kernel (FP32 volumes of at most 256 slices, FP16
  * volumes of an even number of at most 256), 2 register-pipeline forward kernel; split 0
  * automatic (a block's waves share a work item below 16384 items), 1 one wave per item,
- * 2 shared items.  back_sched: 0 four consecutive image columns per block, 1 2 x 2 column
- * patches in 32 x 32 super-tiles (both with a contiguous range per XCD).  A new plan has
- * (0, 1).  Every schedule
- * is deterministic; the forward forms and splits add a ray's terms in different orders, so
- * their results agree to FP32 rounding, not bit for bit. */
+ * 2 shared items.  back_sched bit 0 (DD / SF backprojection): 0 four consecutive image
+ * columns per block, 1 2 x 2 column patches in 32 x 32 super-tiles (both with a contiguous
+ * range per XCD); bit 1 (FDK pd / hpd, FP32 plans): 0 the LDS-staged form (detectors of at
+ * most 1024 rows; taller ones take the direct form), 1 the direct form.  A new plan has
+ * (0, 1).  Every schedule is deterministic; the forward forms and splits, and the two FDK
+ * forms, add terms in different orders, so their results agree to FP32 rounding, not bit
+ * for bit. */
 int ctsim_plan_set_schedule(ctsim_plan* plan, int32_t proj_sched, int32_t back_sched);
 int ctsim_plan_get_schedule(const ctsim_plan* plan, int32_t* proj_sched, int32_t* back_sched);
 
 /* The project's autotuner for 3D plans: times every forward schedule (op proj,dd or
- * proj,tf: automatic, ring whole, ring shared, register whole, register shared) or every
- * backprojector column order (back,dd or back,tf) on the caller's view selection and
+ * proj,tf: automatic, ring whole, ring shared, register whole, register shared), every
+ * backprojector column order (back,dd or back,tf) or both FDK forms (back,pd or back,hpd)
+ * on the caller's view selection and
  * buffers with HIP events on the stream (one warm-up, then reps timed launches each),
  * keeps the fastest in the plan and returns it in *best.  d_out is overwritten; it must not
  * alias d_in.  ms_out / sched_out (may be NULL) receive each candidate's mean time and
