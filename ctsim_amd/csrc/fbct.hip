@@ -13,6 +13,9 @@
 namespace ctsim {
 
 constexpr int kW2 = 64;
+// image lines whose strip holds at least this many pixels on average take a batch each in
+// the forward projection (fewer: several lines share a batch)
+constexpr int kLineBatch = 12;
 
 enum Foot2 { F2_DD = 0, F2_TF = 1 };
 
@@ -54,20 +57,11 @@ __device__ __forceinline__ T fbct_proj_acc(const Geom<T>& g, const View<T>& v, i
         const int incl = wave_incl_scan(cnt);
         const int excl = incl - cnt;
         const int total = rdl_i(incl, kW2 - 1);
-        for (int k0 = 0; k0 < total; k0 += kW2) {
-            const int k = k0 + lane;
-            int j = 0;
-#pragma unroll
-            for (int step = 32; step >= 1; step >>= 1) {
-                const int e = __shfl(excl, j + step, 64);
-                if (e <= k) j += step;
-            }
-            const int lo_j = __shfl(lo, j, 64);
-            const int ex_j = __shfl(excl, j, 64);
-            const int major = m0 + j, minor = lo_j + (k - ex_j);
+        // one batch of 64 candidates: entries in LDS, the covering range of every tile cell,
+        // then each lane's walk over that range
+        auto batch = [&](int major, int minor, bool ok) {
             const int ix = row_major ? minor : major;
             const int iy = row_major ? major : minor;
-            bool ok = k < total;
             int ilo = 0, ihi = -1;
             T p0 = T(0), p1 = T(0), p2 = T(0), p3 = T(0), val = T(0);
             if (ok) {
@@ -124,6 +118,35 @@ __device__ __forceinline__ T fbct_proj_acc(const Geom<T>& g, const View<T>& v, i
                 }
             }
             __builtin_amdgcn_wave_barrier();
+        };
+        const unsigned long long nonempty = __ballot(cnt > 0);
+        const int nlines = __popcll(nonempty);
+        if (total >= kLineBatch * nlines) {
+            // wide strips: one image line per batch.  The pixels of a line project in order,
+            // so the ones covering a cell are consecutive entries and each lane's walk is as
+            // long as its cell's coverage, where a batch spanning several lines walks the
+            // entries of every line in between
+            unsigned long long left = nonempty;
+            while (left) {
+                const int j = __ffsll((long long)left) - 1;
+                left &= left - 1;
+                const int lo_j = rdl_i(lo, j), cnt_j = rdl_i(cnt, j);
+                for (int k0 = 0; k0 < cnt_j; k0 += kW2) batch(m0 + j, lo_j + k0 + lane, k0 + lane < cnt_j);
+            }
+        } else {
+            // narrow strips: consecutive candidates of several lines share a batch
+            for (int k0 = 0; k0 < total; k0 += kW2) {
+                const int k = k0 + lane;
+                int j = 0;
+#pragma unroll
+                for (int step = 32; step >= 1; step >>= 1) {
+                    const int e = __shfl(excl, j + step, 64);
+                    if (e <= k) j += step;
+                }
+                const int lo_j = __shfl(lo, j, 64);
+                const int ex_j = __shfl(excl, j, 64);
+                batch(m0 + j, lo_j + (k - ex_j), k < total);
+            }
         }
     }
     return acc;
