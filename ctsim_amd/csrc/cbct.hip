@@ -894,12 +894,18 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
 
 // ---------------------------------------------------------------------------------
 // Matched backprojection: one wavefront per image column, NZL voxels per lane (z
-// fastest).  The views' footprints of the column are evaluated 64 at a time (one view
+// fastest).  The views' footprints of the column are evaluated 64 or 32 at a time (one view
 // per lane) into LDS.  Per view the wave then combines the footprint's detector columns
 // row by row, s(iv) = sum_u w_u proj(u, iv), into LDS for the rows the column covers
 // (the next view's rows are already loading), and every voxel gathers its v footprint
 // from LDS.  All views accumulate in registers; one store per voxel.
 // ---------------------------------------------------------------------------------
+// views whose footprint records a wave keeps in LDS at a time in the windowed form (tall
+// detectors): 32 of them (lanes 32 .. 63 idle while the records are formed) halve the
+// records' LDS, which otherwise limits its resident waves (measured: 720-view 512^2 x 256
+// SF back 194 -> 173 ms).  Short detectors keep 64 (there the records' second half of the
+// lanes pays: 1.13 vs 1.17 ms per PWLS subset).
+constexpr int kBackViews = 32;
 constexpr int kRowWin = 1019;  // detector rows staged per wave (nv larger: windowed); with the
                                 // zero pads one wave's row buffer is 1024 words
 
@@ -1001,7 +1007,8 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
     // for the detector, so short detectors leave room for more resident waves
     extern __shared__ __align__(16) unsigned char back_dyn_lds[];
     T* srow = reinterpret_cast<T*>(back_dyn_lds);
-    __shared__ ViewCol<T> vbuf[kWavesPerBlock][kWave];
+    constexpr int VB = SHORT ? kWave : kBackViews;  // records per batch
+    __shared__ ViewCol<T> vbuf[kWavesPerBlock][VB];
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int wv = (int)(threadIdx.x / kWave);
     // column order: 0 = four consecutive columns per block (a contiguous run of columns per
@@ -1039,9 +1046,9 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
     T img[NZL];
 #pragma unroll
     for (int c = 0; c < NZL; ++c) img[c] = T(0);
-    for (int b0 = 0; b0 < (masked ? 0 : nviews); b0 += kWave) {
-        // footprint of view b0 + lane on this column
-        bool ok = b0 + lane < nviews;
+    for (int b0 = 0; b0 < (masked ? 0 : nviews); b0 += VB) {
+        // footprint of view b0 + lane on this column (lanes below VB)
+        bool ok = lane < VB && b0 + lane < nviews;
         ViewCol<T> vc{};
         vc.zinc = T(1);
         int iz0 = 0, iz1 = 0, r0 = 0, r1 = 0;
@@ -1092,7 +1099,7 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                 vc.voff = ((unsigned)(lane * g.nu + ilo) * (unsigned)g.nv + (unsigned)r0) * (unsigned)sizeof(Tin);
             }
         }
-        vw[lane] = vc;
+        if (lane < VB) vw[lane] = vc;
         __builtin_amdgcn_wave_barrier();
         unsigned long long mask = __ballot(ok);
         // the first row chunk of the next view is loaded while the current one is used

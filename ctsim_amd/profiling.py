@@ -8,6 +8,8 @@ passes), so that a profile names only the kernels it is about.
                                                        # into the backprojector (as in PwlsOS mixed)
     python -m ctsim_amd.profiling pair-slab [reps]     # proj,tf + back,tf of 24 of 720 views,
                                                        # 512x512x256 volume, 1024x384 detector, FP32
+    python -m ctsim_amd.profiling pair-run [reps]      # the same pair over 48 consecutive views (the
+                                                       # view-to-view locality of the full 720-view pair)
     python -m ctsim_amd.profiling fdk-back [reps]      # back,hpd of 720 filtered views (FP16) into a
                                                        # 512^3 FP32 volume, 1024x768 detector
     python -m ctsim_amd.profiling pwls-update [reps]   # fused Nesterov-SQS update (Huber, FP16 z copy)
@@ -63,6 +65,12 @@ def pair_slab(reps: int) -> None:
     _pair(g, vol, 24, 30, "f32", reps)
 
 
+def pair_run(reps: int) -> None:
+    g = workloads.sf_pair_512()
+    vol = workloads.ellipsoid_phantom(g.reconSize, g.reconSpacing, 20, 2)
+    _pair(g, vol, 48, 1, "f32", reps)
+
+
 def fdk_back(reps: int) -> None:
     g = workloads.fdk_512()
     P = Projector(g)
@@ -89,7 +97,7 @@ def pwls_update(reps: int) -> None:
     synchronize()
 
 
-WORKLOADS = {"copy": copy, "pwls-subset": pwls_subset, "pair-slab": pair_slab, "fdk-back": fdk_back,
+WORKLOADS = {"copy": copy, "pwls-subset": pwls_subset, "pair-slab": pair_slab, "pair-run": pair_run, "fdk-back": fdk_back,
              "pwls-update": pwls_update}
 
 
