@@ -1265,16 +1265,35 @@ __global__ __launch_bounds__(kBlock) void cbct_back_kernel(Geom<T> g, const View
                     rfirst += kWave;
                 }
                 if (lane <= kRowPad) sw[lane == 0 ? 0 : rwe - rw + 1 + lane] = T(0);  // zero pads
-                for (int row = rfirst; row <= rwe; row += kWave) {
-                    const Tin* pr = pv + row;
-                    T s = w0 * ld<T>(pr);
-                    if (nw > 1) s += w1 * ld<T>(pr + g.nv);
-                    if (nw > 2) s += w2 * ld<T>(pr + 2 * g.nv);
-                    if (nw > 3) s += w3 * ld<T>(pr + 3 * g.nv);
-                    for (int k = 4; k < nw; ++k)
-                        s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) * c.ampr *
-                             ld<T>(pr + (size_t)k * g.nv);
-                    sw[row - rw + 1] = s;
+                // the other row chunks, each chunk's loads issued while the one before is
+                // combined: the four footprint columns through one resource from row r0 of
+                // column ilo to the end of the footprint's last column (columns past it read
+                // zero and weigh zero, as in the first chunk; lanes past rwe read row r1)
+                const auto rs = buf_rsrc(bbase + uni((int)c.voff), (nw * g.nv - r0) * (int)sizeof(Tin));
+                const int step = g.nv * (int)sizeof(Tin);
+                Tin q0{}, q1{}, q2{}, q3{};
+                auto ld4 = [&](int row, Tin& a0, Tin& a1, Tin& a2, Tin& a3) {
+                    const int off = (min(row, r1) - r0) * (int)sizeof(Tin);
+                    a0 = buf_ld<Tin>(rs, off);
+                    a1 = buf_ld<Tin>(rs, off + step);
+                    a2 = buf_ld<Tin>(rs, off + 2 * step);
+                    a3 = buf_ld<Tin>(rs, off + 3 * step);
+                };
+                if (rfirst - lane <= rwe) ld4(rfirst, q0, q1, q2, q3);
+                for (int row = rfirst; row - lane <= rwe; row += kWave) {
+                    Tin n0{}, n1{}, n2{}, n3{};
+                    if (row - lane + kWave <= rwe) ld4(row + kWave, n0, n1, n2, n3);
+                    if (row <= rwe) {
+                        T s = w0 * T(to_f(q0)) + w1 * T(to_f(q1)) + w2 * T(to_f(q2)) + w3 * T(to_f(q3));
+                        for (int k = 4; k < nw; ++k)
+                            s += u_weight<T, FOOT>(ilo + k, ilo, ilo + nw - 1, c.p0, c.p1, c.p2, c.p3) * c.ampr *
+                                 ld<T>(pv + (size_t)k * g.nv + row);
+                        sw[row - rw + 1] = s;
+                    }
+                    q0 = n0;
+                    q1 = n1;
+                    q2 = n2;
+                    q3 = n3;
                 }
                 __builtin_amdgcn_wave_barrier();
                 if (rw == r0) next();
