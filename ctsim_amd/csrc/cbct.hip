@@ -437,10 +437,13 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 6 : 1) void cbct_proj_kern
 // + 4 t, and lanes past the buffer range write zero) and kZPad zero words.  Only the loads
 // write words 1 .. 64 NS, so word 0 and the pads stay zero for the whole kernel.  Columns
 // of at most 128 voxels (NS = 2) take four entries (three columns in flight), columns of
-// at most 192 (NS = 3) three: about the same LDS.
+// at most 192 (NS = 3) three: about the same LDS.  Columns of at most 256 (NS = 4) take
+// two (one column in flight): with three, the ring's LDS and 91 registers held the kernel
+// to five waves per SIMD, and the sixth wave measured 7-8% faster than the deeper ring
+// (720-view 512^2 x 256 SF forward 154 -> 143 ms, DD 144 -> 132 ms).
 constexpr int kRingWaves = 6;  // waves per SIMD the ring kernel is compiled for (the LDS of six
                                // blocks per CU; seven would spill registers)
-__host__ __device__ constexpr int ring_len(int ns) { return ns == 2 ? 4 : 3; }
+__host__ __device__ constexpr int ring_len(int ns) { return ns == 2 ? 4 : ns == 4 ? 2 : 3; }
 __host__ __device__ constexpr int ring_entry_words(int ns) { return 1 + ns * kWave + kZPad; }
 
 // A buffer resource as four scalar words, for the inline assembly below.  The loads are
