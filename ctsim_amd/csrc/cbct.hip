@@ -914,7 +914,9 @@ constexpr int kRowWin = 1019;  // detector rows staged per wave (nv larger: wind
 
 
 template <class T>
-struct __align__(16) ViewCol {  // one view's footprint on one image column (80 bytes in FP32)
+// 8-byte aligned: 72 bytes in FP32 rather than 80, so that a short-detector block's 64-view
+// records and row buffers fit eight blocks per CU (eight waves per SIMD)
+struct __align__(8) ViewCol {  // one view's footprint on one image column
     int cells;                  // first footprint cell ilo | cell count nw << 16
     int rows;                   // rows covered r0 | r1 << 16
     int cls;                    // gather class: rows per voxel 2, 3, 4 (0, 1, 2) or the capped loop (3)
