@@ -49,6 +49,28 @@ def test_fdk_filter_matches_fp64_restatement(flat, short):
     assert rel_l2(got64, want) < 1e-11
 
 
+@pytest.mark.parametrize("nu", [100, 200, 500, 1000, 60])
+def test_fdk_filter_lengths_and_ragged_rows(nu):
+    """the fused filter at each filter length it takes (256 .. 2048 taps for 100 .. 1000 cells)
+    and the hipFFT chain below it (60 cells: 128 taps), with a row count that is not a
+    multiple of the fused form's eight-row tiles, against the FP64 restatement"""
+    from oracle import fdk_ref
+    from ctsim_amd import fdk
+    from ctsim_amd.device import DeviceArray
+    g = _geom(flat=nu % 2, short=True, nviews=5, nuv=(nu, 13))
+    rng = np.random.default_rng(nu)
+    sino = rng.uniform(0, 3, (13, nu, 5)).astype(np.float32)
+    H = fdk.fdk_filter_response(g, "hann")
+    want = fdk_ref.fdk_filter(sino, g, H, fdk.fdk_view_weights(g))
+    f32 = fdk.FdkFilter(g, "hann")
+    for dt in ("f32", "f16", "bf16"):
+        d_in = DeviceArray.from_numpy(sino.ravel(order="F"), dt)
+        x = sino if dt == "f32" else d_in.to_numpy().reshape(sino.shape, order="F")
+        ref = want if dt == "f32" else fdk_ref.fdk_filter(x.astype(np.float32), g, H, fdk.fdk_view_weights(g))
+        got = f32(d_in, DeviceArray(d_in.count, "f32")).to_numpy().reshape(sino.shape, order="F")
+        assert rel_l2(got, ref) < 1e-5, (dt, rel_l2(got, ref))
+
+
 def test_fdk_filter_view_ranges_agree_with_whole():
     """filtering views in two ranges gives the bits of filtering them all at once"""
     from ctsim_amd import fdk
