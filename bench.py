@@ -216,7 +216,7 @@ def pair_gups(kinds=("dd", "tf"), reps=2):
                          proj_tflops=round(pt, 2), back_tflops=round(bt, 2),
                          proj_frac=round(pt / PEAK_FP32_TFLOPS, 4), back_frac=round(bt / PEAK_FP32_TFLOPS, 4),
                          schedule={"proj": tp_sched, "back": tb_sched,
-                                   "codes": "proj form|split<<2 (form 0 auto, 1 ring, 2 register; split 0 auto, "
+                                   "codes": "proj form|split<<2 (form 0 auto, 1 ring, 2 register, 3 ring with two row chunks; split 0 auto, "
                                             "1 whole, 2 shared); back 0 linear, 1 super-tile"})
         log(f"pair {kind}: proj {tp:.1f} ms  back {tb:.1f} ms  {out[kind]['gups_pair']} GUPS")
     return out
@@ -575,7 +575,7 @@ def main():
         "kernel_ms_per_subset": {k: round(v, 4) for k, v in kt.items()},
         "autotune": {"schedule": {"proj": sched[0], "back": sched[1]}, "proj": tuning["proj"],
                      "back": tuning["back"],
-                     "codes": "proj form|split<<2 (form 0 auto, 1 ring, 2 register; split 0 auto, 1 whole, "
+                     "codes": "proj form|split<<2 (form 0 auto, 1 ring, 2 register, 3 ring with two row chunks; split 0 auto, 1 whole, "
                               "2 shared); back 0 linear, 1 super-tile; times in ms per 41-view launch"},
         "roofline": roof,
         "accuracy": accuracy,

@@ -417,8 +417,7 @@ int ctsim_plan_back(const ctsim_plan* p, const char* op, int32_t view0, int32_t 
 
 int ctsim_plan_set_schedule(ctsim_plan* p, int32_t proj_sched, int32_t back_sched) {
     if (!p) return fail(CTSIM_EINVAL, "Error: null plan.");
-    if ((proj_sched & 3) == 3 || ((proj_sched >> 2) & 3) == 3 || (proj_sched >> 4) != 0 || back_sched < 0 ||
-        back_sched > 3)
+    if (((proj_sched >> 2) & 3) == 3 || (proj_sched >> 4) != 0 || back_sched < 0 || back_sched > 3)
         return fail(CTSIM_EINVAL, "Error: unknown schedule (proj %d, back %d).", proj_sched, back_sched);
     p->gf.sched_proj = p->gd.sched_proj = proj_sched;
     p->gf.sched_back = p->gd.sched_back = back_sched;
@@ -445,13 +444,14 @@ int ctsim_plan_autotune(ctsim_plan* p, const char* op, int32_t view0, int32_t vs
     if (d_in == d_out) return fail(CTSIM_EINVAL, "Error: the tuning output must not alias the input.");
     if (reps < 1) reps = 1;
     // the forward schedules (form | split << 2): automatic, LDS ring with whole and with shared
-    // items, register pipeline with whole and with shared items; the backward column orders
+    // items, register pipeline with whole and with shared items, the ring with two row chunks
+    // per item (whole items); the backward column orders
     // (DD / SF) or the FDK backprojection's staged and direct forms (pd / hpd), each keeping
     // the other bit of the current schedule
-    static const int kProj[5] = {0, 1 | (1 << 2), 1 | (2 << 2), 2 | (1 << 2), 2 | (2 << 2)};
+    static const int kProj[6] = {0, 1 | (1 << 2), 1 | (2 << 2), 2 | (1 << 2), 2 | (2 << 2), 3 | (1 << 2)};
     const int keep_p = p->gf.sched_proj, keep_b = p->gf.sched_back;
     const int kBack[2] = {pd ? (keep_b & 1) : (keep_b & 2), pd ? (keep_b & 1) | 2 : (keep_b & 2) | 1};
-    const int n = proj ? 5 : 2;
+    const int n = proj ? 6 : 2;
     hipStream_t st = (hipStream_t)stream;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0), "event");

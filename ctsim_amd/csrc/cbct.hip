@@ -584,7 +584,12 @@ __device__ __forceinline__ float ring_gather(const float* zs, int span, int ja, 
 // dword), and the scaling pass converts each lane's two voxels and writes them, scaled, as
 // FP32 to a staging buffer that the row gather reads as the FP32 form reads its slot.  The
 // one or two voxels the widening adds lie outside every row of the wave and weigh zero.
-template <class Tout, int FOOT, int TU, int SPLIT, int NS, int WPB, bool HALF = false, int RING = ring_len(NS)>
+//
+// RPL: detector rows per lane (RPL 64-row chunks per item): a column's voxel range, loads,
+// scaling and weights serve all of the item's rows, so tall detectors pay the per-column
+// set-up once per RPL chunks (with narrower tiles, so that the accumulators' LDS stays).
+template <class Tout, int FOOT, int TU, int SPLIT, int NS, int WPB, bool HALF = false, int RING = ring_len(NS),
+          int RPL = 1>
 __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel(Geom<float> g, const View<float>* __restrict__ views,
                                                                    int vstride, int nviews,
                                                                    const typename std::conditional<HALF, __half, float>::type* __restrict__ vol,
@@ -606,10 +611,11 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
     // length, so that voxel pairs sit on 8-byte boundaries
     constexpr int kStageW = HALF ? 2 + 2 * NS * kWave + kZPad : 0;
     static_assert(!HALF || (kStageW % 2 == 0 && (kRing * kSlotW) % 2 == 0), "staging pairs on 8-byte boundaries");
-    __shared__ __align__(16) T lds_block[WPB * kRing * kSlotW + WPB * kStageW + WPB * TU * kWave + 3 * kWave];
+    constexpr int kCellW = kWave * RPL;  // accumulator words per tile cell: [cell][row chunk][lane]
+    __shared__ __align__(16) T lds_block[WPB * kRing * kSlotW + WPB * kStageW + WPB * TU * kCellW + 3 * kCellW];
     T* const accs = lds_block + WPB * (kRing * kSlotW + kStageW);
     const int nut = (g.nu + TU - 1) / TU;
-    const int nvc = (g.nv + kWave - 1) / kWave;
+    const int nvc = (g.nv + kCellW - 1) / kCellW;
     const int items = nviews * nut * nvc;
     const int bid = blockIdx.x;
     const int wv = (int)(threadIdx.x / kWave);
@@ -622,7 +628,7 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
     const int ut = tile_by_rank(item / (nvc * nviews), min(max((int)floor(-g.wu / T(TU)), 0), nut - 1), nut);
     const int lane = lane_id();
     const int loff = lane * 4;
-    const int iv_a = vc * kWave;
+    const int iv_a = vc * kCellW;
     const int iu_a = ut * TU;
     const int iu_b = min(iu_a + TU, g.nu) - 1;
     const View<T> v = views[(size_t)b * vstride];
@@ -630,8 +636,8 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
     const int m_end = row_major ? g.ny : g.nx;
     const T kL = u_to_slope(g, T(iu_a) - T(0.5));
     const T kR = u_to_slope(g, T(iu_b) + T(0.5));
-    const int iv = iv_a + lane;
-    const int iv_last = min(iv_a + kWave, g.nv) - 1;
+    const int iv = iv_a + lane;  // this lane's row of the first chunk (chunk r: iv + 64 r)
+    const int iv_last = min(iv_a + kCellW, g.nv) - 1;
     T* zr = lds_block + wv * kRing * kSlotW;
     // FP32: word 0 and the pads of every slot stay zero; the loads write words 1 .. 192.
     // HALF: the staging buffer's zero word and pads stay zero; the scaling pass writes the rest
@@ -652,9 +658,9 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
     const T lzh1 = T(2 * kWave + 2 * lane) * g.dz;
     const rsrc_words empty_rs = dma_rsrc(vol, 0);  // a group of loads that only writes zeros
 
-    T* acc = &accs[(wv * TU) * kWave + lane];
+    T* acc = &accs[(wv * TU) * kCellW + lane];
 #pragma unroll
-    for (int t = 0; t < TU; ++t) acc[t * kWave] = T(0);
+    for (int t = 0; t < TU * RPL; ++t) acc[t * kWave] = T(0);
 
     // ring position of the next group to issue (byte address of its slot's word 1; HALF: of
     // the slot's first dword)
@@ -828,51 +834,56 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
                 // row iv against the column's voxels (relative to voxel za): voxel j covers
                 // rows [vst + j zinc, vst + (j + 1) zinc]; at most ceil(1/zinc) + 1 of them
                 // overlap a row, so thick voxels take a fixed unrolled count
-                const T tt = fiv_lo - vst;
-                const int ja = (int)ceil(tt * rz - T(1));
                 const int span = (rdl_i(cp.nbytes, c) >> (HALF ? 1 : 2)) - 1;
-                T q;
-                if (cls == 1) {
-                    q = ring_gather<3>(zs, span, ja, tt, zinc);
-                } else if (cls == 2) {
-                    q = ring_gather<4>(zs, span, ja, tt, zinc);
-                } else if (cls == 0) {
-                    q = ring_gather<2>(zs, span, ja, tt, zinc);
-                } else {
-                    // footprints that may reach the 16-cell cap, and thin voxels
-                    q = T(0);
-                    if (iv <= iv_last) {
-                        const int jb = min((int)ceil((T(iv) + T(0.5) - vst) * rz) - 1, span);
-                        for (int jj = max(ja, 0); jj <= jb; ++jj) {
-                            const T vmin = vst + T(jj) * zinc;
-                            q += (zinc > T(14) ? dd_overlap16(iv, vmin, vmin + zinc, g.nv)
-                                               : dd_overlap(iv, vmin, vmin + zinc)) * zs[jj + 1];
+                const T w0 = rdl(cp.w0, c), w1 = rdl(cp.w1, c), w2 = rdl(cp.w2, c), w3 = rdl(cp.w3, c);
+#pragma unroll
+                for (int rr = 0; rr < RPL; ++rr) {
+                    const int ivr = iv + rr * kWave;
+                    const T tt = (fiv_lo + T(rr * kWave)) - vst;
+                    const int ja = (int)ceil(tt * rz - T(1));
+                    T q;
+                    if (cls == 1) {
+                        q = ring_gather<3>(zs, span, ja, tt, zinc);
+                    } else if (cls == 2) {
+                        q = ring_gather<4>(zs, span, ja, tt, zinc);
+                    } else if (cls == 0) {
+                        q = ring_gather<2>(zs, span, ja, tt, zinc);
+                    } else {
+                        // footprints that may reach the 16-cell cap, and thin voxels
+                        q = T(0);
+                        if (ivr <= iv_last) {
+                            const int jb = min((int)ceil((T(ivr) + T(0.5) - vst) * rz) - 1, span);
+                            for (int jj = max(ja, 0); jj <= jb; ++jj) {
+                                const T vmin = vst + T(jj) * zinc;
+                                q += (zinc > T(14) ? dd_overlap16(ivr, vmin, vmin + zinc, g.nv)
+                                                   : dd_overlap(ivr, vmin, vmin + zinc)) * zs[jj + 1];
+                            }
                         }
+                    }
+                    // tile cells c0 .. c0 + nc; the same on every lane, so each weighted add
+                    // touches one LDS word per lane and bank.  The four words are read and
+                    // updated in one pass (one LDS round trip); only the cells of the footprint
+                    // inside the tile are written back.
+                    T* pa = acc + c0 * kCellW + rr * kWave;
+                    const T a0 = pa[0] + w0 * q, a1 = pa[kCellW] + w1 * q, a2 = pa[2 * kCellW] + w2 * q,
+                            a3 = pa[3 * kCellW] + w3 * q;
+                    pa[0] = a0;
+                    if (nc >= 1) pa[kCellW] = a1;
+                    if (nc >= 2) pa[2 * kCellW] = a2;
+                    if (nc >= 3) pa[3 * kCellW] = a3;
+                    if (nc >= 4) {
+                        // wider than four tile cells (rare): the corners again, from the column
+                        const int col = rdl_i(cp.col, c), cells = rdl_i(cp.cells, c);
+                        const int ilo = cells & 0xffff, ihi = cells >> 16;
+                        T p0, p1, p2, p3, amp, r, d;
+                        int jlo, jhi;
+                        col_corners<T, FOOT>(g, v, col % g.nx, col / g.nx, p0, p1, p2, p3, jlo, jhi, amp, r, d);
+                        const T ampr = amp * frcp(r);
+                        for (int cc = c0 + 4; cc <= c0 + nc; ++cc)
+                            acc[cc * kCellW + rr * kWave] += u_weight<T, FOOT>(iu_a + cc, ilo, ihi, p0, p1, p2, p3) * ampr * q;
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
-                // tile cells c0 .. c0 + nc; the same on every lane, so each weighted add
-                // touches one LDS word per lane and bank.  The four words are read and
-                // updated in one pass (one LDS round trip); only the cells of the footprint
-                // inside the tile are written back.
-                T* pa = acc + c0 * kWave;
-                const T a0 = pa[0] + rdl(cp.w0, c) * q, a1 = pa[kWave] + rdl(cp.w1, c) * q,
-                        a2 = pa[2 * kWave] + rdl(cp.w2, c) * q, a3 = pa[3 * kWave] + rdl(cp.w3, c) * q;
-                pa[0] = a0;
-                if (nc >= 1) pa[kWave] = a1;
-                if (nc >= 2) pa[2 * kWave] = a2;
-                if (nc >= 3) pa[3 * kWave] = a3;
-                if (nc >= 4) {
-                    // wider than four tile cells (rare): the corners again, from the column
-                    const int col = rdl_i(cp.col, c), cells = rdl_i(cp.cells, c);
-                    const int ilo = cells & 0xffff, ihi = cells >> 16;
-                    T p0, p1, p2, p3, amp, r, d;
-                    int jlo, jhi;
-                    col_corners<T, FOOT>(g, v, col % g.nx, col / g.nx, p0, p1, p2, p3, jlo, jhi, amp, r, d);
-                    const T ampr = amp * frcp(r);
-                    for (int cc = c0 + 4; cc <= c0 + nc; ++cc)
-                        acc[cc * kWave] += u_weight<T, FOOT>(iu_a + cc, ilo, ihi, p0, p1, p2, p3) * ampr * q;
-                }
             }
         }
     }
@@ -883,14 +894,17 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
         if (wv > 0) return;
     }
     Tout* out = proj + (size_t)b * g.nu * g.nv;
-    if (iv <= iv_last) {
+#pragma unroll
+    for (int rr = 0; rr < RPL; ++rr) {
+        const int ivr = iv + rr * kWave;
+        if (ivr > iv_last) break;
 #pragma unroll
         for (int t = 0; t < TU; ++t) {
             const int iu = iu_a + t;
-            T sum = acc[t * kWave];
+            T sum = acc[t * kCellW + rr * kWave];
 #pragma unroll
-            for (int w = 1; w < SPLIT; ++w) sum += accs[(w * TU + t) * kWave + lane];
-            if (iu < g.nu) out[(size_t)iu * g.nv + iv] = cvt_store<Tout>(sum);
+            for (int w = 1; w < SPLIT; ++w) sum += accs[(w * TU + t) * kCellW + rr * kWave + lane];
+            if (iu < g.nu) out[(size_t)iu * g.nv + ivr] = cvt_store<Tout>(sum);
         }
     }
 }
@@ -1779,6 +1793,7 @@ __global__ __launch_bounds__(256) void zxy_to_xyz_f64(const double* __restrict__
 namespace ctsim {
 
 constexpr int kTU = 16;  // detector columns per forward-projection wave
+constexpr int kTU2 = 8;  // the same for the two-row-chunk ring form
 // ring depth of the FP16 ring form: one 64-dword slot per column (at most 128 slices), two
 // (at most 256); six entries keep five columns in flight in the LDS the FP32 form's four take
 constexpr int kHalfRing1 = 6, kHalfRing2 = 4;
@@ -1786,7 +1801,7 @@ constexpr int kHalfRing1 = 6, kHalfRing2 = 4;
 template <class Tin, class Tout, class T, int FOOT, int RPL, int SPLIT>
 static void launch_proj(unsigned blocks, hipStream_t st, const Geom<T>& g, const View<T>* views, int vstride,
                         int nviews, const void* vol, const uint8_t* map, void* proj, int exhaustive) {
-    const int form = g.sched_proj & 3;  // 0 auto, 1 ring, 2 register pipeline
+    const int form = g.sched_proj & 3;  // 0 auto, 1 ring, 2 register pipeline, 3 ring with two rows per lane
     // FP16 volumes of an even number of at most 256 slices: the ring form with half slots
     // (auto: the register pipeline, which measured faster at 128 slices)
     if constexpr (std::is_same<Tin, __half>::value && std::is_same<T, float>::value) if (form == 1) {
@@ -1803,6 +1818,27 @@ static void launch_proj(unsigned blocks, hipStream_t st, const Geom<T>& g, const
                                (Tout*)proj, exhaustive);
             return;
         }
+    }
+    // form 3 (FP32 volumes of at most 256 slices, one wave per item): the ring form with two
+    // 64-row chunks per item and eight-cell tiles (proj3_foot sizes the grid for it)
+    if constexpr (std::is_same<Tin, float>::value && std::is_same<T, float>::value) if (form == 3 && SPLIT == 1) {
+        constexpr int kRW = kWavesPerBlock;
+        if (g.nz <= 2 * kWave) {
+            hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU2, 1, 2, kRW, false, ring_len(2), 2>), dim3(blocks),
+                               dim3(kRW * kWave), 0, st, g, views, vstride, nviews, (const float*)vol, map, (Tout*)proj,
+                               exhaustive);
+            return;
+        }
+        if (g.nz <= kFastSpan) {
+            hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU2, 1, 3, kRW, false, ring_len(3), 2>), dim3(blocks),
+                               dim3(kRW * kWave), 0, st, g, views, vstride, nviews, (const float*)vol, map, (Tout*)proj,
+                               exhaustive);
+            return;
+        }
+        hipLaunchKernelGGL((cbct_proj_ring_kernel<Tout, FOOT, kTU2, 1, 4, kRW, false, ring_len(4), 2>), dim3(blocks),
+                           dim3(kRW * kWave), 0, st, g, views, vstride, nviews, (const float*)vol, map, (Tout*)proj,
+                           exhaustive);
+        return;
     }
     // FP32 volumes whose columns fit the three 64-voxel slots take the ring form (auto);
     // four slots (at most 256 slices) for whole work items (auto: 5% faster than the
@@ -1833,16 +1869,33 @@ static void launch_proj(unsigned blocks, hipStream_t st, const Geom<T>& g, const
 template <class Tin, class Tout, class T, int FOOT>
 static void proj3_foot(const Geom<T>& g, const View<T>* views, int vstride, int nviews, const void* vol,
                        const uint8_t* map, void* proj, int exhaustive, hipStream_t st) {
-    const long long nut = (g.nu + kTU - 1) / kTU, nvc = (g.nv + kWave - 1) / kWave;
+    // form 3: eight-cell tiles of two 64-row chunks, one wave per item (FP32 volumes of at
+    // most 256 slices; other volumes take the automatic choice).  Automatic: taken for
+    // detectors over 64 rows whose items need no split (measured on the 720-view
+    // 512^2 x 256 pair at 1024 x 384: SF forward 146 -> 115 ms, DD 134 -> 102 ms)
+    constexpr bool fp32 = std::is_same<Tin, float>::value && std::is_same<T, float>::value;
+    const int form = g.sched_proj & 3, ssplit0 = (g.sched_proj >> 2) & 3;
+    const long long items1 = (long long)nviews * ((g.nu + kTU - 1) / kTU) * ((g.nv + kWave - 1) / kWave);
+    const bool two = fp32 && g.nz <= 4 * kWave &&
+                     (form == 3 || (form == 0 && ssplit0 != 2 && g.nv > kWave && items1 >= 16384));
+    const int tu = two ? kTU2 : kTU, rows = two ? 2 * kWave : kWave;
+    const long long nut = (g.nu + tu - 1) / tu, nvc = (g.nv + rows - 1) / rows;
     const long long items = (long long)nviews * nut * nvc;
+    if (two) {
+        const unsigned blocks = (unsigned)((items + kWavesPerBlock - 1) / kWavesPerBlock);
+        launch_proj<Tin, Tout, T, FOOT, 1, 1>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
+        return;
+    }
+    Geom<T> ga = g;
+    if ((g.sched_proj & 3) == 3) ga.sched_proj = g.sched_proj & ~3;  // form 3 not applicable: automatic
     // few work items (short view subsets, thin detectors): split each item's strip over
     // the waves of a block so that the chip still holds several waves per SIMD; the plan's
     // schedule can force either form
     const int ssplit = (g.sched_proj >> 2) & 3;
     const bool split = ssplit == 1 ? false : (ssplit == 2 ? true : items < 16384);
     const unsigned blocks = (unsigned)(split ? items : (items + kWavesPerBlock - 1) / kWavesPerBlock);
-    if (split) launch_proj<Tin, Tout, T, FOOT, 1, kWavesPerBlock>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
-    else launch_proj<Tin, Tout, T, FOOT, 1, 1>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
+    if (split) launch_proj<Tin, Tout, T, FOOT, 1, kWavesPerBlock>(blocks, st, ga, views, vstride, nviews, vol, map, proj, exhaustive);
+    else launch_proj<Tin, Tout, T, FOOT, 1, 1>(blocks, st, ga, views, vstride, nviews, vol, map, proj, exhaustive);
 }
 
 template <class Tin, class Tout, class T>

@@ -102,7 +102,7 @@ class Projector:
         codes = (C.c_int32 * 8)()
         call("ctsim_plan_autotune", C.c_void_p(self.handle), op.encode(), view0, vstep, nsub, d_in.vp, d_in.dtype,
              d_out.vp, d_out.dtype, int(reps), C.byref(best), ms, codes, 8, C.c_void_p(stream.handle if stream else 0))
-        n = 5 if op.startswith("proj") else 2
+        n = 6 if op.startswith("proj") else 2
         return {"best": int(best.value), "candidates": [(int(codes[i]), round(float(ms[i]), 4)) for i in range(n)]}
 
     # host helpers (MATLAB-order numpy in, MATLAB-order numpy out) --------------------

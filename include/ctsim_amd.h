@@ -186,7 +186,9 @@ int ctsim_plan_destroy(ctsim_plan* plan);
 /* Launch schedules of a 3D plan (no counterpart in the reference, whose CUDA path uses
  * fixed 16 x 16 blocks, geom/dd_cuda/cbct_dd_proj.c:441-442).  proj_sched = form | split << 2:
  * form 0 automatic, 1 LDS-ring forward kernel (FP32 volumes of at most 256 slices, FP16
- * volumes of an even number of at most 256), 2 register-pipeline forward kernel; split 0
+ * volumes of an even number of at most 256), 2 register-pipeline forward kernel, 3 the ring
+ * kernel with two 64-row chunks per item and eight-cell tiles (FP32 volumes of at most 256
+ * slices, one wave per item; others take the automatic form); split 0
  * automatic (a block's waves share a work item below 16384 items), 1 one wave per item,
  * 2 shared items.  back_sched bit 0 (DD / SF backprojection): 0 four consecutive image
  * columns per block, 1 2 x 2 column patches in 32 x 32 super-tiles (both with a contiguous
@@ -199,7 +201,8 @@ int ctsim_plan_set_schedule(ctsim_plan* plan, int32_t proj_sched, int32_t back_s
 int ctsim_plan_get_schedule(const ctsim_plan* plan, int32_t* proj_sched, int32_t* back_sched);
 
 /* The project's autotuner for 3D plans: times every forward schedule (op proj,dd or
- * proj,tf: automatic, ring whole, ring shared, register whole, register shared), every
+ * proj,tf: automatic, ring whole, ring shared, register whole, register shared, ring with two
+ * row chunks), every
  * backprojector column order (back,dd or back,tf) or both FDK forms (back,pd or back,hpd)
  * on the caller's view selection and
  * buffers with HIP events on the stream (one warm-up, then reps timed launches each),

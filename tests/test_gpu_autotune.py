@@ -1,9 +1,10 @@
 """The 3D launch schedules and the autotuner (ctsim_plan_set_schedule / ctsim_plan_autotune).
 
 Every forward schedule (automatic, LDS ring with whole or shared work items, register
-pipeline with whole or shared items) and both backprojector column orders against the
-oracle, on geometries on either side of the ring forms' slice limits (128, 192 and 256
-slices, FP32 and FP16 volumes), with the same tolerance as the other parity tests; the
+pipeline with whole or shared items, the ring with two row chunks per item) and both
+backprojector column orders against the oracle, on geometries on either side of the ring
+forms' slice limits (128, 192 and 256 slices, FP32 and FP16 volumes) and detectors of one
+to three 128-row items, with the same tolerance as the other parity tests; the
 autotuner on a subset launch returns one of its candidates, keeps it in the plan, and the
 schedule cache stores and reuses it."""
 import os
@@ -16,11 +17,13 @@ from test_gpu_parity import _cone_case, _tol, geometry_from_g3
 
 pytestmark = pytest.mark.gpu
 
-PROJ_SCHEDS = [0, 1 | 1 << 2, 1 | 2 << 2, 2 | 1 << 2, 2 | 2 << 2]
+PROJ_SCHEDS = [0, 1 | 1 << 2, 1 | 2 << 2, 2 | 1 << 2, 2 | 2 << 2, 3 | 1 << 2]
 CASES = [
     dict(seed=21, n=(40, 36, 100), nuv=(80, 40), nviews=12),               # ring, two slots
     dict(seed=22, n=(36, 40, 180), nuv=(70, 70), nviews=9, dz=0.3, dv=0.9),  # ring, three slots
     dict(seed=23, n=(30, 34, 240), nuv=(60, 90), nviews=8, dz=0.3, dv=0.9),  # ring, four slots (on request)
+    dict(seed=24, n=(30, 28, 120), nuv=(70, 300), nviews=6, dv=0.25),        # 300 rows: three two-chunk items
+    dict(seed=25, n=(26, 30, 250), nuv=(50, 200), nviews=5, dz=0.25, dv=0.3, dettype=0, couch=True),
 ]
 
 
