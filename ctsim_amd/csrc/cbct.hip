@@ -1882,8 +1882,11 @@ static void proj3_foot(const Geom<T>& g, const View<T>* views, int vstride, int 
     const long long nut = (g.nu + tu - 1) / tu, nvc = (g.nv + rows - 1) / rows;
     const long long items = (long long)nviews * nut * nvc;
     if (two) {
+        // the launch reads the form from the schedule: make it form 3 (the grid is sized for it)
+        Geom<T> g3 = g;
+        g3.sched_proj = (g.sched_proj & ~3) | 3;
         const unsigned blocks = (unsigned)((items + kWavesPerBlock - 1) / kWavesPerBlock);
-        launch_proj<Tin, Tout, T, FOOT, 1, 1>(blocks, st, g, views, vstride, nviews, vol, map, proj, exhaustive);
+        launch_proj<Tin, Tout, T, FOOT, 1, 1>(blocks, st, g3, views, vstride, nviews, vol, map, proj, exhaustive);
         return;
     }
     Geom<T> ga = g;

@@ -59,6 +59,29 @@ def test_every_schedule_matches_the_oracle(case, vdt):
                 assert e <= _tol(f"back,{kind}", g, sino, mp), (kind, sb, e)
 
 
+def test_automatic_forward_takes_the_two_chunk_form_at_scale():
+    """a detector over 64 rows with enough unsplit items (here 342 views x 16 tiles x 3 row
+    chunks = 16416) makes the automatic schedule take the two-chunk ring form: its result is
+    form 3's, bit for bit, and the oracle's to the parity tolerance"""
+    from oracle import oracle as O
+    from ctsim_amd.device import DeviceArray
+    from ctsim_amd.projector import Projector
+    g, vol, sino, mp = _cone_case(seed=26, n=(16, 18, 40), nuv=(256, 130), nviews=342, dz=0.6, dv=0.35)
+    P = Projector(geometry_from_g3(g, mp))
+    d_vol = P.upload_volume(vol)
+    out = DeviceArray(P.n_sino(), "f32")
+    print("automatic two-chunk form", g.n.tolist(), flush=True)
+    for kind in ("dd", "tf"):
+        P.set_schedule(proj_sched=0)
+        P.proj(d_vol, out, f"proj,{kind}")
+        auto = P.download_sino(out)
+        P.set_schedule(proj_sched=3 | 1 << 2)
+        P.proj(d_vol, out, f"proj,{kind}")
+        np.testing.assert_array_equal(auto, P.download_sino(out))
+        want = O.cbct(f"proj,{kind}", g, vol, mp)
+        assert rel_l2(auto, want) <= _tol(f"proj,{kind}", g, vol, mp), kind
+
+
 FDK_CASES = [
     dict(seed=31),
     dict(seed=32, dettype=0, couch=True, offsets=True),
