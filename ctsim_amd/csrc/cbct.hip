@@ -494,13 +494,14 @@ __device__ __forceinline__ void wait_vm() {
 //    vmcnt: NS (ring_len(NS) - 1).
 // ---------------------------------------------------------------------------------
 struct RingCol {
-    int alo, ahi;   // byte address of the column's first needed voxel (ahi: the 16 address bits of a resource)
-    int nbytes;     // bytes of the voxel range [za, zb]
-    int meta;       // gather class | slots - 1 << 2 | cells past the first << 4 | first tile cell << 9
+    int alo;        // low word of the byte address of the column's first needed voxel
+    // gather class | slots - 1 << 2 | cells past the first << 4 | first tile cell << 9 |
+    // zb - za << 16 | (high word of that address - the volume's) << 24
+    int meta;
     float zbase;    // zc of voxel za: za dz + wz dz
-    float r2, zinc, vst;  // r^2, magnified slice pitch, lower edge of voxel za on the detector (rows)
+    float r2, zinc, rz, vst;  // r^2, magnified slice pitch, its reciprocal, lower edge of voxel za on the detector (rows)
     float w0, w1, w2, w3;  // tile weights of the first four footprint cells, times amp / r
-    int col, cells;  // image column and footprint cells ilo | ihi << 16 (wide footprints only)
+    int col;        // image column (wide footprints only)
 };
 
 __device__ __forceinline__ int permute_i(int dst_lane_bytes, int x) {
@@ -599,6 +600,7 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
     using Tin = typename std::conditional<HALF, __half, float>::type;
     static_assert(SPLIT == 1 || SPLIT == WPB, "SPLIT is 1 or the block's wave count");
     static_assert(HALF ? (NS == 1 || NS == 2) : (NS >= 2 && NS <= 4), "slots per column");
+    static_assert((HALF ? 2 : 1) * NS * kWave <= 256 && TU <= 128, "a column's span and first cell fit its meta word");
     // one block of LDS: the waves' rings, (HALF) their staging buffers, then their tile
     // accumulators, then three pad cells.  Accumulators: each lane's row of the tile,
     // [cell][lane] (one word per bank, one writer per word).  The four-cell update below
@@ -704,13 +706,13 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
                 int ilo = 0, ihi = -1, iz0 = 0, iz1 = 0;
                 ok = col_corners<T, FOOT>(g, v, ix, iy, p0, p1, p2, p3, ilo, ihi, amp, r, d);
                 ok = ok && ihi >= iu_a && ilo <= iu_b;
-                T zinc = T(1), vstart = T(0);
+                T zinc = T(1), vstart = T(0), rz = T(1);
                 if (ok) ok = v_window(g, d, v.wz, zinc, vstart, iz0, iz1) && iz0 < iz1 && zinc > T(0);
                 int za = 0, zb = -1;
                 if (ok) {
                     // voxel range needed by this wave's rows (exact ranges; a voxel left out at
                     // the rounding edge has a weight of the order of the rounding)
-                    const T rz = frcp(zinc);
+                    rz = frcp(zinc);
                     za = max((int)ceil((T(iv_a) - T(0.5) - vstart) * rz - T(1)), iz0);
                     zb = min((int)ceil((T(iv_last) + T(0.5) - vstart) * rz) - 1, iz1 - 1);
                     ok = za <= zb;
@@ -732,16 +734,15 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
                     const unsigned long long a =
                         (unsigned long long)(vol + ((size_t)(iy * g.nx + ix) * g.nz + za));
                     cp.alo = (int)(unsigned)(a & 0xffffffffull);
-                    cp.ahi = (int)(unsigned)((a >> 32) & 0xffffull);
-                    cp.nbytes = (span + 1) * (int)sizeof(Tin);
+                    const unsigned ahi_off = (unsigned)(a >> 32) - (unsigned)(empty_rs.y);
                     cp.meta = gather_class(zinc) | ((span >> (HALF ? 7 : 6)) << 2) | ((c1 - c0) << 4) |
-                              ((c0 - iu_a) << 9);
+                              ((c0 - iu_a) << 9) | (span << 16) | (int)(ahi_off << 24);
                     cp.zbase = T(za) * g.dz + wzdz;
                     cp.r2 = r * r;
                     cp.zinc = zinc;
+                    cp.rz = rz;
                     cp.vst = vstart + T(za) * zinc;
                     cp.col = iy * g.nx + ix;
-                    cp.cells = ilo | (ihi << 16);
                 }
             }
             // survivors to lanes 0 .. n-1 in lane order, the others after them
@@ -753,28 +754,27 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
                 const int rank = __popcll(below);
                 const int dst = (ok ? rank : n + (lane - rank)) * 4;
                 cp.alo = permute_i(dst, cp.alo);
-                cp.ahi = permute_i(dst, cp.ahi);
-                cp.nbytes = permute_i(dst, cp.nbytes);
                 cp.meta = permute_i(dst, cp.meta);
                 cp.zbase = permute_f(dst, cp.zbase);
                 cp.r2 = permute_f(dst, cp.r2);
                 cp.zinc = permute_f(dst, cp.zinc);
+                cp.rz = permute_f(dst, cp.rz);
                 cp.vst = permute_f(dst, cp.vst);
                 cp.w0 = permute_f(dst, cp.w0);
                 cp.w1 = permute_f(dst, cp.w1);
                 cp.w2 = permute_f(dst, cp.w2);
                 cp.w3 = permute_f(dst, cp.w3);
                 cp.col = permute_i(dst, cp.col);
-                cp.cells = permute_i(dst, cp.cells);
             }
             // group c's loads into the next ring slot; past the last survivor, a group that
             // writes zeros, so that every column has exactly two groups issued after it
             auto issue = [&](int c) {
                 rsrc_words rs = empty_rs;
                 if (c < n) {
+                    const unsigned mt = (unsigned)rdl_i(cp.meta, c);
                     rs.x = rdl_i(cp.alo, c);
-                    rs.y = rdl_i(cp.ahi, c);
-                    rs.z = rdl_i(cp.nbytes, c);
+                    rs.y = empty_rs.y + (int)(mt >> 24);
+                    rs.z = (int)(((mt >> 16) & 255u) + 1u) * (int)sizeof(Tin);
                 }
                 ring_load<NS>(rs, isl, loff);
                 isl = advance(isl);
@@ -790,9 +790,9 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
                 T* zs = HALF ? zstage : (T*)((char*)zr + slot_off);         // word 0: zero
                 psl = advance(psl);
                 const int meta = rdl_i(cp.meta, c);
-                const int cls = meta & 3, ns = (meta >> 2) & 3, nc = (meta >> 4) & 31, c0 = meta >> 9;
+                const int cls = meta & 3, ns = (meta >> 2) & 3, nc = (meta >> 4) & 31, c0 = (meta >> 9) & 127;
                 const T r2 = rdl(cp.r2, c), zinc = rdl(cp.zinc, c), vst = rdl(cp.vst, c), zbase = rdl(cp.zbase, c);
-                const T rz = frcp(zinc);
+                const T rz = rdl(cp.rz, c);  // frcp(zinc), formed by the candidate's lane
                 if constexpr (HALF) {
                     // each lane's dword is voxels 2 lane, 2 lane + 1 of a slot (zero past the
                     // range): converted, scaled by sqrt(r^2 + zc^2) and stored as an FP32 pair
@@ -834,7 +834,7 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
                 // row iv against the column's voxels (relative to voxel za): voxel j covers
                 // rows [vst + j zinc, vst + (j + 1) zinc]; at most ceil(1/zinc) + 1 of them
                 // overlap a row, so thick voxels take a fixed unrolled count
-                const int span = (rdl_i(cp.nbytes, c) >> (HALF ? 1 : 2)) - 1;
+                const int span = (meta >> 16) & 255;
                 const T w0 = rdl(cp.w0, c), w1 = rdl(cp.w1, c), w2 = rdl(cp.w2, c), w3 = rdl(cp.w3, c);
 #pragma unroll
                 for (int rr = 0; rr < RPL; ++rr) {
@@ -872,12 +872,12 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
                     if (nc >= 2) pa[2 * kCellW] = a2;
                     if (nc >= 3) pa[3 * kCellW] = a3;
                     if (nc >= 4) {
-                        // wider than four tile cells (rare): the corners again, from the column
-                        const int col = rdl_i(cp.col, c), cells = rdl_i(cp.cells, c);
-                        const int ilo = cells & 0xffff, ihi = cells >> 16;
+                        // wider than four tile cells (rare): the corners and cells again, from
+                        // the column (the candidate's own evaluation, so the same values)
+                        const int col = rdl_i(cp.col, c);
                         T p0, p1, p2, p3, amp, r, d;
-                        int jlo, jhi;
-                        col_corners<T, FOOT>(g, v, col % g.nx, col / g.nx, p0, p1, p2, p3, jlo, jhi, amp, r, d);
+                        int ilo, ihi;
+                        col_corners<T, FOOT>(g, v, col % g.nx, col / g.nx, p0, p1, p2, p3, ilo, ihi, amp, r, d);
                         const T ampr = amp * frcp(r);
                         for (int cc = c0 + 4; cc <= c0 + nc; ++cc)
                             acc[cc * kCellW + rr * kWave] += u_weight<T, FOOT>(iu_a + cc, ilo, ihi, p0, p1, p2, p3) * ampr * q;
