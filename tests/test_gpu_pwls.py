@@ -297,7 +297,9 @@ def test_fused_backprojection_update_equals_separate_kernels(mixed, graph, with_
     """ctsim_plan_back_nesterov (the backprojection with the update in its epilogue) agrees with
     ctsim_plan_back followed by the update kernel, masked columns included. Not bit for bit: the
     separate update kernel's four-voxel form sums the penalty's neighbour terms in another
-    order than the fused epilogue's one-voxel form (a last-ulp difference per step)"""
+    order than the fused epilogue's one-voxel form (a last-ulp difference per step).  In the
+    mixed form such a difference can also flip the BF16 rounding of a residual element (a
+    2^-9 relative step of that element), so its bound over the twelve subset updates is 5e-6"""
     from helpers import geometry_from_g3
     from ctsim_amd.pwls import PwlsOS
     g, y, w, x0 = _problem(seed=7)
@@ -308,7 +310,7 @@ def test_fused_backprojection_update_equals_separate_kernels(mixed, graph, with_
     assert a.fused and not b.fused
     xa, pa, _ = a.run(x0.astype(np.float32), its)
     xb, pb, _ = b.run(x0.astype(np.float32), its)
-    assert rel_l2(xa, xb) < 1e-6, rel_l2(xa, xb)
+    assert rel_l2(xa, xb) < (5e-6 if mixed else 1e-6), rel_l2(xa, xb)
     np.testing.assert_allclose(pa, pb, rtol=1e-6)
 
 
