@@ -606,11 +606,10 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
     static_assert(HALF ? (NS == 1 || NS == 2) : (NS >= 2 && NS <= 4), "slots per column");
     static_assert((HALF ? 2 : 1) * NS * kWave <= 256 && TU <= 128, "a column's span and first cell fit its meta word");
     // one block of LDS: the waves' rings, (HALF) their staging buffers, then their tile
-    // accumulators, then three pad cells.  Accumulators: each lane's row of the tile,
-    // [cell][lane] (one word per bank, one writer per word).  The four-cell update below
-    // reads cells c0 .. c0 + 3 whatever c0 is; past a wave's 16 cells it reads the next
-    // wave's cells or the pad (values not used).  The rings come first so that their
-    // addresses fit the LDS instructions' offset fields.
+    // accumulators.  Accumulators: each lane's row of the tile, [cell][lane] (one word per
+    // bank, one writer per word); a column's update reads and writes only its footprint's
+    // cells inside the tile.  The rings come first so that their addresses fit the LDS
+    // instructions' offset fields.
     constexpr int kRing = RING;
     constexpr int kSlotW = HALF ? NS * kWave : ring_entry_words(NS);
     // staging (HALF): an unused word, the zero word, 128 NS voxels, kZPad zero words; even
@@ -618,7 +617,7 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
     constexpr int kStageW = HALF ? 2 + 2 * NS * kWave + kZPad : 0;
     static_assert(!HALF || (kStageW % 2 == 0 && (kRing * kSlotW) % 2 == 0), "staging pairs on 8-byte boundaries");
     constexpr int kCellW = kWave * RPL;  // accumulator words per tile cell: [cell][row chunk][lane]
-    __shared__ __align__(16) T lds_block[WPB * kRing * kSlotW + WPB * kStageW + WPB * TU * kCellW + 3 * kCellW];
+    __shared__ __align__(16) T lds_block[WPB * kRing * kSlotW + WPB * kStageW + WPB * TU * kCellW];
     T* const accs = lds_block + WPB * (kRing * kSlotW + kStageW);
     const int nut = (g.nu + TU - 1) / TU;
     const int nvc = (g.nv + kCellW - 1) / kCellW;
@@ -866,16 +865,28 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
                         }
                     }
                     // tile cells c0 .. c0 + nc; the same on every lane, so each weighted add
-                    // touches one LDS word per lane and bank.  The four words are read and
-                    // updated in one pass (one LDS round trip); only the cells of the footprint
-                    // inside the tile are written back.
+                    // touches one LDS word per lane and bank.  One branch per footprint width
+                    // (uniform), each reading its cells together and writing them back: one LDS
+                    // round trip, and no cell outside the footprint is read or written.
                     T* pa = acc + c0 * kCellW + rr * kWave;
-                    const T a0 = pa[0] + w0 * q, a1 = pa[kCellW] + w1 * q, a2 = pa[2 * kCellW] + w2 * q,
-                            a3 = pa[3 * kCellW] + w3 * q;
-                    pa[0] = a0;
-                    if (nc >= 1) pa[kCellW] = a1;
-                    if (nc >= 2) pa[2 * kCellW] = a2;
-                    if (nc >= 3) pa[3 * kCellW] = a3;
+                    if (nc == 0) {
+                        pa[0] = pa[0] + w0 * q;
+                    } else if (nc == 1) {
+                        const T a0 = pa[0], a1 = pa[kCellW];
+                        pa[0] = a0 + w0 * q;
+                        pa[kCellW] = a1 + w1 * q;
+                    } else if (nc == 2) {
+                        const T a0 = pa[0], a1 = pa[kCellW], a2 = pa[2 * kCellW];
+                        pa[0] = a0 + w0 * q;
+                        pa[kCellW] = a1 + w1 * q;
+                        pa[2 * kCellW] = a2 + w2 * q;
+                    } else {
+                        const T a0 = pa[0], a1 = pa[kCellW], a2 = pa[2 * kCellW], a3 = pa[3 * kCellW];
+                        pa[0] = a0 + w0 * q;
+                        pa[kCellW] = a1 + w1 * q;
+                        pa[2 * kCellW] = a2 + w2 * q;
+                        pa[3 * kCellW] = a3 + w3 * q;
+                    }
                     if (nc >= 4) {
                         // wider than four tile cells (rare): the corners and cells again, from
                         // the column (the candidate's own evaluation, so the same values)
