@@ -174,7 +174,12 @@ __global__ __launch_bounds__(256) void fbct_proj_kernel(Geom<T> g, const View<T>
     const bool valid = wid < nviews * nut;
     const int ut = valid ? wid % nut : 0, b = valid ? wid / nut : 0;
     T acc = T(0);
-    if (valid) acc = fbct_proj_acc<Tin, T, FOOT>(g, views[(size_t)b * vstride], ut, img, exhaustive, wv % S, S);
+    if (valid) {
+        // the view in registers (a wave-uniform index: scalar loads, once per item), so that
+        // the walk's LDS atomics and barriers do not make the compiler reload it every batch
+        const View<T> vw = views[(size_t)uni(b) * vstride];
+        acc = fbct_proj_acc<Tin, T, FOOT>(g, vw, ut, img, exhaustive, wv % S, S);
+    }
     if (S > 1) {
         part[wv][lane] = acc;
         __syncthreads();
@@ -297,13 +302,18 @@ __global__ __launch_bounds__(256) void fbct_proj_batch_kernel(const Slice2<T>* _
     T acc = T(0);
     int si = 0, ut = 0, b = 0;
     if (valid) {
-        si = find_slice(sl, nslices, item);
+        // the item's slice geometry and view in registers (wave-uniform indices: scalar loads,
+        // once per item), so that the walk's LDS atomics and barriers do not make the
+        // compiler reload them from the slice table every batch
+        si = uni(find_slice(sl, nslices, item));
         const Slice2<T>& s = sl[si];
+        const Geom<T> g = s.g;
         const int local = item - s.item0;
-        const int nut = (s.g.nu + kW2 - 1) / kW2;
-        ut = local % nut;
-        b = local / nut;
-        acc = fbct_proj_acc<Tin, T, FOOT>(s.g, s.views[b], ut, img + s.img_off, 0, wv % S, S);
+        const int nut = (g.nu + kW2 - 1) / kW2;
+        ut = uni(local % nut);
+        b = uni(local / nut);
+        const View<T> vw = s.views[b];
+        acc = fbct_proj_acc<Tin, T, FOOT>(g, vw, ut, img + s.img_off, 0, wv % S, S);
     }
     if (S > 1) {
         part[wv][lane] = acc;
