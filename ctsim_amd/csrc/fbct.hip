@@ -64,6 +64,10 @@ __device__ __forceinline__ T fbct_proj_acc(const Geom<T>& g, const View<T>& v, i
             const int iy = row_major ? major : minor;
             int ilo = 0, ihi = -1;
             T p0 = T(0), p1 = T(0), p2 = T(0), p3 = T(0), val = T(0);
+            // the pixel is loaded before its footprint is known (every candidate lies inside
+            // the image), so that the load's latency overlaps the footprint arithmetic
+            T pix = T(0);
+            if (ok) pix = ld<T>(img + (size_t)iy * g.nx + ix);
             if (ok) {
                 const T xc = (T(ix) + g.wx) * g.dx, yc = (T(iy) + g.wy) * g.dy;
                 const Column<T> cg = column_geom(xc, yc, v, g.sad, g.dx);
@@ -81,7 +85,7 @@ __device__ __forceinline__ T fbct_proj_acc(const Geom<T>& g, const View<T>& v, i
                     p0 = tau[0]; p1 = tau[1]; p2 = tau[2]; p3 = tau[3];
                 }
                 ok = ok && ihi >= iu_a && ilo <= iu_b;
-                if (ok) val = cg.amp * ld<T>(img + (size_t)iy * g.nx + ix);
+                if (ok) val = cg.amp * pix;
             }
             ecell[wv][lane] = ok ? (ilo | (ihi << 16)) : 1;
             ep[wv][0][lane] = p0;
