@@ -343,16 +343,21 @@ __global__ __launch_bounds__(256) void fbct_back_batch_kernel(const Slice2<T>* _
     T v = T(0);
     int si = 0, ix = 0, iy = 0;
     if (valid) {
-        si = find_slice(sl, nslices, item);
+        // the item's slice geometry in registers (a wave-uniform index: scalar loads, once
+        // per item) rather than read from the slice table inside the view loop
+        si = uni(find_slice(sl, nslices, item));
         const Slice2<T>& s = sl[si];
+        const Geom<T> g = s.g;
+        const View<T>* views = s.views;
+        const int nviews = s.nviews;
         const int local = item - s.item0;
-        const int nxb = (s.g.nx + kW2 - 1) / kW2;
+        const int nxb = (g.nx + kW2 - 1) / kW2;
         iy = local / nxb;
         ix = (local % nxb) * kW2 + lane;
         const int share = wv % VS;
-        const int per = (s.nviews + VS - 1) / VS;
-        const int b0 = min(share * per, s.nviews), b1 = min(b0 + per, s.nviews);
-        v = fbct_back_pixel<Tin, T, KIND>(s.g, s.views, 1, b0, b1, ix, iy, proj + s.sino_off);
+        const int per = (nviews + VS - 1) / VS;
+        const int b0 = min(share * per, nviews), b1 = min(b0 + per, nviews);
+        v = fbct_back_pixel<Tin, T, KIND>(g, views, 1, b0, b1, ix, iy, proj + s.sino_off);
     }
     if (VS > 1) {
         part[wv][lane] = v;
