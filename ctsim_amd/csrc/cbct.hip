@@ -703,7 +703,10 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
             // quantities the walk needs
             bool ok = k < total;
             RingCol cp{};
-            if (ok && map != nullptr) ok = map[iy * g.nx + ix] != 0;
+            // the map byte is loaded first and tested after the footprint, so that its load
+            // overlaps the footprint arithmetic (lanes it rejects would idle through that code)
+            int in_map = 1;
+            if (ok && map != nullptr) in_map = map[iy * g.nx + ix];
             if (ok) {
                 T p0, p1, p2, p3, amp, r, d;
                 int ilo = 0, ihi = -1, iz0 = 0, iz1 = 0;
@@ -748,6 +751,7 @@ __global__ __launch_bounds__(WPB * kWave, kRingWaves) void cbct_proj_ring_kernel
                     cp.col = iy * g.nx + ix;
                 }
             }
+            ok = ok && in_map != 0;
             // survivors to lanes 0 .. n-1 in lane order, the others after them
             const unsigned long long mask = __ballot(ok);
             const int n = __popcll(mask);
