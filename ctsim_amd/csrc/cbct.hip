@@ -114,19 +114,6 @@ __device__ __forceinline__ T u_weight(int iu, int ilo, int ihi, T p0, T p1, T p2
     return FOOT == FOOT_DD ? dd_weight(iu, p0, p1, ilo, ihi) : sf_weight(iu, p0, p1, p2, p3);
 }
 
-// Tile of rank r in the order c, c+1, c-1, c+2, c-2, ... (then the rest of the longer side)
-// for nut tiles around the tile c of the central ray.
-__device__ __forceinline__ int tile_by_rank(int r, int c, int nut) {
-    const int left = c, right = nut - 1 - c, m = min(left, right);
-    if (r == 0) return c;
-    if (r <= 2 * m) {
-        const int k = (r + 1) >> 1;
-        return (r & 1) ? c + k : c - k;
-    }
-    const int extra = r - 2 * m;
-    return right > left ? c + m + extra : c - m - extra;
-}
-
 // Staged voxels of a window [zw0, zw1] sit at zwin[1 + j - zw0]; zwin[0] and the
 // kZPad words after the last voxel are zero.  gather_row sums, over the K voxels from
 // ja on, (overlap with the row) * staged value, where voxel j covers detector rows

@@ -176,7 +176,9 @@ __global__ __launch_bounds__(256) void fbct_proj_kernel(Geom<T> g, const View<T>
     const int wv = (int)(threadIdx.x / kW2), lane = lane_id();
     const int wid = blockIdx.x * (4 / S) + wv / S;
     const bool valid = wid < nviews * nut;
-    const int ut = valid ? wid % nut : 0, b = valid ? wid / nut : 0;
+    // longest work first: every view's tile of the central ray, then the tiles beside it
+    const int ut = valid ? tile_by_rank(wid / nviews, min(max((int)floor(-g.wu / T(kW2)), 0), nut - 1), nut) : 0;
+    const int b = valid ? wid % nviews : 0;
     T acc = T(0);
     if (valid) {
         // the view in registers (a wave-uniform index: scalar loads, once per item), so that
@@ -314,8 +316,9 @@ __global__ __launch_bounds__(256) void fbct_proj_batch_kernel(const Slice2<T>* _
         const Geom<T> g = s.g;
         const int local = item - s.item0;
         const int nut = (g.nu + kW2 - 1) / kW2;
-        ut = uni(local % nut);
-        b = uni(local / nut);
+        // within the slice, longest work first (the tiles around the central ray's)
+        ut = uni(tile_by_rank(local / s.nviews, min(max((int)floor(-g.wu / T(kW2)), 0), nut - 1), nut));
+        b = uni(local % s.nviews);
         const View<T> vw = s.views[b];
         acc = fbct_proj_acc<Tin, T, FOOT>(g, vw, ut, img + s.img_off, 0, wv % S, S);
     }

@@ -363,6 +363,19 @@ __device__ __forceinline__ double rdl(double x, int lane) { return rdl_d(x, lane
 
 // a value every lane of the wave holds (e.g. read from one LDS address), moved to a
 // scalar register so that comparisons on it become uniform branches
+// Tile of rank r in the order c, c+1, c-1, c+2, c-2, ... (then the rest of the longer side)
+// for nut tiles around the tile c of the central ray.
+__device__ __forceinline__ int tile_by_rank(int r, int c, int nut) {
+    const int left = c, right = nut - 1 - c, m = min(left, right);
+    if (r == 0) return c;
+    if (r <= 2 * m) {
+        const int k = (r + 1) >> 1;
+        return (r & 1) ? c + k : c - k;
+    }
+    const int extra = r - 2 * m;
+    return right > left ? c + m + extra : c - m - extra;
+}
+
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ float uni(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
 __device__ __forceinline__ double uni(double x) {
