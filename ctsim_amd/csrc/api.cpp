@@ -890,7 +890,7 @@ struct ctsim_batch2 {
 namespace {
 template <class T>
 int batch2_build(ctsim_batch2* b, int n, const ctsim_geom* geoms, const int32_t* nviews, const double* const* betas) {
-    std::vector<Slice2<T>> sp(n), sb(n);
+    std::vector<Slice2<T>> sl(n), sp(n), sb(n);
     long long img = 0, sino = 0;
     int ip = 0, ib = 0;
     for (int i = 0; i < n; ++i) {
@@ -908,14 +908,30 @@ int batch2_build(ctsim_batch2* b, int n, const ctsim_geom* geoms, const int32_t*
         s.nviews = nviews[i];
         s.img_off = img;
         s.sino_off = sino;
-        sp[i] = s;
-        sb[i] = s;
-        sp[i].item0 = ip;
-        sb[i].item0 = ib;
-        ip += nviews[i] * ((g.nu + 63) / 64);
-        ib += g.ny * ((g.nx + 63) / 64);
+        sl[i] = s;
         img += (long long)g.nx * g.ny;
         sino += (long long)g.nu * nviews[i];
+    }
+    // work lists with the slices of the longest items first (a forward item's strip grows with
+    // the image, a back item's view loop with the view count), so that the launch ends on
+    // short items; a slice's place in the list changes no output (each keeps its offsets)
+    std::vector<int> po(n), bo(n);
+    for (int i = 0; i < n; ++i) po[i] = bo[i] = i;
+    std::stable_sort(po.begin(), po.end(), [&](int a, int c) {
+        return std::max(geoms[a].nx, geoms[a].ny) > std::max(geoms[c].nx, geoms[c].ny);
+    });
+    std::stable_sort(bo.begin(), bo.end(), [&](int a, int c) { return nviews[a] > nviews[c]; });
+    for (int j = 0; j < n; ++j) {
+        const int i = po[j];
+        sp[j] = sl[i];
+        sp[j].item0 = ip;
+        ip += nviews[i] * ((geoms[i].nu + 63) / 64);
+    }
+    for (int j = 0; j < n; ++j) {
+        const int i = bo[j];
+        sb[j] = sl[i];
+        sb[j].item0 = ib;
+        ib += geoms[i].ny * ((geoms[i].nx + 63) / 64);
     }
     b->nitems_proj = ip;
     b->nitems_back = ib;
