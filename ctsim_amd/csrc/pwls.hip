@@ -60,13 +60,12 @@ static VoxGrid vox_grid(int nx, int ny, int nz) {
     const int bz = 1 << lbz, bx = 256 >> lbz;
     return VoxGrid{dim3((unsigned)((nz + bz - 1) / bz), (unsigned)((nx + bx - 1) / bx), (unsigned)ny), lbz};
 }
-// The dispatcher deals workgroups to the 8 XCDs round robin; the logical block order is
-// remapped so that each XCD walks a contiguous run of y planes, and the y +-1 planes and
-// x +-1 columns a stencil reads come from that XCD's own L2.
+// Blocks in dispatch order (the dispatcher deals them to the 8 XCDs round robin): giving
+// each XCD a contiguous run of y planes, so that a stencil's y +-1 planes come from its own
+// L2, measured 3% slower on the 512 x 512 x 128 Nesterov update.
 __device__ __forceinline__ bool vox_index(int nx, int nz, int lbz, int& ix, int& iy, int& iz, size_t& i) {
     const int gx = (int)gridDim.x, gy = (int)gridDim.y;
-    const int lin = (int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z);
-    const int b = xcd_remap(lin, gx * gy * (int)gridDim.z);
+    const int b = (int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z);
     const int bzi = b % gx, r = b / gx;
     iz = (bzi << lbz) + (int)(threadIdx.x & ((1u << lbz) - 1));
     ix = (r % gy) * (256 >> lbz) + (int)(threadIdx.x >> lbz);
