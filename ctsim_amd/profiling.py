@@ -14,6 +14,10 @@ passes), so that a profile names only the kernels it is about.
                                                        # 512^3 FP32 volume, 1024x768 detector
     python -m ctsim_amd.profiling pwls-update [reps]   # fused Nesterov-SQS update (Huber, FP16 z copy)
                                                        # on 512x512x128 FP32 volumes
+    python -m ctsim_amd.profiling fan-pair [reps]      # proj,dd + back,dd of the 2D fan workload:
+                                                       # 256x256 image, 360 views, 512 bins, FP32
+    python -m ctsim_amd.profiling batch2 [reps]        # proj,dd + back,dd of the 64 uneven 2D slices
+                                                       # (FP16 images and sinograms), schedule (1, 1)
 
 The copy workload calibrates the FETCH_SIZE / WRITE_SIZE counters: it reads and writes
 exactly 256 MiB per launch."""
@@ -97,8 +101,34 @@ def pwls_update(reps: int) -> None:
     synchronize()
 
 
+def fan_pair(reps: int) -> None:
+    g = workloads.fan_256()
+    P = Projector(g)
+    img = DeviceArray.from_numpy(workloads.ellipse_image(256, 0.8, 10, 1).ravel(order="F"), "f32")
+    s = DeviceArray(P.n_sino(), "f32")
+    b = DeviceArray(P.n_img, "f32")
+    for _ in range(reps):
+        P.proj(img, s, "proj,dd")
+        P.back(s, b, "back,dd")
+    synchronize()
+
+
+def batch2(reps: int) -> None:
+    from .batch2 import Batch2
+    geoms, imgs = workloads.batch2_uneven()
+    B = Batch2(geoms)
+    B.set_schedule(1, 1)
+    img = DeviceArray.from_numpy(B.pack_images(imgs), "f16")
+    s = DeviceArray(B.sino_elems, "f16")
+    b = DeviceArray(B.img_elems, "f16")
+    for _ in range(reps):
+        B.proj(img, s, "proj,dd")
+        B.back(s, b, "back,dd")
+    synchronize()
+
+
 WORKLOADS = {"copy": copy, "pwls-subset": pwls_subset, "pair-slab": pair_slab, "pair-run": pair_run, "fdk-back": fdk_back,
-             "pwls-update": pwls_update}
+             "pwls-update": pwls_update, "fan-pair": fan_pair, "batch2": batch2}
 
 
 def main(argv=None) -> int:
