@@ -30,10 +30,11 @@ constexpr int kPmFootU = 16;  // u cells per footprint (iumin + 15, cbct_proj_pm
 constexpr int kPmFootV = 10;  // v rows per voxel (FOOTPRINT_SIZE_V, ct_def.h:82)
 
 // u footprint of column (ix, iy) in view pv (cbct_proj_pmat.c:69-147): cells
-// [iumin, iumax], weights w, ampu, and v0 / f0 and dv/dz of the column; false when the
-// column misses the detector.
-__device__ __forceinline__ bool pmat_ufoot(const PmatConst& c, const PmatView& pv, int ix, int iy, float& ampu,
-                                           int& iumin, int& iumax, float* w, float& v0f, float& dvdz) {
+// [iumin, iumax] covering [umin, umax], ampu, and v0 / f0 and dv/dz of the column; false
+// when the column misses the detector.
+__device__ __forceinline__ bool pmat_ucell(const PmatConst& c, const PmatView& pv, int ix, int iy, float& ampu,
+                                           float& umin, float& umax, int& iumin, int& iumax, float& v0f,
+                                           float& dvdz) {
 #pragma clang fp contract(off)
     const float* pm = pv.pm;
     const float xc = ((float)ix + c.wx) * c.dx;
@@ -43,7 +44,6 @@ __device__ __forceinline__ bool pmat_ufoot(const PmatConst& c, const PmatView& p
     const float f0 = xc * pm[2] + yc * pm[5] + pm[11];
     const float dsp_x = (float)fabs((double)(pv.cx - xc));
     const float dsp_y = (float)fabs((double)(pv.cy - yc));
-    float umin, umax;
     const double dist = sqrt((double)(dsp_x * dsp_x + dsp_y * dsp_y));
     if (fabs((double)dsp_x) > fabs((double)dsp_y)) {
         umin = (u0 - c.dy * pm[3] / 2.0f) / (f0 - c.dy * pm[5] / 2.0f);
@@ -66,24 +66,34 @@ __device__ __forceinline__ bool pmat_ufoot(const PmatConst& c, const PmatView& p
     iumin = (int)floorf(umin);
     iumax = min((int)floorf(umax), c.nu - 1);
     iumax = min(iumax, iumin + kPmFootU - 1);
-    if (iumin == iumax) {
-        w[0] = umax - umin;
-    } else {
-#pragma unroll
-        for (int k = 0; k < kPmFootU; ++k) {
-            const int iu = iumin + k;
-            w[k] = iu == iumin ? (float)((double)iumin + 1.0 - (double)umin)
-                               : (iu == iumax ? umax - (float)iumax : (iu < iumax ? 1.0f : 0.0f));
-        }
-    }
     v0f = v0 / f0;
     dvdz = pm[7] / f0;
     return true;
 }
 
-// v footprint of voxel iz (cbct_proj_pmat.c:155-190): rows [ivmin, ivmax] and weights
-__device__ __forceinline__ bool pmat_vfoot(const PmatConst& c, int iz, float v0f, float dvdz, int& ivmin,
-                                           int& ivmax, float* w) {
+// weight of cell i in [imin, imax] of a footprint covering [lo, hi] (cbct_proj_pmat.c:
+// 126-145 for u, 170-188 for v): the covered fraction of the first and last cells, 1
+// between them, hi - lo for a one-cell footprint
+__device__ __forceinline__ float pmat_weight(float lo, float hi, int imin, int imax, int i) {
+#pragma clang fp contract(off)
+    if (imin == imax) return hi - lo;
+    return i == imin ? (float)((double)(float)imin + 1.0 - (double)lo) : (i == imax ? hi - (float)imax : 1.0f);
+}
+
+// u footprint with its weights w[0 .. iumax - iumin]
+__device__ __forceinline__ bool pmat_ufoot(const PmatConst& c, const PmatView& pv, int ix, int iy, float& ampu,
+                                           int& iumin, int& iumax, float* w, float& v0f, float& dvdz) {
+    float umin, umax;
+    if (!pmat_ucell(c, pv, ix, iy, ampu, umin, umax, iumin, iumax, v0f, dvdz)) return false;
+#pragma unroll
+    for (int k = 0; k < kPmFootU; ++k) w[k] = iumin + k <= iumax ? pmat_weight(umin, umax, iumin, iumax, iumin + k) : 0.0f;
+    return true;
+}
+
+// v footprint of voxel iz (cbct_proj_pmat.c:155-190): rows [ivmin, ivmax] covering
+// [vmin, vmax]
+__device__ __forceinline__ bool pmat_vcell(const PmatConst& c, int iz, float v0f, float dvdz, float& vmin,
+                                           float& vmax, int& ivmin, int& ivmax) {
 #pragma clang fp contract(off)
     const double fz = (double)(float)iz;
     double lo, hi;
@@ -94,7 +104,8 @@ __device__ __forceinline__ bool pmat_vfoot(const PmatConst& c, int iz, float v0f
         lo = (double)v0f + (fz + 0.5) * (double)c.dz * (double)dvdz;
         hi = (double)v0f + (fz - 0.5) * (double)c.dz * (double)dvdz;
     }
-    float vmin = (float)lo, vmax = (float)hi;
+    vmin = (float)lo;
+    vmax = (float)hi;
     const double a = (double)vmin + 0.5, b = (double)vmax + 0.5;
     vmin = (float)(a > 0.0 ? a : 0.0);
     vmax = (float)(b < (double)c.nv ? b : (double)c.nv);
@@ -102,16 +113,16 @@ __device__ __forceinline__ bool pmat_vfoot(const PmatConst& c, int iz, float v0f
     ivmin = (int)floorf(vmin);
     ivmax = min((int)floorf(vmax), c.nv - 1);
     ivmax = min(ivmax, ivmin + kPmFootV - 1);
-    if (ivmin == ivmax) {
-        w[0] = vmax - vmin;
-    } else {
+    return true;
+}
+
+// v footprint with its weights w[0 .. ivmax - ivmin]
+__device__ __forceinline__ bool pmat_vfoot(const PmatConst& c, int iz, float v0f, float dvdz, int& ivmin,
+                                           int& ivmax, float* w) {
+    float vmin, vmax;
+    if (!pmat_vcell(c, iz, v0f, dvdz, vmin, vmax, ivmin, ivmax)) return false;
 #pragma unroll
-        for (int k = 0; k < kPmFootV; ++k) {
-            const int iv = ivmin + k;
-            w[k] = iv == ivmin ? (float)((double)(float)ivmin + 1.0 - (double)vmin)
-                               : (iv == ivmax ? vmax - (float)ivmax : (iv < ivmax ? 1.0f : 0.0f));
-        }
-    }
+    for (int k = 0; k < kPmFootV; ++k) w[k] = ivmin + k <= ivmax ? pmat_weight(vmin, vmax, ivmin, ivmax, ivmin + k) : 0.0f;
     return true;
 }
 
@@ -137,33 +148,169 @@ __global__ __launch_bounds__(256) void pmat_amp_kernel(PmatConst c, const PmatVi
     }
 }
 
-// forward: one wave per (view, image column), lanes are voxels
+// forward: one block per (view, tile of kPtX x kPtY image columns, chunk of kWave
+// slices).  Its threads first take one column each: the u footprint goes to LDS and the
+// block's detector box (the tile's u cells by the chunk's v rows, one row of margin) is
+// reduced there.  The columns are then ranked by the centre of their u footprint and cut
+// into kSeg runs of neighbouring u; in two phases each wave walks one run (runs 0, 2, 4, 6,
+// then 1, 3, 5, 7), its lanes on the chunk's voxels, adding the footprint cells to an LDS
+// patch over the box.  Within one (column, u cell) the lanes hold distinct v rows whenever
+// the v footprint advances at least one row per slice, so a cell that no other wave's
+// current run reaches takes a plain read-add-write, in order; a cell shared with another
+// run, or a column whose slices share a first row, takes an LDS atomic.  After a barrier
+// every patch cell that received anything is added to the view with one FP32 global
+// atomic.  A cell outside the box, or every cell of a block whose box exceeds kPatch,
+// takes a global atomic directly.
+constexpr int kPtX = 16, kPtY = 16;  // image columns per tile
+constexpr int kSeg = 8;              // runs of columns, two per wave
+constexpr int kPatch = 7168;         // LDS patch cells (28 KB; four blocks per CU)
+
+struct PmatCol {
+    float umin, umax, ampu, v0f, dvdz;
+    int iumin, iumax, ok;
+};
+
 __global__ __launch_bounds__(256) void pmat_proj_kernel(PmatConst c, const PmatView* __restrict__ views,
                                                         const float* __restrict__ vol, const uint8_t* __restrict__ map,
-                                                        float* __restrict__ acc) {
+                                                        float* __restrict__ acc, int ntx, int nty) {
 #pragma clang fp contract(off)
-    const int ncol = c.nx * c.ny;
-    const long long wid = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (wid >= (long long)ncol * c.nviews) return;
-    const int b = (int)(wid / ncol), col = (int)(wid - (long long)b * ncol);
-    if (!map[col]) return;
-    const int ix = col % c.nx, iy = col / c.nx;
-    const PmatView pv = views[b];
-    float wu[kPmFootU], wv[kPmFootV];
-    float ampu, v0f, dvdz;
-    int iumin, iumax;
-    if (!pmat_ufoot(c, pv, ix, iy, ampu, iumin, iumax, wu, v0f, dvdz)) return;
+    static_assert(kPtX * kPtY == 256 && kSeg == 2 * (256 / kWave), "one column per thread, two runs per wave");
+    __shared__ PmatCol cols[kPtX * kPtY];
+    __shared__ float ukey[kPtX * kPtY];
+    __shared__ int order[kPtX * kPtY];
+    __shared__ float patch[kPatch];
+    __shared__ int box[4];          // u lo, u hi, v lo, v hi
+    __shared__ int run[2 * kSeg];  // u lo, u hi of each run
+    __shared__ int nok_s;
+    const int t = threadIdx.x;
+    int r = blockIdx.x;
+    const int b = r % c.nviews;
+    r /= c.nviews;
+    const int tx = r % ntx;
+    r /= ntx;
+    const int ty = r % nty;
+    const int z0 = (r / nty) * kWave;
+    const int z1 = min(z0 + kWave, c.nz) - 1;
+    if (t < 2 * kSeg) run[t] = (t & 1) ? INT_MIN : INT_MAX;
+    if (t == 0) {
+        box[0] = INT_MAX;
+        box[1] = INT_MIN;
+        box[2] = INT_MAX;
+        box[3] = INT_MIN;
+        nok_s = 0;
+    }
+    __syncthreads();
+    {
+        const int ix = tx * kPtX + t % kPtX, iy = ty * kPtY + t / kPtX;
+        PmatCol pc;
+        pc.ok = 0;
+        if (ix < c.nx && iy < c.ny && map[iy * c.nx + ix])
+            pc.ok = pmat_ucell(c, views[b], ix, iy, pc.ampu, pc.umin, pc.umax, pc.iumin, pc.iumax, pc.v0f, pc.dvdz);
+        if (pc.ok) {
+            atomicAdd(&nok_s, 1);
+            // v rows of the chunk's end faces; v is linear in z, so they bound every voxel's
+            const float e0 = pc.v0f + ((float)z0 - 0.5f) * c.dz * pc.dvdz;
+            const float e1 = pc.v0f + ((float)z1 + 0.5f) * c.dz * pc.dvdz;
+            const int vlo = max((int)floorf(fminf(e0, e1) + 0.5f) - 1, 0);
+            const int vhi = min((int)floorf(fmaxf(e0, e1) + 0.5f) + 1, c.nv - 1);
+            if (vlo <= vhi) {
+                atomicMin(&box[0], pc.iumin);
+                atomicMax(&box[1], pc.iumax);
+                atomicMin(&box[2], vlo);
+                atomicMax(&box[3], vhi);
+            }
+        }
+        cols[t] = pc;
+        ukey[t] = pc.ok ? pc.umin + pc.umax : HUGE_VALF;
+    }
+    __syncthreads();
+    const bool boxed = box[0] <= box[1] && box[2] <= box[3];
+    const int ulo = box[0], vlo = box[2];
+    const int pu_n = boxed ? box[1] - ulo + 1 : 0, pv_n = boxed ? box[3] - vlo + 1 : 0;
+    const bool use_patch = boxed && (long long)pu_n * pv_n <= kPatch;
+    const int pcells = use_patch ? pu_n * pv_n : 0;
+    const int nok = uni(nok_s);
+    for (int i = t; i < pcells; i += 256) patch[i] = 0.0f;
+    {
+        // rank by footprint centre, ties by thread: the order is a permutation
+        const float kt = ukey[t];
+        int rank = 0;
+        for (int s2 = 0; s2 < kPtX * kPtY; ++s2) {
+            const float ks = ukey[s2];
+            rank += (ks < kt || (ks == kt && s2 < t)) ? 1 : 0;
+        }
+        order[rank] = t;
+    }
+    __syncthreads();
+    // run g holds ranks [ceil(g nok / kSeg), ceil((g + 1) nok / kSeg)); its u cells
+    if (t < nok) {
+        const int g = t * kSeg / nok;
+        const PmatCol& pc = cols[order[t] & 255];
+        atomicMin(&run[2 * g], pc.iumin);
+        atomicMax(&run[2 * g + 1], pc.iumax);
+    }
+    __syncthreads();
     float* pview = acc + (size_t)b * c.nu * c.nv;
-    const float* vc = vol + (size_t)col * c.nz;
-    for (int iz = lane_id(); iz < c.nz; iz += kWave) {
-        const float val = vc[iz];
-        if (val == 0.0f) continue;
-        const float voxel = ampu * val;
-        int ivmin, ivmax;
-        if (!pmat_vfoot(c, iz, v0f, dvdz, ivmin, ivmax, wv)) continue;
-        for (int iu = iumin; iu <= iumax; ++iu)
-            for (int iv = ivmin; iv <= ivmax; ++iv)
-                atomicAdd(&pview[(size_t)iu * c.nv + iv], wu[iu - iumin] * wv[iv - ivmin] * voxel);
+    const int lane = lane_id();
+    const int w = t / kWave;
+    const int iz = z0 + lane;
+    for (int h = 0; h < 2; ++h) {
+        const int g = 2 * w + h;
+        const int p0 = (g * nok + kSeg - 1) / kSeg, p1 = ((g + 1) * nok + kSeg - 1) / kSeg;
+        for (int p = p0; p < p1; ++p) {
+            const int k = uni(order[p]) & 255;
+            const PmatCol pc = cols[k];
+            const int iumin = uni(pc.iumin), iumax = uni(pc.iumax);
+            const int ix = tx * kPtX + k % kPtX, iy = ty * kPtY + k / kPtX;
+            const float val = iz < c.nz ? vol[((size_t)iy * c.nx + ix) * c.nz + iz] : 0.0f;
+            float vmin = 0.0f, vmax = 0.0f;
+            int ivmin = 0, ivmax = -1;
+            const bool vok = iz < c.nz && pmat_vcell(c, iz, pc.v0f, pc.dvdz, vmin, vmax, ivmin, ivmax);
+            // plain read-add-write needs distinct first rows on the lanes that write
+            const int prev_iv = __shfl_up(ivmin, 1);
+            const int prev_ok = __shfl_up(vok ? 1 : 0, 1);
+            const bool plain = !__any(lane > 0 && vok && prev_ok && prev_iv == ivmin);
+            if (!vok || val == 0.0f) continue;
+            const float voxel = pc.ampu * val;
+            const float wv0 = pmat_weight(vmin, vmax, ivmin, ivmax, ivmin);
+            const float wv1 = pmat_weight(vmin, vmax, ivmin, ivmax, ivmax);
+            for (int iu = iumin; iu <= iumax; ++iu) {
+                const float wu = pmat_weight(pc.umin, pc.umax, iumin, iumax, iu);
+                bool own = plain;
+#pragma unroll
+                for (int w2 = 0; w2 < 256 / kWave; ++w2) {
+                    const int g2 = 2 * w2 + h;
+                    if (w2 != w && iu >= run[2 * g2] && iu <= run[2 * g2 + 1]) own = false;
+                }
+                const int pu = iu - ulo;
+                for (int iv = ivmin; iv <= ivmax; ++iv) {
+                    const float add = wu * (iv == ivmin ? wv0 : (iv == ivmax ? wv1 : 1.0f)) * voxel;
+                    const int pv = iv - vlo;
+                    if (use_patch && (unsigned)pu < (unsigned)pu_n && (unsigned)pv < (unsigned)pv_n) {
+                        if (own) {
+                            // the next row's read must follow this write: the LDS runs one
+                            // wave's operations in order, and the empty asm keeps the
+                            // compiler from moving memory operations across it
+                            float* cell = patch + pu * pv_n + pv;
+                            *cell = *cell + add;
+                            asm volatile("" ::: "memory");
+                        } else {
+                            atomicAdd(&patch[pu * pv_n + pv], add);
+                        }
+                    } else {
+                        atomicAdd(&pview[(size_t)iu * c.nv + iv], add);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = t; i < pcells; i += 256) {
+        const float v = patch[i];
+        if (v != 0.0f) {
+            const int pu = i / pv_n;
+            atomicAdd(&pview[(size_t)(ulo + pu) * c.nv + vlo + (i - pu * pv_n)], v);
+        }
     }
 }
 
@@ -303,9 +450,11 @@ hipError_t pmat_launch(int kind, const PmatConst& c, const PmatView* views, cons
     if (kind == PMAT_PROJ_DD) {
         hipError_t e = fill_launch(out, DT_F32, nsino, 0.0, st);
         if (e != hipSuccess) return e;
-        const long long waves = (long long)c.nx * c.ny * c.nviews;
-        hipLaunchKernelGGL(pmat_proj_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, c, views, in, map,
-                           out);
+        const int ntx = (c.nx + kPtX - 1) / kPtX, nty = (c.ny + kPtY - 1) / kPtY;
+        const long long blocks = (long long)c.nviews * ntx * nty * ((c.nz + kWave - 1) / kWave);
+        if (blocks > INT_MAX) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(pmat_proj_kernel, dim3((unsigned)blocks), dim3(256), 0, st, c, views, in, map, out, ntx,
+                           nty);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(pmat_amp_kernel, dim3((unsigned)std::min<size_t>((nsino + 255) / 256, 65536)), dim3(256), 0,

@@ -72,3 +72,32 @@ def test_pmat_ops_match_oracle(oracle_mod, kind):
     assert np.array_equal(got["back,dd"], got["back,tf"])
     with pytest.raises(Exception, match="ray driven method is not supported"):
         mex.cbct_geom_pmat_mex("proj,rd", *args, vol, m)
+
+
+@pytest.mark.parametrize("case", ["tall", "fine"])
+def test_pmat_forward_tiles_chunks_and_fallback(oracle_mod, case):
+    """The forward's tiled form against the oracle where the easy case does not reach:
+    "tall" has 150 slices (three slice chunks, the last one partial) and a detector that
+    cuts the top and bottom slices; "fine" has 0.05 mm cells, so one voxel covers about 30
+    of them (the footprint truncated at 16 u cells and 10 v rows, as the reference does) and
+    a tile's detector box exceeds the LDS patch, which sends every cell to global atomics."""
+    O = oracle_mod
+    from ctsim_amd import mex
+    if case == "tall":
+        n, d, o, nuv, duv, nviews = (24, 20, 150), (1.0, 1.0, 0.9), (0.3, -0.2, 0.0), (48, 180), (1.0, 1.0), 6
+    else:
+        n, d, o, nuv, duv, nviews = (20, 18, 10), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (400, 200), (0.05, 0.05), 4
+    P, cams, sdds, ouv = O.circular_pmat(np.linspace(0, 2 * np.pi, nviews, endpoint=False) + 0.3, 1000.0, 1500.0,
+                                         nuv, duv)
+    m = np.ones(n[:2], bool)
+    g = O.GP(n, d, o, nuv, duv[0], duv[1], P, cams, sdds, ouv, map_=m)
+    rng = np.random.default_rng(5)
+    vol = np.asfortranarray(rng.uniform(0, 1, n).astype(np.float32))
+    args = (n, d, o, nuv, duv, nviews, g.pmat, g.cameras, g.sdds, g.ouv)
+    print(f"pmat forward {case}: n={n} nuv={nuv} views={nviews}", flush=True)
+    got = mex.cbct_geom_pmat_mex("proj,dd", *args, vol, m)
+    want = O.pmat("proj,dd", g, vol)
+    e = rel_l2(got, want)
+    print(f"  proj,dd: rel-L2 vs FP32 oracle {e:.2e}", flush=True)
+    assert np.linalg.norm(want) > 0
+    assert e <= 1e-5
