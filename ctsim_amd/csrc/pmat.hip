@@ -12,13 +12,13 @@
 //
 //  * back,pd / back,hpd and back,dd gather: one wavefront per image column, lanes are the
 //    column's voxels, every view in order; one writer per voxel, no atomics.
-//  * proj,dd: one wavefront per (view, image column), lanes are voxels, each voxel adding
-//    its footprint cells with FP32 global atomics (the reference's voxel-driven form; its
-//    per-cell amplitude is applied afterwards).  Results agree with the reference to FP32
-//    rounding; the order of the atomic additions is not fixed, so the last bits can vary
-//    between runs.  The footprint of a general projection matrix has no cheap inverse
-//    (the circular kernels' strip walk assumes the orbit), and this gateway is not on a
-//    timed path (no MATLAB caller in the reference tree).
+//  * proj,dd: voxel-driven like the reference (its per-cell amplitude applied afterwards),
+//    one block per (view, 16 x 16 image columns, 64 slices), the voxels' footprint cells
+//    summed in an LDS patch over the block's detector box and the patch added to the view
+//    with one FP32 global atomic per cell.  Results agree with the reference to FP32
+//    rounding; the order of the additions is not fixed, so the last bits can vary between
+//    runs.  The footprint of a general projection matrix has no cheap inverse (the circular
+//    kernels' strip walk assumes the orbit), so the form stays a scatter.
 #include <algorithm>
 
 #include "internal.h"
@@ -157,13 +157,16 @@ __global__ __launch_bounds__(256) void pmat_amp_kernel(PmatConst c, const PmatVi
 // patch over the box.  Within one (column, u cell) the lanes hold distinct v rows whenever
 // the v footprint advances at least one row per slice, so a cell that no other wave's
 // current run reaches takes a plain read-add-write, in order; a cell shared with another
-// run, or a column whose slices share a first row, takes an LDS atomic.  After a barrier
+// run, or a column whose slices share a first row, takes an LDS atomic.  (A column with a
+// cell outside the box, or a voxel over more than kFastRows rows, takes atomics for all of
+// its cells.)  After a barrier
 // every patch cell that received anything is added to the view with one FP32 global
 // atomic.  A cell outside the box, or every cell of a block whose box exceeds kPatch,
 // takes a global atomic directly.
 constexpr int kPtX = 16, kPtY = 16;  // image columns per tile
 constexpr int kSeg = 8;              // runs of columns, two per wave
 constexpr int kPatch = 7168;         // LDS patch cells (28 KB; four blocks per CU)
+constexpr int kFastRows = 4;         // v rows per voxel on the in-patch path
 
 struct PmatCol {
     float umin, umax, ampu, v0f, dvdz;
@@ -252,54 +255,92 @@ __global__ __launch_bounds__(256) void pmat_proj_kernel(PmatConst c, const PmatV
     __syncthreads();
     float* pview = acc + (size_t)b * c.nu * c.nv;
     const int lane = lane_id();
-    const int w = t / kWave;
+    const int w = uni(t / kWave);
     const int iz = z0 + lane;
     for (int h = 0; h < 2; ++h) {
         const int g = 2 * w + h;
+        // u cells of the runs the other waves walk in this phase
+        int olo[256 / kWave - 1], ohi[256 / kWave - 1];
+#pragma unroll
+        for (int w2 = 0, o = 0; w2 < 256 / kWave; ++w2) {
+            if (w2 == w) continue;
+            olo[o] = uni(run[2 * (2 * w2 + h)]);
+            ohi[o] = uni(run[2 * (2 * w2 + h) + 1]);
+            ++o;
+        }
         const int p0 = (g * nok + kSeg - 1) / kSeg, p1 = ((g + 1) * nok + kSeg - 1) / kSeg;
         for (int p = p0; p < p1; ++p) {
             const int k = uni(order[p]) & 255;
-            const PmatCol pc = cols[k];
+            PmatCol pc = cols[k];
+            pc.umin = uni(pc.umin);
+            pc.umax = uni(pc.umax);
+            pc.ampu = uni(pc.ampu);
+            pc.v0f = uni(pc.v0f);
+            pc.dvdz = uni(pc.dvdz);
             const int iumin = uni(pc.iumin), iumax = uni(pc.iumax);
             const int ix = tx * kPtX + k % kPtX, iy = ty * kPtY + k / kPtX;
             const float val = iz < c.nz ? vol[((size_t)iy * c.nx + ix) * c.nz + iz] : 0.0f;
             float vmin = 0.0f, vmax = 0.0f;
             int ivmin = 0, ivmax = -1;
             const bool vok = iz < c.nz && pmat_vcell(c, iz, pc.v0f, pc.dvdz, vmin, vmax, ivmin, ivmax);
+            const bool wr = vok && val != 0.0f;
             // plain read-add-write needs distinct first rows on the lanes that write
             const int prev_iv = __shfl_up(ivmin, 1);
             const int prev_ok = __shfl_up(vok ? 1 : 0, 1);
             const bool plain = !__any(lane > 0 && vok && prev_ok && prev_iv == ivmin);
-            if (!vok || val == 0.0f) continue;
+            const int nrows = wr ? ivmax - ivmin + 1 : 0;
+            const bool inbox = use_patch && iumin >= ulo && iumax < ulo + pu_n &&
+                               !__any(wr && (ivmin < vlo || ivmax >= vlo + pv_n || nrows > kFastRows));
             const float voxel = pc.ampu * val;
             const float wv0 = pmat_weight(vmin, vmax, ivmin, ivmax, ivmin);
             const float wv1 = pmat_weight(vmin, vmax, ivmin, ivmax, ivmax);
+            if (inbox) {
+                // every cell of the column lies in the patch and every voxel spans at most
+                // kFastRows rows: rows walked in step, lanes past their last row idle
+                const float wu0 = pmat_weight(pc.umin, pc.umax, iumin, iumax, iumin);
+                const float wu1 = pmat_weight(pc.umin, pc.umax, iumin, iumax, iumax);
+                for (int iu = iumin; iu <= iumax; ++iu) {
+                    const float wu = iu == iumin ? wu0 : (iu == iumax ? wu1 : 1.0f);
+                    bool own = plain;
+#pragma unroll
+                    for (int o = 0; o < 256 / kWave - 1; ++o)
+                        if (iu >= olo[o] && iu <= ohi[o]) own = false;
+                    float* row = patch + (iu - ulo) * pv_n + (ivmin - vlo);
+                    if (own) {
+#pragma unroll
+                        for (int j = 0; j < kFastRows; ++j) {
+                            if (!__any(j < nrows)) break;
+                            if (j < nrows) {
+                                const float add = wu * (j == 0 ? wv0 : (ivmin + j == ivmax ? wv1 : 1.0f)) * voxel;
+                                row[j] = row[j] + add;
+                            }
+                            // the next row's read must follow this write: the LDS runs
+                            // one wave's operations in order, and the empty asm keeps the
+                            // compiler from moving memory operations across it
+                            asm volatile("" ::: "memory");
+                        }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < kFastRows; ++j) {
+                            if (!__any(j < nrows)) break;
+                            if (j < nrows)
+                                atomicAdd(&row[j], wu * (j == 0 ? wv0 : (ivmin + j == ivmax ? wv1 : 1.0f)) * voxel);
+                        }
+                    }
+                }
+                continue;
+            }
+            if (!wr) continue;
             for (int iu = iumin; iu <= iumax; ++iu) {
                 const float wu = pmat_weight(pc.umin, pc.umax, iumin, iumax, iu);
-                bool own = plain;
-#pragma unroll
-                for (int w2 = 0; w2 < 256 / kWave; ++w2) {
-                    const int g2 = 2 * w2 + h;
-                    if (w2 != w && iu >= run[2 * g2] && iu <= run[2 * g2 + 1]) own = false;
-                }
                 const int pu = iu - ulo;
                 for (int iv = ivmin; iv <= ivmax; ++iv) {
                     const float add = wu * (iv == ivmin ? wv0 : (iv == ivmax ? wv1 : 1.0f)) * voxel;
                     const int pv = iv - vlo;
-                    if (use_patch && (unsigned)pu < (unsigned)pu_n && (unsigned)pv < (unsigned)pv_n) {
-                        if (own) {
-                            // the next row's read must follow this write: the LDS runs one
-                            // wave's operations in order, and the empty asm keeps the
-                            // compiler from moving memory operations across it
-                            float* cell = patch + pu * pv_n + pv;
-                            *cell = *cell + add;
-                            asm volatile("" ::: "memory");
-                        } else {
-                            atomicAdd(&patch[pu * pv_n + pv], add);
-                        }
-                    } else {
+                    if (use_patch && (unsigned)pu < (unsigned)pu_n && (unsigned)pv < (unsigned)pv_n)
+                        atomicAdd(&patch[pu * pv_n + pv], add);
+                    else
                         atomicAdd(&pview[(size_t)iu * c.nv + iv], add);
-                    }
                 }
             }
         }

@@ -74,19 +74,26 @@ def test_pmat_ops_match_oracle(oracle_mod, kind):
         mex.cbct_geom_pmat_mex("proj,rd", *args, vol, m)
 
 
-@pytest.mark.parametrize("case", ["tall", "fine"])
+@pytest.mark.parametrize("case", ["tall", "fine", "steep", "thin"])
 def test_pmat_forward_tiles_chunks_and_fallback(oracle_mod, case):
     """The forward's tiled form against the oracle where the easy case does not reach:
     "tall" has 150 slices (three slice chunks, the last one partial) and a detector that
     cuts the top and bottom slices; "fine" has 0.05 mm cells, so one voxel covers about 30
     of them (the footprint truncated at 16 u cells and 10 v rows, as the reference does) and
-    a tile's detector box exceeds the LDS patch, which sends every cell to global atomics."""
+    a tile's detector box exceeds the LDS patch, which sends every cell to global atomics;
+    "steep" has 2 mm slices over 0.5 mm rows (six rows a voxel, past the in-patch row walk,
+    so LDS atomics inside the patch); "thin" has 0.3 mm slices over 1 mm rows (about two
+    slices a row, so neighbouring lanes share a first row and the plain adds are off)."""
     O = oracle_mod
     from ctsim_amd import mex
     if case == "tall":
         n, d, o, nuv, duv, nviews = (24, 20, 150), (1.0, 1.0, 0.9), (0.3, -0.2, 0.0), (48, 180), (1.0, 1.0), 6
-    else:
+    elif case == "fine":
         n, d, o, nuv, duv, nviews = (20, 18, 10), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (400, 200), (0.05, 0.05), 4
+    elif case == "steep":
+        n, d, o, nuv, duv, nviews = (12, 10, 30), (0.5, 0.5, 2.0), (0.0, 0.0, 0.0), (64, 200), (0.5, 0.5), 5
+    else:
+        n, d, o, nuv, duv, nviews = (20, 18, 70), (1.0, 1.0, 0.3), (0.0, 0.0, 0.0), (48, 40), (1.0, 1.0), 5
     P, cams, sdds, ouv = O.circular_pmat(np.linspace(0, 2 * np.pi, nviews, endpoint=False) + 0.3, 1000.0, 1500.0,
                                          nuv, duv)
     m = np.ones(n[:2], bool)
