@@ -80,16 +80,6 @@ __device__ __forceinline__ float pmat_weight(float lo, float hi, int imin, int i
     return i == imin ? (float)((double)(float)imin + 1.0 - (double)lo) : (i == imax ? hi - (float)imax : 1.0f);
 }
 
-// u footprint with its weights w[0 .. iumax - iumin]
-__device__ __forceinline__ bool pmat_ufoot(const PmatConst& c, const PmatView& pv, int ix, int iy, float& ampu,
-                                           int& iumin, int& iumax, float* w, float& v0f, float& dvdz) {
-    float umin, umax;
-    if (!pmat_ucell(c, pv, ix, iy, ampu, umin, umax, iumin, iumax, v0f, dvdz)) return false;
-#pragma unroll
-    for (int k = 0; k < kPmFootU; ++k) w[k] = iumin + k <= iumax ? pmat_weight(umin, umax, iumin, iumax, iumin + k) : 0.0f;
-    return true;
-}
-
 // v footprint of voxel iz (cbct_proj_pmat.c:155-190): rows [ivmin, ivmax] covering
 // [vmin, vmax]
 __device__ __forceinline__ bool pmat_vcell(const PmatConst& c, int iz, float v0f, float dvdz, float& vmin,
@@ -113,16 +103,6 @@ __device__ __forceinline__ bool pmat_vcell(const PmatConst& c, int iz, float v0f
     ivmin = (int)floorf(vmin);
     ivmax = min((int)floorf(vmax), c.nv - 1);
     ivmax = min(ivmax, ivmin + kPmFootV - 1);
-    return true;
-}
-
-// v footprint with its weights w[0 .. ivmax - ivmin]
-__device__ __forceinline__ bool pmat_vfoot(const PmatConst& c, int iz, float v0f, float dvdz, int& ivmin,
-                                           int& ivmax, float* w) {
-    float vmin, vmax;
-    if (!pmat_vcell(c, iz, v0f, dvdz, vmin, vmax, ivmin, ivmax)) return false;
-#pragma unroll
-    for (int k = 0; k < kPmFootV; ++k) w[k] = ivmin + k <= ivmax ? pmat_weight(vmin, vmax, ivmin, ivmax, ivmin + k) : 0.0f;
     return true;
 }
 
@@ -371,23 +351,37 @@ __global__ __launch_bounds__(256) void pmat_back_dd_kernel(PmatConst c, const Pm
     for (int k = 0; k < NZL; ++k) img[k] = 0.0f;
     if (map[col]) {
         const int ix = col % c.nx, iy = col / c.nx;
-        float wu[kPmFootU], wv[kPmFootV];
         for (int b = 0; b < c.nviews; ++b) {
+            // the column's u footprint is the wave's: kept in scalar registers, and the
+            // weights of its first and last cells (1 between) formed once per view
             const PmatView pv = views[b];
-            float ampu, v0f, dvdz;
+            float ampu, umin, umax, v0f, dvdz;
             int iumin, iumax;
-            if (!pmat_ufoot(c, pv, ix, iy, ampu, iumin, iumax, wu, v0f, dvdz)) continue;
+            if (!pmat_ucell(c, pv, ix, iy, ampu, umin, umax, iumin, iumax, v0f, dvdz)) continue;
+            iumin = uni(iumin);
+            iumax = uni(iumax);
+            ampu = uni(ampu);
+            v0f = uni(v0f);
+            dvdz = uni(dvdz);
+            const float wu0 = uni(pmat_weight(umin, umax, iumin, iumax, iumin));
+            const float wu1 = uni(pmat_weight(umin, umax, iumin, iumax, iumax));
             const float* pview = sino + (size_t)b * c.nu * c.nv;
 #pragma unroll
             for (int k = 0; k < NZL; ++k) {
                 const int iz = k * kWave + lane;
                 if (iz >= c.nz) continue;
+                float vmin, vmax;
                 int ivmin, ivmax;
-                if (!pmat_vfoot(c, iz, v0f, dvdz, ivmin, ivmax, wv)) continue;
+                if (!pmat_vcell(c, iz, v0f, dvdz, vmin, vmax, ivmin, ivmax)) continue;
+                const float wv0 = pmat_weight(vmin, vmax, ivmin, ivmax, ivmin);
+                const float wv1 = pmat_weight(vmin, vmax, ivmin, ivmax, ivmax);
                 float voxel = 0.0f;
-                for (int iu = iumin; iu <= iumax; ++iu)
+                for (int iu = iumin; iu <= iumax; ++iu) {
+                    const float wu = iu == iumin ? wu0 : (iu == iumax ? wu1 : 1.0f);
+                    const float* prow = pview + (size_t)iu * c.nv;
                     for (int iv = ivmin; iv <= ivmax; ++iv)
-                        voxel += pview[(size_t)iu * c.nv + iv] * wu[iu - iumin] * wv[iv - ivmin];
+                        voxel += prow[iv] * wu * (iv == ivmin ? wv0 : (iv == ivmax ? wv1 : 1.0f));
+                }
                 img[k] += ampu * voxel;
             }
         }
