@@ -143,6 +143,9 @@ RdConst rd_const(const ctsim_geom& g, int ndim) {
     c.nz = ndim == 3 ? g.nz : 1;
     c.nu = g.nu;
     c.nv = ndim == 3 ? g.nv : 1;
+    c.inv_dx = 1.0 / (double)c.dx;
+    c.inv_dy = 1.0 / (double)c.dy;
+    c.inv_dz = 1.0 / (double)c.dz;
     return c;
 }
 

@@ -64,6 +64,7 @@ struct RayRec {
 struct RdConst {
     float wv, cmin_x, cmin_y, delta, dx, dy, dz, dv;
     int nx, ny, nz, nu, nv;
+    double inv_dx, inv_dy, inv_dz;  // 1 / dx, 1 / dy, 1 / dz in double
 };
 hipError_t cbct_rd_launch(const RayRec* rays, int view0, int vstep, int nviews, const RdConst& c, const void* vol,
                           int vol_dt, void* proj, int proj_dt, hipStream_t st);

@@ -8,7 +8,8 @@
 // discontinuous function of last-ulp differences: the march reproduces the reference's
 // operations one for one.  The per-(view, column) set-up (double sin / atan, float sinf,
 // the slab tests) comes from a host-built table (RayRec); in the kernels FMA contraction
-// is off and divisions and square roots are the correctly rounded ones.
+// is off and divisions and square roots are the correctly rounded ones (the per-sample
+// divisions by the voxel size through div_rn).
 //
 // 3D: one wavefront per (view, detector column, 64 detector rows); lanes are rows.  The
 // ray's in-plane position is the same on every lane (x, y depend on t only), so the
@@ -20,6 +21,13 @@ namespace ctsim {
 
 constexpr int kRdWave = 64;
 constexpr int kRdBlock = 256;
+
+// x / d rounded to float, from the double reciprocal of d: the double product is within
+// 2^-52 (relative) of x / d, while a quotient of two floats is never within 2^-50 of a
+// float rounding midpoint (a midpoint's odd 25-bit significand times d's cannot be x's
+// 24-bit one), so rounding the product to float gives the correctly rounded division.
+// Three instructions in place of the division's scale / reciprocal / fix-up sequence.
+__device__ __forceinline__ float div_rn(float x, double inv_d) { return (float)((double)x * inv_d); }
 
 template <class Tin, class Tout>
 __global__ __launch_bounds__(kRdBlock) void cbct_rd_kernel(const RayRec* __restrict__ rays, int view0, int vstep,
@@ -44,14 +52,14 @@ __global__ __launch_bounds__(kRdBlock) void cbct_rd_kernel(const RayRec* __restr
     for (float t = r.tn; t < r.tf; t = t + c.delta) {
         if (__ballot(!done) == 0ull) break;  // every row of the wave has left the volume
         if (done) continue;
-        const float z = (0.0f + t * rdz - r.czmin) / c.dz;
+        const float z = div_rn(0.0f + t * rdz - r.czmin, c.inv_dz);
         if (z < 1.0f) continue;
         if ((double)z > zlim) {
             done = true;
             continue;
         }
-        const float x = (r.sx + t * r.rdx - c.cmin_x) / c.dx;
-        const float y = (r.sy + t * r.rdy - c.cmin_y) / c.dy;
+        const float x = div_rn(r.sx + t * r.rdx - c.cmin_x, c.inv_dx);
+        const float y = div_rn(r.sy + t * r.rdy - c.cmin_y, c.inv_dy);
         const int ix = (int)floorf(x), iy = (int)floorf(y), iz = (int)floorf(z);
         const float wzl = z - (float)iz;
         if (ix > 0 && ix < c.nx - 1 && iy > 0 && iy < c.ny - 1) {
@@ -83,8 +91,8 @@ __global__ __launch_bounds__(kRdBlock) void fbct_rd_kernel(const RayRec* __restr
     const RayRec r = rays[(size_t)(view0 + (size_t)k * vstep) * c.nu + iu];
     float temp = 0.0f;
     for (float t = r.tn; t < r.tf; t = t + c.delta) {
-        const float x = (r.sx + t * r.rdx - c.cmin_x) / c.dx;
-        const float y = (r.sy + t * r.rdy - c.cmin_y) / c.dy;
+        const float x = div_rn(r.sx + t * r.rdx - c.cmin_x, c.inv_dx);
+        const float y = div_rn(r.sy + t * r.rdy - c.cmin_y, c.inv_dy);
         const int ix = (int)floorf(x), iy = (int)floorf(y);
         if (ix > 0 && ix < c.nx - 1 && iy > 0 && iy < c.ny - 1) {
             const float wxl = x - (float)ix, wyl = y - (float)iy;
