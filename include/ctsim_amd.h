@@ -106,8 +106,9 @@ int ctsim_cbct_back_pd_cuda(const char* op, const int32_t nxyz[3], const float d
  * duv[0], duv[1] for a two-element one.  ops proj,dd | proj,tf (runs dd, as the
  * reference does after its warning) | back,dd | back,tf (dd) | back,pd | back,hpd;
  * proj,rd is refused with the reference's message.  FP32, at most 512 slices.  proj,dd
- * adds voxel footprints with FP32 atomics, so its last bits can vary from run to run;
- * the backprojectors gather (one writer per voxel). */
+ * sums voxel footprints in on-chip patches and adds them to the sinogram with FP32
+ * atomics, so its last bits can vary from run to run; the backprojectors gather (one
+ * writer per voxel). */
 int ctsim_cbct_geom_pmat_mex(const char* op, const int32_t nxyz[3], const float dxyz[3], const float cxyz[3],
                              const int32_t nuv[2], float du, float dv, int32_t noviews, const float* pmat,
                              const float* cameras, const float* sdds, const float* cuvs, const float* in,
